@@ -1,0 +1,98 @@
+/*
+ * tdbem.h — C ABI of the MI355X-native TD-BEM hot paths (libtdbem.so, built from td-bem_amd/csrc).
+ *
+ * The reference (XiaoyongBai/TD-BEM) has no FFI; its hot paths sit behind three C++ classes that
+ * main.cpp drives (src/main.cpp:43-128).  Each entry point below names the member function(s) it
+ * replaces.  Conventions:
+ *   - every function returns TDBEM_OK (0) or a negative status; tdbem_last_error(ctx) gives the
+ *     message (for a failed tdbem_create, tdbem_last_error(NULL) gives it), which the C++ host
+ *     rethrows as `const char*` the way the reference reports errors (src/main.cpp:132-135);
+ *   - ids are 0-based (the deck's 1-based ids are converted on reading, src/InputT.cpp:529-534);
+ *     dof = 3*node + direction;
+ *   - dense matrices are row-major doubles; "host" pointers are ordinary host memory;
+ *   - one host thread per context; each context owns one HIP stream of default priority, so
+ *     independent contexts can run concurrently on one GPU.
+ */
+#ifndef TDBEM_H
+#define TDBEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tdbem_context tdbem_context;
+
+enum {
+    TDBEM_OK = 0,
+    TDBEM_ERR_ARG = -1,     /* bad argument (the reference's thrown const char* cases) */
+    TDBEM_ERR_HIP = -2,     /* HIP runtime failure */
+    TDBEM_ERR_STATE = -3,   /* call out of order */
+    TDBEM_ERR_NUMERIC = -4  /* singular step-0 matrix */
+};
+
+/* Temporal kernels, GreenFunctionT::Compute(type) (src/GreenFunctionT.cpp:119-145). */
+enum {
+    TDBEM_KERNEL_HEAVISIDE = 2, /* TDElementHeavisideLinearT (assemble_step only) */
+    TDBEM_KERNEL_STEP = 4       /* TDElementStepLinearT, the one main.cpp runs
+                                   (src/ElastoDynamicsT.cpp:27) */
+};
+
+/* Problem set-up.  Replaces ElastoDynamicsT::ElastoDynamicsT + TDElementT::SetNQ/SetMaterial
+ * (src/ElastoDynamicsT.cpp:21-43, src/TDElementT.cpp:100-139) and the SolverT constructor with
+ * SetTimeControl and ComputeMaxStep (src/SolverT.cpp:23-51, :101-155, :211-250).
+ * coords [nnd][3]; ien [nel][4 or 8]; etype 1 (4-node) or 2 (8-node); nq in {6,10,20,40}. */
+int tdbem_create(int nnd, const double* coords, int nel, const int* ien, int etype, int nq,
+                 double E, double nu, double rho, int if_ext, double dt, int numstep, int kernel,
+                 tdbem_context** out);
+void tdbem_destroy(tdbem_context* ctx);
+const char* tdbem_last_error(const tdbem_context* ctx);
+
+/* SolverT::GetMaxStep (src/SolverT.cpp:252-255). */
+int tdbem_max_step(const tdbem_context* ctx);
+
+/* ElastoDynamicsT::G_H_Driver(t = step*dt, t_1 = dt) followed by GetGHLinear
+ * (src/ElastoDynamicsT.cpp:108-231, :283-289): dense row blocks of collocation nodes
+ * [lower, upper], each [3(upper-lower+1)][3 nnd].  G1/H1 are fG_1/fH_1 (the phi_1 part, called
+ * G_0/H_0 in main.cpp:100-105), G2/H2 are fG_2/fH_2 (phi_2, main.cpp's G_1/H_1).
+ * The _device variant takes device pointers and does not synchronise. */
+int tdbem_assemble_step(tdbem_context* ctx, int step, int lower, int upper, double* G1,
+                        double* G2, double* H1, double* H2);
+int tdbem_assemble_step_device(tdbem_context* ctx, int step, int lower, int upper, double* G1,
+                               double* G2, double* H1, double* H2);
+
+/* The lag matrices that SolverStepLinearT::UpdateGHLinear accumulates over steps
+ * 1..min(numstep, MaxStep) (src/SolverStepLinearT.cpp:58-97), built on the device in one pass and
+ * kept only inside each node pair's wavefront shell. */
+int tdbem_build_lag_store(tdbem_context* ctx);
+/* Bytes of the shell store (predicted when not yet built). */
+int64_t tdbem_lag_store_bytes(tdbem_context* ctx);
+/* Dense N x N (N = 3 nnd) lag k of G and H on the host: fG_DB[k], fH_DB[k]. */
+int tdbem_get_lag(tdbem_context* ctx, int k, double* G, double* H);
+
+/* SolverStepLinearT::Initialize/SetLoad/Solve_Averaging set-up (src/SolverStepLinearT.cpp:30-56,
+ * :314-393; src/SolverT.cpp:344-377): alpha (a1, a2), displacement (UBC) and traction (FBC)
+ * boundary data, and nrhs in {1,2,4,8} load histories f_r(t) = amp_r * min(1, t / t_rise_r)
+ * (t_rise_r <= 0: Heaviside; amp = t_rise = NULL: the reference's f = 1).  Builds the lag store if
+ * needed and factors the step-0 matrices. */
+int tdbem_march_init(tdbem_context* ctx, double a1, double a2, int n_ubc, const int* ubc_dof,
+                     const double* ubc_val, int n_fbc, const int* fbc_dof, const double* fbc_val,
+                     int nrhs, const double* amp, const double* t_rise);
+/* Advances up to nsteps of main.cpp's step loop (src/main.cpp:89-128: SetLoad + Solve +
+ * RecordResult), stopping after the last step. */
+int tdbem_march_run(tdbem_context* ctx, int nsteps);
+/* Index s of the last solved history vector x_s (0 before the first solve). */
+int tdbem_march_step_index(const tdbem_context* ctx);
+/* Histories of displacement and traction, x_0 .. x_{numstep-1} as RecordResult sees them
+ * (src/SolverStepLinearT.cpp:549-603): U, T are host [numstep][3 nnd]. */
+int tdbem_get_history(tdbem_context* ctx, int rhs, double* U, double* T);
+
+/* Waits for all work queued on the context's stream. */
+int tdbem_synchronize(tdbem_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,825 @@
+// assemble.hip — hot path 1: G/H influence-matrix assembly on CDNA4.
+//
+// Replaces ElastoDynamicsT::G_H_Driver / TDElementStepLinearT::FormGH_* (src/ElastoDynamicsT.cpp:
+// 108-280, src/TDElementStepLinearT.cpp:28-626).  Two products:
+//   * assemble_step_dense: the per-step dense row blocks fG_1, fG_2, fH_1, fH_2 exactly as
+//     G_H_Driver leaves them (parity and the FE-BE matrix export);
+//   * build_lag_store: every lag matrix M_k = phi_1(step k) + phi_2(step k+1) of
+//     SolverStepLinearT::UpdateGHLinear (src/SolverStepLinearT.cpp:58-97), for k = 0..Kc, kept
+//     only inside each node pair's wavefront shell (lags where the Step kernel can be nonzero).
+//
+// Work decomposition (both products): one thread per (collocation node i, element e, lag/part)
+// integrates the element over its quadrature points, skipping points whose Step kernel is zero
+// at that lag (before the P wave arrives or after both waves have passed), and writes per-node
+// 3x3 blocks to a scratch slot; a second kernel gathers, for each (i, column node j), the slots
+// of the elements around j.  The gather is deterministic (fixed order over the node->element
+// adjacency), so results do not depend on scheduling.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+#include "tdbem_internal.h"
+
+namespace tdbem {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(int64_t n, int block = kBlock)
+{
+    return (unsigned)((n + block - 1) / block);
+}
+
+template <int ENND>
+__device__ __forceinline__ int singular_node(const double* X, const double* coll)
+{
+    // src/ElastoDynamicsT.cpp:166-175: the last element node within 1e-9 (squared) wins.
+    int s = -1;
+#pragma unroll
+    for (int a = 0; a < ENND; a++) {
+        const double dx = X[3 * a] - coll[0], dy = X[3 * a + 1] - coll[1],
+                     dz = X[3 * a + 2] - coll[2];
+        if (dx * dx + dy * dy + dz * dz < 1e-9) s = a;
+    }
+    return s;
+}
+
+template <int ENND>
+__device__ __forceinline__ void load_element(const double* coords, const int* ien, int e,
+                                             double* X)
+{
+#pragma unroll
+    for (int a = 0; a < ENND; a++) {
+        const int g = ien[e * ENND + a];
+        X[3 * a] = coords[3 * g];
+        X[3 * a + 1] = coords[3 * g + 1];
+        X[3 * a + 2] = coords[3 * g + 2];
+    }
+}
+
+// Visits every quadrature point of pair (coll, element): f(x, n, wJ, N[ENND]).
+// Regular: nq x nq Gauss tensor rule from the precomputed table (src/TDElementStepLinearT.cpp:
+// 52-61).  Singular: two Duffy triangles of nq x nq (:436-530); 8-node elements are split into
+// bilinear sub-elements whose shape values land on their parent nodes (:149-175, :597-626).
+template <int ENND, typename F>
+__device__ __forceinline__ void for_each_point(const double* X, int sing, const double* geo_e,
+                                               const double* gauss, int nq, F&& f)
+{
+    constexpr int GS = 7 + ENND;
+    if (sing < 0) {
+        const int nq2 = nq * nq;
+        for (int q = 0; q < nq2; q++) {
+            const double* p = geo_e + q * GS;
+            f(p, p + 3, p[6], p + 7);
+        }
+        return;
+    }
+    const double* gx = gauss;
+    const double* gw = gauss + nq;
+    if constexpr (ENND == 4) {
+        for (int tri = 0; tri < 2; tri++)
+            for (int q1 = 0; q1 < nq; q1++)
+                for (int q2 = 0; q2 < nq; q2++) {
+                    double e1, e2, Jt, N[4], x[3], n[3], J;
+                    duffy(sing, tri, gx[q1], gx[q2], e1, e2, Jt);
+                    shape_eval<4>(X, e1, e2, N, x, n, J);
+                    f(x, n, gw[q1] * gw[q2] * Jt * J, N);
+                }
+    } else {
+        int ids[4], nsub, cid;
+        sub_element(sing, 0, ids, nsub, cid);
+        for (int se = 0; se < nsub; se++) {
+            sub_element(sing, se, ids, nsub, cid);
+            double Xs[12];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                Xs[3 * k] = X[3 * ids[k]];
+                Xs[3 * k + 1] = X[3 * ids[k] + 1];
+                Xs[3 * k + 2] = X[3 * ids[k] + 2];
+            }
+            for (int tri = 0; tri < 2; tri++)
+                for (int q1 = 0; q1 < nq; q1++)
+                    for (int q2 = 0; q2 < nq; q2++) {
+                        double e1, e2, Jt, N4[4], x[3], n[3], J, N[8];
+                        duffy(cid, tri, gx[q1], gx[q2], e1, e2, Jt);
+                        shape_eval<4>(Xs, e1, e2, N4, x, n, J);
+#pragma unroll
+                        for (int a = 0; a < 8; a++) N[a] = 0.0;
+#pragma unroll
+                        for (int k = 0; k < 4; k++) N[ids[k]] = N4[k];
+                        f(x, n, gw[q1] * gw[q2] * Jt * J, N);
+                    }
+        }
+    }
+}
+
+// Time-integral combination S for one point: sum over three time levels of cI*I(t) + cJ*J(t).
+// Returns false when the combination is zero: every level with a nonzero coefficient is before
+// the P-wave arrival (t <= r/c_p), or (when the coefficients sum to zero, as for the Step
+// kernel) every such level is in the static regime t - r/c_s >= dt, where HL(t) no longer
+// depends on t (src/GreenFunctionT.cpp:251-314 with b = dt for both waves).
+struct TimeCombo {
+    double t[3];
+    double cI[3], cJ[3];
+    bool static_ok;
+};
+
+__device__ __forceinline__ bool eval_s(const TimeCombo& tc, double r, double dt, double inv_dt,
+                                       const Material& m, double* S)
+{
+    const double inv_cp = 1.0 / m.cp, inv_cs = 1.0 / m.cs;
+    const double ta = r * inv_cp;         // P-wave arrival
+    const double ts = r * inv_cs + dt;    // static from here on
+    bool any = false, all_static = true;
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+        if (tc.cI[l] == 0.0 && tc.cJ[l] == 0.0) continue;
+        if (tc.t[l] > ta) any = true;
+        if (tc.t[l] < ts) all_static = false;
+    }
+    if (!any || (tc.static_ok && all_static)) return false;
+    const double inv_r2 = 1.0 / (r * r);
+#pragma unroll
+    for (int l = 0; l < 6; l++) S[l] = 0.0;
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+        if ((tc.cI[l] == 0.0 && tc.cJ[l] == 0.0) || !(tc.t[l] > ta)) continue;
+        accumulate_hl<double>(tc.t[l], dt, inv_dt, r, inv_r2, inv_cp, inv_cs, inv_cp * inv_cp,
+                              inv_cs * inv_cs, tc.cI[l], tc.cJ[l], S);
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Geometry table: shape functions, positions, unit normals and w*J at the regular Gauss points.
+// ---------------------------------------------------------------------------------------------
+template <int ENND>
+__global__ void k_geom(const double* __restrict__ coords, const int* __restrict__ ien,
+                       const double* __restrict__ gauss, int nq, int nel, double* __restrict__ geo)
+{
+    const int nq2 = nq * nq;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= nel * nq2) return;
+    const int e = tid / nq2, q = tid % nq2;
+    const int q1 = q / nq, q2 = q % nq;
+    double X[3 * ENND], N[ENND], x[3], n[3], J;
+    load_element<ENND>(coords, ien, e, X);
+    shape_eval<ENND>(X, gauss[q1], gauss[q2], N, x, n, J);
+    double* o = geo + ((size_t)e * nq2 + q) * (7 + ENND);
+    o[0] = x[0];
+    o[1] = x[1];
+    o[2] = x[2];
+    o[3] = n[0];
+    o[4] = n[1];
+    o[5] = n[2];
+    o[6] = gauss[nq + q1] * gauss[nq + q2] * J;
+#pragma unroll
+    for (int a = 0; a < ENND; a++) o[7 + a] = N[a];
+}
+
+// ---------------------------------------------------------------------------------------------
+// C-term: C(i) = - sum_e sum_q wJ * (Kelvin traction * dt/2)^T, added by AssembleC to the
+// diagonal block (src/ElastoDynamicsT.cpp:204-205, :261-280).  One block per collocation node,
+// fixed-shape tree reduction.
+// ---------------------------------------------------------------------------------------------
+template <int ENND>
+__global__ void __launch_bounds__(kBlock) k_cterm(const double* __restrict__ coords,
+                                                  const int* __restrict__ ien,
+                                                  const double* __restrict__ geo,
+                                                  const double* __restrict__ gauss, int nq,
+                                                  int nel, double dt, Material m,
+                                                  double* __restrict__ cterm)
+{
+    __shared__ double red[9][kBlock];
+    const int i = blockIdx.x;
+    const double coll[3] = {coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]};
+    double acc[9];
+#pragma unroll
+    for (int v = 0; v < 9; v++) acc[v] = 0.0;
+    const int nq2 = nq * nq;
+    for (int e = threadIdx.x; e < nel; e += blockDim.x) {
+        double X[3 * ENND];
+        load_element<ENND>(coords, ien, e, X);
+        const int sing = singular_node<ENND>(X, coll);
+        for_each_point<ENND>(X, sing, geo + (size_t)e * nq2 * (7 + ENND), gauss, nq,
+                             [&](const double* x, const double* n, double wJ, const double*) {
+                                 double d[3] = {x[0] - coll[0], x[1] - coll[1], x[2] - coll[2]};
+                                 const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                                 const double ir = 1.0 / r;
+                                 d[0] *= ir;
+                                 d[1] *= ir;
+                                 d[2] *= ir;
+                                 double KT[9];
+                                 kelvin_block(d, n, r, dt, m, KT);
+#pragma unroll
+                                 for (int v = 0; v < 9; v++) acc[v] -= wJ * KT[v];
+                             });
+    }
+#pragma unroll
+    for (int v = 0; v < 9; v++) red[v][threadIdx.x] = acc[v];
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+#pragma unroll
+            for (int v = 0; v < 9; v++) red[v][threadIdx.x] += red[v][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 9) cterm[9 * i + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Shell bounds per (node, element) pair: lags k with M_k possibly nonzero.
+// P_n (the Step kernel at step n) is nonzero only for r/c_p < n dt < r/c_s + 2 dt at some point;
+// M_k uses P_k and P_{k+1}, so k runs over [n_lo - 1, n_hi] clipped to [0, Kc].
+// ---------------------------------------------------------------------------------------------
+template <int ENND>
+__global__ void k_pair_range(const double* __restrict__ coords, const int* __restrict__ ien,
+                             const double* __restrict__ geo, const double* __restrict__ gauss,
+                             int nq, int nnd, int nel, double dt, Material m, int Kc,
+                             int2* __restrict__ pair_k)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (int64_t)nnd * nel) return;
+    const int i = (int)(tid / nel), e = (int)(tid % nel);
+    const double coll[3] = {coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]};
+    double X[3 * ENND];
+    load_element<ENND>(coords, ien, e, X);
+    const int sing = singular_node<ENND>(X, coll);
+    double rmin = 1e300, rmax = 0.0;
+    for_each_point<ENND>(X, sing, geo + (size_t)e * nq * nq * (7 + ENND), gauss, nq,
+                         [&](const double* x, const double*, double, const double*) {
+                             const double d0 = x[0] - coll[0], d1 = x[1] - coll[1],
+                                          d2 = x[2] - coll[2];
+                             const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+                             rmin = fmin(rmin, r);
+                             rmax = fmax(rmax, r);
+                         });
+    const double tol = 1e-9;
+    const int n_lo = max(1, (int)floor(rmin / (m.cp * dt) - tol) + 1);
+    const int n_hi = min(Kc, (int)ceil(rmax / (m.cs * dt) + tol) + 1);
+    int2 out;
+    if (n_lo > Kc || n_hi < n_lo) {
+        out.x = 1;
+        out.y = 0;
+    } else {
+        out.x = n_lo - 1;
+        out.y = n_hi;
+    }
+    pair_k[tid] = out;
+}
+
+// Hull of the pair shells of the elements around column node j, for each (i, j).
+__global__ void k_node_hull(const int2* __restrict__ pair_k, const int* __restrict__ adj_off,
+                            const int* __restrict__ adj, int nnd, int nel,
+                            int2* __restrict__ hull)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (int64_t)nnd * nnd) return;
+    const int i = (int)(tid / nnd), j = (int)(tid % nnd);
+    int lo = 1 << 30, hi = -1;
+    for (int a = adj_off[j]; a < adj_off[j + 1]; a++) {
+        const int e = adj[a] >> 3;
+        const int2 pk = pair_k[(int64_t)i * nel + e];
+        if (pk.y < pk.x) continue;
+        lo = min(lo, pk.x);
+        hi = max(hi, pk.y);
+    }
+    hull[tid] = hi < lo ? make_int2(1, 0) : make_int2(lo, hi);
+}
+
+__global__ void k_fill_idx(const int2* __restrict__ hull, const int64_t* __restrict__ off,
+                           int nnd, uint32_t* __restrict__ idx)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (int64_t)nnd * nnd) return;
+    const int j = (int)(tid % nnd);
+    const int2 h = hull[tid];
+    const int64_t o = off[tid];
+    for (int k = h.x; k <= h.y; k++) idx[o + (k - h.x)] = ((uint32_t)j << kKBits) | (uint32_t)k;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Element-pair integration into scratch slots.  Mode LAG: slot = (pair, k) gives
+// M_k = [k>=1](I(t_k) - I(t_{k-1})) + [k+1<=Kc](J(t_{k+1}) - J(t_k)).  Mode STEP: slot =
+// (pair, part) gives phi_1 (part 0) or phi_2 (part 1) of step n; for the Heaviside kernel
+// (src/GreenFunctionT.cpp:126-129) there is no (t - dt) level.
+// ---------------------------------------------------------------------------------------------
+template <int ENND>
+__device__ __forceinline__ void integrate_pair(const double* coords, const int* ien,
+                                               const double* geo, const double* gauss, int nq,
+                                               int i, int e, const TimeCombo& tc, double dt,
+                                               const Material& m, double* out)
+{
+    const double coll[3] = {coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]};
+    double X[3 * ENND];
+    load_element<ENND>(coords, ien, e, X);
+    const int sing = singular_node<ENND>(X, coll);
+    double acc[ENND][kVals];
+#pragma unroll
+    for (int a = 0; a < ENND; a++)
+#pragma unroll
+        for (int v = 0; v < kVals; v++) acc[a][v] = 0.0;
+    const double inv_dt = 1.0 / dt;
+    for_each_point<ENND>(X, sing, geo + (size_t)e * nq * nq * (7 + ENND), gauss, nq,
+                         [&](const double* x, const double* n, double wJ, const double* N) {
+                             double d[3] = {x[0] - coll[0], x[1] - coll[1], x[2] - coll[2]};
+                             const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                             double S[6];
+                             if (!eval_s(tc, r, dt, inv_dt, m, S)) return;
+                             const double ir = 1.0 / r;
+                             d[0] *= ir;
+                             d[1] *= ir;
+                             d[2] *= ir;
+                             double blk[kVals];
+                             blocks_from_s<double>(S, d, n, r, ir, ir * ir, m, blk);
+#pragma unroll
+                             for (int a = 0; a < ENND; a++) {
+                                 const double wN = wJ * N[a];
+#pragma unroll
+                                 for (int v = 0; v < kVals; v++) acc[a][v] += wN * blk[v];
+                             }
+                         });
+#pragma unroll
+    for (int a = 0; a < ENND; a++)
+#pragma unroll
+        for (int v = 0; v < kVals; v++) out[a * kVals + v] = acc[a][v];
+}
+
+template <int ENND>
+__global__ void __launch_bounds__(kBlock)
+    k_pair_lags(const double* __restrict__ coords, const int* __restrict__ ien,
+                const double* __restrict__ geo, const double* __restrict__ gauss, int nq, int nel,
+                double dt, Material m, int Kc, int i0, int64_t nslots,
+                const int* __restrict__ slot_pair, const int64_t* __restrict__ pair_slot,
+                const int2* __restrict__ pair_k, double* __restrict__ scratch)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const int p = slot_pair[s];
+    const int i = i0 + p / nel, e = p % nel;
+    const int k = pair_k[(int64_t)i * nel + e].x + (int)(s - pair_slot[p]);
+    TimeCombo tc;
+    tc.t[0] = (k - 1) * dt;
+    tc.t[1] = k * dt;
+    tc.t[2] = (k + 1) * dt;
+    const double hasI = k >= 1 ? 1.0 : 0.0, hasJ = (k + 1 <= Kc) ? 1.0 : 0.0;
+    tc.cI[0] = -hasI;
+    tc.cJ[0] = 0.0;
+    tc.cI[1] = hasI;
+    tc.cJ[1] = -hasJ;
+    tc.cI[2] = 0.0;
+    tc.cJ[2] = hasJ;
+    tc.static_ok = true;
+    integrate_pair<ENND>(coords, ien, geo, gauss, nq, i, e, tc, dt, m,
+                         scratch + s * (ENND * kVals));
+}
+
+template <int ENND>
+__global__ void __launch_bounds__(kBlock)
+    k_pair_step(const double* __restrict__ coords, const int* __restrict__ ien,
+                const double* __restrict__ geo, const double* __restrict__ gauss, int nq, int nel,
+                double dt, Material m, int step, int heaviside, int i0, int nrow_nodes,
+                double* __restrict__ scratch)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= (int64_t)nrow_nodes * nel * 2) return;
+    const int part = (int)(s & 1);
+    const int64_t p = s >> 1;
+    const int i = i0 + (int)(p / nel), e = (int)(p % nel);
+    TimeCombo tc;
+    tc.t[0] = step * dt - dt;   // fT - fT_1, as Stokes_Step_Linear forms it
+    tc.t[1] = step * dt;
+    tc.t[2] = 0.0;
+    const double prev = heaviside ? 0.0 : -1.0;
+    tc.cI[0] = part == 0 ? prev : 0.0;
+    tc.cJ[0] = part == 1 ? prev : 0.0;
+    tc.cI[1] = part == 0 ? 1.0 : 0.0;
+    tc.cJ[1] = part == 1 ? 1.0 : 0.0;
+    tc.cI[2] = 0.0;
+    tc.cJ[2] = 0.0;
+    tc.static_ok = !heaviside;
+    integrate_pair<ENND>(coords, ien, geo, gauss, nq, i, e, tc, dt, m,
+                         scratch + s * (ENND * kVals));
+}
+
+// Dense row blocks of one step: thread per (row node il, column node j).
+__device__ __forceinline__ void put_block(double* M, int64_t ncol, int il, int j, const double* g,
+                                          bool sym)
+{
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            double v;
+            if (sym) {
+                const int a = p < q ? p : q, b = p < q ? q : p;
+                const int o = a == 0 ? b : (a == 1 ? 2 + b : 5);
+                v = g[o];
+            } else {
+                v = g[3 * p + q];
+            }
+            M[(int64_t)(3 * il + p) * ncol + 3 * j + q] = v;
+        }
+}
+
+template <int ENND>
+__global__ void k_dense_gather(const double* __restrict__ scratch, const int* __restrict__ adj_off,
+                               const int* __restrict__ adj, const double* __restrict__ cterm,
+                               int nnd, int nel, int i0, int nrow_nodes, int add_c, double ext,
+                               double* __restrict__ G1, double* __restrict__ G2,
+                               double* __restrict__ H1, double* __restrict__ H2)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (int64_t)nrow_nodes * nnd) return;
+    const int il = (int)(tid / nnd), j = (int)(tid % nnd);
+    double v1[kVals], v2[kVals];
+#pragma unroll
+    for (int v = 0; v < kVals; v++) v1[v] = v2[v] = 0.0;
+    for (int a = adj_off[j]; a < adj_off[j + 1]; a++) {
+        const int e = adj[a] >> 3, loc = adj[a] & 7;
+        const double* s = scratch + ((int64_t)il * nel + e) * 2 * (ENND * kVals);
+#pragma unroll
+        for (int v = 0; v < kVals; v++) {
+            v1[v] += s[loc * kVals + v];
+            v2[v] += s[ENND * kVals + loc * kVals + v];
+        }
+    }
+    const int i = i0 + il;
+    if (j == i) {
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const double c = (add_c ? cterm[9 * i + 3 * p + q] : 0.0) + (p == q ? ext : 0.0);
+                v1[6 + 3 * p + q] += c;
+                v2[6 + 3 * p + q] += c;
+            }
+    }
+    const int64_t ncol = 3LL * nnd;
+    put_block(G1, ncol, il, j, v1, true);
+    put_block(G2, ncol, il, j, v2, true);
+    put_block(H1, ncol, il, j, v1 + 6, false);
+    put_block(H2, ncol, il, j, v2 + 6, false);
+}
+
+// Finds the row node owning a chunk (row_chunk is sorted, length nnd+1).
+__device__ __forceinline__ int row_of_chunk(const int64_t* row_chunk, int nnd, int64_t c)
+{
+    int lo = 0, hi = nnd;  // invariant: row_chunk[lo] <= c < row_chunk[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (row_chunk[mid] <= c)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Gathers lag-store entries of row nodes [i0, i1): one 64-thread block per chunk.
+template <int ENND>
+__global__ void __launch_bounds__(kChunk)
+    k_store_gather(const double* __restrict__ scratch, const int64_t* __restrict__ pair_slot,
+                   const int2* __restrict__ pair_k, const int* __restrict__ adj_off,
+                   const int* __restrict__ adj, const double* __restrict__ cterm,
+                   const int64_t* __restrict__ row_chunk, const int64_t* __restrict__ row_valid,
+                   const uint32_t* __restrict__ idx, int nnd, int nel, int i0, double ext,
+                   int64_t chunk0, double* __restrict__ vals)
+{
+    const int64_t c = chunk0 + blockIdx.x;
+    const int i = row_of_chunk(row_chunk, nnd, c);
+    const int lane = threadIdx.x;
+    const int64_t in_row = (c - row_chunk[i]) * kChunk + lane;
+    double v[kVals];
+#pragma unroll
+    for (int t = 0; t < kVals; t++) v[t] = 0.0;
+    if (in_row < row_valid[i]) {
+        const uint32_t id = idx[c * kChunk + lane];
+        const int j = (int)(id >> kKBits), k = (int)(id & kMaxLag);
+        for (int a = adj_off[j]; a < adj_off[j + 1]; a++) {
+            const int e = adj[a] >> 3, loc = adj[a] & 7;
+            const int2 pk = pair_k[(int64_t)i * nel + e];
+            if (k < pk.x || k > pk.y) continue;
+            const int64_t p = (int64_t)(i - i0) * nel + e;
+            const double* s = scratch + (pair_slot[p] + (k - pk.x)) * (ENND * kVals) + loc * kVals;
+#pragma unroll
+            for (int t = 0; t < kVals; t++) v[t] += s[t];
+        }
+        if (j == i && k <= 1) {
+            // C-term and the exterior-problem term go to fH_1 and fH_2 of step 1, i.e. into
+            // M_1 (phi_1 part) and M_0 (phi_2 part): src/ElastoDynamicsT.cpp:204-225.
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int q = 0; q < 3; q++)
+                    v[6 + 3 * p + q] += cterm[9 * i + 3 * p + q] + (p == q ? ext : 0.0);
+        }
+    }
+    double* o = vals + c * (kVals * kChunk);
+#pragma unroll
+    for (int t = 0; t < kVals; t++) o[t * kChunk + lane] = v[t];
+}
+
+// Dense N x N G and H of one lag k from the store.
+__global__ void __launch_bounds__(kChunk)
+    k_store_to_dense(const double* __restrict__ vals, const uint32_t* __restrict__ idx,
+                     const int64_t* __restrict__ row_chunk, const int64_t* __restrict__ row_valid,
+                     int nnd, int k, double* __restrict__ G, double* __restrict__ H)
+{
+    const int64_t c = blockIdx.x;
+    const int i = row_of_chunk(row_chunk, nnd, c);
+    const int lane = threadIdx.x;
+    const int64_t in_row = (c - row_chunk[i]) * kChunk + lane;
+    if (in_row >= row_valid[i]) return;
+    const uint32_t id = idx[c * kChunk + lane];
+    if ((int)(id & kMaxLag) != k) return;
+    const int j = (int)(id >> kKBits);
+    double v[kVals];
+    const double* s = vals + c * (kVals * kChunk) + lane;
+#pragma unroll
+    for (int t = 0; t < kVals; t++) v[t] = s[t * kChunk];
+    const int64_t ncol = 3LL * nnd;
+    put_block(G, ncol, i, j, v, true);
+    put_block(H, ncol, i, j, v + 6, false);
+}
+
+template <int ENND>
+void launch_geom(Context& c)
+{
+    const int n = c.nel * c.nq * c.nq;
+    hipLaunchKernelGGL(k_geom<ENND>, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, c.coords.p,
+                       c.ien.p, c.gauss.p, c.nq, c.nel, c.geo.p);
+    TD_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------------------------
+void prepare_geometry(Context& c)
+{
+    const int GS = 7 + c.ennd;
+    c.geo.alloc((size_t)c.nel * c.nq * c.nq * GS);
+    if (c.ennd == 4)
+        launch_geom<4>(c);
+    else
+        launch_geom<8>(c);
+    // node -> (element, local node) adjacency, in element order
+    std::vector<int> cnt(c.nnd + 1, 0);
+    for (int e = 0; e < c.nel; e++)
+        for (int a = 0; a < c.ennd; a++) cnt[c.h_ien[(size_t)e * c.ennd + a] + 1]++;
+    for (int j = 0; j < c.nnd; j++) cnt[j + 1] += cnt[j];
+    c.h_adj_off = cnt;
+    c.h_adj.assign(cnt[c.nnd], 0);
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int e = 0; e < c.nel; e++)
+        for (int a = 0; a < c.ennd; a++) {
+            const int j = c.h_ien[(size_t)e * c.ennd + a];
+            c.h_adj[fill[j]++] = (e << 3) | a;
+        }
+    c.adj_off.upload(c.h_adj_off.data(), c.h_adj_off.size(), c.stream);
+    c.adj.upload(c.h_adj.data(), c.h_adj.size(), c.stream);
+}
+
+void compute_cterm(Context& c)
+{
+    if (c.cterm_ready) return;
+    c.cterm.alloc((size_t)c.nnd * 9);
+    if (c.ennd == 4)
+        hipLaunchKernelGGL(k_cterm<4>, dim3(c.nnd), dim3(kBlock), 0, c.stream, c.coords.p,
+                           c.ien.p, c.geo.p, c.gauss.p, c.nq, c.nel, c.dt, c.mat, c.cterm.p);
+    else
+        hipLaunchKernelGGL(k_cterm<8>, dim3(c.nnd), dim3(kBlock), 0, c.stream, c.coords.p,
+                           c.ien.p, c.geo.p, c.gauss.p, c.nq, c.nel, c.dt, c.mat, c.cterm.p);
+    TD_HIP(hipGetLastError());
+    c.cterm_ready = true;
+}
+
+void compute_pair_ranges(Context& c)
+{
+    if (c.ranges_ready) return;
+    const int64_t np = (int64_t)c.nnd * c.nel;
+    c.pair_k.alloc(np);
+    if (c.ennd == 4)
+        hipLaunchKernelGGL(k_pair_range<4>, dim3(grid_for(np)), dim3(kBlock), 0, c.stream,
+                           c.coords.p, c.ien.p, c.geo.p, c.gauss.p, c.nq, c.nnd, c.nel, c.dt,
+                           c.mat, c.Kc, c.pair_k.p);
+    else
+        hipLaunchKernelGGL(k_pair_range<8>, dim3(grid_for(np)), dim3(kBlock), 0, c.stream,
+                           c.coords.p, c.ien.p, c.geo.p, c.gauss.p, c.nq, c.nnd, c.nel, c.dt,
+                           c.mat, c.Kc, c.pair_k.p);
+    TD_HIP(hipGetLastError());
+    c.ranges_ready = true;
+}
+
+namespace {
+// Scratch budget for the per-pair slots of one row batch.
+constexpr int64_t kScratchBytes = 1LL << 30;
+}  // namespace
+
+void assemble_step_dense(Context& c, int step, int lower, int upper, double* dG1, double* dG2,
+                         double* dH1, double* dH2)
+{
+    compute_cterm(c);
+    const int heaviside = c.kernel == 2;
+    const int add_c = heaviside || step == 1;  // Step kernel: C-term at step 1 only
+    const double ext = (c.if_ext && step == 1) ? 0.5 * c.dt : 0.0;
+    const int64_t per_node = (int64_t)c.nel * 2 * c.ennd * kVals * sizeof(double);
+    const int rows_per_batch = (int)std::max<int64_t>(1, kScratchBytes / per_node);
+    DBuf<double> scratch;
+    scratch.alloc((size_t)std::min(rows_per_batch, upper - lower + 1) * c.nel * 2 * c.ennd *
+                  kVals);
+    const int64_t ncol = 3LL * c.nnd;
+    for (int b0 = lower; b0 <= upper; b0 += rows_per_batch) {
+        const int nb = std::min(rows_per_batch, upper - b0 + 1);
+        const int64_t nslots = (int64_t)nb * c.nel * 2;
+        const int64_t roff = (int64_t)(b0 - lower) * 3 * ncol;
+        if (c.ennd == 4) {
+            hipLaunchKernelGGL(k_pair_step<4>, dim3(grid_for(nslots)), dim3(kBlock), 0, c.stream,
+                               c.coords.p, c.ien.p, c.geo.p, c.gauss.p, c.nq, c.nel, c.dt, c.mat,
+                               step, heaviside, b0, nb, scratch.p);
+            hipLaunchKernelGGL(k_dense_gather<4>, dim3(grid_for((int64_t)nb * c.nnd)),
+                               dim3(kBlock), 0, c.stream, scratch.p, c.adj_off.p, c.adj.p,
+                               c.cterm.p, c.nnd, c.nel, b0, nb, add_c, ext, dG1 + roff,
+                               dG2 + roff, dH1 + roff, dH2 + roff);
+        } else {
+            hipLaunchKernelGGL(k_pair_step<8>, dim3(grid_for(nslots)), dim3(kBlock), 0, c.stream,
+                               c.coords.p, c.ien.p, c.geo.p, c.gauss.p, c.nq, c.nel, c.dt, c.mat,
+                               step, heaviside, b0, nb, scratch.p);
+            hipLaunchKernelGGL(k_dense_gather<8>, dim3(grid_for((int64_t)nb * c.nnd)),
+                               dim3(kBlock), 0, c.stream, scratch.p, c.adj_off.p, c.adj.p,
+                               c.cterm.p, c.nnd, c.nel, b0, nb, add_c, ext, dG1 + roff,
+                               dG2 + roff, dH1 + roff, dH2 + roff);
+        }
+        TD_HIP(hipGetLastError());
+    }
+    // the scratch buffer must outlive the kernels that use it
+    TD_HIP(hipStreamSynchronize(c.stream));
+}
+
+namespace {
+
+struct ShellPlan {
+    std::vector<int2> hull;          // [nnd*nnd]
+    std::vector<int64_t> off;        // [nnd*nnd] entry offsets
+    std::vector<int64_t> row_chunk;  // [nnd+1]
+    std::vector<int64_t> row_valid;  // [nnd]
+    int64_t nonzero = 0;
+};
+
+void plan_shells(Context& c, ShellPlan& P)
+{
+    compute_pair_ranges(c);
+    const int64_t nn = (int64_t)c.nnd * c.nnd;
+    DBuf<int2> hull;
+    hull.alloc(nn);
+    hipLaunchKernelGGL(k_node_hull, dim3(grid_for(nn)), dim3(kBlock), 0, c.stream, c.pair_k.p,
+                       c.adj_off.p, c.adj.p, c.nnd, c.nel, hull.p);
+    TD_HIP(hipGetLastError());
+    P.hull.resize(nn);
+    TD_HIP(hipMemcpyAsync(P.hull.data(), hull.p, nn * sizeof(int2), hipMemcpyDeviceToHost,
+                          c.stream));
+    TD_HIP(hipStreamSynchronize(c.stream));
+    P.off.resize(nn);
+    P.row_chunk.assign(c.nnd + 1, 0);
+    P.row_valid.assign(c.nnd, 0);
+    int64_t chunk = 0;
+    for (int i = 0; i < c.nnd; i++) {
+        int64_t o = chunk * kChunk;
+        const int64_t row0 = o;
+        for (int j = 0; j < c.nnd; j++) {
+            const int2 h = P.hull[(int64_t)i * c.nnd + j];
+            P.off[(int64_t)i * c.nnd + j] = o;
+            if (h.y >= h.x) o += h.y - h.x + 1;
+        }
+        P.row_valid[i] = o - row0;
+        P.nonzero += o - row0;
+        P.row_chunk[i] = chunk;
+        chunk += (o - row0 + kChunk - 1) / kChunk;
+    }
+    P.row_chunk[c.nnd] = chunk;
+}
+
+template <int ENND>
+void fill_store(Context& c, const ShellPlan& P)
+{
+    LagStore& S = c.store;
+    // per-pair slot layout on the host (pair_k copied back once)
+    const int64_t np = (int64_t)c.nnd * c.nel;
+    std::vector<int2> pk(np);
+    TD_HIP(hipMemcpyAsync(pk.data(), c.pair_k.p, np * sizeof(int2), hipMemcpyDeviceToHost,
+                          c.stream));
+    TD_HIP(hipStreamSynchronize(c.stream));
+    const int64_t slot_bytes = (int64_t)ENND * kVals * sizeof(double);
+    DBuf<double> scratch;
+    DBuf<int> slot_pair;
+    DBuf<int64_t> pair_slot;
+    const double ext = c.if_ext ? 0.5 * c.dt : 0.0;
+    int i0 = 0;
+    while (i0 < c.nnd) {
+        // grow the batch while its slots fit the scratch budget
+        int i1 = i0;
+        int64_t slots = 0;
+        while (i1 < c.nnd) {
+            int64_t rs = 0;
+            for (int e = 0; e < c.nel; e++) {
+                const int2 k = pk[(int64_t)i1 * c.nel + e];
+                if (k.y >= k.x) rs += k.y - k.x + 1;
+            }
+            if (i1 > i0 && (slots + rs) * slot_bytes > kScratchBytes) break;
+            slots += rs;
+            i1++;
+        }
+        const int nb = i1 - i0;
+        std::vector<int64_t> h_pair_slot((size_t)nb * c.nel);
+        std::vector<int> h_slot_pair(slots);
+        int64_t s = 0;
+        for (int p = 0; p < nb * c.nel; p++) {
+            h_pair_slot[p] = s;
+            const int2 k = pk[(int64_t)i0 * c.nel + p];
+            if (k.y >= k.x)
+                for (int t = k.x; t <= k.y; t++) h_slot_pair[s++] = p;
+        }
+        scratch.alloc(std::max<int64_t>(slots, 1) * ENND * kVals);
+        slot_pair.upload(h_slot_pair.data(), h_slot_pair.size(), c.stream);
+        pair_slot.upload(h_pair_slot.data(), h_pair_slot.size(), c.stream);
+        if (slots > 0) {
+            hipLaunchKernelGGL(k_pair_lags<ENND>, dim3(grid_for(slots)), dim3(kBlock), 0,
+                               c.stream, c.coords.p, c.ien.p, c.geo.p, c.gauss.p, c.nq, c.nel,
+                               c.dt, c.mat, c.Kc, i0, slots, slot_pair.p, pair_slot.p,
+                               c.pair_k.p, scratch.p);
+            TD_HIP(hipGetLastError());
+        }
+        const int64_t ch0 = P.row_chunk[i0], ch1 = P.row_chunk[i1];
+        if (ch1 > ch0) {
+            hipLaunchKernelGGL(k_store_gather<ENND>, dim3((unsigned)(ch1 - ch0)), dim3(kChunk), 0,
+                               c.stream, scratch.p, pair_slot.p, c.pair_k.p, c.adj_off.p, c.adj.p,
+                               c.cterm.p, S.row_chunk.p, S.row_valid.p, S.idx.p, c.nnd, c.nel,
+                               i0, ext, ch0, S.vals.p);
+            TD_HIP(hipGetLastError());
+        }
+        // host vectors of this batch are reused next iteration: wait for the uploads to land
+        TD_HIP(hipStreamSynchronize(c.stream));
+        i0 = i1;
+    }
+}
+
+}  // namespace
+
+int64_t predict_store_bytes(Context& c)
+{
+    ShellPlan P;
+    plan_shells(c, P);
+    return P.row_chunk[c.nnd] * kChunk * (int64_t)(kVals * sizeof(double) + sizeof(uint32_t));
+}
+
+void build_lag_store(Context& c)
+{
+    if (c.Kc > kMaxLag) throw std::runtime_error("lag count exceeds the store's 10-bit lag index");
+    compute_cterm(c);
+    ShellPlan P;
+    plan_shells(c, P);
+    LagStore& S = c.store;
+    S.n_chunks = P.row_chunk[c.nnd];
+    S.n_entries = S.n_chunks * kChunk;
+    S.nonzero_entries = P.nonzero;
+    S.h_row_chunk = P.row_chunk;
+    S.vals.alloc(std::max<int64_t>(S.n_chunks, 1) * kVals * kChunk);
+    S.idx.alloc(std::max<int64_t>(S.n_entries, 1));
+    TD_HIP(hipMemsetAsync(S.idx.p, 0, S.idx.bytes(), c.stream));
+    S.h_row_valid = P.row_valid;
+    S.row_chunk.upload(P.row_chunk.data(), P.row_chunk.size(), c.stream);
+    S.row_valid.upload(P.row_valid.data(), P.row_valid.size(), c.stream);
+    DBuf<int64_t> d_off;
+    DBuf<int2> d_hull;
+    d_off.upload(P.off.data(), P.off.size(), c.stream);
+    d_hull.upload(P.hull.data(), P.hull.size(), c.stream);
+    const int64_t nn = (int64_t)c.nnd * c.nnd;
+    hipLaunchKernelGGL(k_fill_idx, dim3(grid_for(nn)), dim3(kBlock), 0, c.stream, d_hull.p,
+                       d_off.p, c.nnd, S.idx.p);
+    TD_HIP(hipGetLastError());
+    if (c.ennd == 4)
+        fill_store<4>(c, P);
+    else
+        fill_store<8>(c, P);
+    TD_HIP(hipStreamSynchronize(c.stream));
+    c.store_ready = true;
+}
+
+void lag_dense(Context& c, int k, double* dG, double* dH)
+{
+    if (!c.store_ready) throw std::runtime_error("lag store not built");
+    const int64_t N = 3LL * c.nnd;
+    TD_HIP(hipMemsetAsync(dG, 0, N * N * sizeof(double), c.stream));
+    TD_HIP(hipMemsetAsync(dH, 0, N * N * sizeof(double), c.stream));
+    if (k < 0 || k > c.Kc || c.store.n_chunks == 0) return;
+    hipLaunchKernelGGL(k_store_to_dense, dim3((unsigned)c.store.n_chunks), dim3(kChunk), 0,
+                       c.stream, c.store.vals.p, c.store.idx.p, c.store.row_chunk.p,
+                       c.store.row_valid.p, c.nnd, k, dG, dH);
+    TD_HIP(hipGetLastError());
+}
+
+}  // namespace tdbem

@@ -1,0 +1,333 @@
+// capi.hip — the C ABI of include/tdbem.h over the HIP implementation.
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/tdbem.h"
+#include "tdbem_internal.h"
+
+struct tdbem_context {
+    tdbem::Context c;
+};
+
+namespace tdbem {
+
+void check_hip(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e) + " in " + what);
+}
+
+Context::~Context()
+{
+    march_free(*this);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
+
+thread_local std::string g_create_error;
+
+// Gauss tables of the shape-function quadrature (src/ShapeFunctionT.cpp:83-163).  The 10- and
+// 20-point rows carry the reference's 9 digits, and the 19th 20-point abscissa keeps the
+// reference's sign (+0.993128599), so results match the reference for every nq it accepts.
+const double kG6x[6] = {0.6612093864662645, -0.6612093864662645, -0.2386191860831969,
+                        0.2386191860831969, -0.9324695142031521, 0.9324695142031521};
+const double kG6w[6] = {0.3607615730481386, 0.3607615730481386, 0.4679139345726910,
+                        0.4679139345726910, 0.1713244923791704, 0.1713244923791704};
+const double kG10x[10] = {-0.148874339, 0.148874339, -0.433395394, 0.433395394, -0.679409568,
+                          0.679409568, -0.865063367, 0.865063367, -0.973906529, 0.973906529};
+const double kG10w[10] = {0.295524225, 0.295524225, 0.269266719, 0.269266719, 0.219086363,
+                          0.219086363, 0.149451349, 0.149451349, 0.066671344, 0.066671344};
+const double kG20x[20] = {-0.076526521, 0.076526521, -0.227785851, 0.227785851, -0.373706089,
+                          0.373706089, -0.510867002, 0.510867002, -0.636053681, 0.636053681,
+                          -0.746331906, 0.746331906, -0.839116972, 0.839116972, -0.912234428,
+                          0.912234428, -0.963971927, 0.963971927, 0.993128599, 0.993128599};
+const double kG20w[20] = {0.152753387, 0.152753387, 0.149172986, 0.149172986, 0.142096109,
+                          0.142096109, 0.131688638, 0.131688638, 0.118194532, 0.118194532,
+                          0.10193012, 0.10193012, 0.083276742, 0.083276742, 0.062672048,
+                          0.062672048, 0.04060143, 0.04060143, 0.017614007, 0.017614007};
+const double kG40x[40] = {
+    -0.0387724175060508, 0.0387724175060508, -0.1160840706752552, 0.1160840706752552,
+    -0.1926975807013711, 0.1926975807013711, -0.2681521850072537, 0.2681521850072537,
+    -0.3419940908257585, 0.3419940908257585, -0.4137792043716050, 0.4137792043716050,
+    -0.4830758016861787, 0.4830758016861787, -0.5494671250951282, 0.5494671250951282,
+    -0.6125538896679802, 0.6125538896679802, -0.6719566846141796, 0.6719566846141796,
+    -0.7273182551899271, 0.7273182551899271, -0.7783056514265194, 0.7783056514265194,
+    -0.8246122308333117, 0.8246122308333117, -0.8659595032122595, 0.8659595032122595,
+    -0.9020988069688743, 0.9020988069688743, -0.9328128082786765, 0.9328128082786765,
+    -0.9579168192137917, 0.9579168192137917, -0.9772599499837743, 0.9772599499837743,
+    -0.9907262386994570, 0.9907262386994570, -0.9982377097105593, 0.9982377097105593};
+const double kG40w[40] = {
+    0.0775059479784248, 0.0775059479784248, 0.0770398181642480, 0.0770398181642480,
+    0.0761103619006262, 0.0761103619006262, 0.0747231690579683, 0.0747231690579683,
+    0.0728865823958041, 0.0728865823958041, 0.0706116473912868, 0.0706116473912868,
+    0.0679120458152339, 0.0679120458152339, 0.0648040134566010, 0.0648040134566010,
+    0.0613062424929289, 0.0613062424929289, 0.0574397690993916, 0.0574397690993916,
+    0.0532278469839368, 0.0532278469839368, 0.0486958076350722, 0.0486958076350722,
+    0.0438709081856733, 0.0438709081856733, 0.0387821679744720, 0.0387821679744720,
+    0.0334601952825478, 0.0334601952825478, 0.0279370069800234, 0.0279370069800234,
+    0.0222458491941670, 0.0222458491941670, 0.0164210583819079, 0.0164210583819079,
+    0.0104982845311528, 0.0104982845311528, 0.0045212770985332, 0.0045212770985332};
+
+bool gauss_table(int nq, const double** x, const double** w)
+{
+    switch (nq) {
+    case 6: *x = kG6x; *w = kG6w; return true;
+    case 10: *x = kG10x; *w = kG10w; return true;
+    case 20: *x = kG20x; *w = kG20w; return true;
+    case 40: *x = kG40x; *w = kG40w; return true;
+    default: return false;
+    }
+}
+
+int max_step(const std::vector<double>& X, int nnd, double E, double nu, double rho, double dt)
+{
+    // SolverT::ComputeMaxStep (src/SolverT.cpp:211-250)
+    const double Mu = E / (2.0 * (1.0 + nu));
+    const double Cs = std::sqrt(Mu / rho);
+    double max_dis = 0;
+    for (int i = 0; i < nnd - 1; i++)
+        for (int j = i + 1; j < nnd; j++) {
+            double t = 0;
+            for (int d = 0; d < 3; d++) {
+                const double v = X[3 * i + d] - X[3 * j + d];
+                t += v * v;
+            }
+            t = std::sqrt(t);
+            if (t > max_dis) max_dis = t;
+        }
+    return (int)std::ceil(max_dis / (Cs * dt)) + 1;
+}
+
+template <typename F>
+int guarded(tdbem_context* ctx, F&& f)
+{
+    if (!ctx) return TDBEM_ERR_ARG;
+    try {
+        f(ctx->c);
+        ctx->c.err.clear();
+        return TDBEM_OK;
+    } catch (const std::invalid_argument& e) {
+        ctx->c.err = e.what();
+        return TDBEM_ERR_ARG;
+    } catch (const std::logic_error& e) {
+        ctx->c.err = e.what();
+        return TDBEM_ERR_STATE;
+    } catch (const std::runtime_error& e) {
+        ctx->c.err = e.what();
+        const std::string m = e.what();
+        if (m.find("singular") != std::string::npos) return TDBEM_ERR_NUMERIC;
+        if (m.find("HIP") != std::string::npos) return TDBEM_ERR_HIP;
+        return TDBEM_ERR_ARG;
+    } catch (const std::exception& e) {
+        ctx->c.err = e.what();
+        return TDBEM_ERR_HIP;
+    }
+}
+
+}  // namespace
+}  // namespace tdbem
+
+using namespace tdbem;
+
+extern "C" {
+
+int tdbem_create(int nnd, const double* coords, int nel, const int* ien, int etype, int nq,
+                 double E, double nu, double rho, int if_ext, double dt, int numstep, int kernel,
+                 tdbem_context** out)
+{
+    if (!out) return TDBEM_ERR_ARG;
+    *out = nullptr;
+    tdbem_context* ctx = nullptr;
+    try {
+        const double *gx, *gw;
+        if (nnd <= 0 || nel <= 0 || !coords || !ien)
+            throw std::invalid_argument("empty mesh");
+        if (etype != 1 && etype != 2)
+            throw std::invalid_argument("InputT::ReadInput, unsupported element type");
+        if (!gauss_table(nq, &gx, &gw))
+            throw std::invalid_argument(
+                "ShapeFunctionT::SetNQ, Unsupported number of Gaussian points");
+        if (kernel != TDBEM_KERNEL_STEP && kernel != TDBEM_KERNEL_HEAVISIDE)
+            throw std::invalid_argument("unsupported temporal kernel");
+        if (!(dt > 0) || numstep < 0 || !(E > 0) || !(rho > 0) || !(nu > -1 && nu < 0.5))
+            throw std::invalid_argument("bad material or time step");
+        const int ennd = etype == 1 ? 4 : 8;
+        for (int64_t q = 0; q < (int64_t)nel * ennd; q++)
+            if (ien[q] < 0 || ien[q] >= nnd) throw std::invalid_argument("IEN node id out of range");
+        ctx = new tdbem_context();
+        Context& c = ctx->c;
+        TD_HIP(hipGetDevice(&c.device));
+        TD_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        c.nnd = nnd;
+        c.nel = nel;
+        c.etype = etype;
+        c.ennd = ennd;
+        c.nq = nq;
+        c.kernel = kernel;
+        c.if_ext = if_ext ? 1 : 0;
+        c.numstep = numstep;
+        c.dt = dt;
+        c.mat.E = E;
+        c.mat.nu = nu;
+        c.mat.rho = rho;
+        c.mat.mu = E / (2 * (1 + nu));
+        const double lam = E * nu / ((1 + nu) * (1 - 2 * nu));
+        c.mat.cp = std::sqrt((lam + 2 * c.mat.mu) / rho);
+        c.mat.cs = std::sqrt(c.mat.mu / rho);
+        c.h_coords.assign(coords, coords + 3 * (size_t)nnd);
+        c.h_ien.assign(ien, ien + (size_t)nel * ennd);
+        c.K = max_step(c.h_coords, nnd, E, nu, rho, dt);
+        c.Kc = std::min(c.K, numstep);
+        c.coords.upload(c.h_coords.data(), c.h_coords.size(), c.stream);
+        c.ien.upload(c.h_ien.data(), c.h_ien.size(), c.stream);
+        std::vector<double> g(2 * nq);
+        std::memcpy(g.data(), gx, nq * sizeof(double));
+        std::memcpy(g.data() + nq, gw, nq * sizeof(double));
+        c.gauss.upload(g.data(), g.size(), c.stream);
+        prepare_geometry(c);
+        TD_HIP(hipStreamSynchronize(c.stream));
+        *out = ctx;
+        return TDBEM_OK;
+    } catch (const std::invalid_argument& e) {
+        g_create_error = e.what();
+        delete ctx;
+        return TDBEM_ERR_ARG;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        delete ctx;
+        return TDBEM_ERR_HIP;
+    }
+}
+
+void tdbem_destroy(tdbem_context* ctx)
+{
+    if (!ctx) return;
+    (void)hipStreamSynchronize(ctx->c.stream);
+    delete ctx;
+}
+
+const char* tdbem_last_error(const tdbem_context* ctx)
+{
+    return ctx ? ctx->c.err.c_str() : g_create_error.c_str();
+}
+
+int tdbem_max_step(const tdbem_context* ctx) { return ctx ? ctx->c.K : TDBEM_ERR_ARG; }
+
+int tdbem_assemble_step_device(tdbem_context* ctx, int step, int lower, int upper, double* G1,
+                               double* G2, double* H1, double* H2)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (step < 1) throw std::invalid_argument("step must be >= 1");
+        if (lower < 0 || upper >= c.nnd || upper < lower)
+            throw std::invalid_argument("row range out of bounds");
+        if (!G1 || !G2 || !H1 || !H2) throw std::invalid_argument("null output");
+        assemble_step_dense(c, step, lower, upper, G1, G2, H1, H2);
+    });
+}
+
+int tdbem_assemble_step(tdbem_context* ctx, int step, int lower, int upper, double* G1,
+                        double* G2, double* H1, double* H2)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (step < 1) throw std::invalid_argument("step must be >= 1");
+        if (lower < 0 || upper >= c.nnd || upper < lower)
+            throw std::invalid_argument("row range out of bounds");
+        if (!G1 || !G2 || !H1 || !H2) throw std::invalid_argument("null output");
+        const size_t n = (size_t)3 * (upper - lower + 1) * 3 * c.nnd;
+        DBuf<double> b[4];
+        for (auto& x : b) x.alloc(n);
+        assemble_step_dense(c, step, lower, upper, b[0].p, b[1].p, b[2].p, b[3].p);
+        double* outs[4] = {G1, G2, H1, H2};
+        for (int q = 0; q < 4; q++)
+            TD_HIP(hipMemcpyAsync(outs[q], b[q].p, n * sizeof(double), hipMemcpyDeviceToHost,
+                                  c.stream));
+        TD_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int tdbem_build_lag_store(tdbem_context* ctx)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (c.kernel != TDBEM_KERNEL_STEP)
+            throw std::invalid_argument("the lag store is built for the Step kernel");
+        build_lag_store(c);
+    });
+}
+
+int64_t tdbem_lag_store_bytes(tdbem_context* ctx)
+{
+    int64_t v = -1;
+    int rc = guarded(ctx, [&](Context& c) {
+        if (c.store_ready)
+            v = (int64_t)(c.store.vals.bytes() + c.store.idx.bytes());
+        else
+            v = predict_store_bytes(c);
+    });
+    return rc == TDBEM_OK ? v : rc;
+}
+
+int tdbem_get_lag(tdbem_context* ctx, int k, double* G, double* H)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!c.store_ready) throw std::logic_error("lag store not built");
+        if (k < 0 || k > c.Kc) throw std::invalid_argument("lag index out of range");
+        const size_t N = 3 * (size_t)c.nnd;
+        DBuf<double> g, h;
+        g.alloc(N * N);
+        h.alloc(N * N);
+        lag_dense(c, k, g.p, h.p);
+        TD_HIP(hipMemcpyAsync(G, g.p, N * N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        TD_HIP(hipMemcpyAsync(H, h.p, N * N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        TD_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int tdbem_march_init(tdbem_context* ctx, double a1, double a2, int n_ubc, const int* ubc_dof,
+                     const double* ubc_val, int n_fbc, const int* fbc_dof, const double* fbc_val,
+                     int nrhs, const double* amp, const double* t_rise)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (c.kernel != TDBEM_KERNEL_STEP)
+            throw std::invalid_argument("the march runs the Step kernel (SolverStepLinearT)");
+        if (!(a1 >= 0 && a1 <= 1 && a2 >= 0 && a2 < 1 && a1 + a2 <= 1))
+            throw std::invalid_argument("Alpha is out of range");
+        if (nrhs != 1 && nrhs != 2 && nrhs != 4 && nrhs != 8)
+            throw std::invalid_argument("nrhs must be 1, 2, 4 or 8");
+        if ((n_ubc > 0 && (!ubc_dof || !ubc_val)) || (n_fbc > 0 && (!fbc_dof || !fbc_val)))
+            throw std::invalid_argument("null boundary-condition array");
+        march_init(c, a1, a2, nrhs, amp, t_rise, n_ubc, ubc_dof, ubc_val, n_fbc, fbc_dof,
+                   fbc_val);
+    });
+}
+
+int tdbem_march_run(tdbem_context* ctx, int nsteps)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!c.march) throw std::logic_error("march not initialised");
+        march_run(c, nsteps);
+    });
+}
+
+int tdbem_march_step_index(const tdbem_context* ctx)
+{
+    return ctx ? march_step_count(ctx->c) : TDBEM_ERR_ARG;
+}
+
+int tdbem_get_history(tdbem_context* ctx, int rhs, double* U, double* T)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!c.march) throw std::logic_error("march not initialised");
+        if (!U || !T) throw std::invalid_argument("null output");
+        march_history(c, rhs, U, T);
+    });
+}
+
+int tdbem_synchronize(tdbem_context* ctx)
+{
+    return guarded(ctx, [&](Context& c) { TD_HIP(hipStreamSynchronize(c.stream)); });
+}
+
+}  // extern "C"

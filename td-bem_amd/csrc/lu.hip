@@ -1,0 +1,316 @@
+// lu.hip — the step-0 solve: dense FP64 LU with partial pivoting on the GPU, and solves with it.
+//
+// Stands in for KSPSetOperators + KSPSolve (src/SolverStepLinearT.cpp:503-504).  The reference
+// refactors every step; here the n >= 3 system matrix (2 a1 + a3) M_0 + a1 M_1 (after the column
+// exchange) is the same every step, so it is factored once and inverted once, and each step is
+// one GEMV with the inverse.
+//
+// Blocked right-looking LU (panel width 64): one workgroup factors the panel and swaps whole rows,
+// a triangular solve forms the U12 block row, and an LDS-tiled FP64 GEMM updates the trailing
+// matrix.  Solves use the same pieces column-blocked over the right-hand sides.
+#include <algorithm>
+#include <stdexcept>
+#include <vector>
+
+#include "tdbem_internal.h"
+
+namespace tdbem {
+
+namespace {
+
+constexpr int NB = 64;          // panel width
+constexpr int kPanelThreads = 1024;
+
+// ---------------------------------------------------------------------------------------------
+// Panel factorisation of columns [j0, j0+nb) over rows [j0, n).  Row swaps are applied to whole
+// rows, so the stored factors satisfy P A = L U with L unit lower (below the diagonal) and U upper.
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kPanelThreads)
+    k_panel(double* __restrict__ A, int n, int j0, int nb, int* __restrict__ piv,
+            int* __restrict__ status)
+{
+    __shared__ double sval[kPanelThreads];
+    __shared__ int sidx[kPanelThreads];
+    const int tid = threadIdx.x;
+    for (int c = j0; c < j0 + nb; c++) {
+        // pivot: first row with the largest |a| (ties keep the smaller index)
+        double best = -1.0;
+        int bi = n;
+        for (int r = c + tid; r < n; r += kPanelThreads) {
+            const double v = fabs(A[(int64_t)r * n + c]);
+            if (v > best) {
+                best = v;
+                bi = r;
+            }
+        }
+        sval[tid] = best;
+        sidx[tid] = bi;
+        __syncthreads();
+        for (int s = kPanelThreads / 2; s > 0; s >>= 1) {
+            if (tid < s) {
+                const double ov = sval[tid + s];
+                const int oi = sidx[tid + s];
+                if (ov > sval[tid] || (ov == sval[tid] && oi < sidx[tid])) {
+                    sval[tid] = ov;
+                    sidx[tid] = oi;
+                }
+            }
+            __syncthreads();
+        }
+        const int p = sidx[0];
+        const double pmax = sval[0];
+        __syncthreads();
+        if (tid == 0) {
+            piv[c] = p;
+            if (!(pmax > 0.0)) *status = 1;
+        }
+        if (p != c)
+            for (int col = tid; col < n; col += kPanelThreads) {
+                const double a = A[(int64_t)c * n + col];
+                A[(int64_t)c * n + col] = A[(int64_t)p * n + col];
+                A[(int64_t)p * n + col] = a;
+            }
+        __syncthreads();
+        const double d = A[(int64_t)c * n + c];
+        for (int r = c + 1 + tid; r < n; r += kPanelThreads) {
+            double* row = A + (int64_t)r * n;
+            const double l = row[c] / d;
+            row[c] = l;
+            const double* prow = A + (int64_t)c * n;
+            for (int cc = c + 1; cc < j0 + nb; cc++) row[cc] -= l * prow[cc];
+        }
+        __syncthreads();
+    }
+}
+
+// X = L11^{-1} X for the nb rows starting at j0 (L unit lower, from the panel), columns
+// [c0, c0 + ncols) of a row-major matrix B with leading dimension ldb.  Thread per column.
+__global__ void k_trsm_lower_unit(const double* L, int ldl, int j0, int nb,
+                                  double* B, int ldb, int brow0, int c0, int ncols)
+{
+    __shared__ double Ls[NB][NB + 1];
+    for (int t = threadIdx.x; t < nb * nb; t += blockDim.x) {
+        const int r = t / nb, s = t % nb;
+        Ls[r][s] = L[(int64_t)(j0 + r) * ldl + j0 + s];
+    }
+    __syncthreads();
+    const int col = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= c0 + ncols) return;
+    double* xb = B + (int64_t)brow0 * ldb + col;
+    for (int r = 0; r < nb; r++) {
+        double v = xb[(int64_t)r * ldb];
+        for (int s = 0; s < r; s++) v -= Ls[r][s] * xb[(int64_t)s * ldb];
+        xb[(int64_t)r * ldb] = v;
+    }
+}
+
+// X = U11^{-1} X (U upper with its diagonal) for the nb rows starting at j0.
+__global__ void k_trsm_upper(const double* __restrict__ U, int ldu, int j0, int nb,
+                             double* __restrict__ B, int ldb, int brow0, int c0, int ncols)
+{
+    __shared__ double Us[NB][NB + 1];
+    for (int t = threadIdx.x; t < nb * nb; t += blockDim.x) {
+        const int r = t / nb, s = t % nb;
+        Us[r][s] = U[(int64_t)(j0 + r) * ldu + j0 + s];
+    }
+    __syncthreads();
+    const int col = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= c0 + ncols) return;
+    double* xb = B + (int64_t)brow0 * ldb + col;
+    for (int r = nb - 1; r >= 0; r--) {
+        double v = xb[(int64_t)r * ldb];
+        for (int s = r + 1; s < nb; s++) v -= Us[r][s] * xb[(int64_t)s * ldb];
+        xb[(int64_t)r * ldb] = v / Us[r][r];
+    }
+}
+
+// C[m x n] -= A[m x k] * B[k x n], all row-major with leading dimensions.  64x64 output tile per
+// 256-thread block, 4x4 per thread, K staged through LDS 32 at a time.
+constexpr int TM = 64, TN = 64, TK = 32;
+__global__ void __launch_bounds__(256)
+    k_gemm_sub(int m, int n, int k, const double* __restrict__ A, int lda,
+               const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc)
+{
+    __shared__ double As[TK][TM + 1];
+    __shared__ double Bs[TK][TN + 1];
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const int row0 = blockIdx.y * TM, col0 = blockIdx.x * TN;
+    double acc[4][4] = {};
+    for (int k0 = 0; k0 < k; k0 += TK) {
+        for (int t = threadIdx.x; t < TM * TK; t += 256) {
+            const int r = t / TK, s = t % TK;
+            const int gr = row0 + r, gs = k0 + s;
+            As[s][r] = (gr < m && gs < k) ? A[(int64_t)gr * lda + gs] : 0.0;
+        }
+        for (int t = threadIdx.x; t < TK * TN; t += 256) {
+            const int s = t / TN, cc = t % TN;
+            const int gs = k0 + s, gc = col0 + cc;
+            Bs[s][cc] = (gs < k && gc < n) ? B[(int64_t)gs * ldb + gc] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int s = 0; s < TK; s++) {
+            double a[4], b[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) a[u] = As[s][ty + 16 * u];
+#pragma unroll
+            for (int v = 0; v < 4; v++) b[v] = Bs[s][tx + 16 * v];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) acc[u][v] += a[u] * b[v];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int gr = row0 + ty + 16 * u, gc = col0 + tx + 16 * v;
+            if (gr < m && gc < n) C[(int64_t)gr * ldc + gc] -= acc[u][v];
+        }
+}
+
+void gemm_sub(int m, int n, int k, const double* A, int lda, const double* B, int ldb, double* C,
+              int ldc, hipStream_t s)
+{
+    if (m <= 0 || n <= 0 || k <= 0) return;
+    dim3 grid((n + TN - 1) / TN, (m + TM - 1) / TM);
+    hipLaunchKernelGGL(k_gemm_sub, grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, C, ldc);
+    TD_HIP(hipGetLastError());
+}
+
+// B_out[r][:] = B_in[perm[r]][:]
+__global__ void k_permute_rows(const double* __restrict__ Bin, const int* __restrict__ perm,
+                               int n, int ncols, double* __restrict__ Bout)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * ncols) return;
+    const int r = (int)(t / ncols), c = (int)(t % ncols);
+    Bout[t] = Bin[(int64_t)perm[r] * ncols + c];
+}
+
+__global__ void k_perm_identity(const int* __restrict__ perm, int n, double* __restrict__ B)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * n) return;
+    const int r = (int)(t / n), c = (int)(t % n);
+    B[t] = (perm[r] == c) ? 1.0 : 0.0;
+}
+
+// y = A x + beta y for a dense n x n row-major A: one 256-thread block per row.
+__global__ void __launch_bounds__(256)
+    k_gemv(const double* __restrict__ A, int n, const double* __restrict__ x,
+           double* __restrict__ y, double beta)
+{
+    __shared__ double red[256];
+    const int r = blockIdx.x;
+    const double* row = A + (int64_t)r * n;
+    double s = 0.0;
+    for (int c = threadIdx.x; c < n; c += 256) s += row[c] * x[c];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) y[r] = red[0] + (beta != 0.0 ? beta * y[r] : 0.0);
+}
+
+std::vector<int> host_permutation(const int* dpiv, int n, hipStream_t s)
+{
+    std::vector<int> piv(n), perm(n);
+    TD_HIP(hipMemcpyAsync(piv.data(), dpiv, n * sizeof(int), hipMemcpyDeviceToHost, s));
+    TD_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < n; i++) perm[i] = i;
+    for (int c = 0; c < n; c++) std::swap(perm[c], perm[piv[c]]);
+    return perm;
+}
+
+void solve_inplace(const double* LU, int n, double* B, int ncols, hipStream_t s)
+{
+    const int nblk = (n + NB - 1) / NB;
+    const int tpb = 128;
+    for (int b = 0; b < nblk; b++) {
+        const int j0 = b * NB, nb = std::min(NB, n - j0);
+        hipLaunchKernelGGL(k_trsm_lower_unit, dim3((ncols + tpb - 1) / tpb), dim3(tpb), 0, s, LU,
+                           n, j0, nb, B, ncols, j0, 0, ncols);
+        TD_HIP(hipGetLastError());
+        const int rest = n - j0 - nb;
+        gemm_sub(rest, ncols, nb, LU + (int64_t)(j0 + nb) * n + j0, n, B + (int64_t)j0 * ncols,
+                 ncols, B + (int64_t)(j0 + nb) * ncols, ncols, s);
+    }
+    for (int b = nblk - 1; b >= 0; b--) {
+        const int j0 = b * NB, nb = std::min(NB, n - j0);
+        hipLaunchKernelGGL(k_trsm_upper, dim3((ncols + tpb - 1) / tpb), dim3(tpb), 0, s, LU, n,
+                           j0, nb, B, ncols, j0, 0, ncols);
+        TD_HIP(hipGetLastError());
+        gemm_sub(j0, ncols, nb, LU + j0, n, B + (int64_t)j0 * ncols, ncols, B, ncols, s);
+    }
+}
+
+}  // namespace
+
+void lu_factor(double* A, int n, int* piv, hipStream_t s)
+{
+    DBuf<int> status;
+    status.alloc(1);
+    TD_HIP(hipMemsetAsync(status.p, 0, sizeof(int), s));
+    for (int j0 = 0; j0 < n; j0 += NB) {
+        const int nb = std::min(NB, n - j0);
+        hipLaunchKernelGGL(k_panel, dim3(1), dim3(kPanelThreads), 0, s, A, n, j0, nb, piv,
+                           status.p);
+        TD_HIP(hipGetLastError());
+        const int rest = n - j0 - nb;
+        if (rest > 0) {
+            const int tpb = 128;
+            hipLaunchKernelGGL(k_trsm_lower_unit, dim3((rest + tpb - 1) / tpb), dim3(tpb), 0, s,
+                               A, n, j0, nb, A, n, j0, j0 + nb, rest);
+            TD_HIP(hipGetLastError());
+            gemm_sub(rest, rest, nb, A + (int64_t)(j0 + nb) * n + j0, n,
+                     A + (int64_t)j0 * n + j0 + nb, n, A + (int64_t)(j0 + nb) * n + j0 + nb, n,
+                     s);
+        }
+    }
+    int h_status = 0;
+    TD_HIP(hipMemcpyAsync(&h_status, status.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    TD_HIP(hipStreamSynchronize(s));
+    if (h_status) throw std::runtime_error("step-0 system matrix is singular");
+}
+
+void lu_solve(const double* LU, int n, const int* piv, double* B, int ncols, hipStream_t s)
+{
+    std::vector<int> perm = host_permutation(piv, n, s);
+    DBuf<int> dperm;
+    DBuf<double> tmp;
+    dperm.upload(perm.data(), n, s);
+    tmp.alloc((size_t)n * ncols);
+    const int64_t tot = (int64_t)n * ncols;
+    hipLaunchKernelGGL(k_permute_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, B,
+                       dperm.p, n, ncols, tmp.p);
+    TD_HIP(hipGetLastError());
+    solve_inplace(LU, n, tmp.p, ncols, s);
+    TD_HIP(hipMemcpyAsync(B, tmp.p, tot * sizeof(double), hipMemcpyDeviceToDevice, s));
+    TD_HIP(hipStreamSynchronize(s));
+}
+
+void lu_inverse(const double* LU, int n, const int* piv, double* Ainv, hipStream_t s)
+{
+    std::vector<int> perm = host_permutation(piv, n, s);
+    DBuf<int> dperm;
+    dperm.upload(perm.data(), n, s);
+    const int64_t tot = (int64_t)n * n;
+    hipLaunchKernelGGL(k_perm_identity, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                       dperm.p, n, Ainv);
+    TD_HIP(hipGetLastError());
+    solve_inplace(LU, n, Ainv, n, s);
+    TD_HIP(hipStreamSynchronize(s));
+}
+
+void gemv(const double* A, int n, const double* x, double* y, double beta, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gemv, dim3(n), dim3(256), 0, s, A, n, x, y, beta);
+    TD_HIP(hipGetLastError());
+}
+
+}  // namespace tdbem

@@ -1,0 +1,394 @@
+// march.hip — hot path 2: the history convolution and the step-0 solve of
+// SolverStepLinearT::Solve_Averaging (src/SolverStepLinearT.cpp:314-544), FP64.
+//
+// With u_0 = t_0 = 0 (src/SolverT.cpp:117-125, src/SolverStepLinearT.cpp:50) the three averaged
+// lag sums of the reference ("Eqn M+1", "Eqn M", "Eqn M-1", :408-482) collapse to one pass over
+// the lag store:
+//     hist_s = sum_{k=0}^{min(s,Kc)} ( M^G_k z^t_k - M^H_k z^u_k ),
+//     z_0 = (a2 - a1) x_{s-1},   z_1 = a3 x_{s-1} + a2 x_{s-2},
+//     z_k = y_{s-k} = a1 x_{s-k+1} + a3 x_{s-k} + a2 x_{s-k-1}   (k >= 2),
+// where x = (u, t) after the boundary-condition post-processing.  The y vectors live in a ring of
+// Kc + 2 slots: slot s-1 holds z_1 until x_s is known and then becomes y_{s-1}; slot s holds z_0.
+// Slots of negative time are never written and stay zero, which is what x_m = 0 (m <= 0) needs.
+//
+// Each step: conv (stream the shell store once), r = f(s) gF + gU + hist, x = A^{-1} r with the
+// inverse of the constant n >= 3 matrix, then the post-processing of :512-536.
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <vector>
+
+#include "tdbem_internal.h"
+
+namespace tdbem {
+
+struct March {
+    int N = 0, nrhs = 1, R = 0, s = 0;
+    double a1 = 0, a2 = 0, a3 = 0;
+    std::vector<double> amp, t_rise;
+    DBuf<double> Ainv;      // [N][N] inverse of the n >= 3 system matrix
+    DBuf<double> gF, gU;    // G~ y split into the FBC part (scaled by f) and the UBC part
+    DBuf<double> wF, wU;    // A_2^{-1} (G~_2 y_F), A_2^{-1} (G~_2 y_U) for the first solve
+    DBuf<double> ring;      // [R][nrhs][2][N]
+    DBuf<double> hist, rv, xv;  // [nrhs][N]
+    DBuf<double> U, T;      // [numstep][nrhs][N]
+    DBuf<int> is_ubc, is_fbc;
+    DBuf<double> ubc_val, fbc_val;   // dense per dof
+    DBuf<double> fac;       // [nrhs] load factor of the current step
+    DBuf<double> d_amp, d_rise;
+};
+
+namespace {
+
+constexpr int kConvThreads = 256;
+
+inline unsigned blocks(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+// f_r(t) for every right-hand side: src/SolverT.cpp:379-394 returns 1; a per-RHS amplitude and a
+// linear rise over t_rise (0 = Heaviside) generalise it.
+__global__ void k_fac(const double* __restrict__ amp, const double* __restrict__ rise, double t,
+                      int nrhs, double* __restrict__ fac)
+{
+    const int r = threadIdx.x;
+    if (r >= nrhs) return;
+    const double tr = rise[r];
+    fac[r] = tr > 0 ? amp[r] * (t < tr ? t / tr : 1.0) : amp[r];
+}
+
+__global__ void k_combine(const double* __restrict__ G0, const double* __restrict__ H0,
+                          const double* __restrict__ G1, const double* __restrict__ H1, double c0,
+                          double c1, const int* __restrict__ is_ubc, int N,
+                          double* __restrict__ A, double* __restrict__ Gc, double* __restrict__ Hc)
+{
+    // A = H_comb with UBC columns replaced by -G_comb (ExchangeColumns, src/SolverT.cpp:265-342)
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)N * N) return;
+    const int col = (int)(t % N);
+    const double g = c0 * G0[t] + (G1 ? c1 * G1[t] : 0.0);
+    const double h = c0 * H0[t] + (H1 ? c1 * H1[t] : 0.0);
+    A[t] = is_ubc[col] ? -g : h;
+    Gc[t] = g;
+    Hc[t] = h;
+}
+
+__global__ void k_pack2(const double* __restrict__ a, const double* __restrict__ b, int N,
+                        double* __restrict__ out)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= N) return;
+    out[2 * d] = a[d];
+    out[2 * d + 1] = b[d];
+}
+
+__global__ void k_unpack2(const double* __restrict__ in, int N, double* __restrict__ a,
+                          double* __restrict__ b)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= N) return;
+    a[d] = in[2 * d];
+    b[d] = in[2 * d + 1];
+}
+
+__global__ void k_negate(double* __restrict__ v, int N)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d < N) v[d] = -v[d];
+}
+
+// hist[r][3i+p] = sum over row-i entries of G z^t - H z^u, one block per row node.
+template <int NRHS>
+__global__ void __launch_bounds__(kConvThreads)
+    k_conv(const double* __restrict__ vals, const uint32_t* __restrict__ idx,
+           const int64_t* __restrict__ row_chunk, int N, const double* __restrict__ ring, int R,
+           int s, double* __restrict__ hist)
+{
+    __shared__ double red[kConvThreads / 64][3 * NRHS];
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double acc[NRHS][3];
+#pragma unroll
+    for (int r = 0; r < NRHS; r++) acc[r][0] = acc[r][1] = acc[r][2] = 0.0;
+    const int64_t c1 = row_chunk[i + 1];
+    for (int64_t c = row_chunk[i] + wave; c < c1; c += kConvThreads / 64) {
+        const uint32_t id = idx[c * kChunk + lane];
+        const int j = (int)(id >> kKBits), k = (int)(id & kMaxLag);
+        int slot = (s - k) % R;
+        if (slot < 0) slot += R;
+        const double* v = vals + c * (kVals * kChunk) + lane;
+        const double g00 = v[0], g01 = v[64], g02 = v[128], g11 = v[192], g12 = v[256],
+                     g22 = v[320];
+        const double h00 = v[384], h01 = v[448], h02 = v[512], h10 = v[576], h11 = v[640],
+                     h12 = v[704], h20 = v[768], h21 = v[832], h22 = v[896];
+#pragma unroll
+        for (int r = 0; r < NRHS; r++) {
+            const double* zu = ring + ((int64_t)(slot * NRHS + r) * 2) * N + 3 * j;
+            const double* zt = zu + N;
+            const double u0 = zu[0], u1 = zu[1], u2 = zu[2];
+            const double t0 = zt[0], t1 = zt[1], t2 = zt[2];
+            acc[r][0] += g00 * t0 + g01 * t1 + g02 * t2 - (h00 * u0 + h01 * u1 + h02 * u2);
+            acc[r][1] += g01 * t0 + g11 * t1 + g12 * t2 - (h10 * u0 + h11 * u1 + h12 * u2);
+            acc[r][2] += g02 * t0 + g12 * t1 + g22 * t2 - (h20 * u0 + h21 * u1 + h22 * u2);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NRHS; r++)
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+            double x = acc[r][p];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+            if (lane == 0) red[wave][3 * r + p] = x;
+        }
+    __syncthreads();
+    if (threadIdx.x < 3 * NRHS) {
+        double x = 0.0;
+#pragma unroll
+        for (int w = 0; w < kConvThreads / 64; w++) x += red[w][threadIdx.x];
+        const int r = threadIdx.x / 3, p = threadIdx.x % 3;
+        hist[(int64_t)r * N + 3 * i + p] = x;
+    }
+}
+
+// x_1 = f_r(dt) wF + wU  (the n == 2 solve, src/SolverStepLinearT.cpp:355-378)
+__global__ void k_first(const double* __restrict__ wF, const double* __restrict__ wU,
+                        const double* __restrict__ fac, int N, int nrhs, double* __restrict__ x)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)N * nrhs) return;
+    const int r = (int)(t / N), d = (int)(t % N);
+    x[t] = fac[r] * wF[d] + wU[d];
+}
+
+// r = f_r gF + gU + hist
+__global__ void k_rhs(const double* __restrict__ gF, const double* __restrict__ gU,
+                      const double* __restrict__ hist, const double* __restrict__ fac, int N,
+                      int nrhs, double* __restrict__ rv)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)N * nrhs) return;
+    const int r = (int)(t / N), d = (int)(t % N);
+    rv[t] = fac[r] * gF[d] + gU[d] + hist[t];
+}
+
+// Boundary post-processing (src/SolverStepLinearT.cpp:512-536) and the ring update.
+__global__ void k_finalize(const double* __restrict__ x, const int* __restrict__ is_ubc,
+                           const int* __restrict__ is_fbc, const double* __restrict__ ubc_val,
+                           const double* __restrict__ fbc_val, const double* __restrict__ fac,
+                           int N, int nrhs, int s, int R, double a1, double a2, double a3,
+                           double* __restrict__ U, double* __restrict__ T, double* __restrict__ ring)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)N * nrhs) return;
+    const int r = (int)(t / N), d = (int)(t % N);
+    const double xv = x[t];
+    const double u = is_ubc[d] ? ubc_val[d] : xv;
+    double tr = is_fbc[d] ? fbc_val[d] * fac[r] : 0.0;
+    // fDofHigh is inclusive (src/SolverT.cpp:179) and the test is u_dof < fDofHigh (:527)
+    if (is_ubc[d] && d < N - 1) tr = xv;
+    const int64_t hs = ((int64_t)s * nrhs + r) * N + d;
+    U[hs] = u;
+    T[hs] = tr;
+    const double up = s >= 2 ? U[hs - (int64_t)nrhs * N] : 0.0;
+    const double tp = s >= 2 ? T[hs - (int64_t)nrhs * N] : 0.0;
+    auto at = [&](int m, int part) -> double* {
+        int sl = m % R;
+        if (sl < 0) sl += R;
+        return ring + ((int64_t)(sl * nrhs + r) * 2 + part) * N + d;
+    };
+    *at(s - 1, 0) += a1 * u;            // y_{s-1} complete
+    *at(s - 1, 1) += a1 * tr;
+    *at(s, 0) = a3 * u + a2 * up;       // z_1 of step s+1
+    *at(s, 1) = a3 * tr + a2 * tp;
+    *at(s + 1, 0) = (a2 - a1) * u;      // z_0 of step s+1
+    *at(s + 1, 1) = (a2 - a1) * tr;
+}
+
+template <int NRHS>
+void launch_conv(Context& c, March& M)
+{
+    hipLaunchKernelGGL(k_conv<NRHS>, dim3(c.nnd), dim3(kConvThreads), 0, c.stream,
+                       c.store.vals.p, c.store.idx.p, c.store.row_chunk.p, M.N, M.ring.p, M.R,
+                       M.s, M.hist.p);
+    TD_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void march_free(Context& c)
+{
+    delete c.march;
+    c.march = nullptr;
+}
+
+void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
+                const double* t_rise, int n_ubc, const int* ubc_dof, const double* ubc_val,
+                int n_fbc, const int* fbc_dof, const double* fbc_val)
+{
+    // src/SolverStepLinearT.cpp:317-322
+    if (!(a1 >= 0 && a1 <= 1 && a2 >= 0 && a2 < 1 && a1 + a2 <= 1))
+        throw std::runtime_error("Alpha is out of range");
+    if (nrhs < 1 || nrhs > 8) throw std::runtime_error("nrhs must be in [1, 8]");
+    if (!c.store_ready) build_lag_store(c);
+    march_free(c);
+    c.march = new March();
+    March& M = *c.march;
+    const int N = 3 * c.nnd;
+    M.N = N;
+    M.nrhs = nrhs;
+    M.a1 = a1;
+    M.a2 = a2;
+    M.a3 = 1 - a1 - a2;
+    M.R = c.Kc + 2;
+    M.amp.assign(nrhs, 1.0);
+    M.t_rise.assign(nrhs, 0.0);
+    for (int r = 0; r < nrhs; r++) {
+        if (amp) M.amp[r] = amp[r];
+        if (t_rise) M.t_rise[r] = t_rise[r];
+    }
+    // dense boundary data (INSERT order of SetLoad, src/SolverT.cpp:366-369)
+    std::vector<int> isu(N, 0), isf(N, 0);
+    std::vector<double> uv(N, 0.0), fv(N, 0.0), yF(N, 0.0), yU(N, 0.0);
+    for (int q = 0; q < n_fbc; q++) {
+        const int d = fbc_dof[q];
+        if (d < 0 || d >= N) throw std::runtime_error("FBC dof out of range");
+        isf[d] = 1;
+        fv[d] = fbc_val[q];
+    }
+    for (int q = 0; q < n_ubc; q++) {
+        const int d = ubc_dof[q];
+        if (d < 0 || d >= N) throw std::runtime_error("UBC dof out of range");
+        isu[d] = 1;
+        uv[d] = ubc_val[q];
+    }
+    for (int d = 0; d < N; d++) {
+        yF[d] = (isf[d] && !isu[d]) ? fv[d] : 0.0;
+        yU[d] = isu[d] ? uv[d] : 0.0;
+    }
+    M.is_ubc.upload(isu.data(), N, c.stream);
+    M.is_fbc.upload(isf.data(), N, c.stream);
+    M.ubc_val.upload(uv.data(), N, c.stream);
+    M.fbc_val.upload(fv.data(), N, c.stream);
+    DBuf<double> dyF, dyU;
+    dyF.upload(yF.data(), N, c.stream);
+    dyU.upload(yU.data(), N, c.stream);
+
+    const int64_t NN = (int64_t)N * N;
+    DBuf<double> G0, H0, G1, H1, A, Gc, Hc;
+    G0.alloc(NN);
+    H0.alloc(NN);
+    G1.alloc(NN);
+    H1.alloc(NN);
+    A.alloc(NN);
+    Gc.alloc(NN);
+    Hc.alloc(NN);
+    lag_dense(c, 0, G0.p, H0.p);
+    lag_dense(c, 1, G1.p, H1.p);
+    DBuf<int> piv;
+    piv.alloc(N);
+    M.gF.alloc(N);
+    M.gU.alloc(N);
+    M.wF.alloc(N);
+    M.wU.alloc(N);
+
+    // n == 2: A_2 = M_0 exchanged; gF2 = G_0 y_F, gU2 = -H_0 y_U
+    hipLaunchKernelGGL(k_combine, dim3(blocks(NN)), dim3(256), 0, c.stream, G0.p, H0.p,
+                       (const double*)nullptr, (const double*)nullptr, 1.0, 0.0, M.is_ubc.p, N,
+                       A.p, Gc.p, Hc.p);
+    TD_HIP(hipGetLastError());
+    gemv(Gc.p, N, dyF.p, M.gF.p, 0.0, c.stream);
+    gemv(Hc.p, N, dyU.p, M.gU.p, 0.0, c.stream);
+    hipLaunchKernelGGL(k_negate, dim3(blocks(N)), dim3(256), 0, c.stream, M.gU.p, N);
+    lu_factor(A.p, N, piv.p, c.stream);
+    {
+        DBuf<double> B;
+        B.alloc((size_t)2 * N);
+        hipLaunchKernelGGL(k_pack2, dim3(blocks(N)), dim3(256), 0, c.stream, M.gF.p, M.gU.p, N,
+                           B.p);
+        lu_solve(A.p, N, piv.p, B.p, 2, c.stream);
+        hipLaunchKernelGGL(k_unpack2, dim3(blocks(N)), dim3(256), 0, c.stream, B.p, N, M.wF.p,
+                           M.wU.p);
+        TD_HIP(hipStreamSynchronize(c.stream));
+    }
+    // n >= 3: (2 a1 + a3) M_0 + a1 M_1 (src/SolverStepLinearT.cpp:384-393)
+    hipLaunchKernelGGL(k_combine, dim3(blocks(NN)), dim3(256), 0, c.stream, G0.p, H0.p, G1.p,
+                       H1.p, 2 * a1 + M.a3, a1, M.is_ubc.p, N, A.p, Gc.p, Hc.p);
+    TD_HIP(hipGetLastError());
+    gemv(Gc.p, N, dyF.p, M.gF.p, 0.0, c.stream);
+    gemv(Hc.p, N, dyU.p, M.gU.p, 0.0, c.stream);
+    hipLaunchKernelGGL(k_negate, dim3(blocks(N)), dim3(256), 0, c.stream, M.gU.p, N);
+    lu_factor(A.p, N, piv.p, c.stream);
+    M.Ainv.alloc(NN);
+    lu_inverse(A.p, N, piv.p, M.Ainv.p, c.stream);
+
+    M.ring.alloc((size_t)M.R * nrhs * 2 * N);
+    TD_HIP(hipMemsetAsync(M.ring.p, 0, M.ring.bytes(), c.stream));
+    M.hist.alloc((size_t)nrhs * N);
+    M.rv.alloc((size_t)nrhs * N);
+    M.xv.alloc((size_t)nrhs * N);
+    M.fac.alloc(nrhs);
+    M.d_amp.upload(M.amp.data(), nrhs, c.stream);
+    M.d_rise.upload(M.t_rise.data(), nrhs, c.stream);
+    const int nhist = std::max(c.numstep, 1);
+    M.U.alloc((size_t)nhist * nrhs * N);
+    M.T.alloc((size_t)nhist * nrhs * N);
+    TD_HIP(hipMemsetAsync(M.U.p, 0, M.U.bytes(), c.stream));
+    TD_HIP(hipMemsetAsync(M.T.p, 0, M.T.bytes(), c.stream));
+    TD_HIP(hipStreamSynchronize(c.stream));
+    M.s = 0;
+}
+
+int march_step_count(const Context& c) { return c.march ? c.march->s : 0; }
+
+void march_run(Context& c, int nsteps)
+{
+    if (!c.march) throw std::runtime_error("march not initialised");
+    March& M = *c.march;
+    const int N = M.N;
+    const int64_t tot = (int64_t)N * M.nrhs;
+    for (int it = 0; it < nsteps; it++) {
+        if (M.s + 1 >= c.numstep) break;  // histories hold x_0 .. x_{numstep-1}
+        const int s = ++M.s;
+        hipLaunchKernelGGL(k_fac, dim3(1), dim3(64), 0, c.stream, M.d_amp.p, M.d_rise.p,
+                           s * c.dt, M.nrhs, M.fac.p);
+        if (s == 1) {
+            hipLaunchKernelGGL(k_first, dim3(blocks(tot)), dim3(256), 0, c.stream, M.wF.p, M.wU.p,
+                               M.fac.p, N, M.nrhs, M.xv.p);
+        } else {
+            switch (M.nrhs) {
+            case 1: launch_conv<1>(c, M); break;
+            case 2: launch_conv<2>(c, M); break;
+            case 4: launch_conv<4>(c, M); break;
+            case 8: launch_conv<8>(c, M); break;
+            default: throw std::runtime_error("nrhs must be 1, 2, 4 or 8");
+            }
+            hipLaunchKernelGGL(k_rhs, dim3(blocks(tot)), dim3(256), 0, c.stream, M.gF.p, M.gU.p,
+                               M.hist.p, M.fac.p, N, M.nrhs, M.rv.p);
+            for (int r = 0; r < M.nrhs; r++)
+                gemv(M.Ainv.p, N, M.rv.p + (int64_t)r * N, M.xv.p + (int64_t)r * N, 0.0,
+                     c.stream);
+        }
+        hipLaunchKernelGGL(k_finalize, dim3(blocks(tot)), dim3(256), 0, c.stream, M.xv.p,
+                           M.is_ubc.p, M.is_fbc.p, M.ubc_val.p, M.fbc_val.p, M.fac.p, N, M.nrhs,
+                           s, M.R, M.a1, M.a2, M.a3, M.U.p, M.T.p, M.ring.p);
+        TD_HIP(hipGetLastError());
+    }
+    TD_HIP(hipStreamSynchronize(c.stream));
+}
+
+void march_history(Context& c, int rhs, double* U, double* T)
+{
+    if (!c.march) throw std::runtime_error("march not initialised");
+    March& M = *c.march;
+    if (rhs < 0 || rhs >= M.nrhs) throw std::runtime_error("rhs index out of range");
+    const int N = M.N;
+    for (int s = 0; s < c.numstep; s++) {
+        const int64_t off = ((int64_t)s * M.nrhs + rhs) * N;
+        TD_HIP(hipMemcpyAsync(U + (int64_t)s * N, M.U.p + off, N * sizeof(double),
+                              hipMemcpyDeviceToHost, c.stream));
+        TD_HIP(hipMemcpyAsync(T + (int64_t)s * N, M.T.p + off, N * sizeof(double),
+                              hipMemcpyDeviceToHost, c.stream));
+    }
+    TD_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace tdbem

@@ -1,0 +1,131 @@
+// tdbem_internal.h — host-side context and launch declarations shared by the HIP translation
+// units of libtdbem.  Not part of the public C ABI (include/tdbem.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "tdbem_device.h"
+
+namespace tdbem {
+
+// Throws a std::runtime_error carrying `msg` and the HIP error string when `e` is not success.
+void check_hip(hipError_t e, const char* what);
+#define TD_HIP(call) ::tdbem::check_hip((call), #call)
+
+// Device buffer owned by a context (freed by ~DBuf).
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { reset(); }
+    void reset()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count)
+    {
+        if (count == n && p) return;
+        reset();
+        if (count == 0) return;
+        TD_HIP(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    void upload(const T* h, size_t count, hipStream_t s)
+    {
+        alloc(count);
+        if (count) TD_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Entries of the shell lag store: 15 values per (row node i, column node j, lag k):
+// G[p][q] symmetric (g00 g01 g02 g11 g12 g22) then H[p][q] row-major.  Entries of one row node
+// are ordered by (j, k) and padded to a multiple of 64 (one wavefront); values are stored in
+// 64-entry chunks as [chunk][15][64] so each component load of a wavefront is one 512-byte line.
+constexpr int kVals = 15;
+constexpr int kChunk = 64;
+constexpr int kKBits = 10;  // entry index = (j << kKBits) | k
+constexpr int kMaxLag = (1 << kKBits) - 1;
+
+struct LagStore {
+    int64_t n_entries = 0;        // padded
+    int64_t n_chunks = 0;
+    DBuf<double> vals;            // [n_chunks][15][64]
+    DBuf<uint32_t> idx;           // [n_entries]
+    DBuf<int64_t> row_chunk;      // [nnd + 1] chunk offsets per row node
+    DBuf<int64_t> row_valid;      // [nnd] entries before padding, per row node
+    std::vector<int64_t> h_row_chunk, h_row_valid;
+    int64_t nonzero_entries = 0;  // before padding
+};
+
+struct Context {
+    // problem
+    int nnd = 0, nel = 0, etype = 1, ennd = 4, nq = 6, kernel = 4, if_ext = 0;
+    int numstep = 0, K = 0, Kc = 0;
+    double dt = 0;
+    Material mat{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::string err;
+
+    std::vector<double> h_coords;
+    std::vector<int> h_ien;
+    DBuf<double> coords;   // [nnd][3]
+    DBuf<int> ien;         // [nel][ennd]
+    DBuf<double> gauss;    // [2][nq] abscissae then weights (shape table)
+    DBuf<double> geo;      // [nel][nq*nq][7+ennd]: x(3) n(3) wJ N(ennd)
+    DBuf<int> adj_off;     // [nnd+1]
+    DBuf<int> adj;         // (e << 3) | a
+    std::vector<int> h_adj_off, h_adj;
+    DBuf<double> cterm;    // [nnd][9] sum over elements of the Kelvin C-term
+    bool cterm_ready = false;
+    DBuf<int2> pair_k;     // [nnd*nel] (k_lo, k_hi), k_hi < k_lo when the pair is empty
+    bool ranges_ready = false;
+
+    LagStore store;
+    bool store_ready = false;
+
+    // march state (march.hip)
+    struct March* march = nullptr;
+
+    ~Context();
+};
+
+// assemble.hip
+void prepare_geometry(Context& c);
+void compute_cterm(Context& c);
+void compute_pair_ranges(Context& c);
+void assemble_step_dense(Context& c, int step, int lower, int upper, double* dG1, double* dG2,
+                         double* dH1, double* dH2);
+void build_lag_store(Context& c);
+void lag_dense(Context& c, int k, double* dG, double* dH);   // [N][N] each, device
+int64_t predict_store_bytes(Context& c);
+
+// march.hip
+void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
+                const double* t_rise, int n_ubc, const int* ubc_dof, const double* ubc_val,
+                int n_fbc, const int* fbc_dof, const double* fbc_val);
+void march_run(Context& c, int nsteps);
+int march_step_count(const Context& c);
+void march_history(Context& c, int rhs, double* U, double* T);
+void march_free(Context& c);
+
+// lu.hip: in-place LU with partial pivoting of a dense row-major n x n device matrix, and
+// solves with it.  piv is a device int array of length n.
+void lu_factor(double* A, int n, int* piv, hipStream_t s);
+void lu_solve(const double* LU, int n, const int* piv, double* B, int ncols, hipStream_t s);
+void lu_inverse(const double* LU, int n, const int* piv, double* Ainv, hipStream_t s);
+
+// small dense helpers (lu.hip)
+void gemv(const double* A, int n, const double* x, double* y, double beta, hipStream_t s);
+
+}  // namespace tdbem

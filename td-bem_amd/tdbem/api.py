@@ -1,0 +1,183 @@
+"""ctypes binding of the C ABI in include/tdbem.h (libtdbem.so built from td-bem_amd/csrc).
+
+This is the Python face of the same entry points the C++ host (td-bem_amd/host) calls.  Loading
+fails loudly when the HIP library is missing: there is no CPU fallback.
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parents[1]
+LIB_PATH = PKG / "libtdbem.so"
+
+KERNEL_HEAVISIDE = 2
+KERNEL_STEP = 4
+
+_lib = None
+
+
+class TdbemError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"tdbem error {code}: {msg}")
+        self.code = code
+        self.msg = msg
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(str(LIB_PATH))
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        vp = C.c_void_p
+        L.tdbem_create.argtypes = [C.c_int, dp, C.c_int, ip, C.c_int, C.c_int, C.c_double,
+                                   C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int,
+                                   C.POINTER(vp)]
+        L.tdbem_destroy.argtypes = [vp]
+        L.tdbem_destroy.restype = None
+        L.tdbem_last_error.argtypes = [vp]
+        L.tdbem_last_error.restype = C.c_char_p
+        L.tdbem_max_step.argtypes = [vp]
+        L.tdbem_assemble_step.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp, dp, dp]
+        L.tdbem_assemble_step_device.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]
+        L.tdbem_build_lag_store.argtypes = [vp]
+        L.tdbem_lag_store_bytes.argtypes = [vp]
+        L.tdbem_lag_store_bytes.restype = C.c_int64
+        L.tdbem_get_lag.argtypes = [vp, C.c_int, dp, dp]
+        L.tdbem_march_init.argtypes = [vp, C.c_double, C.c_double, C.c_int, ip, dp, C.c_int, ip,
+                                       dp, C.c_int, dp, dp]
+        L.tdbem_march_run.argtypes = [vp, C.c_int]
+        L.tdbem_march_step_index.argtypes = [vp]
+        L.tdbem_get_history.argtypes = [vp, C.c_int, dp, dp]
+        L.tdbem_synchronize.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class Context:
+    """One problem on the GPU: mesh, material, time step, lag store and march state."""
+
+    def __init__(self, coords, ien, *, E, nu, rho, dt, numstep, nq=6, etype=1, if_ext=0,
+                 kernel=KERNEL_STEP):
+        L = lib()
+        self.coords = np.ascontiguousarray(coords, dtype=np.float64)
+        self.ien = np.ascontiguousarray(ien, dtype=np.int32)
+        self.nnd = self.coords.shape[0]
+        self.N = 3 * self.nnd
+        self.numstep = int(numstep)
+        h = C.c_void_p()
+        rc = L.tdbem_create(self.nnd, _d(self.coords), self.ien.shape[0], _i(self.ien), etype,
+                            nq, E, nu, rho, if_ext, dt, numstep, kernel, C.byref(h))
+        if rc:
+            raise TdbemError(rc, L.tdbem_last_error(None).decode())
+        self._h = h
+
+    @classmethod
+    def from_problem(cls, p, numstep=None):
+        return cls(p.coords, p.ien, E=p.E, nu=p.nu, rho=p.rho, dt=p.dt,
+                   numstep=p.numstep if numstep is None else numstep, nq=p.nq, etype=p.etype,
+                   if_ext=p.if_ext, kernel=p.kernel)
+
+    def _check(self, rc):
+        if rc:
+            raise TdbemError(rc, lib().tdbem_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().tdbem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def max_step(self):
+        return lib().tdbem_max_step(self._h)
+
+    def assemble_step(self, step, lower=0, upper=None):
+        upper = self.nnd - 1 if upper is None else upper
+        shape = (3 * (upper - lower + 1), self.N)
+        out = [np.zeros(shape) for _ in range(4)]
+        self._check(lib().tdbem_assemble_step(self._h, step, lower, upper, *[_d(o) for o in out]))
+        return out
+
+    def build_lag_store(self):
+        self._check(lib().tdbem_build_lag_store(self._h))
+
+    def lag_store_bytes(self):
+        v = lib().tdbem_lag_store_bytes(self._h)
+        if v < 0:
+            self._check(int(v))
+        return int(v)
+
+    def get_lag(self, k):
+        G = np.zeros((self.N, self.N))
+        H = np.zeros((self.N, self.N))
+        self._check(lib().tdbem_get_lag(self._h, k, _d(G), _d(H)))
+        return G, H
+
+    def march_init(self, a1=0.25, a2=0.25, ubc_dof=(), ubc_val=(), fbc_dof=(), fbc_val=(),
+                   amp=None, t_rise=None):
+        ud = np.ascontiguousarray(ubc_dof, dtype=np.int32)
+        uv = np.ascontiguousarray(ubc_val, dtype=np.float64)
+        fd = np.ascontiguousarray(fbc_dof, dtype=np.int32)
+        fv = np.ascontiguousarray(fbc_val, dtype=np.float64)
+        nrhs = 1 if amp is None else len(amp)
+        a = None if amp is None else np.ascontiguousarray(amp, dtype=np.float64)
+        t = None if t_rise is None else np.ascontiguousarray(t_rise, dtype=np.float64)
+        if a is not None and t is None:
+            t = np.zeros(nrhs)
+        self._keep = (ud, uv, fd, fv, a, t)
+        self.nrhs = nrhs
+        self._check(lib().tdbem_march_init(
+            self._h, a1, a2, len(ud), _i(ud), _d(uv), len(fd), _i(fd), _d(fv), nrhs,
+            None if a is None else _d(a), None if t is None else _d(t)))
+
+    def march_run(self, nsteps=None):
+        n = self.numstep if nsteps is None else nsteps
+        self._check(lib().tdbem_march_run(self._h, n))
+
+    @property
+    def step_index(self):
+        return lib().tdbem_march_step_index(self._h)
+
+    def history(self, rhs=0):
+        U = np.zeros((self.numstep, self.N))
+        T = np.zeros((self.numstep, self.N))
+        self._check(lib().tdbem_get_history(self._h, rhs, _d(U), _d(T)))
+        return U, T
+
+    def synchronize(self):
+        self._check(lib().tdbem_synchronize(self._h))
+
+
+def solve_problem(p, amp=None, t_rise=None):
+    """Full reference pipeline (main.cpp:89-128) on the GPU: lag store, march, histories."""
+    ctx = Context.from_problem(p)
+    ctx.march_init(p.a1, p.a2, p.ubc_dof, p.ubc_val, p.fbc_dof, p.fbc_val,
+                   amp=[p.amp] if amp is None else amp,
+                   t_rise=[p.t_rise] if t_rise is None else t_rise)
+    ctx.march_run()
+    out = [ctx.history(r) for r in range(ctx.nrhs)]
+    ctx.close()
+    return out

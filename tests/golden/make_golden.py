@@ -1,0 +1,76 @@
+"""Writes the golden fixtures of this directory from the reference's own compiled classes
+(oracle/_ref/libtdbem_ref.so, built from /root/reference/src by `make -C oracle ref`).
+
+Run in the development container, where the reference tree is mounted:
+    python tests/golden/make_golden.py
+Each fixture holds the inputs and the reference's outputs; no reference source is copied.
+"""
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE.parent))
+sys.path.insert(0, str(HERE.parents[1] / "td-bem_amd"))
+
+import helpers  # noqa: E402
+import oracle_lib as O  # noqa: E402
+
+
+def element_cases(seed=11):
+    rng = np.random.default_rng(seed)
+    X4 = np.array([[0, 0, 0], [1, 0, 0], [1.1, 0.9, 0.15], [0, 1, 0.05]], float)
+    X8 = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0.1], [0, 1, 0], [0.5, 0.02, 0],
+                   [1.02, 0.5, 0.05], [0.5, 1, 0.06], [0, 0.5, 0.01]], float)
+    E, nu, rho = 1.0, 0.25, 1.0
+    dt = 0.3
+    rows = []
+    for kern in (4, 2, 5):
+        for etype, X in ((1, X4), (2, X8)):
+            ennd = 4 if etype == 1 else 8
+            colls = [(rng.uniform(-1, 2, 3), -1), (rng.uniform(-0.5, 1.5, 3) + [0, 0, 0.7], -1)]
+            colls += [(X[a], a) for a in range(ennd)]
+            for coll, sing in colls:
+                for n in (1, 2, 4, 7):
+                    out = O.element(kern, etype, 6, E, nu, rho, coll, X, n * dt, dt, n, sing,
+                                    use_ref=True)
+                    rows.append(dict(kernel=kern, etype=etype, coll=np.asarray(coll, float),
+                                     X=np.pad(X, ((0, 8 - ennd), (0, 0))), n=n, sing=sing,
+                                     out=np.concatenate([np.pad(o.reshape(3, -1),
+                                                                ((0, 0), (0, 24 - o.reshape(3, -1).shape[1]))).ravel()
+                                                         for o in out[:4]] + [out[4].ravel(), out[5].ravel()])))
+    return rows
+
+
+def main():
+    if O.ref() is None:
+        raise SystemExit("oracle/_ref/libtdbem_ref.so is not built (make -C oracle ref)")
+    rows = element_cases()
+    np.savez_compressed(
+        HERE / "elements.npz",
+        kernel=np.array([r["kernel"] for r in rows], np.int32),
+        etype=np.array([r["etype"] for r in rows], np.int32),
+        coll=np.array([r["coll"] for r in rows]), X=np.array([r["X"] for r in rows]),
+        n=np.array([r["n"] for r in rows], np.int32),
+        sing=np.array([r["sing"] for r in rows], np.int32),
+        out=np.array([r["out"] for r in rows]), E=1.0, nu=0.25, rho=1.0, dt=0.3, nq=6)
+
+    # geometry/HalfSpacePlane_1E.txt and a coarse bar: G_H_Driver rows for the first steps
+    for name, p, rows_hi in (("plane1e", helpers.plane_1e(), None),
+                             ("bar_h05", helpers.small_bar(), 7)):
+        upper = p.nnd - 1 if rows_hi is None else rows_hi
+        steps = [1, 2, 3, 9]
+        mats = []
+        for n in steps:
+            mats.append(np.stack(O.assemble_step(4, 1, 6, p.E, p.nu, p.rho, p.coords, p.ien,
+                                                 p.if_ext, 0, upper, n * p.dt, p.dt, n,
+                                                 use_ref=True)))
+        np.savez_compressed(HERE / f"assemble_{name}.npz", coords=p.coords, ien=p.ien, E=p.E,
+                            nu=p.nu, rho=p.rho, dt=p.dt, if_ext=p.if_ext, upper=upper,
+                            steps=np.array(steps, np.int32), GH=np.array(mats))
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,124 @@
+"""GPU parity of the plain FP64 path against the CPU oracle (which is itself pinned to the
+reference's compiled classes by tests/test_oracle_ref.py and the golden fixtures).
+
+Bar: relative Frobenius error <= 1e-10 (the north-star bar for the plain FP64 path); each case
+prints its name before the GPU call so a failing run names it."""
+import numpy as np
+import pytest
+
+import helpers
+import oracle_lib as O
+from tdbem import api
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def _announce(name):
+    print(f"[gpu case] {name}", flush=True)
+
+
+def _ctx(p, numstep=None, kernel=None):
+    return api.Context(p.coords, p.ien, E=p.E, nu=p.nu, rho=p.rho, dt=p.dt,
+                       numstep=p.numstep if numstep is None else numstep, nq=p.nq,
+                       etype=p.etype, if_ext=p.if_ext,
+                       kernel=p.kernel if kernel is None else kernel)
+
+
+def _oracle_step(p, step, lower=0, upper=None, kernel=4):
+    upper = p.nnd - 1 if upper is None else upper
+    return O.assemble_step(kernel, p.etype, p.nq, p.E, p.nu, p.rho, p.coords, p.ien, p.if_ext,
+                           lower, upper, step * p.dt, p.dt, step)
+
+
+@pytest.mark.parametrize("case,steps", [("plane1e", [1, 2, 3, 7]),
+                                        ("bar", [1, 2, 5, 12, 25]),
+                                        ("sphere", [1, 3, 9]),
+                                        ("bar8", [1, 2, 6])])
+def test_assemble_step_matches_oracle(gpu, case, steps):
+    p = {"plane1e": helpers.plane_1e, "bar": helpers.small_bar, "sphere": helpers.small_sphere,
+         "bar8": helpers.small_quadratic_bar}[case]()
+    _announce(f"assemble_step {case} nnd={p.nnd} nel={p.nel}")
+    with _ctx(p) as ctx:
+        got = [ctx.assemble_step(n) for n in steps]
+    ref = [_oracle_step(p, n) for n in steps]
+    for m, name in enumerate(("G1", "G2", "H1", "H2")):
+        err = helpers.rel_fro_set([g[m] for g in got], [r[m] for r in ref])
+        assert err <= TOL, f"{case} {name}: rel err {err:.3e}"
+        peak = max(np.linalg.norm(r[m]) for r in ref)
+        for n, g, r in zip(steps, got, ref):
+            if np.linalg.norm(r[m]) > 1e-6 * peak:
+                e = helpers.rel_fro(g[m], r[m])
+                assert e <= TOL, f"{case} step {n} {name}: rel err {e:.3e}"
+
+
+def test_assemble_step_row_range_and_heaviside(gpu):
+    p = helpers.small_bar()
+    _announce("assemble_step rows 5..17, Heaviside kernel")
+    with _ctx(p, kernel=2) as ctx:
+        got = [ctx.assemble_step(n, 5, 17) for n in (1, 4)]
+    ref = [_oracle_step(p, n, 5, 17, kernel=2) for n in (1, 4)]
+    for m in range(4):
+        assert helpers.rel_fro_set([g[m] for g in got], [r[m] for r in ref]) <= TOL
+
+
+@pytest.mark.parametrize("case", ["bar", "sphere", "plane1e"])
+def test_lag_store_matches_oracle(gpu, case):
+    p = {"bar": helpers.small_bar, "sphere": helpers.small_sphere,
+         "plane1e": helpers.plane_1e}[case]()
+    numstep = 40
+    _announce(f"lag store {case}")
+    with _ctx(p, numstep=numstep) as ctx:
+        K = ctx.max_step
+        assert K == O.max_step(p.E, p.nu, p.rho, p.coords, p.dt)
+        ctx.build_lag_store()
+        MG, MH = O.build_lags(4, p.etype, p.nq, p.E, p.nu, p.rho, p.coords, p.ien, p.if_ext,
+                              p.dt, numstep, K)
+        Kc = min(K, numstep)
+        lags = [ctx.get_lag(k) for k in range(Kc + 1)]
+        assert helpers.rel_fro_set([g for g, _ in lags], list(MG[:Kc + 1])) <= TOL
+        assert helpers.rel_fro_set([h for _, h in lags], list(MH[:Kc + 1])) <= TOL
+        peak = max(np.linalg.norm(MH[k]) for k in range(Kc + 1))
+        for k in range(Kc + 1):
+            if np.linalg.norm(MH[k]) > 1e-6 * peak:
+                assert helpers.rel_fro(lags[k][1], MH[k]) <= TOL, f"H lag {k}"
+        # the store holds fewer entries than the dense lags would
+        assert ctx.lag_store_bytes() < (Kc + 1) * 2 * (3 * p.nnd) ** 2 * 8
+
+
+@pytest.mark.parametrize("case,a1,a2", [("bar", 0.25, 0.25), ("bar", 0.0, 0.0),
+                                        ("sphere", 0.25, 0.25), ("bar", 0.4, 0.1)])
+def test_march_matches_oracle(gpu, case, a1, a2):
+    p = {"bar": helpers.small_bar, "sphere": helpers.small_sphere}[case](numstep=40)
+    _announce(f"march {case} a1={a1} a2={a2}")
+    with _ctx(p) as ctx:
+        K = ctx.max_step
+        ctx.march_init(a1, a2, p.ubc_dof, p.ubc_val, p.fbc_dof, p.fbc_val)
+        ctx.march_run()
+        assert ctx.step_index == p.numstep - 1
+        U, T = ctx.history()
+    MG, MH = O.build_lags(4, p.etype, p.nq, p.E, p.nu, p.rho, p.coords, p.ien, p.if_ext, p.dt,
+                          p.numstep, K)
+    Ur, Tr = O.march(MG, MH, p.numstep, p.dt, p.ubc_dof, p.ubc_val, p.fbc_dof, p.fbc_val, a1, a2)
+    assert helpers.rel_fro(U, Ur) <= TOL
+    assert helpers.rel_fro(T, Tr) <= TOL
+
+
+def test_march_multi_rhs_linearity(gpu):
+    """Four load histories in one march equal four single-RHS marches (amplitude and rise)."""
+    p = helpers.small_bar(numstep=30)
+    amp = [1.0, 0.5, 2.0, 1.3]
+    rise = [0.0, 3 * p.dt, 0.0, 7.5 * p.dt]
+    _announce("march nrhs=4")
+    with _ctx(p) as ctx:
+        ctx.march_init(0.25, 0.25, p.ubc_dof, p.ubc_val, p.fbc_dof, p.fbc_val, amp=amp,
+                       t_rise=rise)
+        ctx.march_run()
+        multi = [ctx.history(r) for r in range(4)]
+    MG, MH = O.build_lags(4, 1, 6, p.E, p.nu, p.rho, p.coords, p.ien, 0, p.dt, p.numstep,
+                          O.max_step(p.E, p.nu, p.rho, p.coords, p.dt))
+    for r in range(4):
+        Ur, Tr = O.march(MG, MH, p.numstep, p.dt, p.ubc_dof, p.ubc_val, p.fbc_dof, p.fbc_val,
+                         amp=amp[r], t_rise=rise[r])
+        assert helpers.rel_fro(multi[r][0], Ur) <= TOL
+        assert helpers.rel_fro(multi[r][1], Tr) <= TOL
