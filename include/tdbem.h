@@ -87,9 +87,24 @@ int tdbem_march_step_index(const tdbem_context* ctx);
 /* Histories of displacement and traction, x_0 .. x_{numstep-1} as RecordResult sees them
  * (src/SolverStepLinearT.cpp:549-603): U, T are host [numstep][3 nnd]. */
 int tdbem_get_history(tdbem_context* ctx, int rhs, double* U, double* T);
+/* One history vector x_s (u and t, host [3 nnd] each): fDis_DB[s], fTrac_DB[s] as RecordResult
+ * gathers them (src/SolverStepLinearT.cpp:558-564). */
+int tdbem_get_state(tdbem_context* ctx, int rhs, int s, double* u, double* t);
 
 /* Waits for all work queued on the context's stream. */
 int tdbem_synchronize(tdbem_context* ctx);
+
+/* Instrumentation (no counterpart in the reference, which has no timers): when on, the lag
+ * kernel, the store gather, the history convolution and the step solve are bracketed with HIP
+ * events on the context stream.  tdbem_get_timings fills out[TDBEM_NUM_TIMINGS] in this order:
+ * store_ms, init_ms, march_ms (host wall clock of build_lag_store / march_init / march_run),
+ * lag_kernel_ms, gather_ms, conv_ms, solve_ms (summed event times), lag_launches,
+ * conv_launches, active_points (quadrature-point x lag evaluations done by the lag kernel),
+ * store_nonzero (stored (i, j, k) blocks before padding), store_padded. */
+#define TDBEM_NUM_TIMINGS 12
+int tdbem_set_profiling(tdbem_context* ctx, int on);
+int tdbem_reset_timings(tdbem_context* ctx);
+int tdbem_get_timings(tdbem_context* ctx, double* out);
 
 #ifdef __cplusplus
 }

@@ -15,6 +15,7 @@
 // of the elements around j.  The gather is deterministic (fixed order over the node->element
 // adjacency), so results do not depend on scheduling.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -307,11 +308,12 @@ __global__ void k_fill_idx(const int2* __restrict__ hull, const int64_t* __restr
 // (src/GreenFunctionT.cpp:126-129) there is no (t - dt) level.
 // ---------------------------------------------------------------------------------------------
 template <int ENND>
-__device__ __forceinline__ void integrate_pair(const double* coords, const int* ien,
-                                               const double* geo, const double* gauss, int nq,
-                                               int i, int e, const TimeCombo& tc, double dt,
-                                               const Material& m, double* out)
+__device__ __forceinline__ int integrate_pair(const double* coords, const int* ien,
+                                              const double* geo, const double* gauss, int nq,
+                                              int i, int e, const TimeCombo& tc, double dt,
+                                              const Material& m, double* out)
 {
+    int active = 0;
     const double coll[3] = {coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]};
     double X[3 * ENND];
     load_element<ENND>(coords, ien, e, X);
@@ -328,6 +330,7 @@ __device__ __forceinline__ void integrate_pair(const double* coords, const int* 
                              const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
                              double S[6];
                              if (!eval_s(tc, r, dt, inv_dt, m, S)) return;
+                             active++;
                              const double ir = 1.0 / r;
                              d[0] *= ir;
                              d[1] *= ir;
@@ -345,6 +348,7 @@ __device__ __forceinline__ void integrate_pair(const double* coords, const int* 
     for (int a = 0; a < ENND; a++)
 #pragma unroll
         for (int v = 0; v < kVals; v++) out[a * kVals + v] = acc[a][v];
+    return active;
 }
 
 template <int ENND>
@@ -353,7 +357,8 @@ __global__ void __launch_bounds__(kBlock)
                 const double* __restrict__ geo, const double* __restrict__ gauss, int nq, int nel,
                 double dt, Material m, int Kc, int i0, int64_t nslots,
                 const int* __restrict__ slot_pair, const int64_t* __restrict__ pair_slot,
-                const int2* __restrict__ pair_k, double* __restrict__ scratch)
+                const int2* __restrict__ pair_k, double* __restrict__ scratch,
+                unsigned long long* __restrict__ counter)
 {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
@@ -372,8 +377,9 @@ __global__ void __launch_bounds__(kBlock)
     tc.cI[2] = 0.0;
     tc.cJ[2] = hasJ;
     tc.static_ok = true;
-    integrate_pair<ENND>(coords, ien, geo, gauss, nq, i, e, tc, dt, m,
-                         scratch + s * (ENND * kVals));
+    const int active = integrate_pair<ENND>(coords, ien, geo, gauss, nq, i, e, tc, dt, m,
+                                            scratch + s * (ENND * kVals));
+    if (counter) atomicAdd(counter, (unsigned long long)active);
 }
 
 template <int ENND>
@@ -718,6 +724,11 @@ void fill_store(Context& c, const ShellPlan& P)
     DBuf<int> slot_pair;
     DBuf<int64_t> pair_slot;
     const double ext = c.if_ext ? 0.5 * c.dt : 0.0;
+    KernelTimer lag_timer(c.stream, c.profile), gather_timer(c.stream, c.profile);
+    if (c.profile) {
+        c.counters.alloc(1);
+        TD_HIP(hipMemsetAsync(c.counters.p, 0, sizeof(unsigned long long), c.stream));
+    }
     int i0 = 0;
     while (i0 < c.nnd) {
         // grow the batch while its slots fit the scratch budget
@@ -747,23 +758,35 @@ void fill_store(Context& c, const ShellPlan& P)
         slot_pair.upload(h_slot_pair.data(), h_slot_pair.size(), c.stream);
         pair_slot.upload(h_pair_slot.data(), h_pair_slot.size(), c.stream);
         if (slots > 0) {
+            lag_timer.begin();
             hipLaunchKernelGGL(k_pair_lags<ENND>, dim3(grid_for(slots)), dim3(kBlock), 0,
                                c.stream, c.coords.p, c.ien.p, c.geo.p, c.gauss.p, c.nq, c.nel,
                                c.dt, c.mat, c.Kc, i0, slots, slot_pair.p, pair_slot.p,
-                               c.pair_k.p, scratch.p);
+                               c.pair_k.p, scratch.p, c.profile ? c.counters.p : nullptr);
             TD_HIP(hipGetLastError());
+            lag_timer.end();
         }
         const int64_t ch0 = P.row_chunk[i0], ch1 = P.row_chunk[i1];
         if (ch1 > ch0) {
+            gather_timer.begin();
             hipLaunchKernelGGL(k_store_gather<ENND>, dim3((unsigned)(ch1 - ch0)), dim3(kChunk), 0,
                                c.stream, scratch.p, pair_slot.p, c.pair_k.p, c.adj_off.p, c.adj.p,
                                c.cterm.p, S.row_chunk.p, S.row_valid.p, S.idx.p, c.nnd, c.nel,
                                i0, ext, ch0, S.vals.p);
             TD_HIP(hipGetLastError());
+            gather_timer.end();
         }
         // host vectors of this batch are reused next iteration: wait for the uploads to land
         TD_HIP(hipStreamSynchronize(c.stream));
         i0 = i1;
+    }
+    if (c.profile) {
+        c.tm.lag_launches += lag_timer.count();
+        c.tm.lag_kernel_ms += lag_timer.collect();
+        c.tm.gather_ms += gather_timer.collect();
+        unsigned long long n = 0;
+        TD_HIP(hipMemcpy(&n, c.counters.p, sizeof(n), hipMemcpyDeviceToHost));
+        c.tm.active_points += (int64_t)n;
     }
 }
 
@@ -779,6 +802,7 @@ int64_t predict_store_bytes(Context& c)
 void build_lag_store(Context& c)
 {
     if (c.Kc > kMaxLag) throw std::runtime_error("lag count exceeds the store's 10-bit lag index");
+    const auto t0 = std::chrono::steady_clock::now();
     compute_cterm(c);
     ShellPlan P;
     plan_shells(c, P);
@@ -807,6 +831,8 @@ void build_lag_store(Context& c)
         fill_store<8>(c, P);
     TD_HIP(hipStreamSynchronize(c.stream));
     c.store_ready = true;
+    c.tm.store_ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 void lag_dense(Context& c, int k, double* dG, double* dH)

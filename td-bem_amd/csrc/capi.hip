@@ -325,9 +325,42 @@ int tdbem_get_history(tdbem_context* ctx, int rhs, double* U, double* T)
     });
 }
 
+int tdbem_get_state(tdbem_context* ctx, int rhs, int s, double* u, double* t)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!c.march) throw std::logic_error("march not initialised");
+        if (!u || !t) throw std::invalid_argument("null output");
+        if (s < 0 || s >= c.numstep) throw std::invalid_argument("step index out of range");
+        march_state(c, rhs, s, u, t);
+    });
+}
+
 int tdbem_synchronize(tdbem_context* ctx)
 {
     return guarded(ctx, [&](Context& c) { TD_HIP(hipStreamSynchronize(c.stream)); });
+}
+
+int tdbem_set_profiling(tdbem_context* ctx, int on)
+{
+    return guarded(ctx, [&](Context& c) { c.profile = on != 0; });
+}
+
+int tdbem_reset_timings(tdbem_context* ctx)
+{
+    return guarded(ctx, [&](Context& c) { c.tm = Context::Timings(); });
+}
+
+int tdbem_get_timings(tdbem_context* ctx, double* out)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!out) throw std::invalid_argument("null output");
+        const auto& t = c.tm;
+        const double v[TDBEM_NUM_TIMINGS] = {
+            t.store_ms, t.init_ms, t.march_ms, t.lag_kernel_ms, t.gather_ms, t.conv_ms,
+            t.solve_ms, (double)t.lag_launches, (double)t.conv_launches, (double)t.active_points,
+            (double)c.store.nonzero_entries, (double)c.store.n_entries};
+        std::memcpy(out, v, sizeof(v));
+    });
 }
 
 }  // extern "C"

@@ -14,6 +14,7 @@
 // Each step: conv (stream the shell store once), r = f(s) gF + gU + hist, x = A^{-1} r with the
 // inverse of the constant n >= 3 matrix, then the post-processing of :512-536.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <stdexcept>
 #include <vector>
@@ -229,6 +230,7 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
         throw std::runtime_error("Alpha is out of range");
     if (nrhs < 1 || nrhs > 8) throw std::runtime_error("nrhs must be in [1, 8]");
     if (!c.store_ready) build_lag_store(c);
+    const auto t0 = std::chrono::steady_clock::now();
     march_free(c);
     c.march = new March();
     March& M = *c.march;
@@ -335,6 +337,8 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
     TD_HIP(hipMemsetAsync(M.T.p, 0, M.T.bytes(), c.stream));
     TD_HIP(hipStreamSynchronize(c.stream));
     M.s = 0;
+    c.tm.init_ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 int march_step_count(const Context& c) { return c.march ? c.march->s : 0; }
@@ -345,6 +349,8 @@ void march_run(Context& c, int nsteps)
     March& M = *c.march;
     const int N = M.N;
     const int64_t tot = (int64_t)N * M.nrhs;
+    const auto t0 = std::chrono::steady_clock::now();
+    KernelTimer conv_timer(c.stream, c.profile), solve_timer(c.stream, c.profile);
     for (int it = 0; it < nsteps; it++) {
         if (M.s + 1 >= c.numstep) break;  // histories hold x_0 .. x_{numstep-1}
         const int s = ++M.s;
@@ -354,6 +360,7 @@ void march_run(Context& c, int nsteps)
             hipLaunchKernelGGL(k_first, dim3(blocks(tot)), dim3(256), 0, c.stream, M.wF.p, M.wU.p,
                                M.fac.p, N, M.nrhs, M.xv.p);
         } else {
+            conv_timer.begin();
             switch (M.nrhs) {
             case 1: launch_conv<1>(c, M); break;
             case 2: launch_conv<2>(c, M); break;
@@ -361,11 +368,14 @@ void march_run(Context& c, int nsteps)
             case 8: launch_conv<8>(c, M); break;
             default: throw std::runtime_error("nrhs must be 1, 2, 4 or 8");
             }
+            conv_timer.end();
             hipLaunchKernelGGL(k_rhs, dim3(blocks(tot)), dim3(256), 0, c.stream, M.gF.p, M.gU.p,
                                M.hist.p, M.fac.p, N, M.nrhs, M.rv.p);
+            solve_timer.begin();
             for (int r = 0; r < M.nrhs; r++)
                 gemv(M.Ainv.p, N, M.rv.p + (int64_t)r * N, M.xv.p + (int64_t)r * N, 0.0,
                      c.stream);
+            solve_timer.end();
         }
         hipLaunchKernelGGL(k_finalize, dim3(blocks(tot)), dim3(256), 0, c.stream, M.xv.p,
                            M.is_ubc.p, M.is_fbc.p, M.ubc_val.p, M.fbc_val.p, M.fac.p, N, M.nrhs,
@@ -373,6 +383,13 @@ void march_run(Context& c, int nsteps)
         TD_HIP(hipGetLastError());
     }
     TD_HIP(hipStreamSynchronize(c.stream));
+    c.tm.march_ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (c.profile) {
+        c.tm.conv_launches += conv_timer.count();
+        c.tm.conv_ms += conv_timer.collect();
+        c.tm.solve_ms += solve_timer.collect();
+    }
 }
 
 void march_history(Context& c, int rhs, double* U, double* T)
@@ -388,6 +405,21 @@ void march_history(Context& c, int rhs, double* U, double* T)
         TD_HIP(hipMemcpyAsync(T + (int64_t)s * N, M.T.p + off, N * sizeof(double),
                               hipMemcpyDeviceToHost, c.stream));
     }
+    TD_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace tdbem
+
+namespace tdbem {
+
+void march_state(Context& c, int rhs, int s, double* u, double* t)
+{
+    if (!c.march) throw std::runtime_error("march not initialised");
+    March& M = *c.march;
+    if (rhs < 0 || rhs >= M.nrhs) throw std::runtime_error("rhs index out of range");
+    const int64_t off = ((int64_t)s * M.nrhs + rhs) * M.N;
+    TD_HIP(hipMemcpyAsync(u, M.U.p + off, M.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    TD_HIP(hipMemcpyAsync(t, M.T.p + off, M.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     TD_HIP(hipStreamSynchronize(c.stream));
 }
 

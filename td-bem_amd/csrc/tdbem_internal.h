@@ -47,6 +47,42 @@ struct DBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
+// Brackets kernel launches with HIP events on one stream; collect() waits and sums the times.
+struct KernelTimer {
+    hipStream_t s = nullptr;
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    KernelTimer(hipStream_t stream, bool enabled) : s(stream), on(enabled) {}
+    ~KernelTimer() { collect(); }
+    void begin()
+    {
+        if (!on) return;
+        hipEvent_t a, b;
+        TD_HIP(hipEventCreate(&a));
+        TD_HIP(hipEventCreate(&b));
+        TD_HIP(hipEventRecord(a, s));
+        ev.emplace_back(a, b);
+    }
+    void end()
+    {
+        if (on && !ev.empty()) TD_HIP(hipEventRecord(ev.back().second, s));
+    }
+    int64_t count() const { return (int64_t)ev.size(); }
+    double collect()
+    {
+        double ms = 0;
+        if (!ev.empty()) (void)hipEventSynchronize(ev.back().second);
+        for (auto& p : ev) {
+            float t = 0;
+            if (hipEventElapsedTime(&t, p.first, p.second) == hipSuccess) ms += t;
+            (void)hipEventDestroy(p.first);
+            (void)hipEventDestroy(p.second);
+        }
+        ev.clear();
+        return ms;
+    }
+};
+
 // Entries of the shell lag store: 15 values per (row node i, column node j, lag k):
 // G[p][q] symmetric (g00 g01 g02 g11 g12 g22) then H[p][q] row-major.  Entries of one row node
 // are ordered by (j, k) and padded to a multiple of 64 (one wavefront); values are stored in
@@ -97,6 +133,16 @@ struct Context {
     // march state (march.hip)
     struct March* march = nullptr;
 
+    // Optional timing of the hot kernels with HIP events on the context stream.
+    bool profile = false;
+    struct Timings {
+        double store_ms = 0, init_ms = 0, march_ms = 0;  // host wall clock, synchronised
+        double lag_kernel_ms = 0, gather_ms = 0, conv_ms = 0, solve_ms = 0;
+        int64_t lag_launches = 0, conv_launches = 0;
+        int64_t active_points = 0;  // (quadrature point, lag) evaluations of the lag kernel
+    } tm;
+    DBuf<unsigned long long> counters;  // device counters of the lag kernel
+
     ~Context();
 };
 
@@ -117,6 +163,7 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
 void march_run(Context& c, int nsteps);
 int march_step_count(const Context& c);
 void march_history(Context& c, int rhs, double* U, double* T);
+void march_state(Context& c, int rhs, int s, double* u, double* t);
 void march_free(Context& c);
 
 // lu.hip: in-place LU with partial pivoting of a dense row-major n x n device matrix, and

@@ -1,0 +1,92 @@
+// tdbem_main.cpp — command-line driver with the step loop of src/main.cpp:22-149 on the GPU.
+//
+//   tdbem_main input.txt [--write-gh] [--history out.bin]
+//
+// Reads the deck and alpha.txt (a1 a2) from the working directory like the reference, marches
+// numstep steps, and writes displacement.txt in the reference's format.  --write-gh also writes
+// the FE-BE matrices G1_step<k>.txt, G2_step<k>.txt, H1_step<k>.txt, H2_step<k>.txt for every
+// assembled step (the calls commented out at main.cpp:113-116, with their G1/G2 naming: file G1
+// holds fG_2 = phi_2 part, file G2 holds fG_1).  --history writes the full-precision u and t
+// histories as raw doubles [2][numstep][3 nnd].  Unlike the reference, errors exit non-zero.
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <vector>
+
+#include "tdbem_host.h"
+
+using namespace TD_BEM;
+
+int main(int argc, char** argv)
+{
+    if (argc <= 1) {
+        std::cerr << "Input file not specified, try ./TD_BEM input.txt" << std::endl;
+        return 2;
+    }
+    bool write_gh = false;
+    const char* history = nullptr;
+    for (int a = 2; a < argc; a++) {
+        if (!std::strcmp(argv[a], "--write-gh"))
+            write_gh = true;
+        else if (!std::strcmp(argv[a], "--history") && a + 1 < argc)
+            history = argv[++a];
+        else {
+            std::cerr << "unknown option " << argv[a] << std::endl;
+            return 2;
+        }
+    }
+    try {
+        ModelManagerT model;
+        model.ReadInput(argv[1]);
+        SolverStepLinearT solver(&model);
+        int dof_low, dof_high;
+        solver.GetDofLowHigh(&dof_low, &dof_high);
+        model.Distributed_BC(dof_low, dof_high);
+        const int num_step = model.GetNumStep();
+        const double dt = model.GetDT();
+        solver.Initialize();
+
+        double a1, a2;
+        std::ifstream alpha("alpha.txt");
+        if (!(alpha >> a1 >> a2)) throw "alpha.txt not found or unreadable (expected: a1 a2)";
+        std::cout << "***********************************\n"
+                  << "alpha_1=" << a1 << "  alpha_2=" << a2 << "\n"
+                  << "***********************************" << std::endl;
+
+        ElastoDynamicsT* ED = write_gh ? new ElastoDynamicsT(&model) : nullptr;
+        const int max_step = solver.GetMaxStep();
+        for (int curr_step = 1; curr_step <= num_step; curr_step++) {
+            if (ED && curr_step <= max_step) {
+                ED->G_H_Driver(curr_step * dt, dt);
+                double *G_0, *G_1, *H_0, *H_1;
+                ED->GetGHLinear(&G_0, &G_1, &H_0, &H_1);
+                const int rows = dof_high - dof_low + 1, cols = 3 * model.GetNND();
+                solver.WriteGH("G1_step", curr_step - 1, rows, cols, G_1);
+                solver.WriteGH("G2_step", curr_step - 1, rows, cols, G_0);
+                solver.WriteGH("H1_step", curr_step - 1, rows, cols, H_1);
+                solver.WriteGH("H2_step", curr_step - 1, rows, cols, H_0);
+            }
+            solver.SetCurrStep(curr_step);
+            solver.UpdateGHLinear(nullptr, nullptr, nullptr, nullptr);
+            solver.SetLoad();
+            solver.Solve(a1, a2);
+            solver.RecordResult();
+        }
+        delete ED;
+        if (history) {
+            std::vector<double> U, T;
+            solver.GetHistory(U, T);
+            std::FILE* f = std::fopen(history, "wb");
+            if (!f) throw "cannot open the history file";
+            std::fwrite(U.data(), sizeof(double), U.size(), f);
+            std::fwrite(T.data(), sizeof(double), T.size(), f);
+            std::fclose(f);
+        }
+    } catch (const char* msg) {
+        std::cerr << msg << std::endl;
+        return 1;
+    }
+    std::cout << "The computation terminated properly" << std::endl;
+    return 0;
+}

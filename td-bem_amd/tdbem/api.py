@@ -53,6 +53,9 @@ def lib():
         L.tdbem_march_step_index.argtypes = [vp]
         L.tdbem_get_history.argtypes = [vp, C.c_int, dp, dp]
         L.tdbem_synchronize.argtypes = [vp]
+        L.tdbem_set_profiling.argtypes = [vp, C.c_int]
+        L.tdbem_reset_timings.argtypes = [vp]
+        L.tdbem_get_timings.argtypes = [vp, dp]
         _lib = L
     return _lib
 
@@ -169,6 +172,21 @@ class Context:
 
     def synchronize(self):
         self._check(lib().tdbem_synchronize(self._h))
+
+    TIMING_KEYS = ("store_ms", "init_ms", "march_ms", "lag_kernel_ms", "gather_ms", "conv_ms",
+                   "solve_ms", "lag_launches", "conv_launches", "active_points", "store_nonzero",
+                   "store_padded")
+
+    def set_profiling(self, on=True):
+        self._check(lib().tdbem_set_profiling(self._h, int(bool(on))))
+
+    def reset_timings(self):
+        self._check(lib().tdbem_reset_timings(self._h))
+
+    def timings(self):
+        out = np.zeros(len(self.TIMING_KEYS))
+        self._check(lib().tdbem_get_timings(self._h, _d(out)))
+        return dict(zip(self.TIMING_KEYS, out.tolist()))
 
 
 def solve_problem(p, amp=None, t_rise=None):

@@ -45,11 +45,12 @@ def test_assemble_step_matches_oracle(gpu, case, steps):
     for m, name in enumerate(("G1", "G2", "H1", "H2")):
         err = helpers.rel_fro_set([g[m] for g in got], [r[m] for r in ref])
         assert err <= TOL, f"{case} {name}: rel err {err:.3e}"
+        # per step, relative to the step's norm but at least 1% of the peak step: late steps
+        # are mostly static-tail pairs whose reference values are cancellation noise
         peak = max(np.linalg.norm(r[m]) for r in ref)
         for n, g, r in zip(steps, got, ref):
-            if np.linalg.norm(r[m]) > 1e-6 * peak:
-                e = helpers.rel_fro(g[m], r[m])
-                assert e <= TOL, f"{case} step {n} {name}: rel err {e:.3e}"
+            e = np.linalg.norm(g[m] - r[m]) / max(np.linalg.norm(r[m]), 1e-2 * peak)
+            assert e <= TOL, f"{case} step {n} {name}: rel err {e:.3e}"
 
 
 def test_assemble_step_row_range_and_heaviside(gpu):
@@ -78,12 +79,14 @@ def test_lag_store_matches_oracle(gpu, case):
         lags = [ctx.get_lag(k) for k in range(Kc + 1)]
         assert helpers.rel_fro_set([g for g, _ in lags], list(MG[:Kc + 1])) <= TOL
         assert helpers.rel_fro_set([h for _, h in lags], list(MH[:Kc + 1])) <= TOL
-        peak = max(np.linalg.norm(MH[k]) for k in range(Kc + 1))
-        for k in range(Kc + 1):
-            if np.linalg.norm(MH[k]) > 1e-6 * peak:
-                assert helpers.rel_fro(lags[k][1], MH[k]) <= TOL, f"H lag {k}"
-        # the store holds fewer entries than the dense lags would
-        assert ctx.lag_store_bytes() < (Kc + 1) * 2 * (3 * p.nnd) ** 2 * 8
+        for M, got, name in ((MG, [g for g, _ in lags], "G"), (MH, [h for _, h in lags], "H")):
+            peak = max(np.linalg.norm(M[k]) for k in range(Kc + 1))
+            for k in range(Kc + 1):
+                e = np.linalg.norm(got[k] - M[k]) / max(np.linalg.norm(M[k]), 1e-2 * peak)
+                assert e <= TOL, f"{name} lag {k}: {e:.3e}"
+        if p.nnd >= 50:
+            # the shell store holds fewer bytes than the dense lags would
+            assert ctx.lag_store_bytes() < (Kc + 1) * 2 * (3 * p.nnd) ** 2 * 8
 
 
 @pytest.mark.parametrize("case,a1,a2", [("bar", 0.25, 0.25), ("bar", 0.0, 0.0),
