@@ -91,6 +91,19 @@ int tdbem_get_history(tdbem_context* ctx, int rhs, double* U, double* T);
  * gathers them (src/SolverStepLinearT.cpp:558-564). */
 int tdbem_get_state(tdbem_context* ctx, int rhs, int s, double* u, double* t);
 
+/* Arithmetic of the history convolution for the next tdbem_march_init (no counterpart in the
+ * reference, which is FP64 throughout):
+ *   TDBEM_PRECISION_FP64 (default): the plain path, nrhs in {1, 2, 4, 8};
+ *   TDBEM_PRECISION_MIXED_BF16X3: lags stored as three-term split BF16 in 48 x 96 tile blocks,
+ *   products on MFMA accumulated in FP32 per block and FP64 across blocks, nrhs in
+ *   {16, 32, 48, 64}.  Assembly, the C-term, the step-0 LU/inverse and the step solve stay FP64. */
+enum { TDBEM_PRECISION_FP64 = 0, TDBEM_PRECISION_MIXED_BF16X3 = 1 };
+int tdbem_set_precision(tdbem_context* ctx, int precision);
+
+/* Diagnostic: one v_mfma_f32_16x16x32_bf16 product D[16][16] = A[16][32] B[32][16] (inputs
+ * rounded to bf16) through the operand lane layouts the mixed convolution assumes. */
+int tdbem_mfma_selftest(const float* A, const float* B, float* D);
+
 /* Waits for all work queued on the context's stream. */
 int tdbem_synchronize(tdbem_context* ctx);
 

@@ -815,6 +815,7 @@ void build_lag_store(Context& c)
     S.idx.alloc(std::max<int64_t>(S.n_entries, 1));
     TD_HIP(hipMemsetAsync(S.idx.p, 0, S.idx.bytes(), c.stream));
     S.h_row_valid = P.row_valid;
+    S.h_hull = P.hull;
     S.row_chunk.upload(P.row_chunk.data(), P.row_chunk.size(), c.stream);
     S.row_valid.upload(P.row_valid.data(), P.row_valid.size(), c.stream);
     DBuf<int64_t> d_off;

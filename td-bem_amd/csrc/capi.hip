@@ -340,6 +340,27 @@ int tdbem_synchronize(tdbem_context* ctx)
     return guarded(ctx, [&](Context& c) { TD_HIP(hipStreamSynchronize(c.stream)); });
 }
 
+int tdbem_set_precision(tdbem_context* ctx, int precision)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (precision != TDBEM_PRECISION_FP64 && precision != TDBEM_PRECISION_MIXED_BF16X3)
+            throw std::invalid_argument("unknown precision");
+        c.precision = precision;
+    });
+}
+
+int tdbem_mfma_selftest(const float* A, const float* B, float* D)
+{
+    if (!A || !B || !D) return TDBEM_ERR_ARG;
+    try {
+        mfma_selftest(A, B, D);
+        return TDBEM_OK;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        return TDBEM_ERR_HIP;
+    }
+}
+
 int tdbem_set_profiling(tdbem_context* ctx, int on)
 {
     return guarded(ctx, [&](Context& c) { c.profile = on != 0; });

@@ -307,6 +307,77 @@ void lu_inverse(const double* LU, int n, const int* piv, double* Ainv, hipStream
     TD_HIP(hipStreamSynchronize(s));
 }
 
+namespace {
+
+// Y[r][i] = sum_j A[i][j] X[r][j]: 32 rows of A per 256-thread block, A and X staged in LDS in
+// 64-column slabs, each thread owns (row, rhs) outputs.  A is read once from HBM.
+constexpr int GR = 32, GC = 64;
+template <int NR>
+__global__ void __launch_bounds__(256)
+    k_gemm_rhs(const double* __restrict__ A, int n, const double* __restrict__ X,
+               double* __restrict__ Y)
+{
+    __shared__ double As[GR][GC + 1];
+    __shared__ double Xs[NR][GC + 1];
+    const int row0 = blockIdx.x * GR;
+    constexpr int per = (GR * NR + 255) / 256;
+    double acc[per];
+#pragma unroll
+    for (int u = 0; u < per; u++) acc[u] = 0.0;
+    for (int c0 = 0; c0 < n; c0 += GC) {
+        for (int t = threadIdx.x; t < GR * GC; t += 256) {
+            const int r = t / GC, cc = t % GC;
+            As[r][cc] = (row0 + r < n && c0 + cc < n) ? A[(int64_t)(row0 + r) * n + c0 + cc] : 0.0;
+        }
+        for (int t = threadIdx.x; t < NR * GC; t += 256) {
+            const int r = t / GC, cc = t % GC;
+            Xs[r][cc] = (c0 + cc < n) ? X[(int64_t)r * n + c0 + cc] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < per; u++) {
+            const int o = threadIdx.x + 256 * u;
+            if (o < GR * NR) {
+                const int r = o % GR, q = o / GR;
+                double s = 0.0;
+#pragma unroll 8
+                for (int cc = 0; cc < GC; cc++) s += As[r][cc] * Xs[q][cc];
+                acc[u] += s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < per; u++) {
+        const int o = threadIdx.x + 256 * u;
+        if (o < GR * NR) {
+            const int r = o % GR, q = o / GR;
+            if (row0 + r < n) Y[(int64_t)q * n + row0 + r] = acc[u];
+        }
+    }
+}
+
+}  // namespace
+
+void gemm_rhs(const double* A, int n, const double* X, double* Y, int nrhs, hipStream_t s)
+{
+    const dim3 grid((n + GR - 1) / GR);
+    switch (nrhs) {
+    case 1: gemv(A, n, X, Y, 0.0, s); return;
+    case 2: hipLaunchKernelGGL(k_gemm_rhs<2>, grid, dim3(256), 0, s, A, n, X, Y); break;
+    case 4: hipLaunchKernelGGL(k_gemm_rhs<4>, grid, dim3(256), 0, s, A, n, X, Y); break;
+    case 8: hipLaunchKernelGGL(k_gemm_rhs<8>, grid, dim3(256), 0, s, A, n, X, Y); break;
+    case 16: hipLaunchKernelGGL(k_gemm_rhs<16>, grid, dim3(256), 0, s, A, n, X, Y); break;
+    case 32: hipLaunchKernelGGL(k_gemm_rhs<32>, grid, dim3(256), 0, s, A, n, X, Y); break;
+    case 48: hipLaunchKernelGGL(k_gemm_rhs<48>, grid, dim3(256), 0, s, A, n, X, Y); break;
+    case 64: hipLaunchKernelGGL(k_gemm_rhs<64>, grid, dim3(256), 0, s, A, n, X, Y); break;
+    default:
+        for (int r = 0; r < nrhs; r++) gemv(A, n, X + (int64_t)r * n, Y + (int64_t)r * n, 0.0, s);
+        return;
+    }
+    TD_HIP(hipGetLastError());
+}
+
 void gemv(const double* A, int n, const double* x, double* y, double beta, hipStream_t s)
 {
     hipLaunchKernelGGL(k_gemv, dim3(n), dim3(256), 0, s, A, n, x, y, beta);

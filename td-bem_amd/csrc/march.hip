@@ -37,6 +37,10 @@ struct March {
     DBuf<double> ubc_val, fbc_val;   // dense per dof
     DBuf<double> fac;       // [nrhs] load factor of the current step
     DBuf<double> d_amp, d_rise;
+    int precision = 0;             // 1: split-BF16 MFMA convolution
+    DBuf<unsigned short> zs;       // split-BF16 mirror of the ring (mixed)
+    DBuf<double> partial;          // per-wave partial sums of the mixed convolution
+    int64_t zslot = 0;
 };
 
 namespace {
@@ -228,7 +232,10 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
     // src/SolverStepLinearT.cpp:317-322
     if (!(a1 >= 0 && a1 <= 1 && a2 >= 0 && a2 < 1 && a1 + a2 <= 1))
         throw std::runtime_error("Alpha is out of range");
-    if (nrhs < 1 || nrhs > 8) throw std::runtime_error("nrhs must be in [1, 8]");
+    if (c.precision == 0 && !(nrhs == 1 || nrhs == 2 || nrhs == 4 || nrhs == 8))
+        throw std::invalid_argument("FP64 march: nrhs must be 1, 2, 4 or 8");
+    if (c.precision == 1 && !(nrhs == 16 || nrhs == 32 || nrhs == 48 || nrhs == 64))
+        throw std::invalid_argument("mixed march: nrhs must be 16, 32, 48 or 64");
     if (!c.store_ready) build_lag_store(c);
     const auto t0 = std::chrono::steady_clock::now();
     march_free(c);
@@ -322,6 +329,14 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
     M.Ainv.alloc(NN);
     lu_inverse(A.p, N, piv.p, M.Ainv.p, c.stream);
 
+    M.precision = c.precision;
+    if (M.precision == 1) {
+        tp_build(c);
+        M.zslot = tp_zslot_elems(c, nrhs);
+        M.zs.alloc((size_t)M.R * M.zslot);
+        TD_HIP(hipMemsetAsync(M.zs.p, 0, M.zs.bytes(), c.stream));
+        M.partial.alloc((size_t)tp_partial_elems(c, nrhs));
+    }
     M.ring.alloc((size_t)M.R * nrhs * 2 * N);
     TD_HIP(hipMemsetAsync(M.ring.p, 0, M.ring.bytes(), c.stream));
     M.hist.alloc((size_t)nrhs * N);
@@ -360,21 +375,30 @@ void march_run(Context& c, int nsteps)
             hipLaunchKernelGGL(k_first, dim3(blocks(tot)), dim3(256), 0, c.stream, M.wF.p, M.wU.p,
                                M.fac.p, N, M.nrhs, M.xv.p);
         } else {
-            conv_timer.begin();
-            switch (M.nrhs) {
-            case 1: launch_conv<1>(c, M); break;
-            case 2: launch_conv<2>(c, M); break;
-            case 4: launch_conv<4>(c, M); break;
-            case 8: launch_conv<8>(c, M); break;
-            default: throw std::runtime_error("nrhs must be 1, 2, 4 or 8");
+            if (M.precision == 1) {
+                // the previous finalize changed ring slots s-2, s-1 and s
+                for (int m = s - 2; m <= s; m++) {
+                    const int sl = ((m % M.R) + M.R) % M.R;
+                    tp_zsplit(c, M.ring.p, N, M.nrhs, sl, M.zs.p + (int64_t)sl * M.zslot);
+                }
+                conv_timer.begin();
+                tp_conv(c, M.zs.p, M.nrhs, M.R, s, M.partial.p, M.hist.p, N);
+                conv_timer.end();
+            } else {
+                conv_timer.begin();
+                switch (M.nrhs) {
+                case 1: launch_conv<1>(c, M); break;
+                case 2: launch_conv<2>(c, M); break;
+                case 4: launch_conv<4>(c, M); break;
+                case 8: launch_conv<8>(c, M); break;
+                default: throw std::runtime_error("nrhs must be 1, 2, 4 or 8");
+                }
+                conv_timer.end();
             }
-            conv_timer.end();
             hipLaunchKernelGGL(k_rhs, dim3(blocks(tot)), dim3(256), 0, c.stream, M.gF.p, M.gU.p,
                                M.hist.p, M.fac.p, N, M.nrhs, M.rv.p);
             solve_timer.begin();
-            for (int r = 0; r < M.nrhs; r++)
-                gemv(M.Ainv.p, N, M.rv.p + (int64_t)r * N, M.xv.p + (int64_t)r * N, 0.0,
-                     c.stream);
+            gemm_rhs(M.Ainv.p, N, M.rv.p, M.xv.p, M.nrhs, c.stream);
             solve_timer.end();
         }
         hipLaunchKernelGGL(k_finalize, dim3(blocks(tot)), dim3(256), 0, c.stream, M.xv.p,

@@ -100,7 +100,22 @@ struct LagStore {
     DBuf<int64_t> row_chunk;      // [nnd + 1] chunk offsets per row node
     DBuf<int64_t> row_valid;      // [nnd] entries before padding, per row node
     std::vector<int64_t> h_row_chunk, h_row_valid;
+    std::vector<int2> h_hull;     // [nnd*nnd] lag range of each node pair
     int64_t nonzero_entries = 0;  // before padding
+};
+
+// Tile-pair store of the mixed-precision convolution (mixed.hip): nodes in Morton order, tiles
+// of 16 nodes, one dense 48 x 96 split-BF16 block per (row tile, column tile, lag).
+struct TPStore {
+    int nT = 0;
+    std::vector<int> perm, pos;      // tile position -> node (-1 = padding); node -> position
+    DBuf<int> d_perm, d_pos;
+    int64_t nblocks = 0;
+    DBuf<unsigned short> vals;       // [nblocks][3 terms][3][3][64][8] bf16
+    DBuf<uint32_t> bidx;             // [nblocks] (J << 10) | k
+    DBuf<int64_t> row_blk;           // [nT + 1]
+    std::vector<int64_t> h_row_blk;
+    bool ready = false;
 };
 
 struct Context {
@@ -129,6 +144,8 @@ struct Context {
 
     LagStore store;
     bool store_ready = false;
+    TPStore tp;
+    int precision = 0;  // 0: FP64 march, 1: split-BF16 MFMA convolution (mixed)
 
     // march state (march.hip)
     struct March* march = nullptr;
@@ -166,6 +183,15 @@ void march_history(Context& c, int rhs, double* U, double* T);
 void march_state(Context& c, int rhs, int s, double* u, double* t);
 void march_free(Context& c);
 
+// mixed.hip
+void tp_build(Context& c);
+void tp_zsplit(Context& c, const double* ring, int N, int nrhs, int slot, unsigned short* zs);
+int64_t tp_zslot_elems(const Context& c, int nrhs);
+int64_t tp_partial_elems(const Context& c, int nrhs);
+void tp_conv(Context& c, const unsigned short* zs, int nrhs, int Rring, int s, double* partial,
+             double* hist, int N);
+void mfma_selftest(const float* A, const float* B, float* D);
+
 // lu.hip: in-place LU with partial pivoting of a dense row-major n x n device matrix, and
 // solves with it.  piv is a device int array of length n.
 void lu_factor(double* A, int n, int* piv, hipStream_t s);
@@ -174,5 +200,7 @@ void lu_inverse(const double* LU, int n, const int* piv, double* Ainv, hipStream
 
 // small dense helpers (lu.hip)
 void gemv(const double* A, int n, const double* x, double* y, double beta, hipStream_t s);
+// Y[r][i] = sum_j A[i][j] X[r][j] for r < nrhs (A n x n row-major; X, Y right-hand-side-major)
+void gemm_rhs(const double* A, int n, const double* X, double* Y, int nrhs, hipStream_t s);
 
 }  // namespace tdbem

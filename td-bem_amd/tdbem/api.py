@@ -13,6 +13,8 @@ LIB_PATH = PKG / "libtdbem.so"
 
 KERNEL_HEAVISIDE = 2
 KERNEL_STEP = 4
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
 
 _lib = None
 
@@ -53,6 +55,9 @@ def lib():
         L.tdbem_march_step_index.argtypes = [vp]
         L.tdbem_get_history.argtypes = [vp, C.c_int, dp, dp]
         L.tdbem_synchronize.argtypes = [vp]
+        L.tdbem_set_precision.argtypes = [vp, C.c_int]
+        L.tdbem_mfma_selftest.argtypes = [C.POINTER(C.c_float)] * 3
+        L.tdbem_get_state.argtypes = [vp, C.c_int, C.c_int, dp, dp]
         L.tdbem_set_profiling.argtypes = [vp, C.c_int]
         L.tdbem_reset_timings.argtypes = [vp]
         L.tdbem_get_timings.argtypes = [vp, dp]
@@ -173,6 +178,16 @@ class Context:
     def synchronize(self):
         self._check(lib().tdbem_synchronize(self._h))
 
+    def set_precision(self, precision):
+        """PRECISION_FP64 or PRECISION_MIXED (split-BF16 MFMA convolution) for march_init."""
+        self._check(lib().tdbem_set_precision(self._h, int(precision)))
+
+    def state(self, s, rhs=0):
+        u = np.zeros(self.N)
+        t = np.zeros(self.N)
+        self._check(lib().tdbem_get_state(self._h, rhs, s, _d(u), _d(t)))
+        return u, t
+
     TIMING_KEYS = ("store_ms", "init_ms", "march_ms", "lag_kernel_ms", "gather_ms", "conv_ms",
                    "solve_ms", "lag_launches", "conv_launches", "active_points", "store_nonzero",
                    "store_padded")
@@ -199,3 +214,17 @@ def solve_problem(p, amp=None, t_rise=None):
     out = [ctx.history(r) for r in range(ctx.nrhs)]
     ctx.close()
     return out
+
+
+def mfma_selftest(A, B):
+    """D = A B through one v_mfma_f32_16x16x32_bf16 with the lane layouts the mixed convolution
+    uses (A 16x32, B 32x16, rounded to bf16)."""
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    D = np.zeros((16, 16), dtype=np.float32)
+    fp = C.POINTER(C.c_float)
+    rc = lib().tdbem_mfma_selftest(A.ctypes.data_as(fp), B.ctypes.data_as(fp),
+                                   D.ctypes.data_as(fp))
+    if rc:
+        raise TdbemError(rc, lib().tdbem_last_error(None).decode())
+    return D

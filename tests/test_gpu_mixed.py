@@ -1,0 +1,51 @@
+"""Mixed-precision march (split-BF16 MFMA convolution): lane-layout self-test of the MFMA
+builtin, then agreement with the plain FP64 path per right-hand side at the stated tolerance
+(relative Frobenius error of the u and t histories <= 1e-6)."""
+import numpy as np
+import pytest
+
+import helpers
+from tdbem import api
+
+pytestmark = pytest.mark.gpu
+MIXED_TOL = 1e-6
+
+
+def test_mfma_lane_layout(gpu):
+    rng = np.random.default_rng(0)
+    A = rng.integers(-8, 9, size=(16, 32)).astype(np.float32)
+    B = rng.integers(-8, 9, size=(32, 16)).astype(np.float32)
+    print("[gpu case] mfma self-test", flush=True)
+    D = api.mfma_selftest(A, B)
+    np.testing.assert_array_equal(D, A @ B)
+
+
+def _fp64_histories(p, amp, rise):
+    out = []
+    for r0 in range(0, len(amp), 8):
+        with api.Context.from_problem(p) as ctx:
+            ctx.march_init(p.a1, p.a2, p.ubc_dof, p.ubc_val, p.fbc_dof, p.fbc_val,
+                           amp=amp[r0:r0 + 8], t_rise=rise[r0:r0 + 8])
+            ctx.march_run()
+            out += [ctx.history(r) for r in range(8)]
+    return out
+
+
+@pytest.mark.parametrize("case,nrhs", [("sphere", 16), ("bar", 32)])
+def test_mixed_march_matches_fp64(gpu, case, nrhs):
+    p = {"sphere": helpers.small_sphere, "bar": helpers.small_bar}[case](numstep=40)
+    rng = np.random.default_rng(3)
+    amp = rng.uniform(0.5, 2.0, nrhs)
+    rise = np.where(rng.uniform(size=nrhs) < 0.5, 0.0, rng.uniform(0, 10 * p.dt, nrhs))
+    print(f"[gpu case] mixed march {case} nrhs={nrhs}", flush=True)
+    with api.Context.from_problem(p) as ctx:
+        ctx.set_precision(api.PRECISION_MIXED)
+        ctx.march_init(p.a1, p.a2, p.ubc_dof, p.ubc_val, p.fbc_dof, p.fbc_val, amp=amp,
+                       t_rise=rise)
+        ctx.march_run()
+        mixed = [ctx.history(r) for r in range(nrhs)]
+    ref = _fp64_histories(p, amp, rise)
+    errs = [max(helpers.rel_fro(m[0], r[0]), helpers.rel_fro(m[1], r[1]))
+            for m, r in zip(mixed, ref)]
+    print(f"mixed vs fp64 max rel err {max(errs):.3e}")
+    assert max(errs) <= MIXED_TOL
