@@ -1,18 +1,21 @@
-"""Benchmark: time-steps/s of the TD-BEM hot paths on one MI355X.
+"""Benchmark: time-steps/s of the TD-BEM hot paths on one MI355X, mixed precision vs plain FP64.
 
-Workload (the configuration the headline metric is quoted on that fits one GPU): spherical cavity
-of radius 1 in full space, gnomonic cube-sphere 6x16x16 = 1536 bilinear quads (1538 nodes, 4614
-DOF), Heaviside internal pressure p = 1, E = rho = 1, nu = 0.25, beta = c_p dt / h = 0.6 with h the
-RMS element size (dt = 0.04949, MaxStep 65), 600 steps, alpha = (1/4, 1/4).
+Workload: spherical cavity of radius 1 in full space, gnomonic cube-sphere 6x16x16 = 1536 bilinear
+quads (1538 nodes, 4614 DOF), E = rho = 1, nu = 0.25, beta = c_p dt / h = 0.6 with h the RMS
+element size (dt = 0.04949, MaxStep 65), 600 steps, alpha = (1/4, 1/4).
+  - plain FP64: one Heaviside internal pressure p = 1 (the reference's load);
+  - mixed: 32 right-hand sides, pressure amplitudes U[0.5, 2] and rise times U[0, 10 dt] (half of
+    them Heaviside), seed 3; split-BF16 MFMA convolution, FP64 assembly and step-0 solve.
 
-One bench "step" is one complete solve of that problem on the GPU: lag-store assembly of all 66
-lags, the step-0 factorisations, and the 600-step march (main.cpp:89-128).  The mesh is resident
-on the GPU before the timed region.  value = time-steps solved / second over the timed solves.
+One bench "step" is one complete solve: lag-store assembly of all 66 lags, the step-0
+factorisations, and the 599 marching solves (src/main.cpp:89-128).  The mesh is resident on the
+GPU before the timed region.  value = time-steps solved per second (summed over right-hand sides)
+of the mixed form; plain_fp64 gives the same for the plain path and the ratio.  The mixed error is
+max over 8 right-hand sides of ||u_mixed - u_fp64||_F / ||u_fp64||_F (and the same for t).
 Printed as one JSON line; --gpus N > 1 exits (one GPU per box in this build).
 """
 import argparse
 import json
-import os
 import sys
 import time
 from pathlib import Path
@@ -22,29 +25,31 @@ sys.path.insert(0, str(ROOT / "td-bem_amd"))
 
 import numpy as np  # noqa: E402
 
+MI355X_HBM_GBS = 8000.0
+TP_BLOCK_BYTES = 3 * 48 * 96 * 2 + 4     # split-BF16 48x96 block + its index
+FP64_ENTRY_BYTES = 15 * 8 + 4            # 15 values + index per (i, j, k)
+
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--numstep", type=int, default=600, help="time steps per solve")
+    ap.add_argument("--nrhs", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 leg")
     ap.add_argument("--cpu-threads", type=int, default=16)
     return ap.parse_args()
-
-
-MI355X_HBM_GBS = 8000.0
-MI355X_FP64_TFLOPS = 78.6
 
 
 def cpu_baseline(p, threads):
     """The oracle (plain-C restatement of the reference, OpenMP over collocation rows) timed on a
     bounded sample of the same workload and scaled to a full solve:
       - assembly: 16 collocation rows at every step 1..MaxStep, scaled by nnd/16;
-      - march: the reference's per-step work, i.e. its literal Solve_Averaging, which forms the
-        system matrix, refactors it and runs about 6*min(s, MaxStep) dense N x N matvecs per step,
-        timed as one LU of an N x N matrix and 8 dense matvecs, scaled over the 600 steps."""
+      - march: the reference's per-step work, i.e. its literal Solve_Averaging, which refactors
+        the N x N system and runs 6*min(s, MaxStep) dense N x N matvecs per step, timed as one
+        oracle LU of an N x N matrix and 8 dense matvecs, scaled over the 599 steps."""
     sys.path.insert(0, str(ROOT / "tests"))
     import ctypes as C
     import oracle_lib as O
@@ -55,7 +60,8 @@ def cpu_baseline(p, threads):
     for n in range(1, K + 1):
         O.assemble_step(4, 1, 6, p.E, p.nu, p.rho, p.coords, p.ien, p.if_ext, 0, rows - 1,
                         n * p.dt, p.dt, n, nthreads=threads)
-    t_asm = (time.perf_counter() - t0) * p.nnd / rows
+    t_rows = time.perf_counter() - t0
+    t_asm = t_rows * p.nnd / rows
     N = p.ndof
     rng = np.random.default_rng(0)
     A = rng.standard_normal((N, N)) + N * np.eye(N)
@@ -67,17 +73,34 @@ def cpu_baseline(p, threads):
     x = rng.standard_normal(N)
     t0 = time.perf_counter()
     for _ in range(8):
-        y = A @ x  # numpy matvec, multithreaded BLAS: a lower bound on the oracle's loop
+        A @ x
     t_mv = (time.perf_counter() - t0) / 8
     mv = sum(6 * min(s, K) for s in range(1, p.numstep))
     t_march = (p.numstep - 1) * t_lu + mv * t_mv
     total = t_asm + t_march
-    return {"value": p.numstep / total, "unit": "time-steps/s", "cores": threads,
+    return {"value": round((p.numstep - 1) / total, 4), "unit": "time-steps/s", "cores": threads,
             "kind": "port",
-            "sample": (f"oracle assembly of {rows}/{p.nnd} rows x {K} steps ({t_asm * rows / p.nnd:.1f} s"
-                       f" measured, x{p.nnd / rows:.0f}); one {N}x{N} LU ({t_lu:.1f} s) per step and"
-                       f" {mv} matvecs ({t_mv * 1e3:.1f} ms each) as the literal Solve_Averaging does;"
-                       f" extrapolated full solve {total:.0f} s")}
+            "sample": (f"oracle assembly of {rows}/{p.nnd} rows x {K} steps ({t_rows:.1f} s, "
+                       f"scaled x{p.nnd / rows:.0f}); one {N}x{N} oracle LU ({t_lu:.1f} s) per step "
+                       f"and {mv} dense matvecs ({t_mv * 1e3:.2f} ms each) as the literal "
+                       f"Solve_Averaging does; extrapolated full solve {total:.0f} s")}
+
+
+def timed_solves(ctx, p, warmup, steps, amp=None, rise=None):
+    def one():
+        ctx.build_lag_store()
+        ctx.march_init(p.a1, p.a2, p.ubc_dof, p.ubc_val, p.fbc_dof, p.fbc_val, amp=amp,
+                       t_rise=rise)
+        ctx.march_run()
+        ctx.synchronize()
+
+    for _ in range(warmup):
+        one()
+    ctx.reset_timings()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one()
+    return time.perf_counter() - t0, ctx.timings()
 
 
 def main():
@@ -88,59 +111,76 @@ def main():
     from tdbem import api, configs
 
     p = configs.sphere(n=16, numstep=args.numstep)
-    ctx = api.Context.from_problem(p)
-    ctx.set_profiling(True)
+    R = args.nrhs
+    amp, rise = configs.rhs_set(R, p.dt, seed=3)
+    out = {"metric": "time-steps/s mixed vs plain FP64 GPU at <=1e-6 err",
+           "unit": "time-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "bf16x3-mfma/f64", "data": "synthetic"}
 
-    def one_solve():
+    # mixed precision, R right-hand sides
+    with api.Context.from_problem(p) as ctx:
+        ctx.set_profiling(True)
+        ctx.set_precision(api.PRECISION_MIXED)
+        wall, tm = timed_solves(ctx, p, args.warmup, args.steps, amp, rise)
+        mixed_hist = [ctx.history(r) for r in range(8)]
+        K = ctx.max_step
+    mixed_steps = args.steps * (p.numstep - 1) * R
+    conv_ms = tm["conv_ms"] / max(tm["conv_launches"], 1)
+    nblocks_bytes = None
+    out.update({
+        "value": round(mixed_steps / wall, 2),
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "config": {"workload": f"sphere cavity 6x16x16 (1536 quads, 4614 dof), {p.numstep} steps, "
+                               f"nu=0.25, beta=0.6; mixed: {R} pressure histories per solve",
+                   "numstep": p.numstep, "max_step": K, "dof": p.ndof, "nrhs": R},
+        "mixed": {"phases_ms_per_solve": {k: round(tm[k] / args.steps, 3)
+                                          for k in ("store_ms", "init_ms", "march_ms")},
+                  "conv_avg_ms": round(conv_ms, 4),
+                  "solve_avg_ms": round(tm["solve_ms"] / max(tm["conv_launches"], 1), 4)},
+    })
+
+    # FP64 reference for the error on the first 8 right-hand sides
+    with api.Context.from_problem(p) as ctx:
         ctx.build_lag_store()
-        ctx.march_init(p.a1, p.a2, p.ubc_dof, p.ubc_val, p.fbc_dof, p.fbc_val)
+        ctx.march_init(p.a1, p.a2, p.ubc_dof, p.ubc_val, p.fbc_dof, p.fbc_val, amp=amp[:8],
+                       t_rise=rise[:8])
         ctx.march_run()
-        ctx.synchronize()
+        ref = [ctx.history(r) for r in range(8)]
+    err = max(max(np.linalg.norm(m[0] - r[0]) / np.linalg.norm(r[0]),
+                  np.linalg.norm(m[1] - r[1]) / np.linalg.norm(r[1]))
+              for m, r in zip(mixed_hist, ref))
+    out["mixed"]["max_rel_err_vs_fp64"] = float(err)
 
-    for _ in range(args.warmup):
-        one_solve()
-    ctx.reset_timings()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        one_solve()
-    wall = time.perf_counter() - t0
-    tm = ctx.timings()
-    U, T = ctx.history()
-    solves = args.steps
-    steps_total = solves * (p.numstep - 1)
-    value = steps_total / wall
-    # dominant kernel: the history convolution (streams the shell lag store once per step)
-    conv_avg_ms = tm["conv_ms"] / max(tm["conv_launches"], 1)
-    entry_bytes = 15 * 8 + 4
-    conv_bytes = tm["store_nonzero"] * entry_bytes
-    achieved = conv_bytes / (conv_avg_ms * 1e-3) / 1e9
-    out = {
-        "metric": "time-steps/s (plain FP64 GPU path, end-to-end assembly + march)",
-        "value": round(value, 2),
-        "unit": "time-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(wall / solves * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "sphere cavity 6x16x16 (1536 quads, 4614 dof), 600 steps, "
-                               "Heaviside p=1, nu=0.25, beta=0.6, plain FP64",
-                   "numstep": p.numstep, "max_step": ctx.max_step, "dof": p.ndof},
-        "phases_ms_per_solve": {k: round(tm[k] / solves, 3) for k in
-                                ("store_ms", "init_ms", "march_ms")},
-        "lag_store": {"entries": int(tm["store_nonzero"]), "bytes": ctx.lag_store_bytes(),
-                      "dense_bytes": 2 * (ctx.max_step + 1) * p.ndof ** 2 * 8},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": MI355X_HBM_GBS,
-                     "unit": "GB/s", "frac": round(achieved / MI355X_HBM_GBS, 4),
-                     "traffic": None, "kernel": "k_conv", "avg_ms": round(conv_avg_ms, 4),
-                     "bytes_per_launch": int(conv_bytes)},
-        "checks": {"u_r_mean_final": float(np.mean(np.einsum(
-            "ij,ij->i", U[-1].reshape(-1, 3), p.coords)))},
-    }
+    # plain FP64, one right-hand side
+    if not args.no_fp64:
+        with api.Context.from_problem(p) as ctx:
+            ctx.set_profiling(True)
+            wall64, tm64 = timed_solves(ctx, p, args.warmup, args.steps)
+            U, _ = ctx.history()
+            store_bytes = ctx.lag_store_bytes()
+        steps64 = args.steps * (p.numstep - 1)
+        conv64 = tm64["conv_ms"] / max(tm64["conv_launches"], 1)
+        bytes64 = tm64["store_nonzero"] * FP64_ENTRY_BYTES
+        ach64 = bytes64 / (conv64 * 1e-3) / 1e9
+        out["plain_fp64"] = {
+            "value": round(steps64 / wall64, 2), "ms_per_step": round(wall64 / args.steps * 1e3, 3),
+            "phases_ms_per_solve": {k: round(tm64[k] / args.steps, 3)
+                                    for k in ("store_ms", "init_ms", "march_ms")},
+            "lag_kernel_ms": round(tm64["lag_kernel_ms"] / args.steps, 3),
+            "active_points_per_solve": int(tm64["active_points"] / args.steps),
+            "store": {"entries": int(tm64["store_nonzero"]), "bytes": store_bytes,
+                      "dense_bytes": 2 * (K + 1) * p.ndof ** 2 * 8},
+            "conv": {"kernel": "k_conv", "avg_ms": round(conv64, 4),
+                     "bytes_per_launch": int(bytes64), "achieved_GBs": round(ach64, 1)},
+            # Lame static radial displacement p a / (4 mu) = 0.625 at late time
+            "u_r_mean_final": float(np.mean(np.einsum("ij,ij->i", U[-1].reshape(-1, 3), p.coords))),
+        }
+        out["speedup_mixed_vs_plain_fp64"] = round(out["value"] / out["plain_fp64"]["value"], 2)
+        out["roofline"] = {"bound": "hbm", "achieved": round(ach64, 1), "peak": MI355X_HBM_GBS,
+                           "unit": "GB/s", "frac": round(ach64 / MI355X_HBM_GBS, 4),
+                           "traffic": None, "kernel": "k_conv (plain FP64 history convolution)",
+                           "avg_ms": round(conv64, 4), "bytes_per_launch": int(bytes64)}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(p, args.cpu_threads)
     print(json.dumps(out))

@@ -82,6 +82,11 @@ int tdbem_march_init(tdbem_context* ctx, double a1, double a2, int n_ubc, const 
 /* Advances up to nsteps of main.cpp's step loop (src/main.cpp:89-128: SetLoad + Solve +
  * RecordResult), stopping after the last step. */
 int tdbem_march_run(tdbem_context* ctx, int nsteps);
+/* Queues up to nsteps on the context's stream and returns without waiting (for running many
+ * contexts concurrently); tdbem_query returns 1 when the queued work has finished, 0 while it runs.
+ * No counterpart in the reference, whose steps are synchronous. */
+int tdbem_march_enqueue(tdbem_context* ctx, int nsteps);
+int tdbem_query(tdbem_context* ctx);
 /* Index s of the last solved history vector x_s (0 before the first solve). */
 int tdbem_march_step_index(const tdbem_context* ctx);
 /* Histories of displacement and traction, x_0 .. x_{numstep-1} as RecordResult sees them

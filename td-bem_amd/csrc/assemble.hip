@@ -803,6 +803,7 @@ void build_lag_store(Context& c)
 {
     if (c.Kc > kMaxLag) throw std::runtime_error("lag count exceeds the store's 10-bit lag index");
     const auto t0 = std::chrono::steady_clock::now();
+    c.tp.ready = false;  // the tile-pair copy is rebuilt from the new store when needed
     compute_cterm(c);
     ShellPlan P;
     plan_shells(c, P);

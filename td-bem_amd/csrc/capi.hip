@@ -294,8 +294,12 @@ int tdbem_march_init(tdbem_context* ctx, double a1, double a2, int n_ubc, const 
             throw std::invalid_argument("the march runs the Step kernel (SolverStepLinearT)");
         if (!(a1 >= 0 && a1 <= 1 && a2 >= 0 && a2 < 1 && a1 + a2 <= 1))
             throw std::invalid_argument("Alpha is out of range");
-        if (nrhs != 1 && nrhs != 2 && nrhs != 4 && nrhs != 8)
-            throw std::invalid_argument("nrhs must be 1, 2, 4 or 8");
+        if (c.precision == TDBEM_PRECISION_FP64 && nrhs != 1 && nrhs != 2 && nrhs != 4 &&
+            nrhs != 8)
+            throw std::invalid_argument("FP64 march: nrhs must be 1, 2, 4 or 8");
+        if (c.precision == TDBEM_PRECISION_MIXED_BF16X3 && nrhs != 16 && nrhs != 32 &&
+            nrhs != 48 && nrhs != 64)
+            throw std::invalid_argument("mixed march: nrhs must be 16, 32, 48 or 64");
         if ((n_ubc > 0 && (!ubc_dof || !ubc_val)) || (n_fbc > 0 && (!fbc_dof || !fbc_val)))
             throw std::invalid_argument("null boundary-condition array");
         march_init(c, a1, a2, nrhs, amp, t_rise, n_ubc, ubc_dof, ubc_val, n_fbc, fbc_dof,
@@ -309,6 +313,24 @@ int tdbem_march_run(tdbem_context* ctx, int nsteps)
         if (!c.march) throw std::logic_error("march not initialised");
         march_run(c, nsteps);
     });
+}
+
+int tdbem_march_enqueue(tdbem_context* ctx, int nsteps)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!c.march) throw std::logic_error("march not initialised");
+        march_run(c, nsteps, false);
+    });
+}
+
+int tdbem_query(tdbem_context* ctx)
+{
+    if (!ctx) return TDBEM_ERR_ARG;
+    const hipError_t e = hipStreamQuery(ctx->c.stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    ctx->c.err = std::string("HIP error: ") + hipGetErrorString(e);
+    return TDBEM_ERR_HIP;
 }
 
 int tdbem_march_step_index(const tdbem_context* ctx)

@@ -25,6 +25,92 @@ constexpr int kPanelThreads = 1024;
 // Panel factorisation of columns [j0, j0+nb) over rows [j0, n).  Row swaps are applied to whole
 // rows, so the stored factors satisfy P A = L U with L unit lower (below the diagonal) and U upper.
 // ---------------------------------------------------------------------------------------------
+// The panel is copied to a column-major scratch P (P[c * m + r], m = n - j0 rows) so that the
+// pivot search, the scaling and the in-panel rank-1 updates are coalesced across threads; the
+// row interchanges are applied to the rest of A afterwards by k_laswp, one thread per column.
+__global__ void __launch_bounds__(kPanelThreads)
+    k_panel_cm(double* __restrict__ A, int n, int j0, int nb, double* __restrict__ P,
+               int* __restrict__ piv, int* __restrict__ status)
+{
+    __shared__ double sval[kPanelThreads];
+    __shared__ int sidx[kPanelThreads];
+    const int tid = threadIdx.x;
+    const int m = n - j0;
+    for (int t = tid; t < m * nb; t += kPanelThreads) {
+        const int r = t / nb, c = t % nb;
+        P[(int64_t)c * m + r] = A[(int64_t)(j0 + r) * n + j0 + c];
+    }
+    __syncthreads();
+    for (int c = 0; c < nb; c++) {
+        double* Pc = P + (int64_t)c * m;
+        double best = -1.0;
+        int bi = m;
+        for (int r = c + tid; r < m; r += kPanelThreads) {
+            const double v = fabs(Pc[r]);
+            if (v > best) {
+                best = v;
+                bi = r;
+            }
+        }
+        sval[tid] = best;
+        sidx[tid] = bi;
+        __syncthreads();
+        for (int s = kPanelThreads / 2; s > 0; s >>= 1) {
+            if (tid < s) {
+                const double ov = sval[tid + s];
+                const int oi = sidx[tid + s];
+                if (ov > sval[tid] || (ov == sval[tid] && oi < sidx[tid])) {
+                    sval[tid] = ov;
+                    sidx[tid] = oi;
+                }
+            }
+            __syncthreads();
+        }
+        const int p = sidx[0];
+        const double pmax = sval[0];
+        __syncthreads();
+        if (tid == 0) {
+            piv[j0 + c] = j0 + p;
+            if (!(pmax > 0.0)) *status = 1;
+        }
+        if (p != c)
+            for (int col = tid; col < nb; col += kPanelThreads) {
+                double* Pk = P + (int64_t)col * m;
+                const double a = Pk[c];
+                Pk[c] = Pk[p];
+                Pk[p] = a;
+            }
+        __syncthreads();
+        const double d = Pc[c];
+        for (int r = c + 1 + tid; r < m; r += kPanelThreads) {
+            const double l = Pc[r] / d;
+            Pc[r] = l;
+            for (int cc = c + 1; cc < nb; cc++) P[(int64_t)cc * m + r] -= l * P[(int64_t)cc * m + c];
+        }
+        __syncthreads();
+    }
+    for (int t = tid; t < m * nb; t += kPanelThreads) {
+        const int r = t / nb, c = t % nb;
+        A[(int64_t)(j0 + r) * n + j0 + c] = P[(int64_t)c * m + r];
+    }
+}
+
+// Row interchanges of panel [j0, j0 + nb) applied, in order, to the columns outside the panel.
+__global__ void k_laswp(double* __restrict__ A, int n, int j0, int nb, const int* __restrict__ piv)
+{
+    int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= n - nb) return;
+    if (col >= j0) col += nb;
+    for (int c = j0; c < j0 + nb; c++) {
+        const int p = piv[c];
+        if (p != c) {
+            const double a = A[(int64_t)c * n + col];
+            A[(int64_t)c * n + col] = A[(int64_t)p * n + col];
+            A[(int64_t)p * n + col] = a;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kPanelThreads)
     k_panel(double* __restrict__ A, int n, int j0, int nb, int* __restrict__ piv,
             int* __restrict__ status)
@@ -256,11 +342,18 @@ void lu_factor(double* A, int n, int* piv, hipStream_t s)
     DBuf<int> status;
     status.alloc(1);
     TD_HIP(hipMemsetAsync(status.p, 0, sizeof(int), s));
+    DBuf<double> P;
+    P.alloc((size_t)n * NB);
     for (int j0 = 0; j0 < n; j0 += NB) {
         const int nb = std::min(NB, n - j0);
-        hipLaunchKernelGGL(k_panel, dim3(1), dim3(kPanelThreads), 0, s, A, n, j0, nb, piv,
+        hipLaunchKernelGGL(k_panel_cm, dim3(1), dim3(kPanelThreads), 0, s, A, n, j0, nb, P.p, piv,
                            status.p);
         TD_HIP(hipGetLastError());
+        if (n - nb > 0) {
+            hipLaunchKernelGGL(k_laswp, dim3((n - nb + 255) / 256), dim3(256), 0, s, A, n, j0, nb,
+                               piv);
+            TD_HIP(hipGetLastError());
+        }
         const int rest = n - j0 - nb;
         if (rest > 0) {
             const int tpb = 128;

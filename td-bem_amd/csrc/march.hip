@@ -358,14 +358,15 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
 
 int march_step_count(const Context& c) { return c.march ? c.march->s : 0; }
 
-void march_run(Context& c, int nsteps)
+void march_run(Context& c, int nsteps, bool wait)
 {
     if (!c.march) throw std::runtime_error("march not initialised");
     March& M = *c.march;
     const int N = M.N;
     const int64_t tot = (int64_t)N * M.nrhs;
     const auto t0 = std::chrono::steady_clock::now();
-    KernelTimer conv_timer(c.stream, c.profile), solve_timer(c.stream, c.profile);
+    // Without a wait the steps are only queued; per-kernel timing needs the wait.
+    KernelTimer conv_timer(c.stream, c.profile && wait), solve_timer(c.stream, c.profile && wait);
     for (int it = 0; it < nsteps; it++) {
         if (M.s + 1 >= c.numstep) break;  // histories hold x_0 .. x_{numstep-1}
         const int s = ++M.s;
@@ -406,6 +407,7 @@ void march_run(Context& c, int nsteps)
                            s, M.R, M.a1, M.a2, M.a3, M.U.p, M.T.p, M.ring.p);
         TD_HIP(hipGetLastError());
     }
+    if (!wait) return;
     TD_HIP(hipStreamSynchronize(c.stream));
     c.tm.march_ms +=
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

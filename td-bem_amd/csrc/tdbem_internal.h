@@ -177,7 +177,7 @@ int64_t predict_store_bytes(Context& c);
 void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
                 const double* t_rise, int n_ubc, const int* ubc_dof, const double* ubc_val,
                 int n_fbc, const int* fbc_dof, const double* fbc_val);
-void march_run(Context& c, int nsteps);
+void march_run(Context& c, int nsteps, bool wait = true);
 int march_step_count(const Context& c);
 void march_history(Context& c, int rhs, double* U, double* T);
 void march_state(Context& c, int rhs, int s, double* u, double* t);

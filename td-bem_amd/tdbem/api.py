@@ -55,6 +55,8 @@ def lib():
         L.tdbem_march_step_index.argtypes = [vp]
         L.tdbem_get_history.argtypes = [vp, C.c_int, dp, dp]
         L.tdbem_synchronize.argtypes = [vp]
+        L.tdbem_march_enqueue.argtypes = [vp, C.c_int]
+        L.tdbem_query.argtypes = [vp]
         L.tdbem_set_precision.argtypes = [vp, C.c_int]
         L.tdbem_mfma_selftest.argtypes = [C.POINTER(C.c_float)] * 3
         L.tdbem_get_state.argtypes = [vp, C.c_int, C.c_int, dp, dp]
@@ -164,6 +166,16 @@ class Context:
     def march_run(self, nsteps=None):
         n = self.numstep if nsteps is None else nsteps
         self._check(lib().tdbem_march_run(self._h, n))
+
+    def march_enqueue(self, nsteps):
+        """Queue nsteps on this context's stream without waiting."""
+        self._check(lib().tdbem_march_enqueue(self._h, nsteps))
+
+    def done(self):
+        rc = lib().tdbem_query(self._h)
+        if rc < 0:
+            self._check(rc)
+        return rc == 1
 
     @property
     def step_index(self):
