@@ -380,7 +380,8 @@ void march_run(Context& c, int nsteps, bool wait)
                 // the previous finalize changed ring slots s-2, s-1 and s
                 for (int m = s - 2; m <= s; m++) {
                     const int sl = ((m % M.R) + M.R) % M.R;
-                    tp_zsplit(c, M.ring.p, N, M.nrhs, sl, M.zs.p + (int64_t)sl * M.zslot);
+                    // k_zsplit adds the slot offset itself: pass the ring base
+                    tp_zsplit(c, M.ring.p, N, M.nrhs, sl, M.zs.p);
                 }
                 conv_timer.begin();
                 tp_conv(c, M.zs.p, M.nrhs, M.R, s, M.partial.p, M.hist.p, N);
