@@ -103,6 +103,10 @@ def timed_solves(ctx, p, warmup, steps, amp=None, rise=None):
     return time.perf_counter() - t0, ctx.timings()
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -119,6 +123,7 @@ def main():
            "dtype": "bf16x3-mfma/f64", "data": "synthetic"}
 
     # mixed precision, R right-hand sides
+    log(f"mixed leg: {R} right-hand sides, {p.numstep} steps")
     with api.Context.from_problem(p) as ctx:
         ctx.set_profiling(True)
         ctx.set_precision(api.PRECISION_MIXED)
@@ -141,6 +146,7 @@ def main():
     })
 
     # FP64 reference for the error on the first 8 right-hand sides
+    log("FP64 reference for the mixed error")
     with api.Context.from_problem(p) as ctx:
         ctx.build_lag_store()
         ctx.march_init(p.a1, p.a2, p.ubc_dof, p.ubc_val, p.fbc_dof, p.fbc_val, amp=amp[:8],
@@ -154,6 +160,7 @@ def main():
 
     # plain FP64, one right-hand side
     if not args.no_fp64:
+        log("plain FP64 leg")
         with api.Context.from_problem(p) as ctx:
             ctx.set_profiling(True)
             wall64, tm64 = timed_solves(ctx, p, args.warmup, args.steps)
@@ -182,6 +189,7 @@ def main():
                            "traffic": None, "kernel": "k_conv (plain FP64 history convolution)",
                            "avg_ms": round(conv64, 4), "bytes_per_launch": int(bytes64)}
     if not args.no_cpu_baseline:
+        log("cpu baseline (oracle sample)")
         out["cpu_baseline"] = cpu_baseline(p, args.cpu_threads)
     print(json.dumps(out))
 

@@ -30,7 +30,7 @@ struct March {
     DBuf<double> Ainv;      // [N][N] inverse of the n >= 3 system matrix
     DBuf<double> gF, gU;    // G~ y split into the FBC part (scaled by f) and the UBC part
     DBuf<double> wF, wU;    // A_2^{-1} (G~_2 y_F), A_2^{-1} (G~_2 y_U) for the first solve
-    DBuf<double> ring;      // [R][nrhs][2][N]
+    DBuf<double> ring;      // [R][nrhs][2][N], part 0 = u, part 1 = t
     DBuf<double> hist, rv, xv;  // [nrhs][N]
     DBuf<double> U, T;      // [numstep][nrhs][N]
     DBuf<int> is_ubc, is_fbc;

@@ -1,5 +1,5 @@
 // mixed.hip — the mixed-precision history convolution: split-BF16 (three terms) MFMA products
-// accumulated in FP32 inside a 48x96 block and in FP64 across blocks and lags, for R right-hand
+// accumulated in FP32 inside a 32-wide k-tile and in FP64 across tiles, blocks and lags, for R right-hand
 // sides at once (R = 16, 32, 48 or 64).
 //
 // Layout.  Nodes are reordered along a Morton curve and grouped into tiles of 16 (48 dofs).  For
@@ -10,8 +10,9 @@
 // The history ring (FP64) is mirrored as split-BF16 fragments per (slot, column tile).
 //
 // Per block the products a1 z1 + a1 z2 + a2 z1 + a1 z3 + a2 z2 + a3 z1 (all terms down to 2^-24
-// relative) run on v_mfma_f32_16x16x32_bf16; the FP32 block result is added to FP64 accumulators,
-// so rounding does not grow with the number of blocks and lags summed.
+// relative) run on v_mfma_f32_16x16x32_bf16; per 32-wide k-tile the leading product and the sum of
+// the corrections leave FP32 for FP64 accumulators, so rounding does not grow with the number of
+// blocks and lags summed.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -126,7 +127,9 @@ __global__ void k_zsplit(const double* __restrict__ ring, int N, int nrhs, int n
     const int J = (int)(t / ((int64_t)nrhs * kCols));
     const int node = perm[J * kTile + c / 6];
     const int part = (c % 6) / 3, q = c % 3;
-    const double v = node < 0 ? 0.0 : ring[((int64_t)(slot * nrhs + r) * 2 + part) * N + 3 * node + q];
+    // block column part 0 multiplies z^t (ring part 1), part 1 multiplies z^u (ring part 0)
+    const int rpart = 1 - part;
+    const double v = node < 0 ? 0.0 : ring[((int64_t)(slot * nrhs + r) * 2 + rpart) * N + 3 * node + q];
     unsigned short b[3];
     split3(v, b);
     const int kt = c >> 5, kk = c & 31, nt = r >> 4, n = r & 15;
@@ -163,11 +166,6 @@ __global__ void __launch_bounds__(64 * kSplitWaves)
         if (slot < 0) slot += Rring;
         const bf16x8* A = tp + b * (kBlockElems / 8);
         const bf16x8* Z = zs + (((int64_t)slot * nT + J) * 3) * 3 * NT * 64;
-        f32x4 c32[3][NT];
-#pragma unroll
-        for (int m = 0; m < 3; m++)
-#pragma unroll
-            for (int n = 0; n < NT; n++) c32[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kt = 0; kt < 3; kt++) {
             bf16x8 a[3][3], z[3][NT];
@@ -179,26 +177,27 @@ __global__ void __launch_bounds__(64 * kSplitWaves)
             for (int t = 0; t < 3; t++)
 #pragma unroll
                 for (int n = 0; n < NT; n++) z[t][n] = Z[((t * 3 + kt) * NT + n) * 64 + lane];
+            // The leading product a1 z1 and the five corrections (2^-8 smaller and below) are
+            // summed in separate FP32 registers and both moved to FP64 after every 32-wide
+            // k-tile: each FP32 rounding then acts on one 32-term dot product only.
 #pragma unroll
             for (int m = 0; m < 3; m++)
 #pragma unroll
                 for (int n = 0; n < NT; n++) {
-                    f32x4 d = c32[m][n];
-                    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][m], z[0][n], d, 0, 0, 0);
-                    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][m], z[1][n], d, 0, 0, 0);
-                    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][m], z[2][n], d, 0, 0, 0);
-                    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][m], z[0][n], d, 0, 0, 0);
-                    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][m], z[1][n], d, 0, 0, 0);
-                    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][m], z[0][n], d, 0, 0, 0);
-                    c32[m][n] = d;
+                    const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+                    const f32x4 dm =
+                        __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][m], z[0][n], zero, 0, 0, 0);
+                    f32x4 dc =
+                        __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][m], z[0][n], zero, 0, 0, 0);
+                    dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][m], z[1][n], dc, 0, 0, 0);
+                    dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][m], z[2][n], dc, 0, 0, 0);
+                    dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][m], z[0][n], dc, 0, 0, 0);
+                    dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][m], z[1][n], dc, 0, 0, 0);
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        acc[m][n][v] += (double)dm[v] + (double)dc[v];
                 }
         }
-#pragma unroll
-        for (int m = 0; m < 3; m++)
-#pragma unroll
-            for (int n = 0; n < NT; n++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) acc[m][n][v] += (double)c32[m][n][v];
     }
     double* out = partial + (((int64_t)I * chunks + sp * kSplitWaves + w) * 3 * NT) * 256;
 #pragma unroll
