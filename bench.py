@@ -10,8 +10,13 @@ element size (dt = 0.04949, MaxStep 65), 600 steps, alpha = (1/4, 1/4).
 One bench "step" is one complete solve: lag-store assembly of all 66 lags, the step-0
 factorisations, and the 599 marching solves (src/main.cpp:89-128).  The mesh is resident on the
 GPU before the timed region.  value = time-steps solved per second (summed over right-hand sides)
-of the mixed form; plain_fp64 gives the same for the plain path and the ratio.  The mixed error is
-max over 8 right-hand sides of ||u_mixed - u_fp64||_F / ||u_fp64||_F (and the same for t).
+of the mixed form; plain_fp64 gives the same for the plain path (one load, as the reference runs)
+and fp64_8rhs for the FP64 path at 8 right-hand sides, its best rate.  The mixed error is max over
+8 right-hand sides of ||u_mixed - u_fp64||_F / ||u_fp64||_F (and the same for t).
+
+roofline: the mixed path's dominant kernel k_conv_mixed, HBM-bound: algorithmic bytes per launch =
+stored 48 x 96 split-BF16 blocks x (27 648 + 4) B, over its mean duration from HIP events on the
+context stream; plain_fp64.roofline gives k_conv the same way (stored entries x 124 B).
 Printed as one JSON line; --gpus N > 1 exits (one GPU per box in this build).
 """
 import argparse
@@ -28,6 +33,10 @@ import numpy as np  # noqa: E402
 MI355X_HBM_GBS = 8000.0
 TP_BLOCK_BYTES = 3 * 48 * 96 * 2 + 4     # split-BF16 48x96 block + its index
 FP64_ENTRY_BYTES = 15 * 8 + 4            # 15 values + index per (i, j, k)
+# HBM bytes per launch (read + written) from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE)
+# on this workload, scaled by the counters of a 1 GiB k_stream_copy (factors 2.0 and 1.0);
+# profiles/k_conv_pmc_sphere1536.md holds the runs.
+MEASURED_TRAFFIC = {"k_conv_mixed": 17.626e9 + 100.1e6, "k_conv": 7.683e9 + 0.1e6}
 
 
 def parse():
@@ -39,6 +48,9 @@ def parse():
     ap.add_argument("--nrhs", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 1 GiB device copy (3 launches) to scale HBM counters by")
     ap.add_argument("--cpu-threads", type=int, default=16)
     return ap.parse_args()
 
@@ -107,6 +119,22 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def copy_calibration(api):
+    """k_stream_copy over 1 GiB, three launches: a plain streaming kernel of known size (1 GiB
+    read + 1 GiB written per launch) whose FETCH_SIZE / WRITE_SIZE counters scale the counters of
+    the kernels above to bytes."""
+    ms = api.stream_copy(1 << 30, 3)
+    return {"kernel": "k_stream_copy", "bytes_read": 1 << 30, "bytes_written": 1 << 30,
+            "launches": 3, "avg_ms": round(ms, 4), "GBs": round(2 * (1 << 30) / ms / 1e6, 1)}
+
+
+def roofline(kernel, avg_ms, nbytes, traffic=None):
+    ach = nbytes / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": MI355X_HBM_GBS, "unit": "GB/s",
+            "frac": round(ach / MI355X_HBM_GBS, 4), "traffic": traffic, "kernel": kernel,
+            "avg_ms": round(avg_ms, 4), "bytes_per_launch": int(nbytes)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -117,48 +145,55 @@ def main():
     p = configs.sphere(n=16, numstep=args.numstep)
     R = args.nrhs
     amp, rise = configs.rhs_set(R, p.dt, seed=3)
+    K = p.max_step()
     out = {"metric": "time-steps/s mixed vs plain FP64 GPU at <=1e-6 err",
            "unit": "time-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "bf16x3-mfma/f64", "data": "synthetic"}
+           "dtype": "bf16x3-mfma/f64", "data": "synthetic",
+           "config": {"workload": f"sphere cavity 6x16x16 (1536 quads, 4614 dof), {p.numstep} "
+                                  f"steps, nu=0.25, beta=0.6; mixed: {R} pressure histories per "
+                                  f"solve", "numstep": p.numstep, "max_step": K, "dof": p.ndof,
+                      "nrhs": R}}
 
-    # mixed precision, R right-hand sides
-    log(f"mixed leg: {R} right-hand sides, {p.numstep} steps")
-    with api.Context.from_problem(p) as ctx:
-        ctx.set_profiling(True)
-        ctx.set_precision(api.PRECISION_MIXED)
-        wall, tm = timed_solves(ctx, p, args.warmup, args.steps, amp, rise)
-        mixed_hist = [ctx.history(r) for r in range(8)]
-        K = ctx.max_step
-    mixed_steps = args.steps * (p.numstep - 1) * R
-    conv_ms = tm["conv_ms"] / max(tm["conv_launches"], 1)
-    nblocks_bytes = None
-    out.update({
-        "value": round(mixed_steps / wall, 2),
-        "ms_per_step": round(wall / args.steps * 1e3, 3),
-        "config": {"workload": f"sphere cavity 6x16x16 (1536 quads, 4614 dof), {p.numstep} steps, "
-                               f"nu=0.25, beta=0.6; mixed: {R} pressure histories per solve",
-                   "numstep": p.numstep, "max_step": K, "dof": p.ndof, "nrhs": R},
-        "mixed": {"phases_ms_per_solve": {k: round(tm[k] / args.steps, 3)
-                                          for k in ("store_ms", "init_ms", "march_ms")},
-                  "conv_avg_ms": round(conv_ms, 4),
-                  "solve_avg_ms": round(tm["solve_ms"] / max(tm["conv_launches"], 1), 4)},
-    })
+    if not args.no_mixed:
+        log(f"mixed leg: {R} right-hand sides, {p.numstep} steps")
+        with api.Context.from_problem(p) as ctx:
+            ctx.set_profiling(True)
+            ctx.set_precision(api.PRECISION_MIXED)
+            wall, tm = timed_solves(ctx, p, args.warmup, args.steps, amp, rise)
+            mixed_hist = [ctx.history(r) for r in range(8)]
+        nconv = max(tm["conv_launches"], 1)
+        conv_ms = tm["conv_ms"] / nconv
+        # per launch: every stored 48 x 96 split-BF16 block (3 x 9216 B) and its 4 B index
+        mixed_bytes = tm["tile_blocks"] * TP_BLOCK_BYTES
+        out.update({
+            "value": round(args.steps * (p.numstep - 1) * R / wall, 2),
+            "ms_per_step": round(wall / args.steps * 1e3, 3),
+            "mixed": {"phases_ms_per_solve": {k: round(tm[k] / args.steps, 3)
+                                              for k in ("store_ms", "init_ms", "march_ms")},
+                      "tile_blocks": int(tm["tile_blocks"]),
+                      "tile_store_bytes": int(tm["tile_blocks"] * (TP_BLOCK_BYTES - 4)),
+                      "conv_avg_ms": round(conv_ms, 4),
+                      "reduce_avg_ms": round(tm["reduce_ms"] / nconv, 4),
+                      "solve_avg_ms": round(tm["solve_ms"] / nconv, 4)},
+        })
+        out["roofline"] = roofline("k_conv_mixed (split-BF16 MFMA history convolution)", conv_ms,
+                                   mixed_bytes, MEASURED_TRAFFIC["k_conv_mixed"])
 
-    # FP64 reference for the error on the first 8 right-hand sides
-    log("FP64 reference for the mixed error")
-    with api.Context.from_problem(p) as ctx:
-        ctx.build_lag_store()
-        ctx.march_init(p.a1, p.a2, p.ubc_dof, p.ubc_val, p.fbc_dof, p.fbc_val, amp=amp[:8],
-                       t_rise=rise[:8])
-        ctx.march_run()
-        ref = [ctx.history(r) for r in range(8)]
-    err = max(max(np.linalg.norm(m[0] - r[0]) / np.linalg.norm(r[0]),
-                  np.linalg.norm(m[1] - r[1]) / np.linalg.norm(r[1]))
-              for m, r in zip(mixed_hist, ref))
-    out["mixed"]["max_rel_err_vs_fp64"] = float(err)
+        # FP64 at 8 right-hand sides: reference for the error, and the FP64 path's best rate
+        log("FP64 8-RHS leg (error reference)")
+        with api.Context.from_problem(p) as ctx:
+            ctx.set_profiling(True)
+            wall8, tm8 = timed_solves(ctx, p, args.warmup, args.steps, amp[:8], rise[:8])
+            ref = [ctx.history(r) for r in range(8)]
+        err = max(max(np.linalg.norm(m[0] - r[0]) / np.linalg.norm(r[0]),
+                      np.linalg.norm(m[1] - r[1]) / np.linalg.norm(r[1]))
+                  for m, r in zip(mixed_hist, ref))
+        out["mixed"]["max_rel_err_vs_fp64"] = float(err)
+        out["fp64_8rhs"] = {"value": round(args.steps * (p.numstep - 1) * 8 / wall8, 2),
+                            "ms_per_step": round(wall8 / args.steps * 1e3, 3),
+                            "conv_avg_ms": round(tm8["conv_ms"] / max(tm8["conv_launches"], 1), 4)}
 
-    # plain FP64, one right-hand side
     if not args.no_fp64:
         log("plain FP64 leg")
         with api.Context.from_problem(p) as ctx:
@@ -166,28 +201,36 @@ def main():
             wall64, tm64 = timed_solves(ctx, p, args.warmup, args.steps)
             U, _ = ctx.history()
             store_bytes = ctx.lag_store_bytes()
-        steps64 = args.steps * (p.numstep - 1)
         conv64 = tm64["conv_ms"] / max(tm64["conv_launches"], 1)
+        # per launch: every stored (i, j, k) entry (15 FP64 values and a 4 B index)
         bytes64 = tm64["store_nonzero"] * FP64_ENTRY_BYTES
-        ach64 = bytes64 / (conv64 * 1e-3) / 1e9
         out["plain_fp64"] = {
-            "value": round(steps64 / wall64, 2), "ms_per_step": round(wall64 / args.steps * 1e3, 3),
+            "value": round(args.steps * (p.numstep - 1) / wall64, 2),
+            "ms_per_step": round(wall64 / args.steps * 1e3, 3),
             "phases_ms_per_solve": {k: round(tm64[k] / args.steps, 3)
                                     for k in ("store_ms", "init_ms", "march_ms")},
             "lag_kernel_ms": round(tm64["lag_kernel_ms"] / args.steps, 3),
             "active_points_per_solve": int(tm64["active_points"] / args.steps),
             "store": {"entries": int(tm64["store_nonzero"]), "bytes": store_bytes,
                       "dense_bytes": 2 * (K + 1) * p.ndof ** 2 * 8},
-            "conv": {"kernel": "k_conv", "avg_ms": round(conv64, 4),
-                     "bytes_per_launch": int(bytes64), "achieved_GBs": round(ach64, 1)},
+            "roofline": roofline("k_conv (plain FP64 history convolution)", conv64, bytes64,
+                                 MEASURED_TRAFFIC["k_conv"]),
             # Lame static radial displacement p a / (4 mu) = 0.625 at late time
-            "u_r_mean_final": float(np.mean(np.einsum("ij,ij->i", U[-1].reshape(-1, 3), p.coords))),
+            "u_r_mean_final": float(np.mean(np.einsum("ij,ij->i", U[-1].reshape(-1, 3),
+                                                      p.coords))),
         }
-        out["speedup_mixed_vs_plain_fp64"] = round(out["value"] / out["plain_fp64"]["value"], 2)
-        out["roofline"] = {"bound": "hbm", "achieved": round(ach64, 1), "peak": MI355X_HBM_GBS,
-                           "unit": "GB/s", "frac": round(ach64 / MI355X_HBM_GBS, 4),
-                           "traffic": None, "kernel": "k_conv (plain FP64 history convolution)",
-                           "avg_ms": round(conv64, 4), "bytes_per_launch": int(bytes64)}
+        if "value" in out:
+            out["speedup_mixed_vs_plain_fp64"] = round(out["value"] / out["plain_fp64"]["value"], 2)
+            if "fp64_8rhs" in out:
+                out["speedup_mixed_vs_fp64_8rhs"] = round(out["value"] / out["fp64_8rhs"]["value"], 2)
+        else:
+            out["value"] = out["plain_fp64"]["value"]
+            out["ms_per_step"] = out["plain_fp64"]["ms_per_step"]
+            out["dtype"] = "f64"
+            out["roofline"] = out["plain_fp64"]["roofline"]
+    if args.copy_calibration:
+        log("copy calibration kernel")
+        out["copy_calibration"] = copy_calibration(api)
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle sample)")
         out["cpu_baseline"] = cpu_baseline(p, args.cpu_threads)

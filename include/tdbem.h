@@ -109,6 +109,12 @@ int tdbem_set_precision(tdbem_context* ctx, int precision);
  * rounded to bf16) through the operand lane layouts the mixed convolution assumes. */
 int tdbem_mfma_selftest(const float* A, const float* B, float* D);
 
+/* Diagnostic: a plain streaming copy (k_stream_copy) of `bytes` bytes (a multiple of 16) between
+ * two device buffers, `launches` times; *avg_ms = mean launch time from HIP events.  It reads and
+ * writes exactly `bytes` per launch, so hardware byte counters of other kernels can be scaled by
+ * what they read for it.  No counterpart in the reference. */
+int tdbem_stream_copy(size_t bytes, int launches, double* avg_ms);
+
 /* Waits for all work queued on the context's stream. */
 int tdbem_synchronize(tdbem_context* ctx);
 
@@ -118,8 +124,10 @@ int tdbem_synchronize(tdbem_context* ctx);
  * store_ms, init_ms, march_ms (host wall clock of build_lag_store / march_init / march_run),
  * lag_kernel_ms, gather_ms, conv_ms, solve_ms (summed event times), lag_launches,
  * conv_launches, active_points (quadrature-point x lag evaluations done by the lag kernel),
- * store_nonzero (stored (i, j, k) blocks before padding), store_padded. */
-#define TDBEM_NUM_TIMINGS 12
+ * store_nonzero (stored (i, j, k) blocks before padding), store_padded, tile_blocks (48 x 96
+ * blocks of the mixed-precision store, 0 until built), reduce_ms (the mixed path's reduction of
+ * per-wave partial sums, summed event times).  conv_ms covers k_conv or k_conv_mixed alone. */
+#define TDBEM_NUM_TIMINGS 14
 int tdbem_set_profiling(tdbem_context* ctx, int on);
 int tdbem_reset_timings(tdbem_context* ctx);
 int tdbem_get_timings(tdbem_context* ctx, double* out);

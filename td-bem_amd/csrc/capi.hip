@@ -127,7 +127,42 @@ int guarded(tdbem_context* ctx, F&& f)
     }
 }
 
+// 16 B per thread per iteration, grid-stride: reads and writes n16 x 16 bytes once each.
+__global__ void __launch_bounds__(256)
+    k_stream_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n16)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+        dst[i] = src[i];
+}
+
 }  // namespace
+
+double stream_copy(size_t bytes, int launches)
+{
+    hipStream_t s;
+    TD_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    double ms = 0;
+    {
+        DBuf<float4> a, b;
+        const int64_t n16 = (int64_t)(bytes / 16);
+        a.alloc((size_t)n16);
+        b.alloc((size_t)n16);
+        TD_HIP(hipMemsetAsync(a.p, 0, a.bytes(), s));
+        KernelTimer t(s, true);
+        for (int l = 0; l < launches; l++) {
+            t.begin();
+            hipLaunchKernelGGL(k_stream_copy, dim3(8192), dim3(256), 0, s, a.p, b.p, n16);
+            t.end();
+        }
+        TD_HIP(hipGetLastError());
+        ms = t.collect() / launches;
+        TD_HIP(hipStreamSynchronize(s));
+    }
+    (void)hipStreamDestroy(s);
+    return ms;
+}
+
 }  // namespace tdbem
 
 using namespace tdbem;
@@ -383,6 +418,18 @@ int tdbem_mfma_selftest(const float* A, const float* B, float* D)
     }
 }
 
+int tdbem_stream_copy(size_t bytes, int launches, double* avg_ms)
+{
+    if (bytes == 0 || bytes % 16 || launches < 1 || !avg_ms) return TDBEM_ERR_ARG;
+    try {
+        *avg_ms = stream_copy(bytes, launches);
+        return TDBEM_OK;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        return TDBEM_ERR_HIP;
+    }
+}
+
 int tdbem_set_profiling(tdbem_context* ctx, int on)
 {
     return guarded(ctx, [&](Context& c) { c.profile = on != 0; });
@@ -401,7 +448,8 @@ int tdbem_get_timings(tdbem_context* ctx, double* out)
         const double v[TDBEM_NUM_TIMINGS] = {
             t.store_ms, t.init_ms, t.march_ms, t.lag_kernel_ms, t.gather_ms, t.conv_ms,
             t.solve_ms, (double)t.lag_launches, (double)t.conv_launches, (double)t.active_points,
-            (double)c.store.nonzero_entries, (double)c.store.n_entries};
+            (double)c.store.nonzero_entries, (double)c.store.n_entries, (double)c.tp.nblocks,
+            t.reduce_ms};
         std::memcpy(out, v, sizeof(v));
     });
 }

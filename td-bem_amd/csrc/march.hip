@@ -366,7 +366,8 @@ void march_run(Context& c, int nsteps, bool wait)
     const int64_t tot = (int64_t)N * M.nrhs;
     const auto t0 = std::chrono::steady_clock::now();
     // Without a wait the steps are only queued; per-kernel timing needs the wait.
-    KernelTimer conv_timer(c.stream, c.profile && wait), solve_timer(c.stream, c.profile && wait);
+    KernelTimer conv_timer(c.stream, c.profile && wait), solve_timer(c.stream, c.profile && wait),
+        reduce_timer(c.stream, c.profile && wait && M.precision == 1);
     for (int it = 0; it < nsteps; it++) {
         if (M.s + 1 >= c.numstep) break;  // histories hold x_0 .. x_{numstep-1}
         const int s = ++M.s;
@@ -384,8 +385,11 @@ void march_run(Context& c, int nsteps, bool wait)
                     tp_zsplit(c, M.ring.p, N, M.nrhs, sl, M.zs.p);
                 }
                 conv_timer.begin();
-                tp_conv(c, M.zs.p, M.nrhs, M.R, s, M.partial.p, M.hist.p, N);
+                tp_conv(c, M.zs.p, M.nrhs, M.R, s, M.partial.p);
                 conv_timer.end();
+                reduce_timer.begin();
+                tp_reduce(c, M.partial.p, M.nrhs, M.hist.p, N);
+                reduce_timer.end();
             } else {
                 conv_timer.begin();
                 switch (M.nrhs) {
@@ -416,6 +420,7 @@ void march_run(Context& c, int nsteps, bool wait)
         c.tm.conv_launches += conv_timer.count();
         c.tm.conv_ms += conv_timer.collect();
         c.tm.solve_ms += solve_timer.collect();
+        c.tm.reduce_ms += reduce_timer.collect();
     }
 }
 

@@ -1,6 +1,6 @@
 // mixed.hip — the mixed-precision history convolution: split-BF16 (three terms) MFMA products
-// accumulated in FP32 inside a 32-wide k-tile and in FP64 across tiles, blocks and lags, for R right-hand
-// sides at once (R = 16, 32, 48 or 64).
+// accumulated in FP32 inside a 32-wide k-tile and in FP64 across tiles, blocks and lags, for R
+// right-hand sides at once (R = 16, 32, 48 or 64).
 //
 // Layout.  Nodes are reordered along a Morton curve and grouped into tiles of 16 (48 dofs).  For
 // each pair of tiles (I, J) the lags k in the union of their node-pair shells are stored as dense
@@ -361,8 +361,7 @@ int tp_nsplit(const Context& c)
     return std::max(1, std::min(64, 2048 / std::max(1, c.tp.nT)));
 }
 
-void tp_conv(Context& c, const unsigned short* zs, int nrhs, int Rring, int s, double* partial,
-             double* hist, int N)
+void tp_conv(Context& c, const unsigned short* zs, int nrhs, int Rring, int s, double* partial)
 {
     const TPStore& T = c.tp;
     const int nsplit = tp_nsplit(c);
@@ -378,7 +377,13 @@ void tp_conv(Context& c, const unsigned short* zs, int nrhs, int Rring, int s, d
     default: throw std::runtime_error("mixed march needs nrhs in {16, 32, 48, 64}");
     }
     TD_HIP(hipGetLastError());
-    const int chunks = nsplit * kSplitWaves;
+}
+
+void tp_reduce(Context& c, const double* partial, int nrhs, double* hist, int N)
+{
+    const TPStore& T = c.tp;
+    const int chunks = tp_nsplit(c) * kSplitWaves;
+    const int NT = nrhs / 16;
     const int64_t n = (int64_t)T.nT * kRows * nrhs;
     hipLaunchKernelGGL(k_tp_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream,
                        partial, T.nT, chunks, NT, T.d_perm.p, N, nrhs, hist);

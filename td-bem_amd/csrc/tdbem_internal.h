@@ -154,7 +154,7 @@ struct Context {
     bool profile = false;
     struct Timings {
         double store_ms = 0, init_ms = 0, march_ms = 0;  // host wall clock, synchronised
-        double lag_kernel_ms = 0, gather_ms = 0, conv_ms = 0, solve_ms = 0;
+        double lag_kernel_ms = 0, gather_ms = 0, conv_ms = 0, solve_ms = 0, reduce_ms = 0;
         int64_t lag_launches = 0, conv_launches = 0;
         int64_t active_points = 0;  // (quadrature point, lag) evaluations of the lag kernel
     } tm;
@@ -188,9 +188,12 @@ void tp_build(Context& c);
 void tp_zsplit(Context& c, const double* ring, int N, int nrhs, int slot, unsigned short* zs);
 int64_t tp_zslot_elems(const Context& c, int nrhs);
 int64_t tp_partial_elems(const Context& c, int nrhs);
-void tp_conv(Context& c, const unsigned short* zs, int nrhs, int Rring, int s, double* partial,
-             double* hist, int N);
+// k_conv_mixed: per-wave FP64 partial sums of the history at step s
+void tp_conv(Context& c, const unsigned short* zs, int nrhs, int Rring, int s, double* partial);
+// fixed-order sum of the partials into hist[nrhs][N]
+void tp_reduce(Context& c, const double* partial, int nrhs, double* hist, int N);
 void mfma_selftest(const float* A, const float* B, float* D);
+double stream_copy(size_t bytes, int launches);
 
 // lu.hip: in-place LU with partial pivoting of a dense row-major n x n device matrix, and
 // solves with it.  piv is a device int array of length n.

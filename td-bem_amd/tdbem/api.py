@@ -59,6 +59,7 @@ def lib():
         L.tdbem_query.argtypes = [vp]
         L.tdbem_set_precision.argtypes = [vp, C.c_int]
         L.tdbem_mfma_selftest.argtypes = [C.POINTER(C.c_float)] * 3
+        L.tdbem_stream_copy.argtypes = [C.c_size_t, C.c_int, dp]
         L.tdbem_get_state.argtypes = [vp, C.c_int, C.c_int, dp, dp]
         L.tdbem_set_profiling.argtypes = [vp, C.c_int]
         L.tdbem_reset_timings.argtypes = [vp]
@@ -202,7 +203,7 @@ class Context:
 
     TIMING_KEYS = ("store_ms", "init_ms", "march_ms", "lag_kernel_ms", "gather_ms", "conv_ms",
                    "solve_ms", "lag_launches", "conv_launches", "active_points", "store_nonzero",
-                   "store_padded")
+                   "store_padded", "tile_blocks", "reduce_ms")
 
     def set_profiling(self, on=True):
         self._check(lib().tdbem_set_profiling(self._h, int(bool(on))))
@@ -240,3 +241,13 @@ def mfma_selftest(A, B):
     if rc:
         raise TdbemError(rc, lib().tdbem_last_error(None).decode())
     return D
+
+
+def stream_copy(nbytes, launches=3):
+    """Plain device copy of nbytes (k_stream_copy), `launches` times; returns the mean launch time
+    in ms.  Its byte counters calibrate those of the other kernels."""
+    ms = C.c_double(0.0)
+    rc = lib().tdbem_stream_copy(C.c_size_t(nbytes), C.c_int(launches), C.byref(ms))
+    if rc:
+        raise TdbemError(rc, lib().tdbem_last_error(None).decode())
+    return ms.value
