@@ -109,6 +109,10 @@ int tdbem_set_precision(tdbem_context* ctx, int precision);
  * rounded to bf16) through the operand lane layouts the mixed convolution assumes. */
 int tdbem_mfma_selftest(const float* A, const float* B, float* D);
 
+/* Diagnostic: one v_mfma_f64_16x16x4 product D[16][16] = A[16][4] B[4][16] through the operand
+ * lane layouts of the step solve. */
+int tdbem_mfma_f64_selftest(const double* A, const double* B, double* D);
+
 /* Diagnostic: a plain streaming copy (k_stream_copy) of `bytes` bytes (a multiple of 16) between
  * two device buffers, `launches` times; *avg_ms = mean launch time from HIP events.  It reads and
  * writes exactly `bytes` per launch, so hardware byte counters of other kernels can be scaled by

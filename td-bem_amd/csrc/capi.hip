@@ -418,6 +418,18 @@ int tdbem_mfma_selftest(const float* A, const float* B, float* D)
     }
 }
 
+int tdbem_mfma_f64_selftest(const double* A, const double* B, double* D)
+{
+    if (!A || !B || !D) return TDBEM_ERR_ARG;
+    try {
+        mfma_f64_selftest(A, B, D);
+        return TDBEM_OK;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        return TDBEM_ERR_HIP;
+    }
+}
+
 int tdbem_stream_copy(size_t bytes, int launches, double* avg_ms)
 {
     if (bytes == 0 || bytes % 16 || launches < 1 || !avg_ms) return TDBEM_ERR_ARG;

@@ -477,4 +477,129 @@ void gemv(const double* A, int n, const double* x, double* y, double beta, hipSt
     TD_HIP(hipGetLastError());
 }
 
+namespace {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// Row-major n x n -> fragment order (zeros in the padding).  One thread per fragment element.
+__global__ void k_to_frag(const double* __restrict__ A, int n, double* __restrict__ Af,
+                          int64_t total)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int e = (int)(t & 7), lane = (int)((t >> 3) & 63);
+    const int64_t fc = t >> 9;
+    const int C = frag_cols(n);
+    const int c = (int)(fc % C), T = (int)(fc / C);
+    const int row = 16 * T + (lane & 15), col = 32 * c + 8 * (lane >> 4) + e;
+    Af[t] = (row < n && col < n) ? A[(int64_t)row * n + col] : 0.0;
+}
+
+constexpr int kApplyWaves = 4;
+
+// parts[p][r][i] = sum_{j in split p} A[i][j] X[r][j].  One wavefront per (16-row tile, column
+// split); per 32-column fragment it loads 4 KiB of A (64 B contiguous per lane) and NT fragments
+// of X, and runs 8 v_mfma_f64_16x16x4 per rhs tile (k index = the 8 columns a lane holds).
+template <int NT>
+__global__ void __launch_bounds__(64 * kApplyWaves)
+    k_apply_frag(const double* __restrict__ Af, int n, const double* __restrict__ Xf, int nrhs,
+                 int nsplit, int cper, double* __restrict__ parts)
+{
+    const int lane = threadIdx.x & 63;
+    const int g = blockIdx.x * kApplyWaves + (threadIdx.x >> 6);
+    const int nTr = frag_rows(n) / 16, C = frag_cols(n);
+    if (g >= nTr * nsplit) return;
+    const int T = g % nTr, p = g / nTr;
+    const int c0 = p * cper, c1 = min(C, c0 + cper);
+    f64x4 acc[NT];
+#pragma unroll
+    for (int q = 0; q < NT; q++) acc[q] = f64x4{0.0, 0.0, 0.0, 0.0};
+    const double* a = Af + ((int64_t)T * C) * 512 + lane * 8;
+    for (int c = c0; c < c1; c++) {
+        double av[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) av[e] = a[(int64_t)c * 512 + e];
+#pragma unroll
+        for (int q = 0; q < NT; q++) {
+            const double* x = Xf + ((int64_t)q * C + c) * 512 + lane * 8;
+            double xv[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) xv[e] = x[e];
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[e], xv[e], acc[q], 0, 0, 0);
+        }
+    }
+    // D fragment of the f64 16x16x4 product: lane = rhs + 16 * (row % 4), element row / 4
+    const int Np = frag_rows(n);
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+        const int r = 16 * q + (lane & 15);
+        if (r >= nrhs) continue;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = 16 * T + (lane >> 4) + 4 * v;
+            parts[((int64_t)p * nrhs + r) * Np + i] = acc[q][v];
+        }
+    }
+}
+
+// One v_mfma_f64_16x16x4 with the lane layouts k_apply_frag assumes, for the layout self-test.
+__global__ void k_mfma_f64_selftest(const double* __restrict__ A, const double* __restrict__ B,
+                                    double* __restrict__ D)
+{
+    const int lane = threadIdx.x;
+    f64x4 d = {0.0, 0.0, 0.0, 0.0};
+    d = __builtin_amdgcn_mfma_f64_16x16x4f64(A[(lane & 15) * 4 + (lane >> 4)],
+                                             B[(lane >> 4) * 16 + (lane & 15)], d, 0, 0, 0);
+    for (int v = 0; v < 4; v++) D[((lane >> 4) + 4 * v) * 16 + (lane & 15)] = d[v];
+}
+
+}  // namespace
+
+int64_t frag_elems(int n, int rows) { return (int64_t)frag_rows(rows) * frag_cols(n) * 32; }
+
+void to_frag(const double* A, int n, double* Af, hipStream_t s)
+{
+    const int64_t total = frag_elems(n, n);
+    hipLaunchKernelGGL(k_to_frag, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, A, n,
+                       Af, total);
+    TD_HIP(hipGetLastError());
+}
+
+int apply_splits(int n)
+{
+    // about 4096 wavefronts, at least 4 fragments (128 columns) per split
+    const int nTr = frag_rows(n) / 16, C = frag_cols(n);
+    return std::max(1, std::min((4096 + nTr - 1) / nTr, C / 4));
+}
+
+void apply_frag(const double* Af, int n, const double* Xf, int nrhs, double* parts,
+                hipStream_t s)
+{
+    const int nsplit = apply_splits(n), C = frag_cols(n);
+    const int cper = (C + nsplit - 1) / nsplit;
+    const int waves = frag_rows(n) / 16 * nsplit;
+    const dim3 grid((waves + kApplyWaves - 1) / kApplyWaves), block(64 * kApplyWaves);
+    switch ((nrhs + 15) / 16) {
+    case 1: hipLaunchKernelGGL(k_apply_frag<1>, grid, block, 0, s, Af, n, Xf, nrhs, nsplit, cper, parts); break;
+    case 2: hipLaunchKernelGGL(k_apply_frag<2>, grid, block, 0, s, Af, n, Xf, nrhs, nsplit, cper, parts); break;
+    case 3: hipLaunchKernelGGL(k_apply_frag<3>, grid, block, 0, s, Af, n, Xf, nrhs, nsplit, cper, parts); break;
+    case 4: hipLaunchKernelGGL(k_apply_frag<4>, grid, block, 0, s, Af, n, Xf, nrhs, nsplit, cper, parts); break;
+    default: throw std::invalid_argument("step solve: at most 64 right-hand sides");
+    }
+    TD_HIP(hipGetLastError());
+}
+
+void mfma_f64_selftest(const double* hA, const double* hB, double* hD)
+{
+    DBuf<double> A, B, D;
+    A.upload(hA, 16 * 4, nullptr);
+    B.upload(hB, 4 * 16, nullptr);
+    D.alloc(16 * 16);
+    hipLaunchKernelGGL(k_mfma_f64_selftest, dim3(1), dim3(64), 0, nullptr, A.p, B.p, D.p);
+    TD_HIP(hipGetLastError());
+    TD_HIP(hipMemcpy(hD, D.p, 16 * 16 * sizeof(double), hipMemcpyDeviceToHost));
+}
+
 }  // namespace tdbem

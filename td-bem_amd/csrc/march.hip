@@ -27,11 +27,14 @@ struct March {
     int N = 0, nrhs = 1, R = 0, s = 0;
     double a1 = 0, a2 = 0, a3 = 0;
     std::vector<double> amp, t_rise;
-    DBuf<double> Ainv;      // [N][N] inverse of the n >= 3 system matrix
+    DBuf<double> Af;        // inverse of the n >= 3 system matrix, FP64-MFMA fragment order
+    DBuf<double> Xf;        // right-hand sides of the step solve, same order (16 per tile row)
+    DBuf<double> parts;     // [nsplit][nrhs][Np] column-split partial solutions
+    int nsplit = 1, Np = 0;
     DBuf<double> gF, gU;    // G~ y split into the FBC part (scaled by f) and the UBC part
     DBuf<double> wF, wU;    // A_2^{-1} (G~_2 y_F), A_2^{-1} (G~_2 y_U) for the first solve
     DBuf<double> ring;      // [R][nrhs][2][N], part 0 = u, part 1 = t
-    DBuf<double> hist, rv, xv;  // [nrhs][N]
+    DBuf<double> hist;      // [nrhs][N]
     DBuf<double> U, T;      // [numstep][nrhs][N]
     DBuf<int> is_ubc, is_fbc;
     DBuf<double> ubc_val, fbc_val;   // dense per dof
@@ -154,29 +157,33 @@ __global__ void __launch_bounds__(kConvThreads)
     }
 }
 
-// x_1 = f_r(dt) wF + wU  (the n == 2 solve, src/SolverStepLinearT.cpp:355-378)
+// x_1 = f_r(dt) wF + wU  (the n == 2 solve, src/SolverStepLinearT.cpp:355-378), written as
+// split 0 of the step-solve partials x[r][Np]
 __global__ void k_first(const double* __restrict__ wF, const double* __restrict__ wU,
-                        const double* __restrict__ fac, int N, int nrhs, double* __restrict__ x)
+                        const double* __restrict__ fac, int N, int Np, int nrhs,
+                        double* __restrict__ x)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)N * nrhs) return;
     const int r = (int)(t / N), d = (int)(t % N);
-    x[t] = fac[r] * wF[d] + wU[d];
+    x[(int64_t)r * Np + d] = fac[r] * wF[d] + wU[d];
 }
 
-// r = f_r gF + gU + hist
+// r = f_r gF + gU + hist, written in the fragment order of the step solve
 __global__ void k_rhs(const double* __restrict__ gF, const double* __restrict__ gU,
                       const double* __restrict__ hist, const double* __restrict__ fac, int N,
-                      int nrhs, double* __restrict__ rv)
+                      int nrhs, double* __restrict__ Xf)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)N * nrhs) return;
     const int r = (int)(t / N), d = (int)(t % N);
-    rv[t] = fac[r] * gF[d] + gU[d] + hist[t];
+    Xf[frag_index(N, r, d)] = fac[r] * gF[d] + gU[d] + hist[t];
 }
 
-// Boundary post-processing (src/SolverStepLinearT.cpp:512-536) and the ring update.
-__global__ void k_finalize(const double* __restrict__ x, const int* __restrict__ is_ubc,
+// The step solution x = sum of the column-split partials (fixed order), the boundary
+// post-processing (src/SolverStepLinearT.cpp:512-536) and the ring update.
+__global__ void k_finalize(const double* __restrict__ parts, int nparts, int Np,
+                           const int* __restrict__ is_ubc,
                            const int* __restrict__ is_fbc, const double* __restrict__ ubc_val,
                            const double* __restrict__ fbc_val, const double* __restrict__ fac,
                            int N, int nrhs, int s, int R, double a1, double a2, double a3,
@@ -185,7 +192,8 @@ __global__ void k_finalize(const double* __restrict__ x, const int* __restrict__
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)N * nrhs) return;
     const int r = (int)(t / N), d = (int)(t % N);
-    const double xv = x[t];
+    double xv = 0.0;
+    for (int p = 0; p < nparts; p++) xv += parts[((int64_t)p * nrhs + r) * Np + d];
     const double u = is_ubc[d] ? ubc_val[d] : xv;
     double tr = is_fbc[d] ? fbc_val[d] * fac[r] : 0.0;
     // fDofHigh is inclusive (src/SolverT.cpp:179) and the test is u_dof < fDofHigh (:527)
@@ -326,8 +334,12 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
     gemv(Hc.p, N, dyU.p, M.gU.p, 0.0, c.stream);
     hipLaunchKernelGGL(k_negate, dim3(blocks(N)), dim3(256), 0, c.stream, M.gU.p, N);
     lu_factor(A.p, N, piv.p, c.stream);
-    M.Ainv.alloc(NN);
-    lu_inverse(A.p, N, piv.p, M.Ainv.p, c.stream);
+    {
+        // the inverse goes to fragment order; Gc holds it in row-major order meanwhile
+        lu_inverse(A.p, N, piv.p, Gc.p, c.stream);
+        M.Af.alloc((size_t)frag_elems(N, N));
+        to_frag(Gc.p, N, M.Af.p, c.stream);
+    }
 
     M.precision = c.precision;
     if (M.precision == 1) {
@@ -340,8 +352,11 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
     M.ring.alloc((size_t)M.R * nrhs * 2 * N);
     TD_HIP(hipMemsetAsync(M.ring.p, 0, M.ring.bytes(), c.stream));
     M.hist.alloc((size_t)nrhs * N);
-    M.rv.alloc((size_t)nrhs * N);
-    M.xv.alloc((size_t)nrhs * N);
+    M.Np = frag_rows(N);
+    M.nsplit = apply_splits(N);
+    M.Xf.alloc((size_t)frag_elems(N, nrhs));
+    TD_HIP(hipMemsetAsync(M.Xf.p, 0, M.Xf.bytes(), c.stream));   // padding stays zero
+    M.parts.alloc((size_t)M.nsplit * nrhs * M.Np);
     M.fac.alloc(nrhs);
     M.d_amp.upload(M.amp.data(), nrhs, c.stream);
     M.d_rise.upload(M.t_rise.data(), nrhs, c.stream);
@@ -375,7 +390,7 @@ void march_run(Context& c, int nsteps, bool wait)
                            s * c.dt, M.nrhs, M.fac.p);
         if (s == 1) {
             hipLaunchKernelGGL(k_first, dim3(blocks(tot)), dim3(256), 0, c.stream, M.wF.p, M.wU.p,
-                               M.fac.p, N, M.nrhs, M.xv.p);
+                               M.fac.p, N, M.Np, M.nrhs, M.parts.p);
         } else {
             if (M.precision == 1) {
                 // the previous finalize changed ring slots s-2, s-1 and s
@@ -402,12 +417,13 @@ void march_run(Context& c, int nsteps, bool wait)
                 conv_timer.end();
             }
             hipLaunchKernelGGL(k_rhs, dim3(blocks(tot)), dim3(256), 0, c.stream, M.gF.p, M.gU.p,
-                               M.hist.p, M.fac.p, N, M.nrhs, M.rv.p);
+                               M.hist.p, M.fac.p, N, M.nrhs, M.Xf.p);
             solve_timer.begin();
-            gemm_rhs(M.Ainv.p, N, M.rv.p, M.xv.p, M.nrhs, c.stream);
+            apply_frag(M.Af.p, N, M.Xf.p, M.nrhs, M.parts.p, c.stream);
             solve_timer.end();
         }
-        hipLaunchKernelGGL(k_finalize, dim3(blocks(tot)), dim3(256), 0, c.stream, M.xv.p,
+        hipLaunchKernelGGL(k_finalize, dim3(blocks(tot)), dim3(256), 0, c.stream, M.parts.p,
+                           s == 1 ? 1 : M.nsplit, M.Np,
                            M.is_ubc.p, M.is_fbc.p, M.ubc_val.p, M.fbc_val.p, M.fac.p, N, M.nrhs,
                            s, M.R, M.a1, M.a2, M.a3, M.U.p, M.T.p, M.ring.p);
         TD_HIP(hipGetLastError());

@@ -206,4 +206,25 @@ void gemv(const double* A, int n, const double* x, double* y, double beta, hipSt
 // Y[r][i] = sum_j A[i][j] X[r][j] for r < nrhs (A n x n row-major; X, Y right-hand-side-major)
 void gemm_rhs(const double* A, int n, const double* X, double* Y, int nrhs, hipStream_t s);
 
+// Step solve with the inverse in v_mfma_f64_16x16x4 fragment order (lu.hip).  A matrix of
+// n x n is held as 16-row x 32-column fragments: element (i, j) of fragment (T, c) sits at
+// ((T * frag_cols(n) + c) * 64 + lane) * 8 + e with i = lane % 16, j = 8 * (lane / 16) + e,
+// rows padded to frag_rows(n) and columns to 32 * frag_cols(n) with zeros.  Right-hand sides
+// use the same order with 16 of them per fragment row ("rhs tile").
+__host__ __device__ inline int frag_rows(int n) { return (n + 15) / 16 * 16; }
+__host__ __device__ inline int frag_cols(int n) { return (n + 31) / 32; }
+__host__ __device__ inline int64_t frag_index(int n, int row, int col)
+{
+    const int T = row >> 4, i = row & 15, c = col >> 5, kk = col & 31;
+    return (((int64_t)T * frag_cols(n) + c) * 64 + i + 16 * (kk >> 3)) * 8 + (kk & 7);
+}
+int64_t frag_elems(int n, int rows);  // doubles of a fragment array of `rows` rows
+void to_frag(const double* A, int n, double* Af, hipStream_t s);
+// column splits of the step solve; its partial sums are parts[split][nrhs][frag_rows(n)]
+int apply_splits(int n);
+// parts[p][r][i] = sum over the columns j of split p of A[i][j] X[r][j]
+void apply_frag(const double* Af, int n, const double* Xf, int nrhs, double* parts,
+                hipStream_t s);
+void mfma_f64_selftest(const double* A, const double* B, double* D);
+
 }  // namespace tdbem

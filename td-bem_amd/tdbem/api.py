@@ -60,6 +60,7 @@ def lib():
         L.tdbem_set_precision.argtypes = [vp, C.c_int]
         L.tdbem_mfma_selftest.argtypes = [C.POINTER(C.c_float)] * 3
         L.tdbem_stream_copy.argtypes = [C.c_size_t, C.c_int, dp]
+        L.tdbem_mfma_f64_selftest.argtypes = [dp, dp, dp]
         L.tdbem_get_state.argtypes = [vp, C.c_int, C.c_int, dp, dp]
         L.tdbem_set_profiling.argtypes = [vp, C.c_int]
         L.tdbem_reset_timings.argtypes = [vp]
@@ -251,3 +252,15 @@ def stream_copy(nbytes, launches=3):
     if rc:
         raise TdbemError(rc, lib().tdbem_last_error(None).decode())
     return ms.value
+
+
+def mfma_f64_selftest(A, B):
+    """D = A B through one v_mfma_f64_16x16x4 with the lane layouts of the step solve
+    (A 16x4, B 4x16)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    D = np.zeros((16, 16))
+    rc = lib().tdbem_mfma_f64_selftest(_d(A), _d(B), _d(D))
+    if rc:
+        raise TdbemError(rc, lib().tdbem_last_error(None).decode())
+    return D

@@ -20,6 +20,15 @@ def test_mfma_lane_layout(gpu):
     np.testing.assert_array_equal(D, A @ B)
 
 
+def test_mfma_f64_lane_layout(gpu):
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((16, 4))
+    B = rng.standard_normal((4, 16))
+    print("[gpu case] mfma f64 self-test", flush=True)
+    D = api.mfma_f64_selftest(A, B)
+    np.testing.assert_allclose(D, A @ B, rtol=1e-14, atol=1e-14)
+
+
 def _fp64_histories(p, amp, rise):
     out = []
     for r0 in range(0, len(amp), 8):
