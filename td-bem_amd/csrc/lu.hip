@@ -5,9 +5,10 @@
 // exchange) is the same every step, so it is factored once and inverted once, and each step is
 // one GEMV with the inverse.
 //
-// Blocked right-looking LU (panel width 64): one workgroup factors the panel and swaps whole rows,
-// a triangular solve forms the U12 block row, and an LDS-tiled FP64 GEMM updates the trailing
-// matrix.  Solves use the same pieces column-blocked over the right-hand sides.
+// Blocked right-looking LU (panel width 64): a recursive panel factorisation with one-workgroup
+// leaves, row interchanges applied to the other columns, a triangular solve for the U12 block row
+// and an LDS-tiled FP64 GEMM for the trailing matrix.  Solves use the same pieces column-blocked
+// over the right-hand sides.
 #include <algorithm>
 #include <stdexcept>
 #include <vector>
@@ -19,79 +20,102 @@ namespace tdbem {
 namespace {
 
 constexpr int NB = 64;          // panel width
-constexpr int kPanelThreads = 1024;
 
 // ---------------------------------------------------------------------------------------------
-// Panel factorisation of columns [j0, j0+nb) over rows [j0, n).  Row swaps are applied to whole
-// rows, so the stored factors satisfy P A = L U with L unit lower (below the diagonal) and U upper.
+// Panel factorisation, recursive (64 -> 32 -> 16 -> 8 columns).  A leaf of kLeafW columns over
+// rows [c0, n) is factored by one workgroup directly in the row-major matrix: per column a block
+// argmax picks the pivot (ties keep the smaller row), the two rows are swapped inside the leaf,
+// and one pass over the rows below scales the column, updates the rest of the leaf and tracks the
+// largest |a| of the next column, so each column costs one sweep of m x kLeafW values (L2
+// resident).  Row swaps are applied to the other columns by k_laswp_range; the stored factors
+// satisfy P A = L U with L unit lower (below the diagonal) and U upper.
 // ---------------------------------------------------------------------------------------------
-// The panel is copied to a column-major scratch P (P[c * m + r], m = n - j0 rows) so that the
-// pivot search, the scaling and the in-panel rank-1 updates are coalesced across threads; the
-// row interchanges are applied to the rest of A afterwards by k_laswp, one thread per column.
-__global__ void __launch_bounds__(kPanelThreads)
-    k_panel_cm(double* __restrict__ A, int n, int j0, int nb, double* __restrict__ P,
-               int* __restrict__ piv, int* __restrict__ status)
+constexpr int kLeafW = 8;
+constexpr int kLeafThreads = 1024;
+constexpr int kLeafUnroll = 4;    // rows in flight per thread
+
+__device__ __forceinline__ void argmax_merge(double& v, int& i, double ov, int oi)
 {
-    __shared__ double sval[kPanelThreads];
-    __shared__ int sidx[kPanelThreads];
-    const int tid = threadIdx.x;
-    const int m = n - j0;
-    for (int t = tid; t < m * nb; t += kPanelThreads) {
-        const int r = t / nb, c = t % nb;
-        P[(int64_t)c * m + r] = A[(int64_t)(j0 + r) * n + j0 + c];
+    if (ov > v || (ov == v && oi < i)) {
+        v = ov;
+        i = oi;
     }
-    __syncthreads();
-    for (int c = 0; c < nb; c++) {
-        double* Pc = P + (int64_t)c * m;
-        double best = -1.0;
-        int bi = m;
-        for (int r = c + tid; r < m; r += kPanelThreads) {
-            const double v = fabs(Pc[r]);
-            if (v > best) {
-                best = v;
-                bi = r;
-            }
+}
+
+__global__ void __launch_bounds__(kLeafThreads)
+    k_leaf(double* __restrict__ A, int n, int c0, int w, int* __restrict__ piv,
+           int* __restrict__ status)
+{
+    __shared__ double sv[kLeafThreads / 64];
+    __shared__ int si[kLeafThreads / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    double best = -1.0;
+    int bi = n;
+    for (int r = c0 + tid; r < n; r += kLeafThreads) argmax_merge(best, bi, fabs(A[(int64_t)r * n + c0]), r);
+    for (int c = c0; c < c0 + w; c++) {
+        const int cc = c - c0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            argmax_merge(best, bi, __shfl_xor(best, off, 64), __shfl_xor(bi, off, 64));
+        if (lane == 0) {
+            sv[wid] = best;
+            si[wid] = bi;
         }
-        sval[tid] = best;
-        sidx[tid] = bi;
         __syncthreads();
-        for (int s = kPanelThreads / 2; s > 0; s >>= 1) {
-            if (tid < s) {
-                const double ov = sval[tid + s];
-                const int oi = sidx[tid + s];
-                if (ov > sval[tid] || (ov == sval[tid] && oi < sidx[tid])) {
-                    sval[tid] = ov;
-                    sidx[tid] = oi;
+        double pm = sv[0];
+        int p = si[0];
+#pragma unroll
+        for (int q = 1; q < kLeafThreads / 64; q++) argmax_merge(pm, p, sv[q], si[q]);
+        if (tid == 0) {
+            piv[c] = p;
+            if (!(pm > 0.0)) *status = 1;
+        }
+        if (p != c && tid < w) {
+            double* a = A + (int64_t)c * n + c0 + tid;
+            double* b = A + (int64_t)p * n + c0 + tid;
+            const double t = *a;
+            *a = *b;
+            *b = t;
+        }
+        __syncthreads();   // swapped rows visible; sv/si free for the next column
+        const double* prow = A + (int64_t)c * n + c0;
+        const double d = prow[cc];
+        double u[kLeafW];
+#pragma unroll
+        for (int j = 0; j < kLeafW; j++) u[j] = (j > cc && j < w) ? prow[j] : 0.0;
+        best = -1.0;
+        bi = n;
+        for (int r0 = c + 1 + tid; r0 < n; r0 += kLeafThreads * kLeafUnroll) {
+            double x[kLeafUnroll][kLeafW];
+#pragma unroll
+            for (int q = 0; q < kLeafUnroll; q++) {
+                const int r = r0 + q * kLeafThreads;
+#pragma unroll
+                for (int j = 0; j < kLeafW; j++)
+                    x[q][j] = (r < n && j >= cc && j < w) ? A[(int64_t)r * n + c0 + j] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < kLeafUnroll; q++) {
+                const int r = r0 + q * kLeafThreads;
+                if (r >= n) continue;
+                double l = 0.0;
+#pragma unroll
+                for (int j = 0; j < kLeafW; j++)
+                    if (j == cc) l = x[q][j] / d;
+                double* row = A + (int64_t)r * n + c0;
+#pragma unroll
+                for (int j = 0; j < kLeafW; j++) {
+                    if (j == cc) {
+                        row[j] = l;
+                    } else if (j > cc && j < w) {
+                        const double v = x[q][j] - l * u[j];
+                        row[j] = v;
+                        if (j == cc + 1) argmax_merge(best, bi, fabs(v), r);
+                    }
                 }
             }
-            __syncthreads();
         }
-        const int p = sidx[0];
-        const double pmax = sval[0];
-        __syncthreads();
-        if (tid == 0) {
-            piv[j0 + c] = j0 + p;
-            if (!(pmax > 0.0)) *status = 1;
-        }
-        if (p != c)
-            for (int col = tid; col < nb; col += kPanelThreads) {
-                double* Pk = P + (int64_t)col * m;
-                const double a = Pk[c];
-                Pk[c] = Pk[p];
-                Pk[p] = a;
-            }
-        __syncthreads();
-        const double d = Pc[c];
-        for (int r = c + 1 + tid; r < m; r += kPanelThreads) {
-            const double l = Pc[r] / d;
-            Pc[r] = l;
-            for (int cc = c + 1; cc < nb; cc++) P[(int64_t)cc * m + r] -= l * P[(int64_t)cc * m + c];
-        }
-        __syncthreads();
-    }
-    for (int t = tid; t < m * nb; t += kPanelThreads) {
-        const int r = t / nb, c = t % nb;
-        A[(int64_t)(j0 + r) * n + j0 + c] = P[(int64_t)c * m + r];
+        __syncthreads();   // the column's updates visible before the next pivot row is read
     }
 }
 
@@ -111,61 +135,19 @@ __global__ void k_laswp(double* __restrict__ A, int n, int j0, int nb, const int
     }
 }
 
-__global__ void __launch_bounds__(kPanelThreads)
-    k_panel(double* __restrict__ A, int n, int j0, int nb, int* __restrict__ piv,
-            int* __restrict__ status)
+// Row interchanges piv[p0 .. p1) applied, in order, to columns [ca, cb).
+__global__ void k_laswp_range(double* __restrict__ A, int n, int p0, int p1, int ca, int cb,
+                              const int* __restrict__ piv)
 {
-    __shared__ double sval[kPanelThreads];
-    __shared__ int sidx[kPanelThreads];
-    const int tid = threadIdx.x;
-    for (int c = j0; c < j0 + nb; c++) {
-        // pivot: first row with the largest |a| (ties keep the smaller index)
-        double best = -1.0;
-        int bi = n;
-        for (int r = c + tid; r < n; r += kPanelThreads) {
-            const double v = fabs(A[(int64_t)r * n + c]);
-            if (v > best) {
-                best = v;
-                bi = r;
-            }
+    const int col = ca + blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= cb) return;
+    for (int c = p0; c < p1; c++) {
+        const int p = piv[c];
+        if (p != c) {
+            const double a = A[(int64_t)c * n + col];
+            A[(int64_t)c * n + col] = A[(int64_t)p * n + col];
+            A[(int64_t)p * n + col] = a;
         }
-        sval[tid] = best;
-        sidx[tid] = bi;
-        __syncthreads();
-        for (int s = kPanelThreads / 2; s > 0; s >>= 1) {
-            if (tid < s) {
-                const double ov = sval[tid + s];
-                const int oi = sidx[tid + s];
-                if (ov > sval[tid] || (ov == sval[tid] && oi < sidx[tid])) {
-                    sval[tid] = ov;
-                    sidx[tid] = oi;
-                }
-            }
-            __syncthreads();
-        }
-        const int p = sidx[0];
-        const double pmax = sval[0];
-        __syncthreads();
-        if (tid == 0) {
-            piv[c] = p;
-            if (!(pmax > 0.0)) *status = 1;
-        }
-        if (p != c)
-            for (int col = tid; col < n; col += kPanelThreads) {
-                const double a = A[(int64_t)c * n + col];
-                A[(int64_t)c * n + col] = A[(int64_t)p * n + col];
-                A[(int64_t)p * n + col] = a;
-            }
-        __syncthreads();
-        const double d = A[(int64_t)c * n + c];
-        for (int r = c + 1 + tid; r < n; r += kPanelThreads) {
-            double* row = A + (int64_t)r * n;
-            const double l = row[c] / d;
-            row[c] = l;
-            const double* prow = A + (int64_t)c * n;
-            for (int cc = c + 1; cc < j0 + nb; cc++) row[cc] -= l * prow[cc];
-        }
-        __syncthreads();
     }
 }
 
@@ -337,18 +319,46 @@ void solve_inplace(const double* LU, int n, double* B, int ncols, hipStream_t s)
 
 }  // namespace
 
+namespace {
+
+void laswp_range(double* A, int n, int p0, int p1, int ca, int cb, const int* piv, hipStream_t s)
+{
+    if (cb <= ca || p1 <= p0) return;
+    hipLaunchKernelGGL(k_laswp_range, dim3((cb - ca + 255) / 256), dim3(256), 0, s, A, n, p0, p1,
+                       ca, cb, piv);
+    TD_HIP(hipGetLastError());
+}
+
+// Factors columns [c0, c0 + w) over rows [c0, n); row swaps reach only these columns.
+void panel_rec(double* A, int n, int c0, int w, int* piv, int* status, hipStream_t s)
+{
+    if (w <= kLeafW) {
+        hipLaunchKernelGGL(k_leaf, dim3(1), dim3(kLeafThreads), 0, s, A, n, c0, w, piv, status);
+        TD_HIP(hipGetLastError());
+        return;
+    }
+    const int h = (w / 2 + kLeafW - 1) / kLeafW * kLeafW;
+    panel_rec(A, n, c0, h, piv, status, s);
+    laswp_range(A, n, c0, c0 + h, c0 + h, c0 + w, piv, s);
+    hipLaunchKernelGGL(k_trsm_lower_unit, dim3(1), dim3(64), 0, s, A, n, c0, h, A, n, c0, c0 + h,
+                       w - h);
+    TD_HIP(hipGetLastError());
+    gemm_sub(n - c0 - h, w - h, h, A + (int64_t)(c0 + h) * n + c0, n,
+             A + (int64_t)c0 * n + c0 + h, n, A + (int64_t)(c0 + h) * n + c0 + h, n, s);
+    panel_rec(A, n, c0 + h, w - h, piv, status, s);
+    laswp_range(A, n, c0 + h, c0 + w, c0, c0 + h, piv, s);
+}
+
+}  // namespace
+
 void lu_factor(double* A, int n, int* piv, hipStream_t s)
 {
     DBuf<int> status;
     status.alloc(1);
     TD_HIP(hipMemsetAsync(status.p, 0, sizeof(int), s));
-    DBuf<double> P;
-    P.alloc((size_t)n * NB);
     for (int j0 = 0; j0 < n; j0 += NB) {
         const int nb = std::min(NB, n - j0);
-        hipLaunchKernelGGL(k_panel_cm, dim3(1), dim3(kPanelThreads), 0, s, A, n, j0, nb, P.p, piv,
-                           status.p);
-        TD_HIP(hipGetLastError());
+        panel_rec(A, n, j0, nb, piv, status.p, s);
         if (n - nb > 0) {
             hipLaunchKernelGGL(k_laswp, dim3((n - nb + 255) / 256), dim3(256), 0, s, A, n, j0, nb,
                                piv);
