@@ -128,10 +128,12 @@ int tdbem_synchronize(tdbem_context* ctx);
  * store_ms, init_ms, march_ms (host wall clock of build_lag_store / march_init / march_run),
  * lag_kernel_ms, gather_ms, conv_ms, solve_ms (summed event times), lag_launches,
  * conv_launches, active_points (quadrature-point x lag evaluations done by the lag kernel),
- * store_nonzero (stored (i, j, k) blocks before padding), store_padded, tile_blocks (48 x 96
- * blocks of the mixed-precision store, 0 until built), reduce_ms (the mixed path's reduction of
- * per-wave partial sums, summed event times).  conv_ms covers k_conv or k_conv_mixed alone. */
-#define TDBEM_NUM_TIMINGS 14
+ * store_nonzero (stored (i, j, k) blocks before padding), store_padded, tile_subtiles (16 x 32
+ * split-BF16 sub-tiles of the mixed-precision store, 0 until built), reduce_ms (the mixed path's
+ * reduction of partial sums, summed event times), tile_blocks (48 x 96 blocks with a stored
+ * sub-tile), tile_entries (steps of the merged per-group lists).  conv_ms covers k_conv or
+ * k_conv_mixed alone. */
+#define TDBEM_NUM_TIMINGS 16
 int tdbem_set_profiling(tdbem_context* ctx, int on);
 int tdbem_reset_timings(tdbem_context* ctx);
 int tdbem_get_timings(tdbem_context* ctx, double* out);

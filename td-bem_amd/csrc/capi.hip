@@ -460,8 +460,8 @@ int tdbem_get_timings(tdbem_context* ctx, double* out)
         const double v[TDBEM_NUM_TIMINGS] = {
             t.store_ms, t.init_ms, t.march_ms, t.lag_kernel_ms, t.gather_ms, t.conv_ms,
             t.solve_ms, (double)t.lag_launches, (double)t.conv_launches, (double)t.active_points,
-            (double)c.store.nonzero_entries, (double)c.store.n_entries, (double)c.tp.nblocks,
-            t.reduce_ms};
+            (double)c.store.nonzero_entries, (double)c.store.n_entries, (double)c.tp.nsub,
+            t.reduce_ms, (double)c.tp.nblocks, (double)c.tp.nentries};
         std::memcpy(out, v, sizeof(v));
     });
 }

@@ -344,6 +344,7 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
     M.precision = c.precision;
     if (M.precision == 1) {
         tp_build(c);
+        tp_plan(c, nrhs);
         M.zslot = tp_zslot_elems(c, nrhs);
         M.zs.alloc((size_t)M.R * M.zslot);
         TD_HIP(hipMemsetAsync(M.zs.p, 0, M.zs.bytes(), c.stream));

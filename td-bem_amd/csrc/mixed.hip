@@ -2,17 +2,23 @@
 // accumulated in FP32 inside a 32-wide k-tile and in FP64 across tiles, blocks and lags, for R
 // right-hand sides at once (R = 16, 32, 48 or 64).
 //
-// Layout.  Nodes are reordered along a Morton curve and grouped into tiles of 16 (48 dofs).  For
-// each pair of tiles (I, J) the lags k in the union of their node-pair shells are stored as dense
-// 48 x 96 blocks A[(3 ii + p)][(6 jj + 3 part + q)] with part 0 = G[p][q] (times z^t) and
-// part 1 = -H[p][q] (times z^u), each value split as a1 + a2 + a3 (bf16, 24 significant bits) and
-// laid out in MFMA fragment order, so a wavefront loads each 16x32 operand as one 1 KiB line.
+// Layout.  Nodes are ordered by recursive coordinate bisection and grouped into tiles of 16
+// (48 dofs).  For each pair of tiles (I, J) and each lag k in the union of their node-pair shells
+// the 48 x 96 block A[(3 ii + p)][(6 jj + 3 part + q)] (part 0 = G[p][q], times z^t; part 1 =
+// -H[p][q], times z^u) is cut into 3 x 3 sub-tiles of 16 x 32; only sub-tiles that some node pair
+// reaches at lag k are stored.  Each value is split as a1 + a2 + a3 (bf16, 24 significant bits)
+// and laid out in MFMA operand order, so a wavefront loads each 16 x 32 operand as one 1 KiB line.
 // The history ring (FP64) is mirrored as split-BF16 fragments per (slot, column tile).
 //
-// Per block the products a1 z1 + a1 z2 + a2 z1 + a1 z3 + a2 z2 + a3 z1 (all terms down to 2^-24
-// relative) run on v_mfma_f32_16x16x32_bf16; per 32-wide k-tile the leading product and the sum of
-// the corrections leave FP32 for FP64 accumulators, so rounding does not grow with the number of
-// blocks and lags summed.
+// Work.  A workgroup takes 4 row tiles (one per wavefront, neighbours in the bisection order)
+// and its wavefronts walk the same merged (J, k) list of the four, each skipping the steps where
+// its row tile has no block.  Walking in step, they fetch the history fragment of
+// (J, slot of s - k) close together in time, so it comes from HBM once and from L1/L2 after.
+//
+// Per sub-tile the products a1 z1 + a1 z2 + a2 z1 + a1 z3 + a2 z2 + a3 z1 (all terms down to
+// 2^-24 relative) run on v_mfma_f32_16x16x32_bf16; the leading product and the sum of the
+// corrections leave FP32 for FP64 accumulators after every sub-tile, so rounding does not grow
+// with the number of blocks and lags summed.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -31,8 +37,8 @@ namespace {
 constexpr int kTile = 16;                 // nodes per tile
 constexpr int kRows = 3 * kTile;          // 48
 constexpr int kCols = 6 * kTile;          // 96
-constexpr int kBlockElems = 3 * 3 * 3 * 512;  // terms x m-tiles x k-tiles x (64 lanes x 8)
-constexpr int kSplitWaves = 4;
+constexpr int kSubElems = 3 * 512;        // bf16 per stored sub-tile: 3 terms x 64 lanes x 8
+constexpr int kGroup = 4;                 // row tiles per workgroup (one per wavefront)
 
 __device__ __forceinline__ unsigned short bf16_rne(float f)
 {
@@ -56,13 +62,6 @@ __device__ __forceinline__ void split3(double v, unsigned short* b)
     b[2] = bf16_rne((float)r2);
 }
 
-// Position of A[r][c] of term t inside a block (16x16x32 A-operand: lane = row + 16 * (k / 8)).
-__device__ __forceinline__ int a_offset(int t, int r, int c)
-{
-    const int m = r >> 4, i = r & 15, kt = c >> 5, kk = c & 31;
-    return ((t * 3 + m) * 3 + kt) * 512 + (i + 16 * (kk >> 3)) * 8 + (kk & 7);
-}
-
 __device__ __forceinline__ int row_of_chunk(const int64_t* row_chunk, int nnd, int64_t c)
 {
     int lo = 0, hi = nnd;
@@ -76,12 +75,15 @@ __device__ __forceinline__ int row_of_chunk(const int64_t* row_chunk, int nnd, i
     return lo;
 }
 
-// FP64 node-pair store -> split-BF16 tile blocks.  One 64-thread block per store chunk.
+// FP64 node-pair store -> split-BF16 sub-tiles.  One 64-thread block per store chunk.  Element
+// (r, c) of a block lives in sub-tile (r / 16, c / 32) at lane (r % 16) + 16 ((c % 32) / 8),
+// element c % 8 (16x16x32 A-operand order).
 __global__ void __launch_bounds__(kChunk)
     k_tp_fill(const double* __restrict__ vals, const uint32_t* __restrict__ idx,
               const int64_t* __restrict__ row_chunk, const int64_t* __restrict__ row_valid,
               int nnd, const int* __restrict__ pos, int nT, const int64_t* __restrict__ pair_off,
-              const int* __restrict__ pair_lo, unsigned short* __restrict__ tp)
+              const int* __restrict__ pair_lo, const uint32_t* __restrict__ sub_off,
+              const uint16_t* __restrict__ mask, unsigned short* __restrict__ tp)
 {
     const int64_t c = blockIdx.x;
     const int i = row_of_chunk(row_chunk, nnd, c);
@@ -91,14 +93,23 @@ __global__ void __launch_bounds__(kChunk)
     const int j = (int)(id >> kKBits), k = (int)(id & kMaxLag);
     const int pi = pos[i], pj = pos[j];
     const int I = pi / kTile, ii = pi % kTile, J = pj / kTile, jj = pj % kTile;
-    const int64_t base = pair_off[(int64_t)I * nT + J];
-    const int64_t blk = base + (k - pair_lo[(int64_t)I * nT + J]);
-    unsigned short* B = tp + blk * kBlockElems;
+    const int64_t blk = pair_off[(int64_t)I * nT + J] + (k - pair_lo[(int64_t)I * nT + J]);
+    const uint32_t first = sub_off[blk];
+    const unsigned msk = mask[blk];
     double v[kVals];
     const double* s = vals + c * (kVals * kChunk) + lane;
 #pragma unroll
     for (int t = 0; t < kVals; t++) v[t] = s[t * kChunk];
     const int sym[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    auto put = [&](int r, int col, const unsigned short* b) {
+        const int bit = (r >> 4) * 3 + (col >> 5);
+        const int q = __popc(msk & ((1u << bit) - 1u));
+        unsigned short* S = tp + ((int64_t)first + q) * kSubElems;
+        const int kk = col & 31;
+        const int o = ((r & 15) + 16 * (kk >> 3)) * 8 + (kk & 7);
+#pragma unroll
+        for (int t = 0; t < 3; t++) S[t * 512 + o] = b[t];
+    };
 #pragma unroll
     for (int p = 0; p < 3; p++)
 #pragma unroll
@@ -107,15 +118,13 @@ __global__ void __launch_bounds__(kChunk)
             split3(v[sym[p][q]], g);
             split3(-v[6 + 3 * p + q], h);
             const int r = 3 * ii + p;
-#pragma unroll
-            for (int t = 0; t < 3; t++) {
-                B[a_offset(t, r, 6 * jj + q)] = g[t];
-                B[a_offset(t, r, 6 * jj + 3 + q)] = h[t];
-            }
+            put(r, 6 * jj + q, g);
+            put(r, 6 * jj + 3 + q, h);
         }
 }
 
-// Split-BF16 copy of ring slot `slot` in B-operand order (lane = rhs + 16 * (k / 8)).
+// Split-BF16 copy of ring slot `slot` in B-operand order (lane = rhs + 16 * (k / 8)), laid out
+// [slot][J][term][k-tile][rhs tile][64 lanes][8].
 __global__ void k_zsplit(const double* __restrict__ ring, int N, int nrhs, int nT,
                          const int* __restrict__ perm, int slot, int NT,
                          unsigned short* __restrict__ zs)
@@ -140,18 +149,15 @@ __global__ void k_zsplit(const double* __restrict__ ring, int N, int nrhs, int n
 }
 
 template <int NT>
-__global__ void __launch_bounds__(64 * kSplitWaves)
-    k_conv_mixed(const bf16x8* __restrict__ tp, const uint32_t* __restrict__ bidx,
-                 const int64_t* __restrict__ row_blk, const bf16x8* __restrict__ zs, int nT,
-                 int Rring, int s, int nsplit, double* __restrict__ partial)
+__global__ void __launch_bounds__(64 * kGroup)
+    k_conv_mixed(const bf16x8* __restrict__ tp, const TPEntry* __restrict__ ent,
+                 const int4* __restrict__ chunk, const bf16x8* __restrict__ zs, int nT, int Rring,
+                 int s, double* __restrict__ partial)
 {
-    const int I = blockIdx.x, sp = blockIdx.y;
+    const int4 cw = chunk[blockIdx.x];
+    const int g = cw.x;
+    const int64_t e0 = cw.y, e1 = cw.z;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t b0 = row_blk[I], b1 = row_blk[I + 1];
-    const int chunks = nsplit * kSplitWaves;
-    const int64_t per = (b1 - b0 + chunks - 1) / chunks;
-    const int64_t lo = b0 + (int64_t)(sp * kSplitWaves + w) * per;
-    const int64_t hi = min(b1, lo + per);
     double acc[3][NT][4];
 #pragma unroll
     for (int m = 0; m < 3; m++)
@@ -159,47 +165,64 @@ __global__ void __launch_bounds__(64 * kSplitWaves)
         for (int n = 0; n < NT; n++)
 #pragma unroll
             for (int v = 0; v < 4; v++) acc[m][n][v] = 0.0;
-    for (int64_t b = lo; b < hi; b++) {
-        const uint32_t id = bidx[b];
-        const int J = (int)(id >> kKBits), k = (int)(id & kMaxLag);
+    for (int64_t e = e0; e < e1; e++) {
+        const TPEntry en = ent[e];
+        const unsigned msk = w == 0 ? en.msk[0] : w == 1 ? en.msk[1] : w == 2 ? en.msk[2] : en.msk[3];
+        if (!msk) continue;
+        const uint32_t first = w == 0 ? en.sub[0] : w == 1 ? en.sub[1] : w == 2 ? en.sub[2] : en.sub[3];
+        const int J = (int)(en.id >> kKBits), k = (int)(en.id & kMaxLag);
         int slot = (s - k) % Rring;
         if (slot < 0) slot += Rring;
-        const bf16x8* A = tp + b * (kBlockElems / 8);
-        const bf16x8* Z = zs + (((int64_t)slot * nT + J) * 3) * 3 * NT * 64;
+        // every stored sub-tile of this wave's block and the history k-tiles it needs are
+        // requested before any is used; the history fragment is shared through L1/L2 with the
+        // other wavefronts of the group, which walk the same list
+        const bf16x8* A = tp + (int64_t)first * (kSubElems / 8) + lane;
+        const bf16x8* Zg = zs + (((int64_t)slot * nT + J) * 3) * 3 * NT * 64 + lane;
+        unsigned kts = 0;
 #pragma unroll
-        for (int kt = 0; kt < 3; kt++) {
-            bf16x8 a[3][3], z[3][NT];
+        for (int m = 0; m < 3; m++) kts |= (msk >> (3 * m)) & 7u;
+        bf16x8 a[9][3], z[3][3][NT];
+        int q = 0;
 #pragma unroll
-            for (int t = 0; t < 3; t++)
+        for (int u = 0; u < 9; u++)
+            if ((msk >> u) & 1u) {
 #pragma unroll
-                for (int m = 0; m < 3; m++) a[t][m] = A[((t * 3 + m) * 3 + kt) * 64 + lane];
+                for (int t = 0; t < 3; t++) a[u][t] = A[(q * 3 + t) * 64];
+                q++;
+            }
 #pragma unroll
-            for (int t = 0; t < 3; t++)
+        for (int kt = 0; kt < 3; kt++)
+            if ((kts >> kt) & 1u) {
 #pragma unroll
-                for (int n = 0; n < NT; n++) z[t][n] = Z[((t * 3 + kt) * NT + n) * 64 + lane];
-            // The leading product a1 z1 and the five corrections (2^-8 smaller and below) are
-            // summed in separate FP32 registers and both moved to FP64 after every 32-wide
-            // k-tile: each FP32 rounding then acts on one 32-term dot product only.
+                for (int t = 0; t < 3; t++)
 #pragma unroll
-            for (int m = 0; m < 3; m++)
+                    for (int n = 0; n < NT; n++) z[kt][t][n] = Zg[((t * 3 + kt) * NT + n) * 64];
+            }
 #pragma unroll
-                for (int n = 0; n < NT; n++) {
-                    const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
-                    const f32x4 dm =
-                        __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][m], z[0][n], zero, 0, 0, 0);
-                    f32x4 dc =
-                        __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][m], z[0][n], zero, 0, 0, 0);
-                    dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][m], z[1][n], dc, 0, 0, 0);
-                    dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][m], z[2][n], dc, 0, 0, 0);
-                    dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][m], z[0][n], dc, 0, 0, 0);
-                    dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][m], z[1][n], dc, 0, 0, 0);
+        for (int u = 0; u < 9; u++) {
+            if (!((msk >> u) & 1u)) continue;
+            const int m = u / 3, kt = u % 3;
+            // the leading product a1 z1 and the five corrections sum in separate FP32 registers;
+            // both move to FP64 after each sub-tile
 #pragma unroll
-                    for (int v = 0; v < 4; v++)
-                        acc[m][n][v] += (double)dm[v] + (double)dc[v];
-                }
+            for (int n = 0; n < NT; n++) {
+                const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+                const f32x4 dm =
+                    __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][0], z[kt][0][n], zero, 0, 0, 0);
+                f32x4 dc =
+                    __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][2], z[kt][0][n], zero, 0, 0, 0);
+                dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][1], z[kt][1][n], dc, 0, 0, 0);
+                dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][0], z[kt][2][n], dc, 0, 0, 0);
+                dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][1], z[kt][0][n], dc, 0, 0, 0);
+                dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][0], z[kt][1][n], dc, 0, 0, 0);
+#pragma unroll
+                for (int v = 0; v < 4; v++) acc[m][n][v] += (double)dm[v] + (double)dc[v];
+            }
         }
     }
-    double* out = partial + (((int64_t)I * chunks + sp * kSplitWaves + w) * 3 * NT) * 256;
+    if (g * kGroup + w >= nT) return;
+    // partial sums of chunk c: [c][wave][3][NT][64][4]
+    double* out = partial + (((int64_t)blockIdx.x * kGroup + w) * 3 * NT) * 256;
 #pragma unroll
     for (int m = 0; m < 3; m++)
 #pragma unroll
@@ -208,9 +231,10 @@ __global__ void __launch_bounds__(64 * kSplitWaves)
             for (int v = 0; v < 4; v++) out[((m * NT + n) * 64 + lane) * 4 + v] = acc[m][n][v];
 }
 
-// hist[r][dof] = sum over the partials of a row tile (fixed order).  16x16 output fragment:
-// lane = col + 16 * (row / 4), element row % 4.
-__global__ void k_tp_reduce(const double* __restrict__ partial, int nT, int chunks, int NT,
+// hist[r][dof] = sum over the chunks of a row tile's group (fixed order).  16x16 output
+// fragment: lane = col + 16 * (row / 4), element row % 4.
+__global__ void k_tp_reduce(const double* __restrict__ partial, int nT,
+                            const int* __restrict__ grp_chunk, int NT,
                             const int* __restrict__ perm, int N, int nrhs,
                             double* __restrict__ hist)
 {
@@ -223,9 +247,10 @@ __global__ void k_tp_reduce(const double* __restrict__ partial, int nT, int chun
     if (node < 0) return;
     const int m = row >> 4, i = row & 15, nt = r >> 4, n = r & 15;
     const int lane = n + 16 * (i >> 2), v = i & 3;
+    const int g = I / kGroup, w = I % kGroup;
     double sum = 0.0;
-    for (int c = 0; c < chunks; c++)
-        sum += partial[((((int64_t)I * chunks + c) * 3 * NT + m * NT + nt) * 64 + lane) * 4 + v];
+    for (int c = grp_chunk[g]; c < grp_chunk[g + 1]; c++)
+        sum += partial[(((((int64_t)c * kGroup + w) * 3 + m) * NT + nt) * 64 + lane) * 4 + v];
     hist[(int64_t)r * N + 3 * node + row % 3] = sum;
 }
 
@@ -245,17 +270,27 @@ __global__ void k_mfma_selftest(const float* __restrict__ A, const float* __rest
     for (int v = 0; v < 4; v++) D[(4 * (lane >> 4) + v) * 16 + (lane & 15)] = d[v];
 }
 
-uint32_t morton3(uint32_t x, uint32_t y, uint32_t z)
+// Recursive coordinate bisection of idx[lo, hi): split across the widest coordinate so that the
+// left part holds a whole number of tiles; leaves are tiles of kTile nodes.
+void bisect(std::vector<int>& idx, int lo, int hi, const std::vector<double>& X)
 {
-    auto spread = [](uint32_t v) {
-        v &= 0x3FF;
-        v = (v | (v << 16)) & 0x030000FF;
-        v = (v | (v << 8)) & 0x0300F00F;
-        v = (v | (v << 4)) & 0x030C30C3;
-        v = (v | (v << 2)) & 0x09249249;
-        return v;
-    };
-    return spread(x) | (spread(y) << 1) | (spread(z) << 2);
+    const int n = hi - lo;
+    if (n <= kTile) return;
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    for (int a = lo; a < hi; a++)
+        for (int d = 0; d < 3; d++) {
+            mn[d] = std::min(mn[d], X[3 * idx[a] + d]);
+            mx[d] = std::max(mx[d], X[3 * idx[a] + d]);
+        }
+    int ax = 0;
+    for (int d = 1; d < 3; d++)
+        if (mx[d] - mn[d] > mx[ax] - mn[ax]) ax = d;
+    std::stable_sort(idx.begin() + lo, idx.begin() + hi,
+                     [&](int a, int b) { return X[3 * a + ax] < X[3 * b + ax]; });
+    const int ntiles = (n + kTile - 1) / kTile;
+    const int left = ntiles / 2 * kTile;
+    bisect(idx, lo, lo + left, X);
+    bisect(idx, lo + left, hi, X);
 }
 
 }  // namespace
@@ -266,79 +301,137 @@ void tp_build(Context& c)
     TPStore& T = c.tp;
     if (T.ready) return;
     const int nnd = c.nnd;
-    // Morton order of the nodes
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int i = 0; i < nnd; i++)
-        for (int d = 0; d < 3; d++) {
-            lo[d] = std::min(lo[d], c.h_coords[3 * i + d]);
-            hi[d] = std::max(hi[d], c.h_coords[3 * i + d]);
-        }
-    std::vector<uint32_t> key(nnd);
-    for (int i = 0; i < nnd; i++) {
-        uint32_t q[3];
-        for (int d = 0; d < 3; d++) {
-            const double span = hi[d] - lo[d];
-            q[d] = span > 0 ? (uint32_t)std::min(1023.0, (c.h_coords[3 * i + d] - lo[d]) / span * 1023.0)
-                            : 0u;
-        }
-        key[i] = morton3(q[0], q[1], q[2]);
-    }
     std::vector<int> order(nnd);
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+    bisect(order, 0, nnd, c.h_coords);
     T.nT = (nnd + kTile - 1) / kTile;
+    T.nG = (T.nT + kGroup - 1) / kGroup;
     T.perm.assign((size_t)T.nT * kTile, -1);
     T.pos.assign(nnd, 0);
     for (int p = 0; p < nnd; p++) {
         T.perm[p] = order[p];
         T.pos[order[p]] = p;
     }
-    // tile-pair lag hulls from the node-pair hulls
     const int nT = T.nT;
-    std::vector<int> plo((size_t)nT * nT, 1 << 30), phi((size_t)nT * nT, -1);
+    // lag range of every sub-tile (m, kt) of every tile pair, from the node-pair hulls: node ii
+    // has rows 3 ii .. 3 ii + 2 (m-tiles (3 ii) / 16 and (3 ii + 2) / 16), node jj columns
+    // 6 jj .. 6 jj + 5 (k-tiles (6 jj) / 32 and (6 jj + 5) / 32)
+    const size_t npair = (size_t)nT * nT;
+    std::vector<int> slo(npair * 9, 1 << 30), shi(npair * 9, -1);
     const auto& H = c.store.h_hull;
-    for (int i = 0; i < nnd; i++) {
-        const int I = T.pos[i] / kTile;
-        for (int j = 0; j < nnd; j++) {
+    for (int pi = 0; pi < nT * kTile; pi++) {
+        const int i = T.perm[pi];
+        if (i < 0) continue;
+        const int I = pi / kTile, ii = pi % kTile;
+        const int m0 = (3 * ii) >> 4, m1 = (3 * ii + 2) >> 4;
+        for (int pj = 0; pj < nT * kTile; pj++) {
+            const int j = T.perm[pj];
+            if (j < 0) continue;
             const int2 h = H[(size_t)i * nnd + j];
             if (h.y < h.x) continue;
-            const size_t q = (size_t)I * nT + T.pos[j] / kTile;
-            plo[q] = std::min(plo[q], h.x);
-            phi[q] = std::max(phi[q], h.y);
+            const int J = pj / kTile, jj = pj % kTile;
+            const int k0 = (6 * jj) >> 5, k1 = (6 * jj + 5) >> 5;
+            int* lo = &slo[((size_t)I * nT + J) * 9];
+            int* hi = &shi[((size_t)I * nT + J) * 9];
+            for (int m = m0; m <= m1; m++)
+                for (int kt = k0; kt <= k1; kt++) {
+                    lo[3 * m + kt] = std::min(lo[3 * m + kt], h.x);
+                    hi[3 * m + kt] = std::max(hi[3 * m + kt], h.y);
+                }
         }
     }
-    std::vector<int64_t> off((size_t)nT * nT, -1);
-    std::vector<uint32_t> bidx;
-    T.h_row_blk.assign(nT + 1, 0);
-    int64_t nb = 0;
-    for (int I = 0; I < nT; I++) {
-        T.h_row_blk[I] = nb;
+    // blocks: every lag of each tile-pair hull; sub-tiles stored where the lag is in range
+    std::vector<int> plo(npair, 1 << 30), phi(npair, -1);
+    std::vector<int64_t> off(npair, -1);
+    std::vector<uint32_t> sub_off;
+    std::vector<uint16_t> msk;
+    int64_t nb = 0, nsub = 0, nz = 0;
+    for (size_t q = 0; q < npair; q++) {
+        for (int u = 0; u < 9; u++) {
+            plo[q] = std::min(plo[q], slo[q * 9 + u]);
+            phi[q] = std::max(phi[q], shi[q * 9 + u]);
+        }
+        if (phi[q] < plo[q]) continue;
+        off[q] = nb;
+        for (int k = plo[q]; k <= phi[q]; k++) {
+            uint16_t mk = 0;
+            for (int u = 0; u < 9; u++)
+                if (slo[q * 9 + u] <= k && k <= shi[q * 9 + u]) mk |= (uint16_t)(1u << u);
+            sub_off.push_back((uint32_t)nsub);
+            msk.push_back(mk);
+            nsub += __builtin_popcount(mk);
+            nz += mk != 0;
+            nb++;
+        }
+    }
+    if (nsub >= (int64_t)UINT32_MAX || nb >= (int64_t)INT32_MAX)
+        throw std::runtime_error("tile store too large");
+    T.nblocks = nz;
+    T.nsub = nsub;
+    // merged (J, k) lists of the groups of 4 row tiles
+    std::vector<TPEntry> ent;
+    std::vector<int64_t> goff(T.nG + 1, 0);
+    for (int g = 0; g < T.nG; g++) {
+        goff[g] = (int64_t)ent.size();
         for (int J = 0; J < nT; J++) {
-            const size_t q = (size_t)I * nT + J;
-            if (phi[q] < plo[q]) continue;
-            off[q] = nb;
-            for (int k = plo[q]; k <= phi[q]; k++) {
-                bidx.push_back(((uint32_t)J << kKBits) | (uint32_t)k);
-                nb++;
+            int klo = 1 << 30, khi = -1;
+            for (int w = 0; w < kGroup; w++) {
+                const int I = g * kGroup + w;
+                if (I >= nT) continue;
+                const size_t q = (size_t)I * nT + J;
+                klo = std::min(klo, plo[q]);
+                khi = std::max(khi, phi[q]);
+            }
+            for (int k = klo; k <= khi; k++) {
+                TPEntry e;
+                e.id = ((uint32_t)J << kKBits) | (uint32_t)k;
+                e.kmask = 0;
+                bool any = false;
+                for (int w = 0; w < kGroup; w++) {
+                    e.sub[w] = 0;
+                    e.msk[w] = 0;
+                    const int I = g * kGroup + w;
+                    if (I >= nT) continue;
+                    const size_t q = (size_t)I * nT + J;
+                    if (off[q] < 0 || k < plo[q] || k > phi[q]) continue;
+                    const int64_t b = off[q] + (k - plo[q]);
+                    if (!msk[b]) continue;
+                    e.sub[w] = sub_off[b];
+                    e.msk[w] = msk[b];
+                    for (int m = 0; m < 3; m++) e.kmask |= (msk[b] >> (3 * m)) & 7u;
+                    any = true;
+                }
+                if (any) ent.push_back(e);
             }
         }
     }
-    T.h_row_blk[nT] = nb;
-    T.nblocks = nb;
+    goff[T.nG] = (int64_t)ent.size();
+    T.nentries = (int64_t)ent.size();
+    T.h_grp_off = goff;
+    // cost of an entry: its stored sub-tiles, the staged history k-tiles, a fixed part
+    T.h_work.resize(ent.size());
+    for (size_t e = 0; e < ent.size(); e++) {
+        int nsubs = 0;
+        for (int w = 0; w < kGroup; w++) nsubs = std::max(nsubs, __builtin_popcount(ent[e].msk[w]));
+        // the slowest wavefront of the step, the history staging, the step's latency
+        T.h_work[e] = (float)(nsubs + 0.5 * __builtin_popcount(ent[e].kmask) + 4.0);
+    }
+    T.plan_nrhs = 0;
     T.d_perm.upload(T.perm.data(), T.perm.size(), c.stream);
     T.d_pos.upload(T.pos.data(), T.pos.size(), c.stream);
-    T.bidx.upload(bidx.data(), bidx.size(), c.stream);
-    T.row_blk.upload(T.h_row_blk.data(), T.h_row_blk.size(), c.stream);
+    T.sub_off.upload(sub_off.data(), sub_off.size(), c.stream);
+    T.mask.upload(msk.data(), msk.size(), c.stream);
+    T.ent.upload(ent.data(), ent.size(), c.stream);
     DBuf<int64_t> d_off;
     DBuf<int> d_plo;
     d_off.upload(off.data(), off.size(), c.stream);
     d_plo.upload(plo.data(), plo.size(), c.stream);
-    T.vals.alloc((size_t)std::max<int64_t>(nb, 1) * kBlockElems);
+    T.vals.alloc((size_t)std::max<int64_t>(nsub, 1) * kSubElems);
     TD_HIP(hipMemsetAsync(T.vals.p, 0, T.vals.bytes(), c.stream));
     if (c.store.n_chunks > 0)
         hipLaunchKernelGGL(k_tp_fill, dim3((unsigned)c.store.n_chunks), dim3(kChunk), 0, c.stream,
                            c.store.vals.p, c.store.idx.p, c.store.row_chunk.p, c.store.row_valid.p,
-                           nnd, T.d_pos.p, nT, d_off.p, d_plo.p, T.vals.p);
+                           nnd, T.d_pos.p, nT, d_off.p, d_plo.p, T.sub_off.p, T.mask.p, T.vals.p);
     TD_HIP(hipGetLastError());
     TD_HIP(hipStreamSynchronize(c.stream));
     T.ready = true;
@@ -355,25 +448,68 @@ void tp_zsplit(Context& c, const double* ring, int N, int nrhs, int slot, unsign
 
 int64_t tp_zslot_elems(const Context& c, int nrhs) { return (int64_t)c.tp.nT * 9 * (nrhs / 16) * 512; }
 
-int tp_nsplit(const Context& c)
+void tp_plan(Context& c, int nrhs)
 {
-    // about 8192 wavefronts in flight over the row tiles
-    return std::max(1, std::min(64, 2048 / std::max(1, c.tp.nT)));
+    TPStore& T = c.tp;
+    if (T.plan_nrhs == nrhs) return;
+    const int NT = nrhs / 16;
+    int per_cu = 1, cus = 1;
+    const void* fn = nullptr;
+    switch (NT) {
+    case 1: fn = reinterpret_cast<const void*>(&k_conv_mixed<1>); break;
+    case 2: fn = reinterpret_cast<const void*>(&k_conv_mixed<2>); break;
+    case 3: fn = reinterpret_cast<const void*>(&k_conv_mixed<3>); break;
+    case 4: fn = reinterpret_cast<const void*>(&k_conv_mixed<4>); break;
+    default: throw std::invalid_argument("mixed march needs nrhs in {16, 32, 48, 64}");
+    }
+    TD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * kGroup, 0));
+    TD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
+    // two rounds of resident workgroups, shared among the groups by their work
+    const int target = std::max(T.nG, 2 * std::max(1, per_cu) * cus);
+    double total = 0;
+    for (float x : T.h_work) total += x;
+    std::vector<int4> ch;
+    std::vector<int> gch(T.nG + 1, 0);
+    for (int g = 0; g < T.nG; g++) {
+        gch[g] = (int)ch.size();
+        const int64_t e0 = T.h_grp_off[g], e1 = T.h_grp_off[g + 1];
+        double wg = 0;
+        for (int64_t e = e0; e < e1; e++) wg += T.h_work[e];
+        const int n = std::max(1, (int)std::lround(total > 0 ? target * wg / total : 1.0));
+        const double step = wg / n;
+        double acc = 0;
+        int64_t start = e0;
+        for (int64_t e = e0; e < e1; e++) {
+            acc += T.h_work[e];
+            // close the chunk once it holds its share (the last one takes the rest)
+            if (acc >= step * ((int)ch.size() - gch[g] + 1) && (int)ch.size() - gch[g] < n - 1) {
+                ch.push_back(make_int4(g, (int)start, (int)(e + 1), 0));
+                start = e + 1;
+            }
+        }
+        ch.push_back(make_int4(g, (int)start, (int)e1, 0));
+    }
+    gch[T.nG] = (int)ch.size();
+    T.nchunks = (int)ch.size();
+    T.chunk.upload(ch.data(), ch.size(), c.stream);
+    T.grp_chunk.upload(gch.data(), gch.size(), c.stream);
+    TD_HIP(hipStreamSynchronize(c.stream));
+    T.plan_nrhs = nrhs;
 }
 
 void tp_conv(Context& c, const unsigned short* zs, int nrhs, int Rring, int s, double* partial)
 {
     const TPStore& T = c.tp;
-    const int nsplit = tp_nsplit(c);
+    if (T.plan_nrhs != nrhs) throw std::logic_error("mixed convolution not planned for nrhs");
     const int NT = nrhs / 16;
-    dim3 grid(T.nT, nsplit);
+    const dim3 grid((unsigned)T.nchunks), block(64 * kGroup);
     const bf16x8* tpv = reinterpret_cast<const bf16x8*>(T.vals.p);
     const bf16x8* z = reinterpret_cast<const bf16x8*>(zs);
     switch (NT) {
-    case 1: hipLaunchKernelGGL(k_conv_mixed<1>, grid, dim3(64 * kSplitWaves), 0, c.stream, tpv, T.bidx.p, T.row_blk.p, z, T.nT, Rring, s, nsplit, partial); break;
-    case 2: hipLaunchKernelGGL(k_conv_mixed<2>, grid, dim3(64 * kSplitWaves), 0, c.stream, tpv, T.bidx.p, T.row_blk.p, z, T.nT, Rring, s, nsplit, partial); break;
-    case 3: hipLaunchKernelGGL(k_conv_mixed<3>, grid, dim3(64 * kSplitWaves), 0, c.stream, tpv, T.bidx.p, T.row_blk.p, z, T.nT, Rring, s, nsplit, partial); break;
-    case 4: hipLaunchKernelGGL(k_conv_mixed<4>, grid, dim3(64 * kSplitWaves), 0, c.stream, tpv, T.bidx.p, T.row_blk.p, z, T.nT, Rring, s, nsplit, partial); break;
+    case 1: hipLaunchKernelGGL(k_conv_mixed<1>, grid, block, 0, c.stream, tpv, T.ent.p, T.chunk.p, z, T.nT, Rring, s, partial); break;
+    case 2: hipLaunchKernelGGL(k_conv_mixed<2>, grid, block, 0, c.stream, tpv, T.ent.p, T.chunk.p, z, T.nT, Rring, s, partial); break;
+    case 3: hipLaunchKernelGGL(k_conv_mixed<3>, grid, block, 0, c.stream, tpv, T.ent.p, T.chunk.p, z, T.nT, Rring, s, partial); break;
+    case 4: hipLaunchKernelGGL(k_conv_mixed<4>, grid, block, 0, c.stream, tpv, T.ent.p, T.chunk.p, z, T.nT, Rring, s, partial); break;
     default: throw std::runtime_error("mixed march needs nrhs in {16, 32, 48, 64}");
     }
     TD_HIP(hipGetLastError());
@@ -382,17 +518,16 @@ void tp_conv(Context& c, const unsigned short* zs, int nrhs, int Rring, int s, d
 void tp_reduce(Context& c, const double* partial, int nrhs, double* hist, int N)
 {
     const TPStore& T = c.tp;
-    const int chunks = tp_nsplit(c) * kSplitWaves;
     const int NT = nrhs / 16;
     const int64_t n = (int64_t)T.nT * kRows * nrhs;
     hipLaunchKernelGGL(k_tp_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream,
-                       partial, T.nT, chunks, NT, T.d_perm.p, N, nrhs, hist);
+                       partial, T.nT, T.grp_chunk.p, NT, T.d_perm.p, N, nrhs, hist);
     TD_HIP(hipGetLastError());
 }
 
 int64_t tp_partial_elems(const Context& c, int nrhs)
 {
-    return (int64_t)c.tp.nT * tp_nsplit(c) * kSplitWaves * 3 * (nrhs / 16) * 256;
+    return (int64_t)c.tp.nchunks * kGroup * 3 * (nrhs / 16) * 256;
 }
 
 void mfma_selftest(const float* hA, const float* hB, float* hD)

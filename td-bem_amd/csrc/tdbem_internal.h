@@ -104,17 +104,38 @@ struct LagStore {
     int64_t nonzero_entries = 0;  // before padding
 };
 
-// Tile-pair store of the mixed-precision convolution (mixed.hip): nodes in Morton order, tiles
-// of 16 nodes, one dense 48 x 96 split-BF16 block per (row tile, column tile, lag).
+// One step of a tile group's merged list: column tile J and lag k (id = J << 10 | k); for each
+// of the group's 4 row tiles the first stored sub-tile of its block at (J, k) and the block's
+// sub-tile mask (0: no block); the k-tiles any of them uses.
+struct TPEntry {
+    uint32_t sub[4];
+    uint16_t msk[4];
+    uint32_t id;
+    uint32_t kmask;
+};
+
+// Tile-pair store of the mixed-precision convolution (mixed.hip): nodes ordered by recursive
+// coordinate bisection into tiles of 16; per (row tile, column tile, lag) a 48 x 96 block of
+// which only the non-zero 16 x 32 sub-tiles are stored, split-BF16, in MFMA operand order.
 struct TPStore {
-    int nT = 0;
+    int nT = 0, nG = 0;              // row tiles, groups of 4 row tiles
     std::vector<int> perm, pos;      // tile position -> node (-1 = padding); node -> position
     DBuf<int> d_perm, d_pos;
-    int64_t nblocks = 0;
-    DBuf<unsigned short> vals;       // [nblocks][3 terms][3][3][64][8] bf16
-    DBuf<uint32_t> bidx;             // [nblocks] (J << 10) | k
-    DBuf<int64_t> row_blk;           // [nT + 1]
-    std::vector<int64_t> h_row_blk;
+    int64_t nblocks = 0;             // blocks with a non-zero sub-tile
+    int64_t nsub = 0;                // stored sub-tiles
+    int64_t nentries = 0;            // merged-list entries over all groups
+    DBuf<unsigned short> vals;       // [nsub][3 terms][64][8] bf16
+    DBuf<uint32_t> sub_off;          // per block (all lags of each tile-pair hull): first sub-tile
+    DBuf<uint16_t> mask;             // per block: bit 3 m + kt set when sub-tile (m, kt) is stored
+    DBuf<TPEntry> ent;               // merged lists, group after group
+    std::vector<int64_t> h_grp_off;  // [nG + 1]
+    std::vector<float> h_work;       // per entry: relative cost (bytes moved)
+    // launch plan of k_conv_mixed for one right-hand-side count: one workgroup per chunk, chunks
+    // of equal work inside each group, as many chunks as workgroups resident at once
+    int plan_nrhs = 0;
+    int nchunks = 0;
+    DBuf<int4> chunk;                // (group, first entry, end entry, 0)
+    DBuf<int> grp_chunk;             // [nG + 1] chunks of each group
     bool ready = false;
 };
 
@@ -187,6 +208,8 @@ void march_free(Context& c);
 void tp_build(Context& c);
 void tp_zsplit(Context& c, const double* ring, int N, int nrhs, int slot, unsigned short* zs);
 int64_t tp_zslot_elems(const Context& c, int nrhs);
+// plans the chunks of k_conv_mixed for nrhs right-hand sides (after tp_build)
+void tp_plan(Context& c, int nrhs);
 int64_t tp_partial_elems(const Context& c, int nrhs);
 // k_conv_mixed: per-wave FP64 partial sums of the history at step s
 void tp_conv(Context& c, const unsigned short* zs, int nrhs, int Rring, int s, double* partial);

@@ -204,7 +204,7 @@ class Context:
 
     TIMING_KEYS = ("store_ms", "init_ms", "march_ms", "lag_kernel_ms", "gather_ms", "conv_ms",
                    "solve_ms", "lag_launches", "conv_launches", "active_points", "store_nonzero",
-                   "store_padded", "tile_blocks", "reduce_ms")
+                   "store_padded", "tile_subtiles", "reduce_ms", "tile_blocks", "tile_entries")
 
     def set_profiling(self, on=True):
         self._check(lib().tdbem_set_profiling(self._h, int(bool(on))))
