@@ -27,7 +27,8 @@ constexpr int NB = 64;          // panel width
 // argmax picks the pivot (ties keep the smaller row), the two rows are swapped inside the leaf,
 // and one pass over the rows below scales the column, updates the rest of the leaf and tracks the
 // largest |a| of the next column, so each column costs one sweep of m x kLeafW values (L2
-// resident).  Row swaps are applied to the other columns by k_laswp_range; the stored factors
+// resident).  Each leaf applies its row swaps to the other columns of its 64-column panel and
+// k_laswp to the columns outside the panel; the stored factors
 // satisfy P A = L U with L unit lower (below the diagonal) and U upper.
 // ---------------------------------------------------------------------------------------------
 constexpr int kLeafW = 8;
@@ -42,8 +43,27 @@ __device__ __forceinline__ void argmax_merge(double& v, int& i, double ov, int o
     }
 }
 
+// The leaf's row interchanges piv[c0 .. c0 + w) applied, in order, to the other columns of its
+// panel [j0, j0 + pnb) (one thread per column; piv is complete after the block barrier).
+__device__ __forceinline__ void leaf_panel_swaps(double* __restrict__ A, int n, int c0, int w,
+                                                 int j0, int pnb, const int* __restrict__ piv)
+{
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t >= pnb - w) return;
+    const int col = j0 + t + (j0 + t >= c0 ? w : 0);
+    for (int c = c0; c < c0 + w; c++) {
+        const int p = piv[c];
+        if (p != c) {
+            const double a = A[(int64_t)c * n + col];
+            A[(int64_t)c * n + col] = A[(int64_t)p * n + col];
+            A[(int64_t)p * n + col] = a;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kLeafThreads)
-    k_leaf(double* __restrict__ A, int n, int c0, int w, int* __restrict__ piv,
+    k_leaf(double* __restrict__ A, int n, int c0, int w, int j0, int pnb, int* __restrict__ piv,
            int* __restrict__ status)
 {
     __shared__ double sv[kLeafThreads / 64];
@@ -117,6 +137,114 @@ __global__ void __launch_bounds__(kLeafThreads)
         }
         __syncthreads();   // the column's updates visible before the next pivot row is read
     }
+    leaf_panel_swaps(A, n, c0, w, j0, pnb, piv);
+}
+
+// The same leaf with the m x kLeafW panel held in registers (thread t owns local rows
+// t + 1024 q, q < RPT): per column one block argmax, one exchange of the two swapped rows through
+// LDS (the new pivot row doubles as the broadcast row), then the update in registers.  The panel
+// is read and written once.
+template <int RPT>
+__global__ void __launch_bounds__(kLeafThreads)
+    k_leaf_reg(double* __restrict__ A, int n, int c0, int w, int j0, int pnb,
+               int* __restrict__ piv, int* __restrict__ status)
+{
+    __shared__ double sv[kLeafThreads / 64];
+    __shared__ int si[kLeafThreads / 64];
+    __shared__ double rowc[kLeafW], rowp[kLeafW];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int m = n - c0;
+    double x[RPT][kLeafW];
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int lr = tid + q * kLeafThreads;
+#pragma unroll
+        for (int j = 0; j < kLeafW; j++)
+            x[q][j] = (lr < m && j < w) ? A[(int64_t)(c0 + lr) * n + c0 + j] : 0.0;
+    }
+    for (int cc = 0; cc < w; cc++) {
+        const int c = c0 + cc;
+        double best = -1.0;
+        int bi = n;
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            const int lr = tid + q * kLeafThreads;
+            double v = 0.0;
+#pragma unroll
+            for (int j = 0; j < kLeafW; j++)
+                if (j == cc) v = x[q][j];
+            if (lr >= cc && lr < m) argmax_merge(best, bi, fabs(v), c0 + lr);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            argmax_merge(best, bi, __shfl_xor(best, off, 64), __shfl_xor(bi, off, 64));
+        if (lane == 0) {
+            sv[wid] = best;
+            si[wid] = bi;
+        }
+        __syncthreads();
+        double pm = sv[0];
+        int p = si[0];
+#pragma unroll
+        for (int q = 1; q < kLeafThreads / 64; q++) argmax_merge(pm, p, sv[q], si[q]);
+        if (tid == 0) {
+            piv[c] = p;
+            if (!(pm > 0.0)) *status = 1;
+        }
+        // rows cc (thread cc, q = 0) and lp = p - c0 trade places; rowp is the new pivot row
+        const int lp = p - c0, tp_ = lp % kLeafThreads, qp = lp / kLeafThreads;
+        if (tid == cc)
+#pragma unroll
+            for (int j = 0; j < kLeafW; j++) rowc[j] = x[0][j];
+        if (tid == tp_)
+#pragma unroll
+            for (int q = 0; q < RPT; q++)
+                if (q == qp)
+#pragma unroll
+                    for (int j = 0; j < kLeafW; j++) rowp[j] = x[q][j];
+        __syncthreads();
+        double u[kLeafW];
+#pragma unroll
+        for (int j = 0; j < kLeafW; j++) u[j] = rowp[j];
+        if (tid == tp_)
+#pragma unroll
+            for (int q = 0; q < RPT; q++)
+                if (q == qp)
+#pragma unroll
+                    for (int j = 0; j < kLeafW; j++) x[q][j] = rowc[j];
+        if (tid == cc)
+#pragma unroll
+            for (int j = 0; j < kLeafW; j++) x[0][j] = u[j];
+        double d = 0.0;
+#pragma unroll
+        for (int j = 0; j < kLeafW; j++)
+            if (j == cc) d = u[j];
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            const int lr = tid + q * kLeafThreads;
+            if (lr <= cc || lr >= m) continue;
+            double l = 0.0;
+#pragma unroll
+            for (int j = 0; j < kLeafW; j++)
+                if (j == cc) l = x[q][j] / d;
+#pragma unroll
+            for (int j = 0; j < kLeafW; j++) {
+                if (j == cc)
+                    x[q][j] = l;
+                else if (j > cc && j < w)
+                    x[q][j] -= l * u[j];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int lr = tid + q * kLeafThreads;
+        if (lr >= m) continue;
+#pragma unroll
+        for (int j = 0; j < kLeafW; j++)
+            if (j < w) A[(int64_t)(c0 + lr) * n + c0 + j] = x[q][j];
+    }
+    leaf_panel_swaps(A, n, c0, w, j0, pnb, piv);
 }
 
 // Row interchanges of panel [j0, j0 + nb) applied, in order, to the columns outside the panel.
@@ -135,61 +263,70 @@ __global__ void k_laswp(double* __restrict__ A, int n, int j0, int nb, const int
     }
 }
 
-// Row interchanges piv[p0 .. p1) applied, in order, to columns [ca, cb).
-__global__ void k_laswp_range(double* __restrict__ A, int n, int p0, int p1, int ca, int cb,
-                              const int* __restrict__ piv)
-{
-    const int col = ca + blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= cb) return;
-    for (int c = p0; c < p1; c++) {
-        const int p = piv[c];
-        if (p != c) {
-            const double a = A[(int64_t)c * n + col];
-            A[(int64_t)c * n + col] = A[(int64_t)p * n + col];
-            A[(int64_t)p * n + col] = a;
-        }
-    }
-}
-
-// X = L11^{-1} X for the nb rows starting at j0 (L unit lower, from the panel), columns
-// [c0, c0 + ncols) of a row-major matrix B with leading dimension ldb.  Thread per column.
-__global__ void k_trsm_lower_unit(const double* L, int ldl, int j0, int nb,
-                                  double* B, int ldb, int brow0, int c0, int ncols)
+// X = L11^{-1} X for the nb <= 64 rows starting at j0 (L unit lower, from the panel), columns
+// [c0, c0 + ncols) of a row-major matrix B with leading dimension ldb.  Thread per column; the
+// column is held in registers and L11 in LDS, both with their rows in reverse order, so forward
+// substitution runs as the back substitution of k_trsm_upper.  L and B may be one matrix but the
+// regions read through L and written through B do not overlap.
+__global__ void __launch_bounds__(128)
+    k_trsm_lower_unit(const double* __restrict__ L, int ldl, int j0, int nb,
+                      double* __restrict__ B, int ldb, int brow0, int c0, int ncols)
 {
     __shared__ double Ls[NB][NB + 1];
-    for (int t = threadIdx.x; t < nb * nb; t += blockDim.x) {
-        const int r = t / nb, s = t % nb;
-        Ls[r][s] = L[(int64_t)(j0 + r) * ldl + j0 + s];
+    for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+        const int r = t / NB, s = t % NB;
+        // Ls[r][s] = L11[nb-1-r][nb-1-s] for s > r (strictly lower part of L11), 1 on the
+        // diagonal, identity padding for rows past nb
+        Ls[r][s] = (s > r && s < nb) ? L[(int64_t)(j0 + nb - 1 - r) * ldl + j0 + nb - 1 - s]
+                                     : (r == s ? 1.0 : 0.0);
     }
     __syncthreads();
     const int col = c0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= c0 + ncols) return;
     double* xb = B + (int64_t)brow0 * ldb + col;
-    for (int r = 0; r < nb; r++) {
-        double v = xb[(int64_t)r * ldb];
-        for (int s = 0; s < r; s++) v -= Ls[r][s] * xb[(int64_t)s * ldb];
-        xb[(int64_t)r * ldb] = v;
+    double x[NB];
+#pragma unroll
+    for (int r = 0; r < NB; r++) x[r] = r < nb ? xb[(int64_t)(nb - 1 - r) * ldb] : 0.0;
+#pragma unroll
+    for (int r = NB - 1; r >= 0; r--) {
+        double v = x[r];
+#pragma unroll
+        for (int s = r + 1; s < NB; s++) v -= Ls[r][s] * x[s];
+        x[r] = v / Ls[r][r];
     }
+#pragma unroll
+    for (int r = 0; r < NB; r++)
+        if (r < nb) xb[(int64_t)(nb - 1 - r) * ldb] = x[r];
 }
 
-// X = U11^{-1} X (U upper with its diagonal) for the nb rows starting at j0.
-__global__ void k_trsm_upper(const double* __restrict__ U, int ldu, int j0, int nb,
-                             double* __restrict__ B, int ldb, int brow0, int c0, int ncols)
+// X = U11^{-1} X (U upper with its diagonal) for the nb <= 64 rows starting at j0.
+__global__ void __launch_bounds__(128)
+    k_trsm_upper(const double* __restrict__ U, int ldu, int j0, int nb, double* __restrict__ B,
+                 int ldb, int brow0, int c0, int ncols)
 {
     __shared__ double Us[NB][NB + 1];
-    for (int t = threadIdx.x; t < nb * nb; t += blockDim.x) {
-        const int r = t / nb, s = t % nb;
-        Us[r][s] = U[(int64_t)(j0 + r) * ldu + j0 + s];
+    for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+        const int r = t / NB, s = t % NB;
+        // identity outside the nb x nb block keeps the unrolled solve exact for nb < 64
+        Us[r][s] = (r < nb && s < nb) ? U[(int64_t)(j0 + r) * ldu + j0 + s] : (r == s ? 1.0 : 0.0);
     }
     __syncthreads();
     const int col = c0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= c0 + ncols) return;
     double* xb = B + (int64_t)brow0 * ldb + col;
-    for (int r = nb - 1; r >= 0; r--) {
-        double v = xb[(int64_t)r * ldb];
-        for (int s = r + 1; s < nb; s++) v -= Us[r][s] * xb[(int64_t)s * ldb];
-        xb[(int64_t)r * ldb] = v / Us[r][r];
+    double x[NB];
+#pragma unroll
+    for (int r = 0; r < NB; r++) x[r] = r < nb ? xb[(int64_t)r * ldb] : 0.0;
+#pragma unroll
+    for (int r = NB - 1; r >= 0; r--) {
+        double v = x[r];
+#pragma unroll
+        for (int s = r + 1; s < NB; s++) v -= Us[r][s] * x[s];
+        x[r] = v / Us[r][r];
     }
+#pragma unroll
+    for (int r = 0; r < NB; r++)
+        if (r < nb) xb[(int64_t)r * ldb] = x[r];
 }
 
 // C[m x n] -= A[m x k] * B[k x n], all row-major with leading dimensions.  64x64 output tile per
@@ -321,32 +458,33 @@ void solve_inplace(const double* LU, int n, double* B, int ncols, hipStream_t s)
 
 namespace {
 
-void laswp_range(double* A, int n, int p0, int p1, int ca, int cb, const int* piv, hipStream_t s)
-{
-    if (cb <= ca || p1 <= p0) return;
-    hipLaunchKernelGGL(k_laswp_range, dim3((cb - ca + 255) / 256), dim3(256), 0, s, A, n, p0, p1,
-                       ca, cb, piv);
-    TD_HIP(hipGetLastError());
-}
-
-// Factors columns [c0, c0 + w) over rows [c0, n); row swaps reach only these columns.
-void panel_rec(double* A, int n, int c0, int w, int* piv, int* status, hipStream_t s)
+// Factors columns [c0, c0 + w) over rows [c0, n) inside panel [j0, j0 + pnb); the leaves apply
+// their row swaps to every column of the panel.
+void panel_rec(double* A, int n, int c0, int w, int j0, int pnb, int* piv, int* status,
+               hipStream_t s)
 {
     if (w <= kLeafW) {
-        hipLaunchKernelGGL(k_leaf, dim3(1), dim3(kLeafThreads), 0, s, A, n, c0, w, piv, status);
+        const int rpt = (n - c0 + kLeafThreads - 1) / kLeafThreads;
+        const dim3 g1(1), b1(kLeafThreads);
+        switch (rpt) {
+        case 1: hipLaunchKernelGGL(k_leaf_reg<1>, g1, b1, 0, s, A, n, c0, w, j0, pnb, piv, status); break;
+        case 2: hipLaunchKernelGGL(k_leaf_reg<2>, g1, b1, 0, s, A, n, c0, w, j0, pnb, piv, status); break;
+        case 3: hipLaunchKernelGGL(k_leaf_reg<3>, g1, b1, 0, s, A, n, c0, w, j0, pnb, piv, status); break;
+        case 4: hipLaunchKernelGGL(k_leaf_reg<4>, g1, b1, 0, s, A, n, c0, w, j0, pnb, piv, status); break;
+        case 5: hipLaunchKernelGGL(k_leaf_reg<5>, g1, b1, 0, s, A, n, c0, w, j0, pnb, piv, status); break;
+        default: hipLaunchKernelGGL(k_leaf, g1, b1, 0, s, A, n, c0, w, j0, pnb, piv, status); break;
+        }
         TD_HIP(hipGetLastError());
         return;
     }
     const int h = (w / 2 + kLeafW - 1) / kLeafW * kLeafW;
-    panel_rec(A, n, c0, h, piv, status, s);
-    laswp_range(A, n, c0, c0 + h, c0 + h, c0 + w, piv, s);
+    panel_rec(A, n, c0, h, j0, pnb, piv, status, s);
     hipLaunchKernelGGL(k_trsm_lower_unit, dim3(1), dim3(64), 0, s, A, n, c0, h, A, n, c0, c0 + h,
                        w - h);
     TD_HIP(hipGetLastError());
     gemm_sub(n - c0 - h, w - h, h, A + (int64_t)(c0 + h) * n + c0, n,
              A + (int64_t)c0 * n + c0 + h, n, A + (int64_t)(c0 + h) * n + c0 + h, n, s);
-    panel_rec(A, n, c0 + h, w - h, piv, status, s);
-    laswp_range(A, n, c0 + h, c0 + w, c0, c0 + h, piv, s);
+    panel_rec(A, n, c0 + h, w - h, j0, pnb, piv, status, s);
 }
 
 }  // namespace
@@ -358,7 +496,7 @@ void lu_factor(double* A, int n, int* piv, hipStream_t s)
     TD_HIP(hipMemsetAsync(status.p, 0, sizeof(int), s));
     for (int j0 = 0; j0 < n; j0 += NB) {
         const int nb = std::min(NB, n - j0);
-        panel_rec(A, n, j0, nb, piv, status.p, s);
+        panel_rec(A, n, j0, nb, j0, nb, piv, status.p, s);
         if (n - nb > 0) {
             hipLaunchKernelGGL(k_laswp, dim3((n - nb + 255) / 256), dim3(256), 0, s, A, n, j0, nb,
                                piv);

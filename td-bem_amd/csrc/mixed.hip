@@ -174,7 +174,8 @@ __global__ void __launch_bounds__(64 * kGroup)
         int slot = (s - k) % Rring;
         if (slot < 0) slot += Rring;
         // every stored sub-tile of this wave's block and the history k-tiles it needs are
-        // requested before any is used; the history fragment is shared through L1/L2 with the
+        // requested before any is used; the store is read once per step (non-temporal, so it
+        // does not evict the history), the history fragment is shared through L1/L2 with the
         // other wavefronts of the group, which walk the same list
         const bf16x8* A = tp + (int64_t)first * (kSubElems / 8) + lane;
         const bf16x8* Zg = zs + (((int64_t)slot * nT + J) * 3) * 3 * NT * 64 + lane;
@@ -187,7 +188,7 @@ __global__ void __launch_bounds__(64 * kGroup)
         for (int u = 0; u < 9; u++)
             if ((msk >> u) & 1u) {
 #pragma unroll
-                for (int t = 0; t < 3; t++) a[u][t] = A[(q * 3 + t) * 64];
+                for (int t = 0; t < 3; t++) a[u][t] = __builtin_nontemporal_load(&A[(q * 3 + t) * 64]);
                 q++;
             }
 #pragma unroll
@@ -464,8 +465,8 @@ void tp_plan(Context& c, int nrhs)
     }
     TD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * kGroup, 0));
     TD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
-    // two rounds of resident workgroups, shared among the groups by their work
-    const int target = std::max(T.nG, 2 * std::max(1, per_cu) * cus);
+    // four rounds of resident workgroups, shared among the groups by their work
+    const int target = std::max(T.nG, 4 * std::max(1, per_cu) * cus);
     double total = 0;
     for (float x : T.h_work) total += x;
     std::vector<int4> ch;

@@ -4,12 +4,14 @@ This is the Python face of the same entry points the C++ host (td-bem_amd/host) 
 fails loudly when the HIP library is missing: there is no CPU fallback.
 """
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
 PKG = Path(__file__).resolve().parents[1]
-LIB_PATH = PKG / "libtdbem.so"
+# TDBEM_LIBRARY selects another build of the same library (same C ABI)
+LIB_PATH = Path(os.environ.get("TDBEM_LIBRARY", str(PKG / "libtdbem.so")))
 
 KERNEL_HEAVISIDE = 2
 KERNEL_STEP = 4
