@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrivals per second")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (3 launches) to scale HBM counters by")
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -129,6 +132,32 @@ def copy_calibration(api):
     ms = api.stream_copy(1 << 30, 3)
     return {"kernel": "k_stream_copy", "bytes_read": 1 << 30, "bytes_written": 1 << 30,
             "launches": 3, "avg_ms": round(ms, 4), "GBs": round(2 * (1 << 30) / ms / 1e6, 1)}
+
+
+def serve_leg(api, configs, serve, nreq, rate):
+    """Continuous-batching serving of mixed-precision solves (bar and sphere geometries of 72 to
+    864 elements, 100 to 300 steps, 16 or 32 load cases, Poisson arrivals), against the same
+    requests solved one after another on the plain FP64 path with one load case each."""
+    probs = configs.serving_mix(nreq, seed=5)
+    reqs = serve.poisson_requests(probs, rate=rate, seed=5)
+    res, summary = serve.Server(max_active=8).run(reqs)
+    t0 = time.perf_counter()
+    steps = 0
+    for p in probs:
+        with api.Context.from_problem(p) as ctx:
+            ctx.build_lag_store()
+            ctx.march_init(p.a1, p.a2, p.ubc_dof, p.ubc_val, p.fbc_dof, p.fbc_val)
+            ctx.march_run()
+            ctx.synchronize()
+        steps += p.numstep - 1
+    serial = time.perf_counter() - t0
+    summary.update({"arrival_rate_per_s": rate,
+                    "elements": [int(p.nel) for p in probs],
+                    "plain_fp64_serial": {"time_steps": steps, "seconds": round(serial, 3),
+                                          "time_steps_per_s": round(steps / serial, 2)}})
+    summary["speedup_vs_plain_fp64_serial"] = round(summary["time_steps_per_s"] /
+                                                    (steps / serial), 2)
+    return summary
 
 
 def roofline(kernel, avg_ms, nbytes, traffic=None):
@@ -236,6 +265,10 @@ def main():
             out["ms_per_step"] = out["plain_fp64"]["ms_per_step"]
             out["dtype"] = "f64"
             out["roofline"] = out["plain_fp64"]["roofline"]
+    if not args.no_serve:
+        log(f"serving leg: {args.serve_requests} requests at {args.serve_rate}/s")
+        from tdbem import serve
+        out["serving"] = serve_leg(api, configs, serve, args.serve_requests, args.serve_rate)
     if args.copy_calibration:
         log("copy calibration kernel")
         out["copy_calibration"] = copy_calibration(api)

@@ -129,3 +129,18 @@ def halfspace_scenarios(count=64, seed=4, el_range=(400, 4000), step_range=(100,
                            np.zeros(0), fdof, fval,
                            meta={"h": h, "radius": radius, "depth": depth}))
     return out
+
+
+def serving_mix(count=16, seed=5, step_range=(100, 300)):
+    """Solve requests for the serving loop, drawn from the bar and sphere geometries at several
+    sizes (72 to 864 elements), step counts U[step_range], 16 or 32 load cases each."""
+    rng = np.random.default_rng(seed)
+    shapes = [("bar", 0.5), ("bar", 0.25), ("sphere", 8), ("sphere", 10), ("sphere", 12)]
+    out = []
+    for _ in range(count):
+        kind, size = shapes[int(rng.integers(len(shapes)))]
+        steps = int(rng.integers(step_range[0], step_range[1] + 1))
+        p = bar(h=size, numstep=steps) if kind == "bar" else sphere(n=size, numstep=steps)
+        p.meta["nrhs"] = int(rng.choice([16, 32]))
+        out.append(p)
+    return out

@@ -143,15 +143,17 @@ def summarize(results, makespan):
             "latency_p99_s": round(float(np.percentile(lat, 99)), 3)}
 
 
-def poisson_requests(problems, rate, nrhs, seed=5, precision=api.PRECISION_MIXED):
+def poisson_requests(problems, rate, nrhs=None, seed=5, precision=api.PRECISION_MIXED):
     """Requests over the given problems with exponential inter-arrival times (rate per second),
-    per-request load amplitudes U[0.5, 2] and rise times U[0, 10 dt]."""
+    per-request load amplitudes U[0.5, 2] and rise times U[0, 10 dt]; nrhs None takes each
+    problem's meta["nrhs"]."""
     rng = np.random.default_rng(seed)
     t = 0.0
     reqs = []
     for i, p in enumerate(problems):
         t += rng.exponential(1.0 / rate)
-        amp = rng.uniform(0.5, 2.0, nrhs)
-        rise = np.where(rng.uniform(size=nrhs) < 0.5, 0.0, rng.uniform(0, 10 * p.dt, nrhs))
+        n = nrhs if nrhs is not None else int(p.meta.get("nrhs", 16))
+        amp = rng.uniform(0.5, 2.0, n)
+        rise = np.where(rng.uniform(size=n) < 0.5, 0.0, rng.uniform(0, 10 * p.dt, n))
         reqs.append(Request(i, p, t, amp, rise, precision))
     return reqs
