@@ -122,6 +122,23 @@ int tdbem_stream_copy(size_t bytes, int launches, double* avg_ms);
 /* Waits for all work queued on the context's stream. */
 int tdbem_synchronize(tdbem_context* ctx);
 
+/* Batched march (no counterpart in the reference, which runs one problem per process): many
+ * independent contexts, each after a mixed-precision tdbem_march_init with the same nrhs, are
+ * stepped together, one step of all of them per seven kernel launches, captured as a hipGraph of
+ * steps_per_graph steps that is replayed until every context reaches its own last step.  The
+ * contexts must stay alive and must not be marched on their own while the batch exists; their
+ * histories (tdbem_get_history) hold the results.  tdbem_batch_tune times the tunables (workgroup
+ * rounds of the convolution, wavefronts of the step solve) on the batch and keeps the fastest;
+ * report (may be NULL) receives {rounds, solve waves, convolution ms, solve ms}.
+ * tdbem_batch_run returns the number of graph launches in *launches (may be NULL); max_replays
+ * <= 0 runs to the end. */
+typedef struct tdbem_batch tdbem_batch;
+int tdbem_batch_create(tdbem_context* const* ctxs, int count, tdbem_batch** out);
+int tdbem_batch_tune(tdbem_batch* b, double* report);
+int tdbem_batch_set_params(tdbem_batch* b, int rounds, int solve_waves);
+int tdbem_batch_run(tdbem_batch* b, int steps_per_graph, int max_replays, int* launches);
+int tdbem_batch_destroy(tdbem_batch* b);
+
 /* Instrumentation (no counterpart in the reference, which has no timers): when on, the lag
  * kernel, the store gather, the history convolution and the step solve are bracketed with HIP
  * events on the context stream.  tdbem_get_timings fills out[TDBEM_NUM_TIMINGS] in this order:

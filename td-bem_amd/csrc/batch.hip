@@ -1,0 +1,422 @@
+// batch.hip — many independent time-marching scenarios (uneven DOF and step counts) stepped
+// together.  Each scenario is a context with a mixed-precision march; one step of every scenario
+// is seven launches in total, whatever the number of scenarios:
+//   advance + load factors, split-BF16 history copy, MFMA convolution, reduction, right-hand
+//   sides, step solve, finalize,
+// each launch covering all scenarios through a table of per-scenario descriptors and a device
+// step counter per scenario (scenarios past their last step drop out).  A run captures
+// `steps_per_graph` steps of this sequence into one hipGraph and replays it, so one graph launch
+// advances every scenario by that many steps.  The convolution work of all scenarios is cut into
+// one work-balanced chunk table (one workgroup per chunk); the number of workgroup rounds and the
+// width of the step solve's column splits are picked by batch_tune, which times the candidates on
+// the batch itself.
+#include <algorithm>
+#include <chrono>
+#include <memory>
+#include <stdexcept>
+#include <vector>
+
+#include "step_device.h"
+
+namespace tdbem {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+// scenario of block b: the last sc with off[sc] <= b (off[0] = 0, off[S] = total blocks)
+__device__ __forceinline__ int scen_of(const int* __restrict__ off, int S, int b)
+{
+    int lo = 0, hi = S;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= b)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Advances every scenario with steps left (s + 1 < numstep) and forms its load factors.
+__global__ void k_b_step(const BatchScen* __restrict__ D, int* __restrict__ sdev,
+                         int* __restrict__ act)
+{
+    const int sc = blockIdx.x;
+    const BatchScen& d = D[sc];
+    __shared__ int s_new, a;
+    if (threadIdx.x == 0) {
+        int s = sdev[sc];
+        a = s + 1 < d.numstep;
+        if (a) {
+            s += 1;
+            sdev[sc] = s;
+        }
+        act[sc] = a;
+        s_new = s;
+    }
+    __syncthreads();
+    const int r = threadIdx.x;
+    if (a && r < d.nrhs) d.fac[r] = load_factor(d.amp[r], d.rise[r], s_new * d.dt);
+}
+
+// The three ring slots s - 2 .. s changed by the previous step, in split-BF16 operand order.
+__global__ void k_b_zsplit(const BatchScen* __restrict__ D, const int* __restrict__ off, int S,
+                           const int* __restrict__ sdev, const int* __restrict__ act)
+{
+    const int sc = scen_of(off, S, blockIdx.x);
+    if (!act[sc]) return;
+    const BatchScen& d = D[sc];
+    const int64_t t = (int64_t)(blockIdx.x - off[sc]) * kThreads + threadIdx.x;
+    const int64_t per = (int64_t)d.nT * kCols * d.nrhs;
+    if (t >= 3 * per) return;
+    const int m = (int)(t / per);
+    int slot = (sdev[sc] - 2 + m) % d.R;
+    if (slot < 0) slot += d.R;
+    zsplit_elem(d.ring, d.N, d.nrhs, d.nT, d.perm, slot, d.nrhs / 16, d.zs, t - m * per);
+}
+
+// One workgroup per chunk (chunk.w = scenario), one wavefront per row tile of its group.
+template <int NT>
+__global__ void __launch_bounds__(64 * kGroup)
+    k_b_conv(const BatchScen* __restrict__ D, const int4* __restrict__ chunk,
+             const int* __restrict__ sdev, const int* __restrict__ act, double* __restrict__ partial)
+{
+    const int4 cw = chunk[blockIdx.x];
+    if (!act[cw.w]) return;
+    const BatchScen& d = D[cw.w];
+    const int w = threadIdx.x >> 6;
+    conv_chunk<NT>(reinterpret_cast<const bf16x8*>(d.tp), d.ent,
+                   reinterpret_cast<const bf16x8*>(d.zs), d.nT, d.R, sdev[cw.w], cw.y, cw.z, w,
+                   cw.x * kGroup + w < d.nT,
+                   partial + (((int64_t)blockIdx.x * kGroup + w) * 3 * NT) * 256);
+}
+
+__global__ void k_b_reduce(const BatchScen* __restrict__ D, const int* __restrict__ off, int S,
+                           const int* __restrict__ act, const double* __restrict__ partial,
+                           const int* __restrict__ gch)
+{
+    const int sc = scen_of(off, S, blockIdx.x);
+    if (!act[sc]) return;
+    const BatchScen& d = D[sc];
+    const int64_t t = (int64_t)(blockIdx.x - off[sc]) * kThreads + threadIdx.x;
+    if (t >= (int64_t)d.nT * kRows * d.nrhs) return;
+    reduce_elem(partial, gch + d.gch0, d.nrhs / 16, d.perm, d.N, d.nrhs, d.hist, t);
+}
+
+__global__ void k_b_rhs(const BatchScen* __restrict__ D, const int* __restrict__ off, int S,
+                        const int* __restrict__ act)
+{
+    const int sc = scen_of(off, S, blockIdx.x);
+    if (!act[sc]) return;
+    const BatchScen& d = D[sc];
+    const int64_t t = (int64_t)(blockIdx.x - off[sc]) * kThreads + threadIdx.x;
+    if (t < (int64_t)d.N * d.nrhs) rhs_elem(d.gF, d.gU, d.hist, d.fac, d.N, d.Xf, t);
+}
+
+template <int NT>
+__global__ void __launch_bounds__(64 * kApplyWaves)
+    k_b_apply(const BatchScen* __restrict__ D, const int* __restrict__ off, int S,
+              const int* __restrict__ act)
+{
+    const int sc = scen_of(off, S, blockIdx.x);
+    if (!act[sc]) return;
+    const BatchScen& d = D[sc];
+    const int g = (blockIdx.x - off[sc]) * kApplyWaves + (threadIdx.x >> 6);
+    if (g < frag_rows(d.N) / 16 * d.nsplit)
+        apply_wave<NT>(d.Af, d.N, d.Xf, d.nrhs, d.nsplit, g, d.parts);
+}
+
+__global__ void k_b_finalize(const BatchScen* __restrict__ D, const int* __restrict__ off, int S,
+                             const int* __restrict__ sdev, const int* __restrict__ act)
+{
+    const int sc = scen_of(off, S, blockIdx.x);
+    if (!act[sc]) return;
+    const BatchScen& d = D[sc];
+    const int64_t t = (int64_t)(blockIdx.x - off[sc]) * kThreads + threadIdx.x;
+    if (t < (int64_t)d.N * d.nrhs)
+        finalize_elem(d.parts, d.nsplit, d.Np, d.is_ubc, d.is_fbc, d.ubc_val, d.fbc_val, d.fac,
+                      d.N, d.nrhs, sdev[sc], d.R, d.a1, d.a2, d.a3, d.U, d.T, d.ring, t);
+}
+
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+struct Batch {
+    std::vector<Context*> ctx;
+    int S = 0, nrhs = 0, device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<BatchScen> h_desc;
+    DBuf<BatchScen> desc;
+    DBuf<int> sdev, act;
+    std::vector<std::unique_ptr<DBuf<double>>> parts;
+    DBuf<int> off_z, off_r, off_rhs, off_a;
+    int nb_z = 0, nb_r = 0, nb_rhs = 0, nb_a = 0;
+    DBuf<int4> chunk;
+    DBuf<int> gch;
+    DBuf<double> partial;
+    int nchunks = 0;
+    int rounds = 4, apply_waves = 4096;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_steps = 0;
+
+    ~Batch()
+    {
+        drop_graph();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void drop_graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graph_steps = 0;
+    }
+};
+
+namespace {
+
+// chunk table, partial sums, step-solve splits and block tables for the current tunables
+void batch_plan(Batch& B)
+{
+    B.drop_graph();
+    const int NT = B.nrhs / 16;
+    const int target = B.rounds * tp_resident_groups(B.nrhs, B.device);
+    std::vector<double> work(B.S);
+    double total = 0;
+    for (int sc = 0; sc < B.S; sc++) total += work[sc] = tp_work(*B.ctx[sc]);
+    std::vector<int4> ch;
+    std::vector<int> gch;
+    std::vector<int> oz(B.S + 1, 0), orr(B.S + 1, 0), orhs(B.S + 1, 0), oa(B.S + 1, 0);
+    for (int sc = 0; sc < B.S; sc++) {
+        Context& c = *B.ctx[sc];
+        BatchScen& d = B.h_desc[sc];
+        d.gch0 = (int)gch.size();
+        const int share = total > 0 ? (int)std::lround(target * work[sc] / total) : 1;
+        tp_chunks(c, share, sc, ch, gch);
+        d.nsplit = apply_splits(d.N, B.apply_waves);
+        const size_t need = (size_t)d.nsplit * d.nrhs * d.Np;
+        if (!B.parts[sc]) B.parts[sc].reset(new DBuf<double>());
+        B.parts[sc]->alloc(need);
+        d.parts = B.parts[sc]->p;
+        oz[sc + 1] = oz[sc] + (int)ceil_div(3LL * d.nT * kCols * d.nrhs, kThreads);
+        orr[sc + 1] = orr[sc] + (int)ceil_div((int64_t)d.nT * kRows * d.nrhs, kThreads);
+        orhs[sc + 1] = orhs[sc] + (int)ceil_div((int64_t)d.N * d.nrhs, kThreads);
+        oa[sc + 1] = oa[sc] + (int)ceil_div((int64_t)frag_rows(d.N) / 16 * d.nsplit, kApplyWaves);
+    }
+    B.nchunks = (int)ch.size();
+    B.chunk.upload(ch.data(), ch.size(), B.stream);
+    B.gch.upload(gch.data(), gch.size(), B.stream);
+    B.partial.alloc((size_t)B.nchunks * kGroup * 3 * NT * 256);
+    B.off_z.upload(oz.data(), oz.size(), B.stream);
+    B.off_r.upload(orr.data(), orr.size(), B.stream);
+    B.off_rhs.upload(orhs.data(), orhs.size(), B.stream);
+    B.off_a.upload(oa.data(), oa.size(), B.stream);
+    B.nb_z = oz[B.S];
+    B.nb_r = orr[B.S];
+    B.nb_rhs = orhs[B.S];
+    B.nb_a = oa[B.S];
+    B.desc.upload(B.h_desc.data(), B.h_desc.size(), B.stream);
+    TD_HIP(hipStreamSynchronize(B.stream));
+}
+
+void launch_conv(Batch& B)
+{
+    const dim3 g((unsigned)B.nchunks), b(64 * kGroup);
+    switch (B.nrhs / 16) {
+    case 1: hipLaunchKernelGGL(k_b_conv<1>, g, b, 0, B.stream, B.desc.p, B.chunk.p, B.sdev.p, B.act.p, B.partial.p); break;
+    case 2: hipLaunchKernelGGL(k_b_conv<2>, g, b, 0, B.stream, B.desc.p, B.chunk.p, B.sdev.p, B.act.p, B.partial.p); break;
+    case 3: hipLaunchKernelGGL(k_b_conv<3>, g, b, 0, B.stream, B.desc.p, B.chunk.p, B.sdev.p, B.act.p, B.partial.p); break;
+    case 4: hipLaunchKernelGGL(k_b_conv<4>, g, b, 0, B.stream, B.desc.p, B.chunk.p, B.sdev.p, B.act.p, B.partial.p); break;
+    default: throw std::invalid_argument("batch: nrhs must be 16, 32, 48 or 64");
+    }
+    TD_HIP(hipGetLastError());
+}
+
+void launch_apply(Batch& B)
+{
+    const dim3 g((unsigned)B.nb_a), b(64 * kApplyWaves);
+    switch (B.nrhs / 16) {
+    case 1: hipLaunchKernelGGL(k_b_apply<1>, g, b, 0, B.stream, B.desc.p, B.off_a.p, B.S, B.act.p); break;
+    case 2: hipLaunchKernelGGL(k_b_apply<2>, g, b, 0, B.stream, B.desc.p, B.off_a.p, B.S, B.act.p); break;
+    case 3: hipLaunchKernelGGL(k_b_apply<3>, g, b, 0, B.stream, B.desc.p, B.off_a.p, B.S, B.act.p); break;
+    case 4: hipLaunchKernelGGL(k_b_apply<4>, g, b, 0, B.stream, B.desc.p, B.off_a.p, B.S, B.act.p); break;
+    default: throw std::invalid_argument("batch: nrhs must be 16, 32, 48 or 64");
+    }
+    TD_HIP(hipGetLastError());
+}
+
+// one step of every scenario with steps left
+void launch_step(Batch& B)
+{
+    hipLaunchKernelGGL(k_b_step, dim3(B.S), dim3(64), 0, B.stream, B.desc.p, B.sdev.p, B.act.p);
+    hipLaunchKernelGGL(k_b_zsplit, dim3(B.nb_z), dim3(kThreads), 0, B.stream, B.desc.p,
+                       B.off_z.p, B.S, B.sdev.p, B.act.p);
+    launch_conv(B);
+    hipLaunchKernelGGL(k_b_reduce, dim3(B.nb_r), dim3(kThreads), 0, B.stream, B.desc.p,
+                       B.off_r.p, B.S, B.act.p, B.partial.p, B.gch.p);
+    hipLaunchKernelGGL(k_b_rhs, dim3(B.nb_rhs), dim3(kThreads), 0, B.stream, B.desc.p,
+                       B.off_rhs.p, B.S, B.act.p);
+    launch_apply(B);
+    hipLaunchKernelGGL(k_b_finalize, dim3(B.nb_rhs), dim3(kThreads), 0, B.stream, B.desc.p,
+                       B.off_rhs.p, B.S, B.sdev.p, B.act.p);
+    TD_HIP(hipGetLastError());
+}
+
+// device step counters and activity flags from the contexts' host step indices
+void batch_sync_state(Batch& B)
+{
+    std::vector<int> s(B.S), a(B.S);
+    for (int sc = 0; sc < B.S; sc++) {
+        s[sc] = march_step_count(*B.ctx[sc]);
+        a[sc] = s[sc] + 1 < B.ctx[sc]->numstep;
+    }
+    TD_HIP(hipMemcpyAsync(B.sdev.p, s.data(), B.S * sizeof(int), hipMemcpyHostToDevice, B.stream));
+    TD_HIP(hipMemcpyAsync(B.act.p, a.data(), B.S * sizeof(int), hipMemcpyHostToDevice, B.stream));
+    TD_HIP(hipStreamSynchronize(B.stream));
+}
+
+template <typename F>
+double time_launches(Batch& B, F&& launch, int reps)
+{
+    launch();  // warm-up
+    hipEvent_t e0, e1;
+    TD_HIP(hipEventCreate(&e0));
+    TD_HIP(hipEventCreate(&e1));
+    TD_HIP(hipEventRecord(e0, B.stream));
+    for (int r = 0; r < reps; r++) launch();
+    TD_HIP(hipEventRecord(e1, B.stream));
+    TD_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    TD_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms / reps;
+}
+
+}  // namespace
+
+Batch* batch_create(const std::vector<Context*>& ctxs)
+{
+    if (ctxs.empty()) throw std::invalid_argument("batch: no contexts");
+    std::unique_ptr<Batch> B(new Batch());
+    B->ctx = ctxs;
+    B->S = (int)ctxs.size();
+    B->device = ctxs[0]->device;
+    B->nrhs = 0;
+    for (Context* c : ctxs) {
+        if (!c) throw std::invalid_argument("batch: null context");
+        if (c->device != B->device) throw std::invalid_argument("batch: contexts on different GPUs");
+        BatchScen d{};
+        march_describe(*c, d);
+        if (B->nrhs == 0) B->nrhs = d.nrhs;
+        if (d.nrhs != B->nrhs) throw std::invalid_argument("batch: contexts differ in nrhs");
+        TD_HIP(hipStreamSynchronize(c->stream));
+    }
+    TD_HIP(hipSetDevice(B->device));
+    TD_HIP(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+    // step 1 (the n == 2 solve) of scenarios that have not started, on their own streams
+    for (Context* c : ctxs)
+        if (march_step_count(*c) == 0) march_run(*c, 1);
+    B->h_desc.resize(B->S);
+    B->parts.resize(B->S);
+    for (int sc = 0; sc < B->S; sc++) {
+        Context& c = *ctxs[sc];
+        BatchScen& d = B->h_desc[sc];
+        march_describe(c, d);
+        d.tp = c.tp.vals.p;
+        d.ent = c.tp.ent.p;
+        d.perm = c.tp.d_perm.p;
+        d.nT = c.tp.nT;
+    }
+    B->sdev.alloc(B->S);
+    B->act.alloc(B->S);
+    batch_plan(*B);
+    return B.release();
+}
+
+void batch_destroy(Batch* B) { delete B; }
+
+void batch_tune(Batch& B, double* report)
+{
+    batch_sync_state(B);
+    const int rounds_c[] = {1, 2, 4, 8};
+    const int waves_c[] = {1024, 2048, 4096, 8192};
+    double best_conv = 1e300, best_apply = 1e300;
+    int best_r = B.rounds, best_w = B.apply_waves;
+    for (int r : rounds_c) {
+        B.rounds = r;
+        batch_plan(B);
+        const double ms = time_launches(B, [&] { launch_conv(B); }, 3);
+        if (ms < best_conv) {
+            best_conv = ms;
+            best_r = r;
+        }
+    }
+    B.rounds = best_r;
+    for (int w : waves_c) {
+        B.apply_waves = w;
+        batch_plan(B);
+        const double ms = time_launches(B, [&] { launch_apply(B); }, 3);
+        if (ms < best_apply) {
+            best_apply = ms;
+            best_w = w;
+        }
+    }
+    B.apply_waves = best_w;
+    batch_plan(B);
+    if (report) {
+        report[0] = best_r;
+        report[1] = best_w;
+        report[2] = best_conv;
+        report[3] = best_apply;
+    }
+}
+
+void batch_set_params(Batch& B, int rounds, int apply_waves)
+{
+    if (rounds < 1 || rounds > 64 || apply_waves < 64 || apply_waves > (1 << 20))
+        throw std::invalid_argument("batch: tunables out of range");
+    B.rounds = rounds;
+    B.apply_waves = apply_waves;
+    batch_plan(B);
+}
+
+int batch_run(Batch& B, int steps_per_graph, int max_replays)
+{
+    if (steps_per_graph < 1 || steps_per_graph > 4096)
+        throw std::invalid_argument("batch: steps_per_graph out of range");
+    batch_sync_state(B);
+    int remaining = 0;
+    for (Context* c : B.ctx)
+        remaining = std::max(remaining, c->numstep - 1 - march_step_count(*c));
+    if (remaining <= 0) return 0;
+    if (!B.exec || B.graph_steps != steps_per_graph) {
+        B.drop_graph();
+        TD_HIP(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int k = 0; k < steps_per_graph; k++) launch_step(B);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(B.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        TD_HIP(hipStreamEndCapture(B.stream, &B.graph));
+        TD_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+        B.graph_steps = steps_per_graph;
+    }
+    int replays = (int)ceil_div(remaining, steps_per_graph);
+    if (max_replays > 0) replays = std::min(replays, max_replays);
+    for (int r = 0; r < replays; r++) TD_HIP(hipGraphLaunch(B.exec, B.stream));
+    TD_HIP(hipStreamSynchronize(B.stream));
+    std::vector<int> s(B.S);
+    TD_HIP(hipMemcpy(s.data(), B.sdev.p, B.S * sizeof(int), hipMemcpyDeviceToHost));
+    for (int sc = 0; sc < B.S; sc++) march_set_step(*B.ctx[sc], s[sc]);
+    return replays;
+}
+
+}  // namespace tdbem

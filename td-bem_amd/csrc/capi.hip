@@ -1,14 +1,21 @@
 // capi.hip — the C ABI of include/tdbem.h over the HIP implementation.
 #include <cmath>
+#include <memory>
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/tdbem.h"
 #include "tdbem_internal.h"
 
 struct tdbem_context {
     tdbem::Context c;
+};
+
+struct tdbem_batch {
+    tdbem::Batch* b = nullptr;
+    ~tdbem_batch() { tdbem::batch_destroy(b); }
 };
 
 namespace tdbem {
@@ -166,6 +173,26 @@ double stream_copy(size_t bytes, int launches)
 }  // namespace tdbem
 
 using namespace tdbem;
+
+namespace {
+template <typename F>
+int batch_guarded(F&& f)
+{
+    try {
+        f();
+        return TDBEM_OK;
+    } catch (const std::invalid_argument& e) {
+        g_create_error = e.what();
+        return TDBEM_ERR_ARG;
+    } catch (const std::logic_error& e) {
+        g_create_error = e.what();
+        return TDBEM_ERR_STATE;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        return TDBEM_ERR_HIP;
+    }
+}
+}  // namespace
 
 extern "C" {
 
@@ -464,6 +491,50 @@ int tdbem_get_timings(tdbem_context* ctx, double* out)
             t.reduce_ms, (double)c.tp.nblocks, (double)c.tp.nentries};
         std::memcpy(out, v, sizeof(v));
     });
+}
+
+
+int tdbem_batch_create(tdbem_context* const* ctxs, int count, tdbem_batch** out)
+{
+    if (!ctxs || count < 1 || !out) return TDBEM_ERR_ARG;
+    *out = nullptr;
+    return batch_guarded([&] {
+        std::vector<Context*> v;
+        for (int i = 0; i < count; i++) {
+            if (!ctxs[i]) throw std::invalid_argument("batch: null context");
+            v.push_back(&ctxs[i]->c);
+        }
+        std::unique_ptr<tdbem_batch> b(new tdbem_batch());
+        b->b = batch_create(v);
+        *out = b.release();
+    });
+}
+
+int tdbem_batch_tune(tdbem_batch* b, double* report)
+{
+    if (!b) return TDBEM_ERR_ARG;
+    return batch_guarded([&] { batch_tune(*b->b, report); });
+}
+
+int tdbem_batch_set_params(tdbem_batch* b, int rounds, int solve_waves)
+{
+    if (!b) return TDBEM_ERR_ARG;
+    return batch_guarded([&] { batch_set_params(*b->b, rounds, solve_waves); });
+}
+
+int tdbem_batch_run(tdbem_batch* b, int steps_per_graph, int max_replays, int* launches)
+{
+    if (!b) return TDBEM_ERR_ARG;
+    return batch_guarded([&] {
+        const int n = batch_run(*b->b, steps_per_graph, max_replays);
+        if (launches) *launches = n;
+    });
+}
+
+int tdbem_batch_destroy(tdbem_batch* b)
+{
+    delete b;
+    return TDBEM_OK;
 }
 
 }  // extern "C"

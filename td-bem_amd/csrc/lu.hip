@@ -13,7 +13,7 @@
 #include <stdexcept>
 #include <vector>
 
-#include "tdbem_internal.h"
+#include "step_device.h"
 
 namespace tdbem {
 
@@ -548,77 +548,6 @@ void lu_inverse(const double* LU, int n, const int* piv, double* Ainv, hipStream
     TD_HIP(hipStreamSynchronize(s));
 }
 
-namespace {
-
-// Y[r][i] = sum_j A[i][j] X[r][j]: 32 rows of A per 256-thread block, A and X staged in LDS in
-// 64-column slabs, each thread owns (row, rhs) outputs.  A is read once from HBM.
-constexpr int GR = 32, GC = 64;
-template <int NR>
-__global__ void __launch_bounds__(256)
-    k_gemm_rhs(const double* __restrict__ A, int n, const double* __restrict__ X,
-               double* __restrict__ Y)
-{
-    __shared__ double As[GR][GC + 1];
-    __shared__ double Xs[NR][GC + 1];
-    const int row0 = blockIdx.x * GR;
-    constexpr int per = (GR * NR + 255) / 256;
-    double acc[per];
-#pragma unroll
-    for (int u = 0; u < per; u++) acc[u] = 0.0;
-    for (int c0 = 0; c0 < n; c0 += GC) {
-        for (int t = threadIdx.x; t < GR * GC; t += 256) {
-            const int r = t / GC, cc = t % GC;
-            As[r][cc] = (row0 + r < n && c0 + cc < n) ? A[(int64_t)(row0 + r) * n + c0 + cc] : 0.0;
-        }
-        for (int t = threadIdx.x; t < NR * GC; t += 256) {
-            const int r = t / GC, cc = t % GC;
-            Xs[r][cc] = (c0 + cc < n) ? X[(int64_t)r * n + c0 + cc] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < per; u++) {
-            const int o = threadIdx.x + 256 * u;
-            if (o < GR * NR) {
-                const int r = o % GR, q = o / GR;
-                double s = 0.0;
-#pragma unroll 8
-                for (int cc = 0; cc < GC; cc++) s += As[r][cc] * Xs[q][cc];
-                acc[u] += s;
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int u = 0; u < per; u++) {
-        const int o = threadIdx.x + 256 * u;
-        if (o < GR * NR) {
-            const int r = o % GR, q = o / GR;
-            if (row0 + r < n) Y[(int64_t)q * n + row0 + r] = acc[u];
-        }
-    }
-}
-
-}  // namespace
-
-void gemm_rhs(const double* A, int n, const double* X, double* Y, int nrhs, hipStream_t s)
-{
-    const dim3 grid((n + GR - 1) / GR);
-    switch (nrhs) {
-    case 1: gemv(A, n, X, Y, 0.0, s); return;
-    case 2: hipLaunchKernelGGL(k_gemm_rhs<2>, grid, dim3(256), 0, s, A, n, X, Y); break;
-    case 4: hipLaunchKernelGGL(k_gemm_rhs<4>, grid, dim3(256), 0, s, A, n, X, Y); break;
-    case 8: hipLaunchKernelGGL(k_gemm_rhs<8>, grid, dim3(256), 0, s, A, n, X, Y); break;
-    case 16: hipLaunchKernelGGL(k_gemm_rhs<16>, grid, dim3(256), 0, s, A, n, X, Y); break;
-    case 32: hipLaunchKernelGGL(k_gemm_rhs<32>, grid, dim3(256), 0, s, A, n, X, Y); break;
-    case 48: hipLaunchKernelGGL(k_gemm_rhs<48>, grid, dim3(256), 0, s, A, n, X, Y); break;
-    case 64: hipLaunchKernelGGL(k_gemm_rhs<64>, grid, dim3(256), 0, s, A, n, X, Y); break;
-    default:
-        for (int r = 0; r < nrhs; r++) gemv(A, n, X + (int64_t)r * n, Y + (int64_t)r * n, 0.0, s);
-        return;
-    }
-    TD_HIP(hipGetLastError());
-}
-
 void gemv(const double* A, int n, const double* x, double* y, double beta, hipStream_t s)
 {
     hipLaunchKernelGGL(k_gemv, dim3(n), dim3(256), 0, s, A, n, x, y, beta);
@@ -626,8 +555,6 @@ void gemv(const double* A, int n, const double* x, double* y, double beta, hipSt
 }
 
 namespace {
-
-typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 // Row-major n x n -> fragment order (zeros in the padding).  One thread per fragment element.
 __global__ void k_to_frag(const double* __restrict__ A, int n, double* __restrict__ Af,
@@ -643,53 +570,14 @@ __global__ void k_to_frag(const double* __restrict__ A, int n, double* __restric
     Af[t] = (row < n && col < n) ? A[(int64_t)row * n + col] : 0.0;
 }
 
-constexpr int kApplyWaves = 4;
-
-// parts[p][r][i] = sum_{j in split p} A[i][j] X[r][j].  One wavefront per (16-row tile, column
-// split); per 32-column fragment it loads 4 KiB of A (64 B contiguous per lane) and NT fragments
-// of X, and runs 8 v_mfma_f64_16x16x4 per rhs tile (k index = the 8 columns a lane holds).
+// Step solve (see apply_wave): one wavefront per (16-row tile, column split).
 template <int NT>
 __global__ void __launch_bounds__(64 * kApplyWaves)
     k_apply_frag(const double* __restrict__ Af, int n, const double* __restrict__ Xf, int nrhs,
-                 int nsplit, int cper, double* __restrict__ parts)
+                 int nsplit, double* __restrict__ parts)
 {
-    const int lane = threadIdx.x & 63;
     const int g = blockIdx.x * kApplyWaves + (threadIdx.x >> 6);
-    const int nTr = frag_rows(n) / 16, C = frag_cols(n);
-    if (g >= nTr * nsplit) return;
-    const int T = g % nTr, p = g / nTr;
-    const int c0 = p * cper, c1 = min(C, c0 + cper);
-    f64x4 acc[NT];
-#pragma unroll
-    for (int q = 0; q < NT; q++) acc[q] = f64x4{0.0, 0.0, 0.0, 0.0};
-    const double* a = Af + ((int64_t)T * C) * 512 + lane * 8;
-    for (int c = c0; c < c1; c++) {
-        double av[8];
-#pragma unroll
-        for (int e = 0; e < 8; e++) av[e] = a[(int64_t)c * 512 + e];
-#pragma unroll
-        for (int q = 0; q < NT; q++) {
-            const double* x = Xf + ((int64_t)q * C + c) * 512 + lane * 8;
-            double xv[8];
-#pragma unroll
-            for (int e = 0; e < 8; e++) xv[e] = x[e];
-#pragma unroll
-            for (int e = 0; e < 8; e++)
-                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[e], xv[e], acc[q], 0, 0, 0);
-        }
-    }
-    // D fragment of the f64 16x16x4 product: lane = rhs + 16 * (row % 4), element row / 4
-    const int Np = frag_rows(n);
-#pragma unroll
-    for (int q = 0; q < NT; q++) {
-        const int r = 16 * q + (lane & 15);
-        if (r >= nrhs) continue;
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int i = 16 * T + (lane >> 4) + 4 * v;
-            parts[((int64_t)p * nrhs + r) * Np + i] = acc[q][v];
-        }
-    }
+    if (g < frag_rows(n) / 16 * nsplit) apply_wave<NT>(Af, n, Xf, nrhs, nsplit, g, parts);
 }
 
 // One v_mfma_f64_16x16x4 with the lane layouts k_apply_frag assumes, for the layout self-test.
@@ -715,25 +603,24 @@ void to_frag(const double* A, int n, double* Af, hipStream_t s)
     TD_HIP(hipGetLastError());
 }
 
-int apply_splits(int n)
+int apply_splits(int n, int target_waves)
 {
-    // about 4096 wavefronts, at least 4 fragments (128 columns) per split
+    // about target_waves wavefronts, at least 4 fragments (128 columns) per split
     const int nTr = frag_rows(n) / 16, C = frag_cols(n);
-    return std::max(1, std::min((4096 + nTr - 1) / nTr, C / 4));
+    return std::max(1, std::min((target_waves + nTr - 1) / nTr, C / 4));
 }
 
 void apply_frag(const double* Af, int n, const double* Xf, int nrhs, double* parts,
                 hipStream_t s)
 {
-    const int nsplit = apply_splits(n), C = frag_cols(n);
-    const int cper = (C + nsplit - 1) / nsplit;
+    const int nsplit = apply_splits(n);
     const int waves = frag_rows(n) / 16 * nsplit;
     const dim3 grid((waves + kApplyWaves - 1) / kApplyWaves), block(64 * kApplyWaves);
     switch ((nrhs + 15) / 16) {
-    case 1: hipLaunchKernelGGL(k_apply_frag<1>, grid, block, 0, s, Af, n, Xf, nrhs, nsplit, cper, parts); break;
-    case 2: hipLaunchKernelGGL(k_apply_frag<2>, grid, block, 0, s, Af, n, Xf, nrhs, nsplit, cper, parts); break;
-    case 3: hipLaunchKernelGGL(k_apply_frag<3>, grid, block, 0, s, Af, n, Xf, nrhs, nsplit, cper, parts); break;
-    case 4: hipLaunchKernelGGL(k_apply_frag<4>, grid, block, 0, s, Af, n, Xf, nrhs, nsplit, cper, parts); break;
+    case 1: hipLaunchKernelGGL(k_apply_frag<1>, grid, block, 0, s, Af, n, Xf, nrhs, nsplit, parts); break;
+    case 2: hipLaunchKernelGGL(k_apply_frag<2>, grid, block, 0, s, Af, n, Xf, nrhs, nsplit, parts); break;
+    case 3: hipLaunchKernelGGL(k_apply_frag<3>, grid, block, 0, s, Af, n, Xf, nrhs, nsplit, parts); break;
+    case 4: hipLaunchKernelGGL(k_apply_frag<4>, grid, block, 0, s, Af, n, Xf, nrhs, nsplit, parts); break;
     default: throw std::invalid_argument("step solve: at most 64 right-hand sides");
     }
     TD_HIP(hipGetLastError());

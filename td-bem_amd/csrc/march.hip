@@ -19,7 +19,7 @@
 #include <stdexcept>
 #include <vector>
 
-#include "tdbem_internal.h"
+#include "step_device.h"
 
 namespace tdbem {
 
@@ -52,15 +52,11 @@ constexpr int kConvThreads = 256;
 
 inline unsigned blocks(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
-// f_r(t) for every right-hand side: src/SolverT.cpp:379-394 returns 1; a per-RHS amplitude and a
-// linear rise over t_rise (0 = Heaviside) generalise it.
 __global__ void k_fac(const double* __restrict__ amp, const double* __restrict__ rise, double t,
                       int nrhs, double* __restrict__ fac)
 {
     const int r = threadIdx.x;
-    if (r >= nrhs) return;
-    const double tr = rise[r];
-    fac[r] = tr > 0 ? amp[r] * (t < tr ? t / tr : 1.0) : amp[r];
+    if (r < nrhs) fac[r] = load_factor(amp[r], rise[r], t);
 }
 
 __global__ void k_combine(const double* __restrict__ G0, const double* __restrict__ H0,
@@ -175,13 +171,10 @@ __global__ void k_rhs(const double* __restrict__ gF, const double* __restrict__ 
                       int nrhs, double* __restrict__ Xf)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)N * nrhs) return;
-    const int r = (int)(t / N), d = (int)(t % N);
-    Xf[frag_index(N, r, d)] = fac[r] * gF[d] + gU[d] + hist[t];
+    if (t < (int64_t)N * nrhs) rhs_elem(gF, gU, hist, fac, N, Xf, t);
 }
 
-// The step solution x = sum of the column-split partials (fixed order), the boundary
-// post-processing (src/SolverStepLinearT.cpp:512-536) and the ring update.
+// The step's end (see finalize_elem).
 __global__ void k_finalize(const double* __restrict__ parts, int nparts, int Np,
                            const int* __restrict__ is_ubc,
                            const int* __restrict__ is_fbc, const double* __restrict__ ubc_val,
@@ -190,30 +183,9 @@ __global__ void k_finalize(const double* __restrict__ parts, int nparts, int Np,
                            double* __restrict__ U, double* __restrict__ T, double* __restrict__ ring)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)N * nrhs) return;
-    const int r = (int)(t / N), d = (int)(t % N);
-    double xv = 0.0;
-    for (int p = 0; p < nparts; p++) xv += parts[((int64_t)p * nrhs + r) * Np + d];
-    const double u = is_ubc[d] ? ubc_val[d] : xv;
-    double tr = is_fbc[d] ? fbc_val[d] * fac[r] : 0.0;
-    // fDofHigh is inclusive (src/SolverT.cpp:179) and the test is u_dof < fDofHigh (:527)
-    if (is_ubc[d] && d < N - 1) tr = xv;
-    const int64_t hs = ((int64_t)s * nrhs + r) * N + d;
-    U[hs] = u;
-    T[hs] = tr;
-    const double up = s >= 2 ? U[hs - (int64_t)nrhs * N] : 0.0;
-    const double tp = s >= 2 ? T[hs - (int64_t)nrhs * N] : 0.0;
-    auto at = [&](int m, int part) -> double* {
-        int sl = m % R;
-        if (sl < 0) sl += R;
-        return ring + ((int64_t)(sl * nrhs + r) * 2 + part) * N + d;
-    };
-    *at(s - 1, 0) += a1 * u;            // y_{s-1} complete
-    *at(s - 1, 1) += a1 * tr;
-    *at(s, 0) = a3 * u + a2 * up;       // z_1 of step s+1
-    *at(s, 1) = a3 * tr + a2 * tp;
-    *at(s + 1, 0) = (a2 - a1) * u;      // z_0 of step s+1
-    *at(s + 1, 1) = (a2 - a1) * tr;
+    if (t < (int64_t)N * nrhs)
+        finalize_elem(parts, nparts, Np, is_ubc, is_fbc, ubc_val, fbc_val, fac, N, nrhs, s, R, a1,
+                      a2, a3, U, T, ring, t);
 }
 
 template <int NRHS>
@@ -373,6 +345,45 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
 }
 
 int march_step_count(const Context& c) { return c.march ? c.march->s : 0; }
+
+void march_describe(Context& c, BatchScen& d)
+{
+    if (!c.march || c.march->precision != 1)
+        throw std::logic_error("batch: every context needs a mixed-precision march_init");
+    March& M = *c.march;
+    d.ring = M.ring.p;
+    d.hist = M.hist.p;
+    d.gF = M.gF.p;
+    d.gU = M.gU.p;
+    d.Xf = M.Xf.p;
+    d.Af = M.Af.p;
+    d.parts = nullptr;
+    d.is_ubc = M.is_ubc.p;
+    d.is_fbc = M.is_fbc.p;
+    d.ubc_val = M.ubc_val.p;
+    d.fbc_val = M.fbc_val.p;
+    d.fac = M.fac.p;
+    d.amp = M.d_amp.p;
+    d.rise = M.d_rise.p;
+    d.U = M.U.p;
+    d.T = M.T.p;
+    d.N = M.N;
+    d.Np = M.Np;
+    d.nrhs = M.nrhs;
+    d.R = M.R;
+    d.numstep = c.numstep;
+    d.nsplit = 0;
+    d.dt = c.dt;
+    d.a1 = M.a1;
+    d.a2 = M.a2;
+    d.a3 = M.a3;
+    d.zs = M.zs.p;
+}
+
+void march_set_step(Context& c, int s)
+{
+    if (c.march) c.march->s = s;
+}
 
 void march_run(Context& c, int nsteps, bool wait)
 {

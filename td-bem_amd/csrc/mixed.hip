@@ -25,42 +25,11 @@
 #include <stdexcept>
 #include <vector>
 
-#include "tdbem_internal.h"
+#include "step_device.h"
 
 namespace tdbem {
 
-typedef __attribute__((__vector_size__(8 * sizeof(short)))) short bf16x8;
-typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
-
 namespace {
-
-constexpr int kTile = 16;                 // nodes per tile
-constexpr int kRows = 3 * kTile;          // 48
-constexpr int kCols = 6 * kTile;          // 96
-constexpr int kSubElems = 3 * 512;        // bf16 per stored sub-tile: 3 terms x 64 lanes x 8
-constexpr int kGroup = 4;                 // row tiles per workgroup (one per wavefront)
-
-__device__ __forceinline__ unsigned short bf16_rne(float f)
-{
-    unsigned int u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
-
-__device__ __forceinline__ float bf16_to_float(unsigned short b)
-{
-    return __uint_as_float(((unsigned int)b) << 16);
-}
-
-// v = b1 + b2 + b3 + O(2^-24 |v|), each a bf16; residuals formed exactly in FP64.
-__device__ __forceinline__ void split3(double v, unsigned short* b)
-{
-    b[0] = bf16_rne((float)v);
-    const double r1 = v - (double)bf16_to_float(b[0]);
-    b[1] = bf16_rne((float)r1);
-    const double r2 = r1 - (double)bf16_to_float(b[1]);
-    b[2] = bf16_rne((float)r2);
-}
 
 __device__ __forceinline__ int row_of_chunk(const int64_t* row_chunk, int nnd, int64_t c)
 {
@@ -123,31 +92,17 @@ __global__ void __launch_bounds__(kChunk)
         }
 }
 
-// Split-BF16 copy of ring slot `slot` in B-operand order (lane = rhs + 16 * (k / 8)), laid out
-// [slot][J][term][k-tile][rhs tile][64 lanes][8].
+// Split-BF16 copy of ring slot `slot` (see zsplit_elem).
 __global__ void k_zsplit(const double* __restrict__ ring, int N, int nrhs, int nT,
                          const int* __restrict__ perm, int slot, int NT,
                          unsigned short* __restrict__ zs)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nT * kCols * nrhs) return;
-    const int r = (int)(t % nrhs);
-    const int c = (int)((t / nrhs) % kCols);
-    const int J = (int)(t / ((int64_t)nrhs * kCols));
-    const int node = perm[J * kTile + c / 6];
-    const int part = (c % 6) / 3, q = c % 3;
-    // block column part 0 multiplies z^t (ring part 1), part 1 multiplies z^u (ring part 0)
-    const int rpart = 1 - part;
-    const double v = node < 0 ? 0.0 : ring[((int64_t)(slot * nrhs + r) * 2 + rpart) * N + 3 * node + q];
-    unsigned short b[3];
-    split3(v, b);
-    const int kt = c >> 5, kk = c & 31, nt = r >> 4, n = r & 15;
-    const int lane = n + 16 * (kk >> 3), e = kk & 7;
-    unsigned short* base = zs + (((int64_t)slot * nT + J) * 3) * 3 * NT * 512;
-#pragma unroll
-    for (int tm = 0; tm < 3; tm++) base[((tm * 3 + kt) * NT + nt) * 512 + lane * 8 + e] = b[tm];
+    zsplit_elem(ring, N, nrhs, nT, perm, slot, NT, zs, t);
 }
 
+// One workgroup per chunk of a group's merged list, one wavefront per row tile (conv_chunk).
 template <int NT>
 __global__ void __launch_bounds__(64 * kGroup)
     k_conv_mixed(const bf16x8* __restrict__ tp, const TPEntry* __restrict__ ent,
@@ -155,85 +110,13 @@ __global__ void __launch_bounds__(64 * kGroup)
                  int s, double* __restrict__ partial)
 {
     const int4 cw = chunk[blockIdx.x];
-    const int g = cw.x;
-    const int64_t e0 = cw.y, e1 = cw.z;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double acc[3][NT][4];
-#pragma unroll
-    for (int m = 0; m < 3; m++)
-#pragma unroll
-        for (int n = 0; n < NT; n++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) acc[m][n][v] = 0.0;
-    for (int64_t e = e0; e < e1; e++) {
-        const TPEntry en = ent[e];
-        const unsigned msk = w == 0 ? en.msk[0] : w == 1 ? en.msk[1] : w == 2 ? en.msk[2] : en.msk[3];
-        if (!msk) continue;
-        const uint32_t first = w == 0 ? en.sub[0] : w == 1 ? en.sub[1] : w == 2 ? en.sub[2] : en.sub[3];
-        const int J = (int)(en.id >> kKBits), k = (int)(en.id & kMaxLag);
-        int slot = (s - k) % Rring;
-        if (slot < 0) slot += Rring;
-        // every stored sub-tile of this wave's block and the history k-tiles it needs are
-        // requested before any is used; the store is read once per step (non-temporal, so it
-        // does not evict the history), the history fragment is shared through L1/L2 with the
-        // other wavefronts of the group, which walk the same list
-        const bf16x8* A = tp + (int64_t)first * (kSubElems / 8) + lane;
-        const bf16x8* Zg = zs + (((int64_t)slot * nT + J) * 3) * 3 * NT * 64 + lane;
-        unsigned kts = 0;
-#pragma unroll
-        for (int m = 0; m < 3; m++) kts |= (msk >> (3 * m)) & 7u;
-        bf16x8 a[9][3], z[3][3][NT];
-        int q = 0;
-#pragma unroll
-        for (int u = 0; u < 9; u++)
-            if ((msk >> u) & 1u) {
-#pragma unroll
-                for (int t = 0; t < 3; t++) a[u][t] = __builtin_nontemporal_load(&A[(q * 3 + t) * 64]);
-                q++;
-            }
-#pragma unroll
-        for (int kt = 0; kt < 3; kt++)
-            if ((kts >> kt) & 1u) {
-#pragma unroll
-                for (int t = 0; t < 3; t++)
-#pragma unroll
-                    for (int n = 0; n < NT; n++) z[kt][t][n] = Zg[((t * 3 + kt) * NT + n) * 64];
-            }
-#pragma unroll
-        for (int u = 0; u < 9; u++) {
-            if (!((msk >> u) & 1u)) continue;
-            const int m = u / 3, kt = u % 3;
-            // the leading product a1 z1 and the five corrections sum in separate FP32 registers;
-            // both move to FP64 after each sub-tile
-#pragma unroll
-            for (int n = 0; n < NT; n++) {
-                const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
-                const f32x4 dm =
-                    __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][0], z[kt][0][n], zero, 0, 0, 0);
-                f32x4 dc =
-                    __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][2], z[kt][0][n], zero, 0, 0, 0);
-                dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][1], z[kt][1][n], dc, 0, 0, 0);
-                dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][0], z[kt][2][n], dc, 0, 0, 0);
-                dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][1], z[kt][0][n], dc, 0, 0, 0);
-                dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][0], z[kt][1][n], dc, 0, 0, 0);
-#pragma unroll
-                for (int v = 0; v < 4; v++) acc[m][n][v] += (double)dm[v] + (double)dc[v];
-            }
-        }
-    }
-    if (g * kGroup + w >= nT) return;
+    const int w = threadIdx.x >> 6;
     // partial sums of chunk c: [c][wave][3][NT][64][4]
-    double* out = partial + (((int64_t)blockIdx.x * kGroup + w) * 3 * NT) * 256;
-#pragma unroll
-    for (int m = 0; m < 3; m++)
-#pragma unroll
-        for (int n = 0; n < NT; n++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) out[((m * NT + n) * 64 + lane) * 4 + v] = acc[m][n][v];
+    conv_chunk<NT>(tp, ent, zs, nT, Rring, s, cw.y, cw.z, w, cw.x * kGroup + w < nT,
+                   partial + (((int64_t)blockIdx.x * kGroup + w) * 3 * NT) * 256);
 }
 
-// hist[r][dof] = sum over the chunks of a row tile's group (fixed order).  16x16 output
-// fragment: lane = col + 16 * (row / 4), element row % 4.
+// hist[r][dof] = sum over the chunks of a row tile's group (see reduce_elem).
 __global__ void k_tp_reduce(const double* __restrict__ partial, int nT,
                             const int* __restrict__ grp_chunk, int NT,
                             const int* __restrict__ perm, int N, int nrhs,
@@ -241,18 +124,7 @@ __global__ void k_tp_reduce(const double* __restrict__ partial, int nT,
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nT * kRows * nrhs) return;
-    const int r = (int)(t % nrhs);
-    const int row = (int)((t / nrhs) % kRows);
-    const int I = (int)(t / ((int64_t)nrhs * kRows));
-    const int node = perm[I * kTile + row / 3];
-    if (node < 0) return;
-    const int m = row >> 4, i = row & 15, nt = r >> 4, n = r & 15;
-    const int lane = n + 16 * (i >> 2), v = i & 3;
-    const int g = I / kGroup, w = I % kGroup;
-    double sum = 0.0;
-    for (int c = grp_chunk[g]; c < grp_chunk[g + 1]; c++)
-        sum += partial[(((((int64_t)c * kGroup + w) * 3 + m) * NT + nt) * 64 + lane) * 4 + v];
-    hist[(int64_t)r * N + 3 * node + row % 3] = sum;
+    reduce_elem(partial, grp_chunk, NT, perm, N, nrhs, hist, t);
 }
 
 // One 16x16x32 bf16 product with the lane layouts assumed above, for the layout self-test.
@@ -449,14 +321,11 @@ void tp_zsplit(Context& c, const double* ring, int N, int nrhs, int slot, unsign
 
 int64_t tp_zslot_elems(const Context& c, int nrhs) { return (int64_t)c.tp.nT * 9 * (nrhs / 16) * 512; }
 
-void tp_plan(Context& c, int nrhs)
+int tp_resident_groups(int nrhs, int device)
 {
-    TPStore& T = c.tp;
-    if (T.plan_nrhs == nrhs) return;
-    const int NT = nrhs / 16;
     int per_cu = 1, cus = 1;
     const void* fn = nullptr;
-    switch (NT) {
+    switch (nrhs / 16) {
     case 1: fn = reinterpret_cast<const void*>(&k_conv_mixed<1>); break;
     case 2: fn = reinterpret_cast<const void*>(&k_conv_mixed<2>); break;
     case 3: fn = reinterpret_cast<const void*>(&k_conv_mixed<3>); break;
@@ -464,15 +333,27 @@ void tp_plan(Context& c, int nrhs)
     default: throw std::invalid_argument("mixed march needs nrhs in {16, 32, 48, 64}");
     }
     TD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * kGroup, 0));
-    TD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
-    // four rounds of resident workgroups, shared among the groups by their work
-    const int target = std::max(T.nG, 4 * std::max(1, per_cu) * cus);
+    TD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    return std::max(1, per_cu) * std::max(1, cus);
+}
+
+double tp_work(const Context& c)
+{
     double total = 0;
-    for (float x : T.h_work) total += x;
-    std::vector<int4> ch;
-    std::vector<int> gch(T.nG + 1, 0);
+    for (float x : c.tp.h_work) total += x;
+    return total;
+}
+
+void tp_chunks(const Context& c, int target, int sc, std::vector<int4>& ch, std::vector<int>& gch)
+{
+    const TPStore& T = c.tp;
+    target = std::max(target, T.nG);
+    const double total = tp_work(c);
+    const size_t gbase = gch.size();
+    gch.resize(gbase + T.nG + 1, 0);
     for (int g = 0; g < T.nG; g++) {
-        gch[g] = (int)ch.size();
+        const int first = (int)ch.size();
+        gch[gbase + g] = first;
         const int64_t e0 = T.h_grp_off[g], e1 = T.h_grp_off[g + 1];
         double wg = 0;
         for (int64_t e = e0; e < e1; e++) wg += T.h_work[e];
@@ -483,19 +364,32 @@ void tp_plan(Context& c, int nrhs)
         for (int64_t e = e0; e < e1; e++) {
             acc += T.h_work[e];
             // close the chunk once it holds its share (the last one takes the rest)
-            if (acc >= step * ((int)ch.size() - gch[g] + 1) && (int)ch.size() - gch[g] < n - 1) {
-                ch.push_back(make_int4(g, (int)start, (int)(e + 1), 0));
+            const int made = (int)ch.size() - first;
+            if (acc >= step * (made + 1) && made < n - 1) {
+                ch.push_back(make_int4(g, (int)start, (int)(e + 1), sc));
                 start = e + 1;
             }
         }
-        ch.push_back(make_int4(g, (int)start, (int)e1, 0));
+        ch.push_back(make_int4(g, (int)start, (int)e1, sc));
     }
-    gch[T.nG] = (int)ch.size();
+    gch[gbase + T.nG] = (int)ch.size();
+}
+
+void tp_plan(Context& c, int nrhs, int rounds)
+{
+    TPStore& T = c.tp;
+    if (T.plan_nrhs == nrhs && T.plan_rounds == rounds) return;
+    // `rounds` rounds of resident workgroups, shared among the groups by their work
+    const int target = rounds * tp_resident_groups(nrhs, c.device);
+    std::vector<int4> ch;
+    std::vector<int> gch;
+    tp_chunks(c, target, 0, ch, gch);
     T.nchunks = (int)ch.size();
     T.chunk.upload(ch.data(), ch.size(), c.stream);
     T.grp_chunk.upload(gch.data(), gch.size(), c.stream);
     TD_HIP(hipStreamSynchronize(c.stream));
     T.plan_nrhs = nrhs;
+    T.plan_rounds = rounds;
 }
 
 void tp_conv(Context& c, const unsigned short* zs, int nrhs, int Rring, int s, double* partial)

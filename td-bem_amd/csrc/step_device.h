@@ -1,0 +1,273 @@
+// step_device.h — device code of one marching step, shared by the single-context kernels
+// (march.hip, mixed.hip, lu.hip) and the batched kernels over many scenarios (batch.hip).
+#pragma once
+
+#include "tdbem_internal.h"
+
+namespace tdbem {
+
+typedef __attribute__((__vector_size__(8 * sizeof(short)))) short bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kTile = 16;                 // nodes per tile of the mixed-precision store
+constexpr int kRows = 3 * kTile;          // 48
+constexpr int kCols = 6 * kTile;          // 96
+constexpr int kSubElems = 3 * 512;        // bf16 per stored sub-tile: 3 terms x 64 lanes x 8
+constexpr int kGroup = 4;                 // row tiles per convolution workgroup (one per wave)
+constexpr int kApplyWaves = 4;            // wavefronts per step-solve workgroup
+
+// f_r(t): src/SolverT.cpp:379-394 returns 1; a per-RHS amplitude and a linear rise over t_rise
+// (0 = Heaviside) generalise it.
+__device__ __forceinline__ double load_factor(double amp, double rise, double t)
+{
+    return rise > 0 ? amp * (t < rise ? t / rise : 1.0) : amp;
+}
+
+__device__ __forceinline__ unsigned short bf16_rne(float f)
+{
+    unsigned int u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ float bf16_to_float(unsigned short b)
+{
+    return __uint_as_float(((unsigned int)b) << 16);
+}
+
+// v = b1 + b2 + b3 + O(2^-24 |v|), each a bf16; residuals formed exactly in FP64.
+__device__ __forceinline__ void split3(double v, unsigned short* b)
+{
+    b[0] = bf16_rne((float)v);
+    const double r1 = v - (double)bf16_to_float(b[0]);
+    b[1] = bf16_rne((float)r1);
+    const double r2 = r1 - (double)bf16_to_float(b[1]);
+    b[2] = bf16_rne((float)r2);
+}
+
+// Element t (< nT * 96 * nrhs) of the split-BF16 copy of ring slot `slot`, in B-operand order
+// (lane = rhs + 16 * (k / 8)), laid out [slot][J][term][k-tile][rhs tile][64 lanes][8].  The ring
+// holds [slot][rhs][part][N] with part 0 = u, part 1 = t; block column part 0 multiplies z^t.
+__device__ __forceinline__ void zsplit_elem(const double* __restrict__ ring, int N, int nrhs,
+                                            int nT, const int* __restrict__ perm, int slot,
+                                            int NT, unsigned short* __restrict__ zs, int64_t t)
+{
+    const int r = (int)(t % nrhs);
+    const int c = (int)((t / nrhs) % kCols);
+    const int J = (int)(t / ((int64_t)nrhs * kCols));
+    const int node = perm[J * kTile + c / 6];
+    const int part = (c % 6) / 3, q = c % 3;
+    const int rpart = 1 - part;
+    const double v =
+        node < 0 ? 0.0 : ring[((int64_t)(slot * nrhs + r) * 2 + rpart) * N + 3 * node + q];
+    unsigned short b[3];
+    split3(v, b);
+    const int kt = c >> 5, kk = c & 31, nt = r >> 4, n = r & 15;
+    const int lane = n + 16 * (kk >> 3), e = kk & 7;
+    unsigned short* base = zs + (((int64_t)slot * nT + J) * 3) * 3 * NT * 512;
+#pragma unroll
+    for (int tm = 0; tm < 3; tm++) base[((tm * 3 + kt) * NT + nt) * 512 + lane * 8 + e] = b[tm];
+}
+
+// One wavefront's share of a convolution chunk: the steps [e0, e1) of its group's merged list,
+// for row tile `w` of the group, at step s; writes its 48 x (16 NT) FP64 partial sums to `out`
+// ([3][NT][64][4]) unless the row tile does not exist.
+template <int NT>
+__device__ __forceinline__ void conv_chunk(const bf16x8* __restrict__ tp,
+                                           const TPEntry* __restrict__ ent,
+                                           const bf16x8* __restrict__ zs, int nT, int Rring, int s,
+                                           int64_t e0, int64_t e1, int w, bool write,
+                                           double* __restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    double acc[3][NT][4];
+#pragma unroll
+    for (int m = 0; m < 3; m++)
+#pragma unroll
+        for (int n = 0; n < NT; n++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) acc[m][n][v] = 0.0;
+    for (int64_t e = e0; e < e1; e++) {
+        const TPEntry en = ent[e];
+        const unsigned msk = w == 0 ? en.msk[0] : w == 1 ? en.msk[1] : w == 2 ? en.msk[2] : en.msk[3];
+        if (!msk) continue;
+        const uint32_t first = w == 0 ? en.sub[0] : w == 1 ? en.sub[1] : w == 2 ? en.sub[2] : en.sub[3];
+        const int J = (int)(en.id >> kKBits), k = (int)(en.id & kMaxLag);
+        int slot = (s - k) % Rring;
+        if (slot < 0) slot += Rring;
+        // every stored sub-tile of this wave's block and the history k-tiles it needs are
+        // requested before any is used; the store is read once per step (non-temporal, so it
+        // does not evict the history), the history fragment is shared through L1/L2 with the
+        // other wavefronts of the group, which walk the same list
+        const bf16x8* A = tp + (int64_t)first * (kSubElems / 8) + lane;
+        const bf16x8* Zg = zs + (((int64_t)slot * nT + J) * 3) * 3 * NT * 64 + lane;
+        unsigned kts = 0;
+#pragma unroll
+        for (int m = 0; m < 3; m++) kts |= (msk >> (3 * m)) & 7u;
+        bf16x8 a[9][3], z[3][3][NT];
+        int q = 0;
+#pragma unroll
+        for (int u = 0; u < 9; u++)
+            if ((msk >> u) & 1u) {
+#pragma unroll
+                for (int t = 0; t < 3; t++)
+                    a[u][t] = __builtin_nontemporal_load(&A[(q * 3 + t) * 64]);
+                q++;
+            }
+#pragma unroll
+        for (int kt = 0; kt < 3; kt++)
+            if ((kts >> kt) & 1u) {
+#pragma unroll
+                for (int t = 0; t < 3; t++)
+#pragma unroll
+                    for (int n = 0; n < NT; n++) z[kt][t][n] = Zg[((t * 3 + kt) * NT + n) * 64];
+            }
+#pragma unroll
+        for (int u = 0; u < 9; u++) {
+            if (!((msk >> u) & 1u)) continue;
+            const int m = u / 3, kt = u % 3;
+            // the leading product a1 z1 and the five corrections sum in separate FP32 registers;
+            // both move to FP64 after each sub-tile
+#pragma unroll
+            for (int n = 0; n < NT; n++) {
+                const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+                const f32x4 dm =
+                    __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][0], z[kt][0][n], zero, 0, 0, 0);
+                f32x4 dc =
+                    __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][2], z[kt][0][n], zero, 0, 0, 0);
+                dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][1], z[kt][1][n], dc, 0, 0, 0);
+                dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][0], z[kt][2][n], dc, 0, 0, 0);
+                dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][1], z[kt][0][n], dc, 0, 0, 0);
+                dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][0], z[kt][1][n], dc, 0, 0, 0);
+#pragma unroll
+                for (int v = 0; v < 4; v++) acc[m][n][v] += (double)dm[v] + (double)dc[v];
+            }
+        }
+    }
+    if (!write) return;
+#pragma unroll
+    for (int m = 0; m < 3; m++)
+#pragma unroll
+        for (int n = 0; n < NT; n++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) out[((m * NT + n) * 64 + lane) * 4 + v] = acc[m][n][v];
+}
+
+// Element t (< nT * 48 * nrhs) of hist[r][dof] = the fixed-order sum over the chunks of the row
+// tile's group; partial of chunk c, wave w: [c][w][3][NT][64][4], 16x16 output fragment
+// lane = col + 16 * (row / 4), element row % 4.  grp_chunk: first chunk of each group.
+__device__ __forceinline__ void reduce_elem(const double* __restrict__ partial,
+                                            const int* __restrict__ grp_chunk, int NT,
+                                            const int* __restrict__ perm, int N, int nrhs,
+                                            double* __restrict__ hist, int64_t t)
+{
+    const int r = (int)(t % nrhs);
+    const int row = (int)((t / nrhs) % kRows);
+    const int I = (int)(t / ((int64_t)nrhs * kRows));
+    const int node = perm[I * kTile + row / 3];
+    if (node < 0) return;
+    const int m = row >> 4, i = row & 15, nt = r >> 4, n = r & 15;
+    const int lane = n + 16 * (i >> 2), v = i & 3;
+    const int g = I / kGroup, w = I % kGroup;
+    double sum = 0.0;
+    for (int c = grp_chunk[g]; c < grp_chunk[g + 1]; c++)
+        sum += partial[(((((int64_t)c * kGroup + w) * 3 + m) * NT + nt) * 64 + lane) * 4 + v];
+    hist[(int64_t)r * N + 3 * node + row % 3] = sum;
+}
+
+// Element t (< N * nrhs): r = f_r gF + gU + hist, written in the fragment order of the step solve.
+__device__ __forceinline__ void rhs_elem(const double* __restrict__ gF,
+                                         const double* __restrict__ gU,
+                                         const double* __restrict__ hist,
+                                         const double* __restrict__ fac, int N,
+                                         double* __restrict__ Xf, int64_t t)
+{
+    const int r = (int)(t / N), d = (int)(t % N);
+    Xf[frag_index(N, r, d)] = fac[r] * gF[d] + gU[d] + hist[t];
+}
+
+// Wavefront g (< frag_rows(n) / 16 * nsplit) of parts[p][r][i] = sum over the columns j of
+// split p of A[i][j] X[r][j], A and X in fragment order: per 32-column fragment it loads 4 KiB of
+// A (64 B contiguous per lane) and NT fragments of X and runs 8 v_mfma_f64_16x16x4 per rhs tile.
+template <int NT>
+__device__ __forceinline__ void apply_wave(const double* __restrict__ Af, int n,
+                                           const double* __restrict__ Xf, int nrhs, int nsplit,
+                                           int g, double* __restrict__ parts)
+{
+    const int lane = threadIdx.x & 63;
+    const int nTr = frag_rows(n) / 16, C = frag_cols(n);
+    const int cper = (C + nsplit - 1) / nsplit;
+    const int T = g % nTr, p = g / nTr;
+    const int c0 = p * cper, c1 = min(C, c0 + cper);
+    f64x4 acc[NT];
+#pragma unroll
+    for (int q = 0; q < NT; q++) acc[q] = f64x4{0.0, 0.0, 0.0, 0.0};
+    const double* a = Af + ((int64_t)T * C) * 512 + lane * 8;
+    for (int c = c0; c < c1; c++) {
+        double av[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) av[e] = a[(int64_t)c * 512 + e];
+#pragma unroll
+        for (int q = 0; q < NT; q++) {
+            const double* x = Xf + ((int64_t)q * C + c) * 512 + lane * 8;
+            double xv[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) xv[e] = x[e];
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[e], xv[e], acc[q], 0, 0, 0);
+        }
+    }
+    // D fragment of the f64 16x16x4 product: lane = rhs + 16 * (row % 4), element row / 4
+    const int Np = frag_rows(n);
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+        const int r = 16 * q + (lane & 15);
+        if (r >= nrhs) continue;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = 16 * T + (lane >> 4) + 4 * v;
+            parts[((int64_t)p * nrhs + r) * Np + i] = acc[q][v];
+        }
+    }
+}
+
+// Element t (< N * nrhs) of the step's end: x = sum of the column-split partials (fixed order),
+// the boundary post-processing (src/SolverStepLinearT.cpp:512-536) and the ring update.
+__device__ __forceinline__ void finalize_elem(const double* __restrict__ parts, int nparts,
+                                              int Np, const int* __restrict__ is_ubc,
+                                              const int* __restrict__ is_fbc,
+                                              const double* __restrict__ ubc_val,
+                                              const double* __restrict__ fbc_val,
+                                              const double* __restrict__ fac, int N, int nrhs,
+                                              int s, int R, double a1, double a2, double a3,
+                                              double* __restrict__ U, double* __restrict__ T,
+                                              double* __restrict__ ring, int64_t t)
+{
+    const int r = (int)(t / N), d = (int)(t % N);
+    double xv = 0.0;
+    for (int p = 0; p < nparts; p++) xv += parts[((int64_t)p * nrhs + r) * Np + d];
+    const double u = is_ubc[d] ? ubc_val[d] : xv;
+    double tr = is_fbc[d] ? fbc_val[d] * fac[r] : 0.0;
+    // fDofHigh is inclusive (src/SolverT.cpp:179) and the test is u_dof < fDofHigh (:527)
+    if (is_ubc[d] && d < N - 1) tr = xv;
+    const int64_t hs = ((int64_t)s * nrhs + r) * N + d;
+    U[hs] = u;
+    T[hs] = tr;
+    const double up = s >= 2 ? U[hs - (int64_t)nrhs * N] : 0.0;
+    const double tp = s >= 2 ? T[hs - (int64_t)nrhs * N] : 0.0;
+    auto at = [&](int m, int part) -> double* {
+        int sl = m % R;
+        if (sl < 0) sl += R;
+        return ring + ((int64_t)(sl * nrhs + r) * 2 + part) * N + d;
+    };
+    *at(s - 1, 0) += a1 * u;            // y_{s-1} complete
+    *at(s - 1, 1) += a1 * tr;
+    *at(s, 0) = a3 * u + a2 * up;       // z_1 of step s+1
+    *at(s, 1) = a3 * tr + a2 * tp;
+    *at(s + 1, 0) = (a2 - a1) * u;      // z_0 of step s+1
+    *at(s + 1, 1) = (a2 - a1) * tr;
+}
+
+}  // namespace tdbem

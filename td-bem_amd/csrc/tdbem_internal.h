@@ -132,7 +132,7 @@ struct TPStore {
     std::vector<float> h_work;       // per entry: relative cost (bytes moved)
     // launch plan of k_conv_mixed for one right-hand-side count: one workgroup per chunk, chunks
     // of equal work inside each group, as many chunks as workgroups resident at once
-    int plan_nrhs = 0;
+    int plan_nrhs = 0, plan_rounds = 0;
     int nchunks = 0;
     DBuf<int4> chunk;                // (group, first entry, end entry, 0)
     DBuf<int> grp_chunk;             // [nG + 1] chunks of each group
@@ -204,12 +204,63 @@ void march_history(Context& c, int rhs, double* U, double* T);
 void march_state(Context& c, int rhs, int s, double* u, double* t);
 void march_free(Context& c);
 
+// One scenario of a batched march, as the batched kernels see it (batch.hip): the march state
+// and the mixed-precision store of one context.
+struct BatchScen {
+    // march (march.hip)
+    double* ring;
+    double* hist;
+    const double* gF;
+    const double* gU;
+    double* Xf;
+    const double* Af;
+    double* parts;          // column-split partial solutions, owned by the batch
+    const int* is_ubc;
+    const int* is_fbc;
+    const double* ubc_val;
+    const double* fbc_val;
+    double* fac;
+    const double* amp;
+    const double* rise;
+    double* U;
+    double* T;
+    int N, Np, nrhs, R, numstep, nsplit;
+    double dt, a1, a2, a3;
+    // mixed-precision store (mixed.hip)
+    const void* tp;
+    const TPEntry* ent;
+    unsigned short* zs;
+    const int* perm;
+    int nT, gch0;           // gch0: this scenario's first entry in the batch's group-chunk table
+};
+// fills the march part of d (requires a mixed-precision march_init); march_set_step stores the
+// step a batch reached
+void march_describe(Context& c, BatchScen& d);
+void march_set_step(Context& c, int s);
+
+// batch.hip: scenarios stepped together (see the file's header)
+struct Batch;
+Batch* batch_create(const std::vector<Context*>& ctxs);
+void batch_destroy(Batch* b);
+// times the tunables on the batch; report = {rounds, apply waves, conv ms, apply ms}
+void batch_tune(Batch& b, double* report);
+void batch_set_params(Batch& b, int rounds, int apply_waves);
+// steps every scenario to its last step (or max_replays graph launches); returns the launches
+int batch_run(Batch& b, int steps_per_graph, int max_replays);
+
 // mixed.hip
 void tp_build(Context& c);
 void tp_zsplit(Context& c, const double* ring, int N, int nrhs, int slot, unsigned short* zs);
 int64_t tp_zslot_elems(const Context& c, int nrhs);
 // plans the chunks of k_conv_mixed for nrhs right-hand sides (after tp_build)
-void tp_plan(Context& c, int nrhs);
+void tp_plan(Context& c, int nrhs, int rounds = 4);
+// appends `target` work-balanced chunks of context c's merged lists (chunk.w = sc) and the
+// absolute first chunk of each of its groups (nG + 1 entries) to gch
+void tp_chunks(const Context& c, int target, int sc, std::vector<int4>& chunk,
+               std::vector<int>& gch);
+// resident workgroups of k_conv_mixed for nrhs right-hand sides over the whole GPU
+int tp_resident_groups(int nrhs, int device);
+double tp_work(const Context& c);
 int64_t tp_partial_elems(const Context& c, int nrhs);
 // k_conv_mixed: per-wave FP64 partial sums of the history at step s
 void tp_conv(Context& c, const unsigned short* zs, int nrhs, int Rring, int s, double* partial);
@@ -226,8 +277,6 @@ void lu_inverse(const double* LU, int n, const int* piv, double* Ainv, hipStream
 
 // small dense helpers (lu.hip)
 void gemv(const double* A, int n, const double* x, double* y, double beta, hipStream_t s);
-// Y[r][i] = sum_j A[i][j] X[r][j] for r < nrhs (A n x n row-major; X, Y right-hand-side-major)
-void gemm_rhs(const double* A, int n, const double* X, double* Y, int nrhs, hipStream_t s);
 
 // Step solve with the inverse in v_mfma_f64_16x16x4 fragment order (lu.hip).  A matrix of
 // n x n is held as 16-row x 32-column fragments: element (i, j) of fragment (T, c) sits at
@@ -244,7 +293,7 @@ __host__ __device__ inline int64_t frag_index(int n, int row, int col)
 int64_t frag_elems(int n, int rows);  // doubles of a fragment array of `rows` rows
 void to_frag(const double* A, int n, double* Af, hipStream_t s);
 // column splits of the step solve; its partial sums are parts[split][nrhs][frag_rows(n)]
-int apply_splits(int n);
+int apply_splits(int n, int target_waves = 4096);
 // parts[p][r][i] = sum over the columns j of split p of A[i][j] X[r][j]
 void apply_frag(const double* Af, int n, const double* Xf, int nrhs, double* parts,
                 hipStream_t s);

@@ -63,6 +63,11 @@ def lib():
         L.tdbem_mfma_selftest.argtypes = [C.POINTER(C.c_float)] * 3
         L.tdbem_stream_copy.argtypes = [C.c_size_t, C.c_int, dp]
         L.tdbem_mfma_f64_selftest.argtypes = [dp, dp, dp]
+        L.tdbem_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
+        L.tdbem_batch_tune.argtypes = [vp, dp]
+        L.tdbem_batch_set_params.argtypes = [vp, C.c_int, C.c_int]
+        L.tdbem_batch_run.argtypes = [vp, C.c_int, C.c_int, ip]
+        L.tdbem_batch_destroy.argtypes = [vp]
         L.tdbem_get_state.argtypes = [vp, C.c_int, C.c_int, dp, dp]
         L.tdbem_set_profiling.argtypes = [vp, C.c_int]
         L.tdbem_reset_timings.argtypes = [vp]
@@ -266,3 +271,54 @@ def mfma_f64_selftest(A, B):
     if rc:
         raise TdbemError(rc, lib().tdbem_last_error(None).decode())
     return D
+
+
+class Batch:
+    """Scenarios marched together (tdbem_batch_*): each context after a mixed-precision
+    march_init with the same number of right-hand sides; results stay in the contexts."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        arr = (C.c_void_p * len(self.contexts))(*[c._h for c in self.contexts])
+        h = C.c_void_p()
+        rc = lib().tdbem_batch_create(arr, len(self.contexts), C.byref(h))
+        if rc:
+            raise TdbemError(rc, lib().tdbem_last_error(None).decode())
+        self._h = h
+
+    def _check(self, rc):
+        if rc:
+            raise TdbemError(rc, lib().tdbem_last_error(None).decode())
+
+    def tune(self):
+        """Times the tunables on this batch and keeps the fastest."""
+        rep = np.zeros(4)
+        self._check(lib().tdbem_batch_tune(self._h, _d(rep)))
+        return {"rounds": int(rep[0]), "solve_waves": int(rep[1]), "conv_ms": float(rep[2]),
+                "solve_ms": float(rep[3])}
+
+    def set_params(self, rounds, solve_waves):
+        self._check(lib().tdbem_batch_set_params(self._h, int(rounds), int(solve_waves)))
+
+    def run(self, steps_per_graph=16, max_replays=0):
+        n = C.c_int(0)
+        self._check(lib().tdbem_batch_run(self._h, int(steps_per_graph), int(max_replays),
+                                          C.byref(n)))
+        return n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().tdbem_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
