@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenario leg")
+    ap.add_argument("--batch-scenarios", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrivals per second")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -160,6 +162,49 @@ def serve_leg(api, configs, serve, nreq, rate):
     return summary
 
 
+def batch_leg(api, configs, count, nrhs=16):
+    """Uneven half-space cavity scenarios (free plane 8x8 + spherical cavity of radius U[0.5,
+    1.5] at depth U[2, 4], 200 to 800 elements, 100 to 400 steps, nu U[0.2, 0.35], seed 4),
+    each with nrhs pressure histories, marched together by one batch: assembly and step-0
+    solves per scenario, autotuning, then hipGraph replays of 16 steps of all scenarios."""
+    scen = configs.halfspace_scenarios(count=count, seed=4, el_range=(200, 800),
+                                       step_range=(100, 400))
+    t0 = time.perf_counter()
+    ctxs, store = [], 0
+    for i, p in enumerate(scen):
+        ctx = api.Context.from_problem(p)
+        ctx.set_precision(api.PRECISION_MIXED)
+        amp, rise = configs.rhs_set(nrhs, p.dt, seed=40 + i)
+        ctx.build_lag_store()
+        store += ctx.lag_store_bytes()
+        ctx.march_init(p.a1, p.a2, p.ubc_dof, p.ubc_val, p.fbc_dof, p.fbc_val, amp=amp,
+                       t_rise=rise)
+        ctxs.append(ctx)
+    t_setup = time.perf_counter() - t0
+    b = api.Batch(ctxs)
+    t1 = time.perf_counter()
+    tuned = b.tune()
+    t_tune = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    launches = b.run(steps_per_graph=16)
+    t_march = time.perf_counter() - t2
+    total = time.perf_counter() - t0
+    assert all(c.step_index == p.numstep - 1 for c, p in zip(ctxs, scen))
+    steps = sum((p.numstep - 1) * nrhs for p in scen)
+    b.close()
+    for c in ctxs:
+        c.close()
+    return {"scenarios": count, "nrhs": nrhs, "elements_total": int(sum(p.nel for p in scen)),
+            "elements_range": [int(min(p.nel for p in scen)), int(max(p.nel for p in scen))],
+            "steps_range": [int(min(p.numstep for p in scen)), int(max(p.numstep for p in scen))],
+            "fp64_store_bytes": int(store), "time_steps": int(steps),
+            "time_steps_per_s": round(steps / total, 2),
+            "march_time_steps_per_s": round(steps / t_march, 2),
+            "seconds": {"assembly_and_step0": round(t_setup, 3), "tune": round(t_tune, 3),
+                        "march": round(t_march, 3)},
+            "graph_launches": launches, "tuned": tuned}
+
+
 def roofline(kernel, avg_ms, nbytes, traffic=None):
     ach = nbytes / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": MI355X_HBM_GBS, "unit": "GB/s",
@@ -265,6 +310,12 @@ def main():
             out["ms_per_step"] = out["plain_fp64"]["ms_per_step"]
             out["dtype"] = "f64"
             out["roofline"] = out["plain_fp64"]["roofline"]
+    if not args.no_batch:
+        log(f"batched leg: {args.batch_scenarios} half-space scenarios")
+        out["batched"] = batch_leg(api, configs, args.batch_scenarios)
+        if "plain_fp64" in out:
+            out["batched"]["speedup_vs_plain_fp64"] = round(
+                out["batched"]["time_steps_per_s"] / out["plain_fp64"]["value"], 2)
     if not args.no_serve:
         log(f"serving leg: {args.serve_requests} requests at {args.serve_rate}/s")
         from tdbem import serve
