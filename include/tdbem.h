@@ -96,12 +96,16 @@ int tdbem_get_history(tdbem_context* ctx, int rhs, double* U, double* T);
  * gathers them (src/SolverStepLinearT.cpp:558-564). */
 int tdbem_get_state(tdbem_context* ctx, int rhs, int s, double* u, double* t);
 
-/* Arithmetic of the history convolution for the next tdbem_march_init (no counterpart in the
- * reference, which is FP64 throughout):
+/* Arithmetic of the lag store and the history convolution for the next tdbem_build_lag_store /
+ * tdbem_march_init (no counterpart in the reference, which is FP64 throughout):
  *   TDBEM_PRECISION_FP64 (default): the plain path, nrhs in {1, 2, 4, 8};
- *   TDBEM_PRECISION_MIXED_BF16X3: lags stored as three-term split BF16 in 48 x 96 tile blocks,
- *   products on MFMA accumulated in FP32 per block and FP64 across blocks, nrhs in
- *   {16, 32, 48, 64}.  Assembly, the C-term, the step-0 LU/inverse and the step solve stay FP64. */
+ *   TDBEM_PRECISION_MIXED_BF16X3: regular element pairs farther than two element diameters are
+ *   integrated with FP32 time integrals and kernels (FP64 distances, FP64 sums); the lags are
+ *   stored as three-term split BF16 16 x 32 sub-tiles, products on MFMA accumulated in FP32 per
+ *   sub-tile and in FP64 across sub-tiles and lags, nrhs in {16, 32, 48, 64}.  Singular and
+ *   near-singular pairs, the C-term, the step-0 LU/inverse and the step solve stay FP64.
+ * Changing the precision discards a lag store built with the other one.  tdbem_assemble_step
+ * is always FP64. */
 enum { TDBEM_PRECISION_FP64 = 0, TDBEM_PRECISION_MIXED_BF16X3 = 1 };
 int tdbem_set_precision(tdbem_context* ctx, int precision);
 

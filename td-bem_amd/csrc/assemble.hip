@@ -153,6 +153,57 @@ __device__ __forceinline__ bool eval_s(const TimeCombo& tc, double r, double dt,
     return true;
 }
 
+// eval_s with the time integrals in FP32 (accumulate_hl_f32); same zero tests, in FP64.
+__device__ __forceinline__ bool eval_s_f32(const TimeCombo& tc, double r, double dt,
+                                           const Material& m, float* S)
+{
+    const double inv_cp = 1.0 / m.cp, inv_cs = 1.0 / m.cs;
+    const double ta = r * inv_cp;
+    const double ts = r * inv_cs + dt;
+    bool any = false, all_static = true;
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+        if (tc.cI[l] == 0.0 && tc.cJ[l] == 0.0) continue;
+        if (tc.t[l] > ta) any = true;
+        if (tc.t[l] < ts) all_static = false;
+    }
+    if (!any || (tc.static_ok && all_static)) return false;
+    const float inv_r2 = (float)(1.0 / (r * r));
+    const float fdt = (float)dt, finv_dt = (float)(1.0 / dt);
+#pragma unroll
+    for (int l = 0; l < 6; l++) S[l] = 0.f;
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+        if ((tc.cI[l] == 0.0 && tc.cJ[l] == 0.0) || !(tc.t[l] > ta)) continue;
+        accumulate_hl_f32(tc.t[l], r, fdt, finv_dt, inv_r2, inv_cp, inv_cs, (float)tc.cI[l],
+                          (float)tc.cJ[l], S);
+    }
+    return true;
+}
+
+// A regular pair is integrated in FP32 (mixed precision) unless the collocation point lies
+// within two element diameters of the element's centroid ("near-singular"): there the kernels
+// vary fast across the element and FP64 is kept, as for singular pairs.
+template <int ENND>
+__device__ __forceinline__ bool near_singular(const double* X, const double* coll)
+{
+    double c[3] = {0.0, 0.0, 0.0}, d2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < ENND; a++) {
+        c[0] += X[3 * a];
+        c[1] += X[3 * a + 1];
+        c[2] += X[3 * a + 2];
+#pragma unroll
+        for (int b = a + 1; b < ENND; b++) {
+            const double dx = X[3 * a] - X[3 * b], dy = X[3 * a + 1] - X[3 * b + 1],
+                         dz = X[3 * a + 2] - X[3 * b + 2];
+            d2 = fmax(d2, dx * dx + dy * dy + dz * dz);
+        }
+    }
+    const double dx = c[0] / ENND - coll[0], dy = c[1] / ENND - coll[1], dz = c[2] / ENND - coll[2];
+    return dx * dx + dy * dy + dz * dz < 4.0 * d2;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Geometry table: shape functions, positions, unit normals and w*J at the regular Gauss points.
 // ---------------------------------------------------------------------------------------------
@@ -311,7 +362,7 @@ template <int ENND>
 __device__ __forceinline__ int integrate_pair(const double* coords, const int* ien,
                                               const double* geo, const double* gauss, int nq,
                                               int i, int e, const TimeCombo& tc, double dt,
-                                              const Material& m, double* out)
+                                              const Material& m, double* out, bool fp32 = false)
 {
     int active = 0;
     const double coll[3] = {coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]};
@@ -324,6 +375,39 @@ __device__ __forceinline__ int integrate_pair(const double* coords, const int* i
 #pragma unroll
         for (int v = 0; v < kVals; v++) acc[a][v] = 0.0;
     const double inv_dt = 1.0 / dt;
+    if (fp32 && sing < 0 && !near_singular<ENND>(X, coll)) {
+        // regular pair, mixed precision: distance and retarded times in FP64, the time
+        // integrals and the 15 block values in FP32, sums over the points in FP64
+        for_each_point<ENND>(X, sing, geo + (size_t)e * nq * nq * (7 + ENND), gauss, nq,
+                             [&](const double* x, const double* n, double wJ, const double* N) {
+                                 const double d0 = x[0] - coll[0], d1 = x[1] - coll[1],
+                                              d2 = x[2] - coll[2];
+                                 const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+                                 float S[6];
+                                 if (!eval_s_f32(tc, r, dt, m, S)) return;
+                                 active++;
+                                 const double ir = 1.0 / r;
+                                 const float rh[3] = {(float)(d0 * ir), (float)(d1 * ir),
+                                                      (float)(d2 * ir)};
+                                 const float nf[3] = {(float)n[0], (float)n[1], (float)n[2]};
+                                 const float irf = (float)ir;
+                                 float blk[kVals];
+                                 blocks_from_s<float>(S, rh, nf, (float)r, irf, irf * irf, m,
+                                                      blk);
+#pragma unroll
+                                 for (int a = 0; a < ENND; a++) {
+                                     const float wN = (float)(wJ * N[a]);
+#pragma unroll
+                                     for (int v = 0; v < kVals; v++)
+                                         acc[a][v] += (double)(wN * blk[v]);
+                                 }
+                             });
+#pragma unroll
+        for (int a = 0; a < ENND; a++)
+#pragma unroll
+            for (int v = 0; v < kVals; v++) out[a * kVals + v] = acc[a][v];
+        return active;
+    }
     for_each_point<ENND>(X, sing, geo + (size_t)e * nq * nq * (7 + ENND), gauss, nq,
                          [&](const double* x, const double* n, double wJ, const double* N) {
                              double d[3] = {x[0] - coll[0], x[1] - coll[1], x[2] - coll[2]};
@@ -358,7 +442,7 @@ __global__ void __launch_bounds__(kBlock)
                 double dt, Material m, int Kc, int i0, int64_t nslots,
                 const int* __restrict__ slot_pair, const int64_t* __restrict__ pair_slot,
                 const int2* __restrict__ pair_k, double* __restrict__ scratch,
-                unsigned long long* __restrict__ counter)
+                unsigned long long* __restrict__ counter, int fp32)
 {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
@@ -378,7 +462,7 @@ __global__ void __launch_bounds__(kBlock)
     tc.cJ[2] = hasJ;
     tc.static_ok = true;
     const int active = integrate_pair<ENND>(coords, ien, geo, gauss, nq, i, e, tc, dt, m,
-                                            scratch + s * (ENND * kVals));
+                                            scratch + s * (ENND * kVals), fp32 != 0);
     if (counter) atomicAdd(counter, (unsigned long long)active);
 }
 
@@ -762,7 +846,8 @@ void fill_store(Context& c, const ShellPlan& P)
             hipLaunchKernelGGL(k_pair_lags<ENND>, dim3(grid_for(slots)), dim3(kBlock), 0,
                                c.stream, c.coords.p, c.ien.p, c.geo.p, c.gauss.p, c.nq, c.nel,
                                c.dt, c.mat, c.Kc, i0, slots, slot_pair.p, pair_slot.p,
-                               c.pair_k.p, scratch.p, c.profile ? c.counters.p : nullptr);
+                               c.pair_k.p, scratch.p, c.profile ? c.counters.p : nullptr,
+                               c.precision == 1 ? 1 : 0);
             TD_HIP(hipGetLastError());
             lag_timer.end();
         }
@@ -804,6 +889,7 @@ void build_lag_store(Context& c)
     if (c.Kc > kMaxLag) throw std::runtime_error("lag count exceeds the store's 10-bit lag index");
     const auto t0 = std::chrono::steady_clock::now();
     c.tp.ready = false;  // the tile-pair copy is rebuilt from the new store when needed
+    c.store_precision = c.precision;
     compute_cterm(c);
     ShellPlan P;
     plan_shells(c, P);

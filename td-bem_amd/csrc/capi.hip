@@ -430,6 +430,11 @@ int tdbem_set_precision(tdbem_context* ctx, int precision)
         if (precision != TDBEM_PRECISION_FP64 && precision != TDBEM_PRECISION_MIXED_BF16X3)
             throw std::invalid_argument("unknown precision");
         c.precision = precision;
+        // the lag store's regular pairs are assembled in the precision of the context
+        if (c.store_ready && c.store_precision != precision) {
+            c.store_ready = false;
+            c.tp.ready = false;
+        }
     });
 }
 

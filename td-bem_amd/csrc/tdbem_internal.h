@@ -167,6 +167,7 @@ struct Context {
     bool store_ready = false;
     TPStore tp;
     int precision = 0;  // 0: FP64 march, 1: split-BF16 MFMA convolution (mixed)
+    int store_precision = 0;  // precision the lag store was assembled with
 
     // march state (march.hip)
     struct March* march = nullptr;

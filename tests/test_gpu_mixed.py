@@ -58,3 +58,31 @@ def test_mixed_march_matches_fp64(gpu, case, nrhs):
             for m, r in zip(mixed, ref)]
     print(f"mixed vs fp64 max rel err {max(errs):.3e}")
     assert max(errs) <= MIXED_TOL
+
+
+@pytest.mark.parametrize("case", ["sphere", "bar"])
+def test_fp32_regular_quadrature_store(gpu, case):
+    """Mixed precision assembles regular far pairs with FP32 time integrals and kernels; the lag
+    store stays within 1e-6 of the FP64 store over all lags (relative Frobenius), and switching
+    back to FP64 rebuilds the FP64 store exactly."""
+    p = {"sphere": helpers.small_sphere, "bar": helpers.small_bar}[case](numstep=40)
+    print(f"[gpu case] fp32 quadrature store {case}", flush=True)
+    with api.Context.from_problem(p) as c64, api.Context.from_problem(p) as c32:
+        c64.build_lag_store()
+        c32.set_precision(api.PRECISION_MIXED)
+        c32.build_lag_store()
+        num = den = 0.0
+        for k in range(c64.max_step + 1):
+            G64, H64 = c64.get_lag(k)
+            G32, H32 = c32.get_lag(k)
+            num += np.linalg.norm(G32 - G64) ** 2 + np.linalg.norm(H32 - H64) ** 2
+            den += np.linalg.norm(G64) ** 2 + np.linalg.norm(H64) ** 2
+        err = np.sqrt(num / den)
+        print(f"fp32 store rel err {err:.3e}")
+        assert 0 < err <= 1e-6
+        c32.set_precision(api.PRECISION_FP64)
+        c32.build_lag_store()
+        for k in range(c64.max_step + 1):
+            G64, H64 = c64.get_lag(k)
+            G, H = c32.get_lag(k)
+            assert np.array_equal(G, G64) and np.array_equal(H, H64)
