@@ -35,8 +35,10 @@ enum {
 /* Temporal kernels, GreenFunctionT::Compute(type) (src/GreenFunctionT.cpp:119-145). */
 enum {
     TDBEM_KERNEL_HEAVISIDE = 2, /* TDElementHeavisideLinearT (assemble_step only) */
-    TDBEM_KERNEL_STEP = 4       /* TDElementStepLinearT, the one main.cpp runs
+    TDBEM_KERNEL_STEP = 4,      /* TDElementStepLinearT, the one main.cpp runs
                                    (src/ElastoDynamicsT.cpp:27) */
+    TDBEM_KERNEL_BSPLINE = 5    /* TDElementRampHeaviLinearT: cubic B-spline pulse
+                                   (src/TDElementRampHeaviLinearT.cpp:77-108; assemble_step only) */
 };
 
 /* Problem set-up.  Replaces ElastoDynamicsT::ElastoDynamicsT + TDElementT::SetNQ/SetMaterial

@@ -435,6 +435,81 @@ __device__ __forceinline__ int integrate_pair(const double* coords, const int* i
     return active;
 }
 
+// The reference's 20-point time rule (src/GreenFunctionT.cpp:1042-1164, 9 digits), for the
+// device and (below) the host.
+__constant__ double kTime20x[20] = {
+    -0.076526521, 0.076526521, -0.227785851, 0.227785851, -0.373706089, 0.373706089,
+    -0.510867002, 0.510867002, -0.636053681, 0.636053681, -0.746331906, 0.746331906,
+    -0.839116972, 0.839116972, -0.912234428, 0.912234428, -0.963971927, 0.963971927,
+    -0.993128599, 0.993128599};
+__constant__ double kTime20w[20] = {
+    0.152753387, 0.152753387, 0.149172986, 0.149172986, 0.142096109, 0.142096109, 0.131688638,
+    0.131688638, 0.118194532, 0.118194532, 0.10193012, 0.10193012, 0.083276742, 0.083276742,
+    0.062672048, 0.062672048, 0.04060143, 0.04060143, 0.017614007, 0.017614007};
+
+const double kTime20xHost[20] = {
+    -0.076526521, 0.076526521, -0.227785851, 0.227785851, -0.373706089, 0.373706089,
+    -0.510867002, 0.510867002, -0.636053681, 0.636053681, -0.746331906, 0.746331906,
+    -0.839116972, 0.839116972, -0.912234428, 0.912234428, -0.963971927, 0.963971927,
+    -0.993128599, 0.993128599};
+const double kTime20wHost[20] = {
+    0.152753387, 0.152753387, 0.149172986, 0.149172986, 0.142096109, 0.142096109, 0.131688638,
+    0.131688638, 0.118194532, 0.118194532, 0.10193012, 0.10193012, 0.083276742, 0.083276742,
+    0.062672048, 0.062672048, 0.04060143, 0.04060143, 0.017614007, 0.017614007};
+
+// B-spline kernel (Compute(5), src/TDElementRampHeaviLinearT.cpp): phi_{part+1} blocks of pair
+// (i, e) at time t, Int_Stokes_CubicBSpline_Linear (src/GreenFunctionT.cpp:625-658) at every
+// spatial point.  The Kelvin parts go to the C-term (assemble_step_dense).
+template <int ENND>
+__device__ __forceinline__ void integrate_pair_bspline(const double* coords, const int* ien,
+                                                       const double* geo, const double* gauss,
+                                                       int nq, int i, int e, double t, double dt,
+                                                       const Material& m, int part, double* out)
+{
+    const double coll[3] = {coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]};
+    double X[3 * ENND];
+    load_element<ENND>(coords, ien, e, X);
+    const int sing = singular_node<ENND>(X, coll);
+    double acc[ENND][kVals];
+#pragma unroll
+    for (int a = 0; a < ENND; a++)
+#pragma unroll
+        for (int v = 0; v < kVals; v++) acc[a][v] = 0.0;
+    const double bw = 0.7 * dt;
+    for_each_point<ENND>(X, sing, geo + (size_t)e * nq * nq * (7 + ENND), gauss, nq,
+                         [&](const double* x, const double* n, double wJ, const double* N) {
+                             double d[3] = {x[0] - coll[0], x[1] - coll[1], x[2] - coll[2]};
+                             const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                             const double ir = 1.0 / r;
+                             d[0] *= ir;
+                             d[1] *= ir;
+                             d[2] *= ir;
+                             double blk[kVals];
+#pragma unroll
+                             for (int v = 0; v < kVals; v++) blk[v] = 0.0;
+                             for (int q = 0; q < 20; q++) {
+                                 const double ab = kTime20x[q], w = kTime20w[q];
+                                 const double phi = part == 0 ? 0.5 * (1 - ab) : 0.5 * (1 + ab);
+                                 const double tt = 0.5 * (1 + ab) * dt;
+                                 double b[kVals];
+                                 bspline_blocks(t - tt, bw, r, d, n, m, b);
+                                 const double wt = w * (dt * 0.5) * phi;
+#pragma unroll
+                                 for (int v = 0; v < kVals; v++) blk[v] += wt * b[v];
+                             }
+#pragma unroll
+                             for (int a = 0; a < ENND; a++) {
+                                 const double wN = wJ * N[a];
+#pragma unroll
+                                 for (int v = 0; v < kVals; v++) acc[a][v] += wN * blk[v];
+                             }
+                         });
+#pragma unroll
+    for (int a = 0; a < ENND; a++)
+#pragma unroll
+        for (int v = 0; v < kVals; v++) out[a * kVals + v] = acc[a][v];
+}
+
 template <int ENND>
 __global__ void __launch_bounds__(kBlock)
     k_pair_lags(const double* __restrict__ coords, const int* __restrict__ ien,
@@ -470,7 +545,7 @@ template <int ENND>
 __global__ void __launch_bounds__(kBlock)
     k_pair_step(const double* __restrict__ coords, const int* __restrict__ ien,
                 const double* __restrict__ geo, const double* __restrict__ gauss, int nq, int nel,
-                double dt, Material m, int step, int heaviside, int i0, int nrow_nodes,
+                double dt, Material m, int step, int kernel, int i0, int nrow_nodes,
                 double* __restrict__ scratch)
 {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -478,6 +553,12 @@ __global__ void __launch_bounds__(kBlock)
     const int part = (int)(s & 1);
     const int64_t p = s >> 1;
     const int i = i0 + (int)(p / nel), e = (int)(p % nel);
+    if (kernel == 5) {
+        integrate_pair_bspline<ENND>(coords, ien, geo, gauss, nq, i, e, step * dt, dt, m, part,
+                                     scratch + s * (ENND * kVals));
+        return;
+    }
+    const int heaviside = kernel == 2;
     TimeCombo tc;
     tc.t[0] = step * dt - dt;   // fT - fT_1, as Stokes_Step_Linear forms it
     tc.t[1] = step * dt;
@@ -517,7 +598,8 @@ __device__ __forceinline__ void put_block(double* M, int64_t ncol, int il, int j
 template <int ENND>
 __global__ void k_dense_gather(const double* __restrict__ scratch, const int* __restrict__ adj_off,
                                const int* __restrict__ adj, const double* __restrict__ cterm,
-                               int nnd, int nel, int i0, int nrow_nodes, int add_c, double ext,
+                               int nnd, int nel, int i0, int nrow_nodes, double c1s, double c2s,
+                               double ext,
                                double* __restrict__ G1, double* __restrict__ G2,
                                double* __restrict__ H1, double* __restrict__ H2)
 {
@@ -542,9 +624,9 @@ __global__ void k_dense_gather(const double* __restrict__ scratch, const int* __
         for (int p = 0; p < 3; p++)
 #pragma unroll
             for (int q = 0; q < 3; q++) {
-                const double c = (add_c ? cterm[9 * i + 3 * p + q] : 0.0) + (p == q ? ext : 0.0);
-                v1[6 + 3 * p + q] += c;
-                v2[6 + 3 * p + q] += c;
+                const double c = cterm[9 * i + 3 * p + q], x = p == q ? ext : 0.0;
+                v1[6 + 3 * p + q] += (c1s != 0.0 ? c1s * c : 0.0) + x;
+                v2[6 + 3 * p + q] += (c2s != 0.0 ? c2s * c : 0.0) + x;
             }
     }
     const int64_t ncol = 3LL * nnd;
@@ -714,8 +796,26 @@ void assemble_step_dense(Context& c, int step, int lower, int upper, double* dG1
                          double* dH1, double* dH2)
 {
     compute_cterm(c);
-    const int heaviside = c.kernel == 2;
-    const int add_c = heaviside || step == 1;  // Step kernel: C-term at step 1 only
+    // C-term (Kelvin * dt/2 summed over the elements) into H1 and H2: Step kernel at step 1
+    // only, Heaviside every step; B-spline every step with its own phi_1 / phi_2 weights of the
+    // spline pulse, Int_CubicB_Linear (src/GreenFunctionT.cpp:733-759), over dt/2
+    double c1s = 0.0, c2s = 0.0;
+    if (c.kernel == 2 || (c.kernel == 4 && step == 1)) {
+        c1s = c2s = 1.0;
+    } else if (c.kernel == 5) {
+        const double t = step * c.dt, bw = 0.7 * c.dt, J = c.dt * 0.5;
+        double g1 = 0.0, g2 = 0.0;
+        for (int q = 0; q < 20; q++) {
+            const double ab = kTime20xHost[q], w = kTime20wHost[q];
+            const double phi1 = 0.5 * (1 - ab), phi2 = 0.5 * (1 + ab);
+            double g, dg;
+            cubic_b1(t - phi2 * c.dt, bw, g, dg);
+            g1 += phi1 * g * w * J;
+            g2 += phi2 * g * w * J;
+        }
+        c1s = g1 / (0.5 * c.dt);
+        c2s = g2 / (0.5 * c.dt);
+    }
     const double ext = (c.if_ext && step == 1) ? 0.5 * c.dt : 0.0;
     const int64_t per_node = (int64_t)c.nel * 2 * c.ennd * kVals * sizeof(double);
     const int rows_per_batch = (int)std::max<int64_t>(1, kScratchBytes / per_node);
@@ -730,18 +830,18 @@ void assemble_step_dense(Context& c, int step, int lower, int upper, double* dG1
         if (c.ennd == 4) {
             hipLaunchKernelGGL(k_pair_step<4>, dim3(grid_for(nslots)), dim3(kBlock), 0, c.stream,
                                c.coords.p, c.ien.p, c.geo.p, c.gauss.p, c.nq, c.nel, c.dt, c.mat,
-                               step, heaviside, b0, nb, scratch.p);
+                               step, c.kernel, b0, nb, scratch.p);
             hipLaunchKernelGGL(k_dense_gather<4>, dim3(grid_for((int64_t)nb * c.nnd)),
                                dim3(kBlock), 0, c.stream, scratch.p, c.adj_off.p, c.adj.p,
-                               c.cterm.p, c.nnd, c.nel, b0, nb, add_c, ext, dG1 + roff,
+                               c.cterm.p, c.nnd, c.nel, b0, nb, c1s, c2s, ext, dG1 + roff,
                                dG2 + roff, dH1 + roff, dH2 + roff);
         } else {
             hipLaunchKernelGGL(k_pair_step<8>, dim3(grid_for(nslots)), dim3(kBlock), 0, c.stream,
                                c.coords.p, c.ien.p, c.geo.p, c.gauss.p, c.nq, c.nel, c.dt, c.mat,
-                               step, heaviside, b0, nb, scratch.p);
+                               step, c.kernel, b0, nb, scratch.p);
             hipLaunchKernelGGL(k_dense_gather<8>, dim3(grid_for((int64_t)nb * c.nnd)),
                                dim3(kBlock), 0, c.stream, scratch.p, c.adj_off.p, c.adj.p,
-                               c.cterm.p, c.nnd, c.nel, b0, nb, add_c, ext, dG1 + roff,
+                               c.cterm.p, c.nnd, c.nel, b0, nb, c1s, c2s, ext, dG1 + roff,
                                dG2 + roff, dH1 + roff, dH2 + roff);
         }
         TD_HIP(hipGetLastError());

@@ -212,7 +212,8 @@ int tdbem_create(int nnd, const double* coords, int nel, const int* ien, int ety
         if (!gauss_table(nq, &gx, &gw))
             throw std::invalid_argument(
                 "ShapeFunctionT::SetNQ, Unsupported number of Gaussian points");
-        if (kernel != TDBEM_KERNEL_STEP && kernel != TDBEM_KERNEL_HEAVISIDE)
+        if (kernel != TDBEM_KERNEL_STEP && kernel != TDBEM_KERNEL_HEAVISIDE &&
+            kernel != TDBEM_KERNEL_BSPLINE)
             throw std::invalid_argument("unsupported temporal kernel");
         if (!(dt > 0) || numstep < 0 || !(E > 0) || !(rho > 0) || !(nu > -1 && nu < 0.5))
             throw std::invalid_argument("bad material or time step");

@@ -172,6 +172,134 @@ __device__ __forceinline__ void blocks_from_s(const R* S, const R* rh, const R* 
                                  (p == q ? de : R(0));
 }
 
+// ---------------------------------------------------------------------------------------------
+// Cubic B-spline temporal kernel (Compute(5), src/GreenFunctionT.cpp:522-585, 625-658, 733-998):
+// the Stokes solution for a cubic B-spline load pulse of width bw = 0.7 dt, integrated against
+// the linear hats by the 20-point time rule.
+// ---------------------------------------------------------------------------------------------
+__host__ __device__ inline double heav(double t) { return t >= 0 ? 1.0 : 0.0; }
+
+// B1(t / T) and its time derivative (src/GreenFunctionT.cpp:780-815)
+__host__ __device__ inline void cubic_b1(double t, double T, double& B1, double& DB1)
+{
+    t = t / T;
+    const double tp2 = t * t, tp3 = tp2 * t;
+    const double h0 = heav(t), h1 = heav(t - 0.25), h2 = heav(t - 0.5), h3 = heav(t - 0.75),
+                 h4 = heav(t - 1);
+    B1 = (h0 - h1) * (32.0 / 3.0) * tp3;
+    B1 += (h1 - h2) * (-32 * tp3 + 32 * tp2 + -8 * t + 2.0 / 3.0);
+    B1 += (h2 - h3) * (32 * tp3 + -64 * tp2 + 40 * t + -22.0 / 3.0);
+    B1 += (h3 - h4) * (-32.0 / 3.0 * tp3 + 32 * tp2 + -32 * t + 32.0 / 3.0);
+    double d = (h0 - h1) * 32 * tp2;
+    d += (h1 - h2) * (-96 * tp2 + 64 * t - 8);
+    d += (h2 - h3) * (96 * tp2 - 128 * t + 40);
+    d += (h3 - h4) * (-32 * tp2 + 64 * t - 32);
+    DB1 = d / T;
+}
+
+// One spline segment [T1, T2] of the time convolution of the spline with the Stokes time
+// function (the four blocks of src/GreenFunctionT.cpp:857-995); t1, t2 = r/c_p, r/c_s.
+__device__ __forceinline__ double bspline_segment(double t, double r, double t1, double t2,
+                                                  double T1, double T2, double pc, double pq,
+                                                  double pl, double p0)
+{
+    const double tp2 = t * t, tp3 = tp2 * t, tp4 = tp3 * t, tp5 = tp4 * t;
+    const double a2 = t1 * t1, a3 = a2 * t1, a4 = a3 * t1, a5 = a4 * t1;
+    const double b2 = t2 * t2, b3 = b2 * t2, b4 = b3 * t2, b5 = b4 * t2;
+    const double J1c = 4 * a5 - 15 * a4 * t + 20 * a3 * tp2 - 10 * a2 * tp3;
+    const double J2c = 4 * b5 - 15 * b4 * t + 20 * b3 * tp2 - 10 * b2 * tp3;
+    const double J1q = -3 * a4 + 8 * a3 * t - 6 * a2 * tp2;
+    const double J2q = -3 * b4 + 8 * b3 * t - 6 * b2 * tp2;
+    const double J1l = 2 * a3 - 3 * a2 * t;
+    const double J2l = 2 * b3 - 3 * b2 * t;
+    const double J10 = -a2, J20 = -b2;
+    const double c2 = T1 * T1, c3 = c2 * T1, c4 = c3 * T1, c5 = c4 * T1;
+    const double d2 = T2 * T2, d3 = d2 * T2, d4 = d3 * T2, d5 = d4 * T2;
+    const double K1c = tp5 - 5 * t * c4 + 4 * c5;
+    const double K2c = tp5 - 5 * t * d4 + 4 * d5;
+    const double K1q = tp4 - 4 * t * c3 + 3 * c4;
+    const double K2q = tp4 - 4 * t * d3 + 3 * d4;
+    const double K1l = (t - T1) * (t - T1) * (t + 2 * T1);
+    const double K2l = (t - T2) * (t - T2) * (t + 2 * T2);
+    const double K10 = (t - T1) * (t - T1), K20 = (t - T2) * (t - T2);
+    const double h11 = heav(t - t1 - T1), h21 = heav(t - t2 - T1);
+    const double h12 = heav(t - t1 - T2), h22 = heav(t - t2 - T2);
+    const double rr = r * r;
+    double Ic = h11 * (J1c + K1c) - h21 * (J2c + K1c);
+    Ic -= (h12 * (J1c + K2c) - h22 * (J2c + K2c));
+    Ic = Ic / (20.0 * rr);
+    double Iq = h11 * (J1q + K1q) - h21 * (J2q + K1q);
+    Iq -= (h12 * (J1q + K2q) - h22 * (J2q + K2q));
+    Iq = Iq / (12.0 * rr);
+    double Il = h11 * (J1l + K1l) - h21 * (J2l + K1l);
+    Il -= (h12 * (J1l + K2l) - h22 * (J2l + K2l));
+    Il = Il / (6.0 * rr);
+    double I0 = h11 * (J10 + K10) - h21 * (J20 + K10);
+    I0 -= (h12 * (J10 + K20) - h22 * (J20 + K20));
+    I0 = I0 / (2.0 * rr);
+    return pc * Ic + pq * Iq + pl * Il + p0 * I0;
+}
+
+// src/GreenFunctionT.cpp:817-998: the first segment carries only the cubic term.
+__device__ __forceinline__ double cubic_b1_int(double r, double cp, double cs, double t, double T)
+{
+    const double t1 = r / cp, t2 = r / cs;
+    const double Tp2 = T * T, Tp3 = Tp2 * T;
+    double B = bspline_segment(t, r, t1, t2, 0, 0.25 * T, 32.0 / (3.0 * Tp3), 0, 0, 0);
+    B += bspline_segment(t, r, t1, t2, 0.25 * T, 0.5 * T, -32.0 / Tp3, 32.0 / Tp2, -8.0 / T,
+                         2.0 / 3.0);
+    B += bspline_segment(t, r, t1, t2, 0.5 * T, 0.75 * T, 32.0 / Tp3, -64.0 / Tp2, 40 / T,
+                         -22.0 / 3.0);
+    B += bspline_segment(t, r, t1, t2, 0.75 * T, T, -32.0 / (3.0 * Tp3), 32 / Tp2, -32.0 / T,
+                         32.0 / 3.0);
+    return B;
+}
+
+// Stokes_CubicBSpline (src/GreenFunctionT.cpp:522-585) at time t, written as the 15 block
+// values of one point in the global block layout (G[p][q] = U[q][p], H[p][q] = T[q][p]).
+__device__ __forceinline__ void bspline_blocks(double t, double bw, double r, const double* rh,
+                                               const double* n, const Material& m, double* out)
+{
+    const double cp = m.cp, cs = m.cs;
+    double GC1, GC2, DGC1, DGC2;
+    cubic_b1(t - r / cp, bw, GC1, DGC1);
+    cubic_b1(t - r / cs, bw, GC2, DGC2);
+    const double I1 = cubic_b1_int(r, cp, cs, t, bw);
+    const double DI2 = GC1 / (cp * cp) - GC2 / (cs * cs);
+    const double DI3 = GC2;
+    const double k = cs / cp;
+    const double TI2 = GC2 - k * k * GC1;
+    const double TI3 = DGC2 - k * k * k * DGC1;
+    const double TI4 = GC1 + (r / cp) * DGC1;
+    const double TI5 = GC2 + (r / cs) * DGC2;
+    const double rdn = rh[0] * n[0] + rh[1] * n[1] + rh[2] * n[2];
+    const double divider = 4 * kPi * m.rho * r;
+    const double RR = r * r;
+    // U is symmetric: G = U^T = U
+    const int gi[6] = {0, 0, 0, 1, 1, 2}, gk[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int v = 0; v < 6; v++) {
+        const int i = gi[v], kk = gk[v];
+        const double dik = i == kk ? 1.0 : 0.0, rirk = rh[i] * rh[kk];
+        out[v] = ((3 * rirk - dik) * I1 + rirk * DI2 + dik / (cs * cs) * DI3) / divider;
+    }
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const int i = q, kk = p;  // H[p][q] = T[q][p]
+            const double dik = i == kk ? 1.0 : 0.0, rirk = rh[i] * rh[kk];
+            const double nirk = n[i] * rh[kk], nkri = n[kk] * rh[i];
+            const double s1 = -6 * cs * cs * (5 * rirk * rdn - (nirk + nkri + dik * rdn)) / RR;
+            const double s2 = 2 * (6 * rirk * rdn - (nirk + nkri + dik * rdn)) / RR;
+            const double s3 = 2 * rirk * rdn / (cs * r);
+            const double s4 = -nirk * (1 - 2 * k * k) / RR;
+            const double s5 = -(dik * rdn + nkri) / RR;
+            out[6 + 3 * p + q] =
+                (s1 * I1 + s2 * TI2 + s3 * TI3 + s4 * TI4 + s5 * TI5) / (4.0 * kPi);
+        }
+}
+
 // Kelvin traction * dt/2 (src/GreenFunctionT.cpp:205-231), returned transposed as the C-term
 // block adds it: C[p][q] -= wJ * K[q][p] (src/TDElementStepLinearT.cpp:103-109).
 __device__ __forceinline__ void kelvin_block(const double* rh, const double* n, double r,

@@ -288,6 +288,18 @@ void TDElementT::Form(int singular)
             fC_2[3 * i + j] = h2[i * ncol + 3 * ennd + j];
         }
     }
+    // Heaviside and B-spline singular elements subtract their Kelvin part from the singular
+    // node's H columns instead of C (src/TDElementHeavisideLinearT.cpp:489-497,
+    // src/TDElementRampHeaviLinearT.cpp:499-506); the Step kernel keeps it in C
+    if (singular >= 0 && fKernel != TDBEM_KERNEL_STEP) {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                fHE_1[i * 3 * ennd + 3 * singular + j] += fC_1[3 * i + j];
+                fHE_2[i * 3 * ennd + 3 * singular + j] += fC_2[3 * i + j];
+                fC_1[3 * i + j] = 0.0;
+                fC_2[3 * i + j] = 0.0;
+            }
+    }
 }
 
 void TDElementT::GetGHLinear(const double** GE_1, const double** GE_2, const double** HE_1,

@@ -134,6 +134,12 @@ public:
     explicit TDElementHeavisideLinearT(int type) : TDElementT(type, TDBEM_KERNEL_HEAVISIDE) {}
 };
 
+// src/TDElementRampHeaviLinearT.h: the cubic B-spline pulse (GreenFunctionT::Compute(5))
+class TDElementRampHeaviLinearT : public TDElementT {
+public:
+    explicit TDElementRampHeaviLinearT(int type) : TDElementT(type, TDBEM_KERNEL_BSPLINE) {}
+};
+
 // src/ElastoDynamicsT.h:19-55.  G_H_Driver fills the four dense row blocks on the GPU and copies
 // them to host buffers owned by this object (valid until the next call).
 class ElastoDynamicsT {

@@ -15,6 +15,7 @@ LIB_PATH = Path(os.environ.get("TDBEM_LIBRARY", str(PKG / "libtdbem.so")))
 
 KERNEL_HEAVISIDE = 2
 KERNEL_STEP = 4
+KERNEL_BSPLINE = 5
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 

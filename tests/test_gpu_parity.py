@@ -68,6 +68,24 @@ def test_assemble_step_row_range_and_heaviside(gpu):
         assert helpers.rel_fro_set([g[m] for g in got], [r[m] for r in ref]) <= TOL
 
 
+@pytest.mark.parametrize("case,steps", [("plane1e", [1, 2, 3, 5]),
+                                        ("bar", [1, 2, 4]),
+                                        ("bar8", [1, 2])])
+def test_assemble_step_bspline_matches_oracle(gpu, case, steps):
+    """The cubic B-spline kernel (Compute(5), TDElementRampHeaviLinearT): Stokes part by the
+    20-point time rule at every point, Kelvin part into the C-term with its own phi_1 / phi_2
+    spline weights every step."""
+    p = {"plane1e": helpers.plane_1e, "bar": lambda: helpers.small_bar(h=1.0),
+         "bar8": helpers.small_quadratic_bar}[case]()
+    _announce(f"assemble_step B-spline {case} nnd={p.nnd} nel={p.nel}")
+    with _ctx(p, kernel=api.KERNEL_BSPLINE) as ctx:
+        got = [ctx.assemble_step(n) for n in steps]
+    ref = [_oracle_step(p, n, kernel=api.KERNEL_BSPLINE) for n in steps]
+    for m, name in enumerate(("G1", "G2", "H1", "H2")):
+        err = helpers.rel_fro_set([g[m] for g in got], [r[m] for r in ref])
+        assert err <= TOL, f"{case} {name}: rel err {err:.3e}"
+
+
 @pytest.mark.parametrize("case", ["bar", "sphere", "plane1e"])
 def test_lag_store_matches_oracle(gpu, case):
     p = {"bar": helpers.small_bar, "sphere": helpers.small_sphere,
