@@ -78,7 +78,7 @@ __global__ void k_b_zsplit(const BatchScen* __restrict__ D, const int* __restric
 
 // One workgroup per chunk (chunk.w = scenario), one wavefront per row tile of its group.
 template <int NT>
-__global__ void __launch_bounds__(64 * kGroup)
+__global__ void __launch_bounds__(64 * kGroup, NT <= 2 ? 2 : 1)
     k_b_conv(const BatchScen* __restrict__ D, const int4* __restrict__ chunk,
              const int* __restrict__ sdev, const int* __restrict__ act, double* __restrict__ partial)
 {

@@ -104,7 +104,7 @@ __global__ void k_zsplit(const double* __restrict__ ring, int N, int nrhs, int n
 
 // One workgroup per chunk of a group's merged list, one wavefront per row tile (conv_chunk).
 template <int NT>
-__global__ void __launch_bounds__(64 * kGroup)
+__global__ void __launch_bounds__(64 * kGroup, NT <= 2 ? 2 : 1)
     k_conv_mixed(const bf16x8* __restrict__ tp, const TPEntry* __restrict__ ent,
                  const int4* __restrict__ chunk, const bf16x8* __restrict__ zs, int nT, int Rring,
                  int s, double* __restrict__ partial)

@@ -72,7 +72,10 @@ __device__ __forceinline__ void zsplit_elem(const double* __restrict__ ring, int
 
 // One wavefront's share of a convolution chunk: the steps [e0, e1) of its group's merged list,
 // for row tile `w` of the group, at step s; writes its 48 x (16 NT) FP64 partial sums to `out`
-// ([3][NT][64][4]) unless the row tile does not exist.
+// ([3][NT][64][4]) unless the row tile does not exist.  All four wavefronts of the workgroup
+// call it together (it has a barrier per step): the history k-tiles of a step are loaded from
+// HBM once per workgroup into one of two LDS buffers, one step ahead of their use, and shared by
+// the four row tiles; each wavefront streams its own stored sub-tiles.
 template <int NT>
 __device__ __forceinline__ void conv_chunk(const bf16x8* __restrict__ tp,
                                            const TPEntry* __restrict__ ent,
@@ -80,6 +83,10 @@ __device__ __forceinline__ void conv_chunk(const bf16x8* __restrict__ tp,
                                            int64_t e0, int64_t e1, int w, bool write,
                                            double* __restrict__ out)
 {
+    // one (slot, column tile) history block: [term][k-tile][rhs tile] lines of 64 x 16 B
+    constexpr int kLines = 9 * NT;
+    constexpr int kPer = (kLines + kGroup - 1) / kGroup;   // lines per wavefront
+    __shared__ bf16x8 zl[2][kLines * 64];
     const int lane = threadIdx.x & 63;
     double acc[3][NT][4];
 #pragma unroll
@@ -88,24 +95,41 @@ __device__ __forceinline__ void conv_chunk(const bf16x8* __restrict__ tp,
         for (int n = 0; n < NT; n++)
 #pragma unroll
             for (int v = 0; v < 4; v++) acc[m][n][v] = 0.0;
-    for (int64_t e = e0; e < e1; e++) {
-        const TPEntry en = ent[e];
-        const unsigned msk = w == 0 ? en.msk[0] : w == 1 ? en.msk[1] : w == 2 ? en.msk[2] : en.msk[3];
-        if (!msk) continue;
-        const uint32_t first = w == 0 ? en.sub[0] : w == 1 ? en.sub[1] : w == 2 ? en.sub[2] : en.sub[3];
+    // wavefront w stages lines w, w + 4, ... of a block; line L = (term * 3 + k-tile) * NT + n
+    bf16x8 zn[kPer];
+    auto zload = [&](const TPEntry& en) {
         const int J = (int)(en.id >> kKBits), k = (int)(en.id & kMaxLag);
         int slot = (s - k) % Rring;
         if (slot < 0) slot += Rring;
-        // every stored sub-tile of this wave's block and the history k-tiles it needs are
-        // requested before any is used; the store is read once per step (non-temporal, so it
-        // does not evict the history), the history fragment is shared through L1/L2 with the
-        // other wavefronts of the group, which walk the same list
-        const bf16x8* A = tp + (int64_t)first * (kSubElems / 8) + lane;
-        const bf16x8* Zg = zs + (((int64_t)slot * nT + J) * 3) * 3 * NT * 64 + lane;
-        unsigned kts = 0;
+        const bf16x8* Zg = zs + ((int64_t)slot * nT + J) * (kLines * 64) + lane;
 #pragma unroll
-        for (int m = 0; m < 3; m++) kts |= (msk >> (3 * m)) & 7u;
-        bf16x8 a[9][3], z[3][3][NT];
+        for (int q = 0; q < kPer; q++) {
+            const int L = q * kGroup + w;
+            if (L < kLines && ((en.kmask >> ((L / NT) % 3)) & 1u)) zn[q] = Zg[L * 64];
+        }
+    };
+    auto zstore = [&](int buf, unsigned kmask) {
+#pragma unroll
+        for (int q = 0; q < kPer; q++) {
+            const int L = q * kGroup + w;
+            if (L < kLines && ((kmask >> ((L / NT) % 3)) & 1u)) zl[buf][L * 64 + lane] = zn[q];
+        }
+    };
+    if (e0 < e1) {
+        const TPEntry en = ent[e0];
+        zload(en);
+        zstore(0, en.kmask);
+    }
+    __syncthreads();
+    for (int64_t e = e0; e < e1; e++) {
+        const int cur = (int)((e - e0) & 1);
+        const TPEntry en = ent[e];
+        const unsigned msk = w == 0 ? en.msk[0] : w == 1 ? en.msk[1] : w == 2 ? en.msk[2] : en.msk[3];
+        const uint32_t first = w == 0 ? en.sub[0] : w == 1 ? en.sub[1] : w == 2 ? en.sub[2] : en.sub[3];
+        // this wavefront's stored sub-tiles of the step (read once: non-temporal, so they do not
+        // evict history blocks), then the next step's history lines, then the products
+        const bf16x8* A = tp + (int64_t)first * (kSubElems / 8) + lane;
+        bf16x8 a[9][3];
         int q = 0;
 #pragma unroll
         for (int u = 0; u < 9; u++)
@@ -115,35 +139,49 @@ __device__ __forceinline__ void conv_chunk(const bf16x8* __restrict__ tp,
                     a[u][t] = __builtin_nontemporal_load(&A[(q * 3 + t) * 64]);
                 q++;
             }
+        const bool more = e + 1 < e1;
+        unsigned knext = 0;
+        if (more) {
+            const TPEntry nx = ent[e + 1];
+            knext = nx.kmask;
+            zload(nx);
+        }
+        if (msk) {
+            const bf16x8* Z = zl[cur] + lane;
 #pragma unroll
-        for (int kt = 0; kt < 3; kt++)
-            if ((kts >> kt) & 1u) {
+            for (int kt = 0; kt < 3; kt++) {
+                const unsigned rows = (msk >> kt) & 0x49u;   // bits kt, 3 + kt, 6 + kt
+                if (!rows) continue;
+                bf16x8 z[3][NT];
 #pragma unroll
                 for (int t = 0; t < 3; t++)
 #pragma unroll
-                    for (int n = 0; n < NT; n++) z[kt][t][n] = Zg[((t * 3 + kt) * NT + n) * 64];
-            }
+                    for (int n = 0; n < NT; n++) z[t][n] = Z[((t * 3 + kt) * NT + n) * 64];
 #pragma unroll
-        for (int u = 0; u < 9; u++) {
-            if (!((msk >> u) & 1u)) continue;
-            const int m = u / 3, kt = u % 3;
-            // the leading product a1 z1 and the five corrections sum in separate FP32 registers;
-            // both move to FP64 after each sub-tile
+                for (int m = 0; m < 3; m++) {
+                    const int u = 3 * m + kt;
+                    if (!((msk >> u) & 1u)) continue;
+                    // the leading product a1 z1 and the five corrections sum in separate FP32
+                    // registers; both move to FP64 after each sub-tile
 #pragma unroll
-            for (int n = 0; n < NT; n++) {
-                const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
-                const f32x4 dm =
-                    __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][0], z[kt][0][n], zero, 0, 0, 0);
-                f32x4 dc =
-                    __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][2], z[kt][0][n], zero, 0, 0, 0);
-                dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][1], z[kt][1][n], dc, 0, 0, 0);
-                dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][0], z[kt][2][n], dc, 0, 0, 0);
-                dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][1], z[kt][0][n], dc, 0, 0, 0);
-                dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][0], z[kt][1][n], dc, 0, 0, 0);
+                    for (int n = 0; n < NT; n++) {
+                        const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+                        const f32x4 dm =
+                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][0], z[0][n], zero, 0, 0, 0);
+                        f32x4 dc =
+                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][2], z[0][n], zero, 0, 0, 0);
+                        dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][1], z[1][n], dc, 0, 0, 0);
+                        dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][0], z[2][n], dc, 0, 0, 0);
+                        dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][1], z[0][n], dc, 0, 0, 0);
+                        dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][0], z[1][n], dc, 0, 0, 0);
 #pragma unroll
-                for (int v = 0; v < 4; v++) acc[m][n][v] += (double)dm[v] + (double)dc[v];
+                        for (int v = 0; v < 4; v++) acc[m][n][v] += (double)dm[v] + (double)dc[v];
+                    }
+                }
             }
         }
+        if (more) zstore(cur ^ 1, knext);
+        __syncthreads();
     }
     if (!write) return;
 #pragma unroll
