@@ -37,8 +37,12 @@ enum {
     TDBEM_KERNEL_HEAVISIDE = 2, /* TDElementHeavisideLinearT (assemble_step only) */
     TDBEM_KERNEL_STEP = 4,      /* TDElementStepLinearT, the one main.cpp runs
                                    (src/ElastoDynamicsT.cpp:27) */
-    TDBEM_KERNEL_BSPLINE = 5    /* TDElementRampHeaviLinearT: cubic B-spline pulse
+    TDBEM_KERNEL_BSPLINE = 5,   /* TDElementRampHeaviLinearT: cubic B-spline pulse
                                    (src/TDElementRampHeaviLinearT.cpp:77-108; assemble_step only) */
+    TDBEM_KERNEL_RAMP = 6       /* ramped-Heaviside pulse of slope 2/dt: Int_Stokes_RampedHeaviside_Linear
+                                   and Int_Kelvin_RampedHeaviside_Linear (src/GreenFunctionT.cpp:
+                                   589-622, 661-680), which no Compute case reaches, in the element
+                                   loop of TDElementRampHeaviLinearT (assemble_step only) */
 };
 
 /* Problem set-up.  Replaces ElastoDynamicsT::ElastoDynamicsT + TDElementT::SetNQ/SetMaterial
@@ -124,6 +128,16 @@ int tdbem_mfma_f64_selftest(const double* A, const double* B, double* D);
  * writes exactly `bytes` per launch, so hardware byte counters of other kernels can be scaled by
  * what they read for it.  No counterpart in the reference. */
 int tdbem_stream_copy(size_t bytes, int launches, double* avg_ms);
+
+/* Diagnostic: the step-0 dense solver on its own.  A is n x n row-major (host), B is n x nrhs
+ * row-major (host), overwritten with X = A^{-1} B.  mode 0: the LU of the march (no row
+ * interchanges when partial pivoting would make none, else partial pivoting) and two triangular
+ * solves, as in the n == 2 solve; TDBEM_SOLVE_INVERSE: X = (A^{-1}) B through the explicit
+ * inverse the march applies every step; TDBEM_SOLVE_PIVOTED: factor with partial pivoting in
+ * every case.  Stands in for KSPSolve (src/SolverStepLinearT.cpp:503-504); TDBEM_ERR_NUMERIC
+ * when A is singular. */
+enum { TDBEM_SOLVE_INVERSE = 1, TDBEM_SOLVE_PIVOTED = 2 };
+int tdbem_dense_solve(int n, const double* A, int nrhs, double* B, int mode);
 
 /* Waits for all work queued on the context's stream. */
 int tdbem_synchronize(tdbem_context* ctx);

@@ -548,6 +548,129 @@ void or_kelvin_bspline_linear(const or_material* m, const double* dir, const dou
     kelvin_bspline(m, &s, t, dt, K1, K2);
 }
 
+/* --- ramped-Heaviside kernels: src/GreenFunctionT.cpp:449-519, 589-622, 661-680, 705-731 ---- */
+/* Load g(t) = min(max(k t, 0), 1) with slope k = 2/dt (SetTime, :111-117). */
+static void stokes_ramped_heaviside(const or_material* m, const or_spatial* s, double t,
+                                    double k, double* U, double* Tr)
+{
+    /* src/GreenFunctionT.cpp:449-519 */
+    double cp = m->cp, cs = m->cs, r = s->r;
+    double HC1 = heav(t - r / cp), HC2 = heav(t - r / cs);
+    double HKC1 = heav(t - 1 / k - r / cp), HKC2 = heav(t - 1 / k - r / cs);
+    double GC1 = k * (t - r / cp) * (HC1 - HKC1) + HKC1;
+    double GC2 = k * (t - r / cs) * (HC2 - HKC2) + HKC2;
+    double DGC1 = k * (HC1 - HKC1), DGC2 = k * (HC2 - HKC2);
+    double I1, DI2, DI3, TI2, TI3, TI4, TI5, rdn, divider;
+    int i, kk;
+    I1 = k * ((pow(t / r, 2) - 1 / pow(cp, 2)) * t / 2 - (pow(t / r, 3) - 1 / pow(cp, 3)) * r / 3) * HC1;
+    I1 = I1 - k * ((pow(t / r, 2) - 1 / pow(cs, 2)) * t / 2 - (pow(t / r, 3) - 1 / pow(cs, 3)) * r / 3) * HC2;
+    I1 = I1 + ((1 - k * t) * (pow(t - 1 / k, 2) / pow(r, 2) - 1 / pow(cp, 2)) / 2 +
+               k * r * (pow(t - 1 / k, 3) / pow(r, 3) - 1 / pow(cp, 3)) / 3) * HKC1;
+    I1 = I1 - ((1 - k * t) * (pow(t - 1 / k, 2) / pow(r, 2) - 1 / pow(cs, 2)) / 2 +
+               k * r * (pow(t - 1 / k, 3) / pow(r, 3) - 1 / pow(cs, 3)) / 3) * HKC2;
+    DI2 = GC1 / pow(cp, 2) - GC2 / pow(cs, 2);
+    DI3 = GC2;
+    TI2 = GC2 - pow(cs / cp, 2) * GC1;
+    TI3 = DGC2 - pow(cs / cp, 3) * DGC1;
+    TI4 = GC1 + (r / cp) * DGC1;
+    TI5 = GC2 + (r / cs) * DGC2;
+    rdn = dot3(s->rv, s->n);
+    divider = 4 * M_PI * m->rho * r;
+    for (i = 0; i < 3; i++)
+        for (kk = 0; kk < 3; kk++) {
+            double dik = (i == kk) ? 1.0 : 0.0;
+            double RR = pow(r, 2);
+            double rirk = s->rv[i] * s->rv[kk];
+            double nirk = s->n[i] * s->rv[kk];
+            double nkri = s->n[kk] * s->rv[i];
+            double s1 = -6 * pow(cs, 2) * (5 * rirk * rdn - (nirk + nkri + dik * rdn)) / RR;
+            double s2 = 2 * (6 * rirk * rdn - (nirk + nkri + dik * rdn)) / RR;
+            double s3 = 2 * rirk * rdn / (cs * r);
+            double s4 = -nirk * (1 - 2 * pow(cs / cp, 2)) / RR;
+            double s5 = -(dik * rdn + nkri) / RR;
+            U[i * 3 + kk] = ((3 * rirk - dik) * I1 + rirk * DI2 + dik / pow(cs, 2) * DI3) / divider;
+            Tr[i * 3 + kk] = (s1 * I1 + s2 * TI2 + s3 * TI3 + s4 * TI4 + s5 * TI5) / (4.0 * M_PI);
+        }
+}
+
+static void ramp_linear(const or_material* m, const or_spatial* s, double t, double dt,
+                        double* DU1, double* DU2, double* DT1, double* DT2)
+{
+    /* src/GreenFunctionT.cpp:589-622: the 40-point time rule over the hats. */
+    const double *gx, *gw;
+    double U[9], Tr[9];
+    int q, i;
+    or_gauss_time(40, &gx, &gw);
+    for (i = 0; i < 9; i++) DU1[i] = DU2[i] = DT1[i] = DT2[i] = 0;
+    for (q = 0; q < 40; q++) {
+        double ab = gx[q], w = gw[q];
+        double phi1 = 0.5 * (1 - ab), phi2 = 0.5 * (1 + ab), J = dt * 0.5;
+        stokes_ramped_heaviside(m, s, t - phi2 * dt, 2 / dt, U, Tr);
+        for (i = 0; i < 9; i++) {
+            DU1[i] += w * J * phi1 * U[i];
+            DU2[i] += w * J * phi2 * U[i];
+            DT1[i] += w * J * phi1 * Tr[i];
+            DT2[i] += w * J * phi2 * Tr[i];
+        }
+    }
+}
+
+void or_ramp_weights(double t, double dt, double g[2])
+{
+    /* Int_RampedHeaviside_Linear (src/GreenFunctionT.cpp:705-731). */
+    const double *gx, *gw;
+    double k = 2 / dt;
+    int q;
+    or_gauss_time(40, &gx, &gw);
+    g[0] = g[1] = 0;
+    for (q = 0; q < 40; q++) {
+        double ab = gx[q], w = gw[q];
+        double phi1 = 0.5 * (1 - ab), phi2 = 0.5 * (1 + ab), J = dt * 0.5;
+        double tt = t - phi2 * dt;
+        double gg = k * tt * (heav(tt) - heav(tt - 1 / k)) + heav(tt - 1 / k);
+        g[0] += phi1 * gg * w * J;
+        g[1] += phi2 * gg * w * J;
+    }
+}
+
+static void kelvin_ramp(const or_material* m, const or_spatial* s, double t, double dt,
+                        double* K1, double* K2)
+{
+    /* Int_Kelvin_RampedHeaviside_Linear (src/GreenFunctionT.cpp:661-680). */
+    double g[2], KT[9];
+    int i;
+    or_ramp_weights(t, dt, g);
+    kelvin_scaled(m, s, 1.0, KT);
+    for (i = 0; i < 9; i++) {
+        K1[i] = KT[i] * g[0];
+        K2[i] = KT[i] * g[1];
+    }
+}
+
+void or_stokes_ramp_point(const or_material* m, const double* dir, const double* nrm, double t,
+                          double dt, double U[9], double T[9])
+{
+    or_spatial s;
+    set_spatial(&s, dir, nrm);
+    stokes_ramped_heaviside(m, &s, t, 2 / dt, U, T);
+}
+
+void or_stokes_ramp_linear(const or_material* m, const double* dir, const double* nrm, double t,
+                           double dt, double DU1[9], double DU2[9], double DT1[9], double DT2[9])
+{
+    or_spatial s;
+    set_spatial(&s, dir, nrm);
+    ramp_linear(m, &s, t, dt, DU1, DU2, DT1, DT2);
+}
+
+void or_kelvin_ramp_linear(const or_material* m, const double* dir, const double* nrm, double t,
+                           double dt, double K1[9], double K2[9])
+{
+    or_spatial s;
+    set_spatial(&s, dir, nrm);
+    kelvin_ramp(m, &s, t, dt, K1, K2);
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* Element integration                                                                         */
 /* ------------------------------------------------------------------------------------------ */
@@ -587,9 +710,12 @@ static void eval_kernel(const or_time* tm, const or_spatial* s, double* DU1, dou
         kelvin_scaled(tm->m, s, 0.5 * tm->dt, K1);
         memcpy(K2, K1, 9 * sizeof(double));
         heaviside_linear(tm->m, s, tm->t, tm->dt, DU1, DU2, DT1, DT2);
-    } else {
+    } else if (tm->kernel == OR_KERNEL_BSPLINE) {
         bspline_linear(tm->m, s, tm->t, tm->dt, DU1, DU2, DT1, DT2);
         kelvin_bspline(tm->m, s, tm->t, tm->dt, K1, K2);
+    } else {
+        ramp_linear(tm->m, s, tm->t, tm->dt, DU1, DU2, DT1, DT2);
+        kelvin_ramp(tm->m, s, tm->t, tm->dt, K1, K2);
     }
 }
 
@@ -705,7 +831,8 @@ int or_element(int kernel, int etype, int nq, const or_material* m, const double
     const double *gx, *gw;
     int ennd, i;
     if (etype != 1 && etype != 2) return -1;
-    if (kernel != OR_KERNEL_STEP && kernel != OR_KERNEL_HEAVISIDE && kernel != OR_KERNEL_BSPLINE)
+    if (kernel != OR_KERNEL_STEP && kernel != OR_KERNEL_HEAVISIDE && kernel != OR_KERNEL_BSPLINE &&
+        kernel != OR_KERNEL_RAMP)
         return -1;
     if (or_gauss_shape(nq, &gx, &gw)) return -1;
     ennd = etype == 1 ? 4 : 8;

@@ -20,7 +20,9 @@ extern "C" {
 enum {
     OR_KERNEL_HEAVISIDE = 2, /* Kelvin_Linear + Stokes_Heaviside_Linear, TDElementHeavisideLinearT */
     OR_KERNEL_STEP = 4,      /* Kelvin_Linear + Stokes_Step_Linear, TDElementStepLinearT (main path) */
-    OR_KERNEL_BSPLINE = 5    /* cubic B-spline, TDElementRampHeaviLinearT */
+    OR_KERNEL_BSPLINE = 5,   /* cubic B-spline, TDElementRampHeaviLinearT */
+    OR_KERNEL_RAMP = 6       /* ramped Heaviside (no Compute case reaches it) in the element loop
+                              * of TDElementRampHeaviLinearT */
 };
 
 typedef struct {
@@ -62,6 +64,16 @@ void or_stokes_bspline_linear(const or_material* m, const double* dir, const dou
                               double DT1[9], double DT2[9]);
 void or_kelvin_bspline_linear(const or_material* m, const double* dir, const double* nrm,
                               double t, double dt, double K1[9], double K2[9]);
+/* Ramped-Heaviside load g(t) = min(max(2 t / dt, 0), 1): Stokes_RampedHeaviside at time t
+ * (src/GreenFunctionT.cpp:449-519), Int_Stokes_RampedHeaviside_Linear (:589-622),
+ * Int_Kelvin_RampedHeaviside_Linear (:661-680) and Int_RampedHeaviside_Linear (:705-731). */
+void or_stokes_ramp_point(const or_material* m, const double* dir, const double* nrm, double t,
+                          double dt, double U[9], double T[9]);
+void or_stokes_ramp_linear(const or_material* m, const double* dir, const double* nrm, double t,
+                           double dt, double DU1[9], double DU2[9], double DT1[9], double DT2[9]);
+void or_kelvin_ramp_linear(const or_material* m, const double* dir, const double* nrm, double t,
+                           double dt, double K1[9], double K2[9]);
+void or_ramp_weights(double t, double dt, double g[2]);
 
 /* One (collocation point, element) pair: TDElement{Step,Heaviside,RampHeavi}LinearT::
  * FormGH_Regular / FormGH_Singular (src/TDElementStepLinearT.cpp:28-626 and the two variants).

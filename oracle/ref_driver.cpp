@@ -96,6 +96,29 @@ int ref_green(int kernel, double E, double nu, double rho, const double* dir, co
     return 0;
 }
 
+// The ramped-Heaviside functions of GreenFunctionT, which no Compute case reaches: the 40-point
+// time integrals Int_Stokes_RampedHeaviside_Linear (src/GreenFunctionT.cpp:589-622) and
+// Int_Kelvin_RampedHeaviside_Linear (:661-680), out = DU1|DU2|DT1|DT2|K1|K2; the closed form
+// Stokes_RampedHeaviside (:449-519) at time t, point = U|T; and Int_RampedHeaviside_Linear
+// (:705-731), g = g1|g2.
+int ref_green_ramp(double E, double nu, double rho, const double* dir, const double* nrm,
+                   double t, double dt, double* out, double* point, double* g)
+{
+    GreenFunctionT gf;
+    gf.SetMaterial(E, nu, rho);
+    gf.SetSpatial(dir, nrm);
+    gf.SetTime(t, dt);
+    gf.Int_Stokes_RampedHeaviside_Linear(out, out + 9, out + 18, out + 27);
+    gf.Int_Kelvin_RampedHeaviside_Linear();
+    double *k1, *k2;
+    gf.GetKelvinRampedLinear(&k1, &k2);
+    std::memcpy(out + 36, k1, 72);
+    std::memcpy(out + 45, k2, 72);
+    gf.Stokes_RampedHeaviside(point, point + 9, t);
+    gf.Int_RampedHeaviside_Linear(g[0], g[1]);
+    return 0;
+}
+
 // ShapeFunctionT::Evaluate(eta1, eta2): out = N[ennd] | x[3] | n[3] | J.
 int ref_shape_eval(int etype, const double* X, double e1, double e2, double* out)
 {

@@ -457,14 +457,42 @@ const double kTime20wHost[20] = {
     0.131688638, 0.118194532, 0.118194532, 0.10193012, 0.10193012, 0.083276742, 0.083276742,
     0.062672048, 0.062672048, 0.04060143, 0.04060143, 0.017614007, 0.017614007};
 
-// B-spline kernel (Compute(5), src/TDElementRampHeaviLinearT.cpp): phi_{part+1} blocks of pair
-// (i, e) at time t, Int_Stokes_CubicBSpline_Linear (src/GreenFunctionT.cpp:625-658) at every
-// spatial point.  The Kelvin parts go to the C-term (assemble_step_dense).
+// The reference's 40-point rule (src/GreenFunctionT.cpp:1082-1139; the same values as the
+// 40-point shape-function rule, src/ShapeFunctionT.cpp:134-159), for the ramped-Heaviside kernel.
+__constant__ double kTime40x[40] = {
+    -0.0387724175060508, 0.0387724175060508, -0.1160840706752552, 0.1160840706752552,
+    -0.1926975807013711, 0.1926975807013711, -0.2681521850072537, 0.2681521850072537,
+    -0.3419940908257585, 0.3419940908257585, -0.4137792043716050, 0.4137792043716050,
+    -0.4830758016861787, 0.4830758016861787, -0.5494671250951282, 0.5494671250951282,
+    -0.6125538896679802, 0.6125538896679802, -0.6719566846141796, 0.6719566846141796,
+    -0.7273182551899271, 0.7273182551899271, -0.7783056514265194, 0.7783056514265194,
+    -0.8246122308333117, 0.8246122308333117, -0.8659595032122595, 0.8659595032122595,
+    -0.9020988069688743, 0.9020988069688743, -0.9328128082786765, 0.9328128082786765,
+    -0.9579168192137917, 0.9579168192137917, -0.9772599499837743, 0.9772599499837743,
+    -0.9907262386994570, 0.9907262386994570, -0.9982377097105593, 0.9982377097105593};
+__constant__ double kTime40w[40] = {
+    0.0775059479784248, 0.0775059479784248, 0.0770398181642480, 0.0770398181642480,
+    0.0761103619006262, 0.0761103619006262, 0.0747231690579683, 0.0747231690579683,
+    0.0728865823958041, 0.0728865823958041, 0.0706116473912868, 0.0706116473912868,
+    0.0679120458152339, 0.0679120458152339, 0.0648040134566010, 0.0648040134566010,
+    0.0613062424929289, 0.0613062424929289, 0.0574397690993916, 0.0574397690993916,
+    0.0532278469839368, 0.0532278469839368, 0.0486958076350722, 0.0486958076350722,
+    0.0438709081856733, 0.0438709081856733, 0.0387821679744720, 0.0387821679744720,
+    0.0334601952825478, 0.0334601952825478, 0.0279370069800234, 0.0279370069800234,
+    0.0222458491941670, 0.0222458491941670, 0.0164210583819079, 0.0164210583819079,
+    0.0104982845311528, 0.0104982845311528, 0.0045212770985332, 0.0045212770985332};
+
+// Pulse kernels of the TDElementRampHeaviLinearT element loop: kernel 5, the cubic B-spline
+// (Compute(5): Int_Stokes_CubicBSpline_Linear, src/GreenFunctionT.cpp:625-658, 20-point time
+// rule), and kernel 6, the ramped Heaviside (Int_Stokes_RampedHeaviside_Linear, :589-622,
+// 40-point time rule).  phi_{part+1} blocks of pair (i, e) at time t at every spatial point; the
+// Kelvin parts go to the C-term (assemble_step_dense).
 template <int ENND>
-__device__ __forceinline__ void integrate_pair_bspline(const double* coords, const int* ien,
-                                                       const double* geo, const double* gauss,
-                                                       int nq, int i, int e, double t, double dt,
-                                                       const Material& m, int part, double* out)
+__device__ __forceinline__ void integrate_pair_pulse(const double* coords, const int* ien,
+                                                     const double* geo, const double* gauss,
+                                                     int nq, int i, int e, double t, double dt,
+                                                     const Material& m, int part, int kernel,
+                                                     double* out)
 {
     const double coll[3] = {coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]};
     double X[3 * ENND];
@@ -475,7 +503,9 @@ __device__ __forceinline__ void integrate_pair_bspline(const double* coords, con
     for (int a = 0; a < ENND; a++)
 #pragma unroll
         for (int v = 0; v < kVals; v++) acc[a][v] = 0.0;
-    const double bw = 0.7 * dt;
+    const bool ramp = kernel == 6;
+    const double bw = 0.7 * dt, slope = 2 / dt;
+    const int ntq = ramp ? 40 : 20;
     for_each_point<ENND>(X, sing, geo + (size_t)e * nq * nq * (7 + ENND), gauss, nq,
                          [&](const double* x, const double* n, double wJ, const double* N) {
                              double d[3] = {x[0] - coll[0], x[1] - coll[1], x[2] - coll[2]};
@@ -487,12 +517,16 @@ __device__ __forceinline__ void integrate_pair_bspline(const double* coords, con
                              double blk[kVals];
 #pragma unroll
                              for (int v = 0; v < kVals; v++) blk[v] = 0.0;
-                             for (int q = 0; q < 20; q++) {
-                                 const double ab = kTime20x[q], w = kTime20w[q];
+                             for (int q = 0; q < ntq; q++) {
+                                 const double ab = ramp ? kTime40x[q] : kTime20x[q];
+                                 const double w = ramp ? kTime40w[q] : kTime20w[q];
                                  const double phi = part == 0 ? 0.5 * (1 - ab) : 0.5 * (1 + ab);
                                  const double tt = 0.5 * (1 + ab) * dt;
                                  double b[kVals];
-                                 bspline_blocks(t - tt, bw, r, d, n, m, b);
+                                 if (ramp)
+                                     ramp_blocks(t - tt, slope, r, d, n, m, b);
+                                 else
+                                     bspline_blocks(t - tt, bw, r, d, n, m, b);
                                  const double wt = w * (dt * 0.5) * phi;
 #pragma unroll
                                  for (int v = 0; v < kVals; v++) blk[v] += wt * b[v];
@@ -553,9 +587,9 @@ __global__ void __launch_bounds__(kBlock)
     const int part = (int)(s & 1);
     const int64_t p = s >> 1;
     const int i = i0 + (int)(p / nel), e = (int)(p % nel);
-    if (kernel == 5) {
-        integrate_pair_bspline<ENND>(coords, ien, geo, gauss, nq, i, e, step * dt, dt, m, part,
-                                     scratch + s * (ENND * kVals));
+    if (kernel == 5 || kernel == 6) {
+        integrate_pair_pulse<ENND>(coords, ien, geo, gauss, nq, i, e, step * dt, dt, m, part,
+                                   kernel, scratch + s * (ENND * kVals));
         return;
     }
     const int heaviside = kernel == 2;
@@ -797,8 +831,9 @@ void assemble_step_dense(Context& c, int step, int lower, int upper, double* dG1
 {
     compute_cterm(c);
     // C-term (Kelvin * dt/2 summed over the elements) into H1 and H2: Step kernel at step 1
-    // only, Heaviside every step; B-spline every step with its own phi_1 / phi_2 weights of the
-    // spline pulse, Int_CubicB_Linear (src/GreenFunctionT.cpp:733-759), over dt/2
+    // only, Heaviside every step; B-spline and ramp every step with the phi_1 / phi_2 weights of
+    // their load pulse, Int_CubicB_Linear (src/GreenFunctionT.cpp:733-759) and
+    // Int_RampedHeaviside_Linear (:705-731), over dt/2
     double c1s = 0.0, c2s = 0.0;
     if (c.kernel == 2 || (c.kernel == 4 && step == 1)) {
         c1s = c2s = 1.0;
@@ -812,6 +847,19 @@ void assemble_step_dense(Context& c, int step, int lower, int upper, double* dG1
             cubic_b1(t - phi2 * c.dt, bw, g, dg);
             g1 += phi1 * g * w * J;
             g2 += phi2 * g * w * J;
+        }
+        c1s = g1 / (0.5 * c.dt);
+        c2s = g2 / (0.5 * c.dt);
+    } else if (c.kernel == 6) {
+        const double t = step * c.dt, k = 2 / c.dt, J = c.dt * 0.5;
+        const double *gx, *gw;
+        gauss_table(40, &gx, &gw);
+        double g1 = 0.0, g2 = 0.0;
+        for (int q = 0; q < 40; q++) {
+            const double phi1 = 0.5 * (1 - gx[q]), phi2 = 0.5 * (1 + gx[q]);
+            const double g = ramp_load(t - phi2 * c.dt, k);
+            g1 += phi1 * g * gw[q] * J;
+            g2 += phi2 * g * gw[q] * J;
         }
         c1s = g1 / (0.5 * c.dt);
         c2s = g2 / (0.5 * c.dt);
@@ -1005,19 +1053,22 @@ void build_lag_store(Context& c)
     S.h_hull = P.hull;
     S.row_chunk.upload(P.row_chunk.data(), P.row_chunk.size(), c.stream);
     S.row_valid.upload(P.row_valid.data(), P.row_valid.size(), c.stream);
-    DBuf<int64_t> d_off;
-    DBuf<int2> d_hull;
-    d_off.upload(P.off.data(), P.off.size(), c.stream);
-    d_hull.upload(P.hull.data(), P.hull.size(), c.stream);
+    S.d_off.upload(P.off.data(), P.off.size(), c.stream);
+    S.d_hull.upload(P.hull.data(), P.hull.size(), c.stream);
     const int64_t nn = (int64_t)c.nnd * c.nnd;
-    hipLaunchKernelGGL(k_fill_idx, dim3(grid_for(nn)), dim3(kBlock), 0, c.stream, d_hull.p,
-                       d_off.p, c.nnd, S.idx.p);
+    hipLaunchKernelGGL(k_fill_idx, dim3(grid_for(nn)), dim3(kBlock), 0, c.stream, S.d_hull.p,
+                       S.d_off.p, c.nnd, S.idx.p);
     TD_HIP(hipGetLastError());
     if (c.ennd == 4)
         fill_store<4>(c, P);
     else
         fill_store<8>(c, P);
     TD_HIP(hipStreamSynchronize(c.stream));
+    if (c.precision != 1) {
+        // only the mixed path's tile-pair copy reads them again
+        S.d_off.reset();
+        S.d_hull.reset();
+    }
     c.store_ready = true;
     c.tm.store_ms +=
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

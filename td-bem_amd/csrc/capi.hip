@@ -78,6 +78,8 @@ const double kG40w[40] = {
     0.0222458491941670, 0.0222458491941670, 0.0164210583819079, 0.0164210583819079,
     0.0104982845311528, 0.0104982845311528, 0.0045212770985332, 0.0045212770985332};
 
+}  // namespace
+
 bool gauss_table(int nq, const double** x, const double** w)
 {
     switch (nq) {
@@ -88,6 +90,8 @@ bool gauss_table(int nq, const double** x, const double** w)
     default: return false;
     }
 }
+
+namespace {
 
 int max_step(const std::vector<double>& X, int nnd, double E, double nu, double rho, double dt)
 {
@@ -213,7 +217,7 @@ int tdbem_create(int nnd, const double* coords, int nel, const int* ien, int ety
             throw std::invalid_argument(
                 "ShapeFunctionT::SetNQ, Unsupported number of Gaussian points");
         if (kernel != TDBEM_KERNEL_STEP && kernel != TDBEM_KERNEL_HEAVISIDE &&
-            kernel != TDBEM_KERNEL_BSPLINE)
+            kernel != TDBEM_KERNEL_BSPLINE && kernel != TDBEM_KERNEL_RAMP)
             throw std::invalid_argument("unsupported temporal kernel");
         if (!(dt > 0) || numstep < 0 || !(E > 0) || !(rho > 0) || !(nu > -1 && nu < 0.5))
             throw std::invalid_argument("bad material or time step");
@@ -457,6 +461,23 @@ int tdbem_mfma_f64_selftest(const double* A, const double* B, double* D)
     try {
         mfma_f64_selftest(A, B, D);
         return TDBEM_OK;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        return TDBEM_ERR_HIP;
+    }
+}
+
+int tdbem_dense_solve(int n, const double* A, int nrhs, double* B, int mode)
+{
+    if (n < 1 || nrhs < 1 || !A || !B || (mode & ~3)) return TDBEM_ERR_ARG;
+    try {
+        dense_solve(n, A, nrhs, B, (mode & TDBEM_SOLVE_INVERSE) != 0,
+                    (mode & TDBEM_SOLVE_PIVOTED) != 0);
+        return TDBEM_OK;
+    } catch (const std::runtime_error& e) {
+        g_create_error = e.what();
+        return std::string(e.what()).find("singular") != std::string::npos ? TDBEM_ERR_NUMERIC
+                                                                           : TDBEM_ERR_HIP;
     } catch (const std::exception& e) {
         g_create_error = e.what();
         return TDBEM_ERR_HIP;

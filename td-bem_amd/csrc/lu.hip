@@ -1,14 +1,18 @@
-// lu.hip — the step-0 solve: dense FP64 LU with partial pivoting on the GPU, and solves with it.
+// lu.hip — the step-0 solve: dense FP64 LU on the GPU, and solves with it.
 //
 // Stands in for KSPSetOperators + KSPSolve (src/SolverStepLinearT.cpp:503-504).  The reference
 // refactors every step; here the n >= 3 system matrix (2 a1 + a3) M_0 + a1 M_1 (after the column
 // exchange) is the same every step, so it is factored once and inverted once, and each step is
-// one GEMV with the inverse.
+// one product with the inverse.
 //
-// Blocked right-looking LU (panel width 64): a recursive panel factorisation with one-workgroup
-// leaves, row interchanges applied to the other columns, a triangular solve for the U12 block row
-// and an LDS-tiled FP64 GEMM for the trailing matrix.  Solves use the same pieces column-blocked
-// over the right-hand sides.
+// lu_factor first runs a blocked LU without row interchanges (panel width 64: diagonal block in
+// LDS, the rows below and the block row to the right by triangular solves, an LDS-tiled FP64 GEMM
+// for the trailing matrix) and checks every multiplier for |l| <= 1: then partial pivoting would
+// have kept every row in place and the factors are the partially pivoted ones.  Otherwise it
+// restores the matrix and runs blocked right-looking LU with partial pivoting: a recursive panel
+// factorisation with one-workgroup register leaves, row interchanges applied to the other
+// columns, a triangular solve for the U12 block row and the same GEMM.  Solves use the same
+// pieces column-blocked over the right-hand sides.
 #include <algorithm>
 #include <stdexcept>
 #include <vector>
@@ -86,6 +90,8 @@ __global__ void __launch_bounds__(kLeafThreads)
         int p = si[0];
 #pragma unroll
         for (int q = 1; q < kLeafThreads / 64; q++) argmax_merge(pm, p, sv[q], si[q]);
+        // no candidate (a column of NaN after a zero pivot): keep the row, the matrix is singular
+        if (p >= n) p = c;
         if (tid == 0) {
             piv[c] = p;
             if (!(pm > 0.0)) *status = 1;
@@ -118,10 +124,11 @@ __global__ void __launch_bounds__(kLeafThreads)
             for (int q = 0; q < kLeafUnroll; q++) {
                 const int r = r0 + q * kLeafThreads;
                 if (r >= n) continue;
-                double l = 0.0;
+                double xc = 0.0;
 #pragma unroll
                 for (int j = 0; j < kLeafW; j++)
-                    if (j == cc) l = x[q][j] / d;
+                    if (j == cc) xc = x[q][j];
+                const double l = xc / d;
                 double* row = A + (int64_t)r * n + c0;
 #pragma unroll
                 for (int j = 0; j < kLeafW; j++) {
@@ -187,6 +194,8 @@ __global__ void __launch_bounds__(kLeafThreads)
         int p = si[0];
 #pragma unroll
         for (int q = 1; q < kLeafThreads / 64; q++) argmax_merge(pm, p, sv[q], si[q]);
+        // no candidate (a column of NaN after a zero pivot): keep the row, the matrix is singular
+        if (p >= n) p = c;
         if (tid == 0) {
             piv[c] = p;
             if (!(pm > 0.0)) *status = 1;
@@ -223,10 +232,11 @@ __global__ void __launch_bounds__(kLeafThreads)
         for (int q = 0; q < RPT; q++) {
             const int lr = tid + q * kLeafThreads;
             if (lr <= cc || lr >= m) continue;
-            double l = 0.0;
+            double xc = 0.0;
 #pragma unroll
             for (int j = 0; j < kLeafW; j++)
-                if (j == cc) l = x[q][j] / d;
+                if (j == cc) xc = x[q][j];
+            const double l = xc / d;
 #pragma unroll
             for (int j = 0; j < kLeafW; j++) {
                 if (j == cc)
@@ -487,9 +497,142 @@ void panel_rec(double* A, int n, int c0, int w, int j0, int pnb, int* piv, int* 
     panel_rec(A, n, c0 + h, w - h, j0, pnb, piv, status, s);
 }
 
+// ---------------------------------------------------------------------------------------------
+// Blocked LU without row interchanges.  Partial pivoting leaves a row in place exactly when no
+// entry below the diagonal exceeds the diagonal in magnitude (ties keep the diagonal), i.e. when
+// every multiplier satisfies |l| <= 1.  The step-0 matrices of the march are of that kind (their
+// multipliers stay well below 1), so lu_factor first factors without interchanges, checking
+// |l| <= 1 for every multiplier, and falls back to the pivoted panels only when a check fails.
+// Per panel of NB columns: the NB x NB diagonal block in LDS (one wavefront), the rows below it
+// against U11 (thread per row), the block row right of it against L11, and the trailing GEMM.
+// ---------------------------------------------------------------------------------------------
+
+// Diagonal block A[j0 .. j0 + nb) x [j0 .. j0 + nb), unit-lower L and upper U in place, in LDS.
+// Four wavefronts: lane r of wavefront q updates row r in the columns 16 q .. 16 q + 15.
+__global__ void __launch_bounds__(256)
+    k_diag_nopiv(double* __restrict__ A, int n, int j0, int nb, unsigned* __restrict__ bad)
+{
+    __shared__ double S[NB][NB + 1];
+    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
+        const int r = e / NB, c = e % NB;
+        S[r][c] = (r < nb && c < nb) ? A[(int64_t)(j0 + r) * n + j0 + c] : 0.0;
+    }
+    __syncthreads();
+    const int r = threadIdx.x & 63, q0 = (threadIdx.x >> 6) * 16;
+    bool ok = true;
+    for (int c = 0; c < nb; c++) {
+        double l = 0.0;
+        const bool act = r > c && r < nb;
+        if (act) {
+            l = S[r][c] / S[c][c];
+            ok = ok && fabs(l) <= 1.0;   // false for NaN as well
+        }
+        __syncthreads();                 // every S[r][c] read before the owner replaces it
+        if (act) {
+#pragma unroll
+            for (int jj = 0; jj < 16; jj++) {
+                const int j = q0 + jj;
+                if (j == c)
+                    S[r][j] = l;
+                else if (j > c)
+                    S[r][j] -= l * S[c][j];
+            }
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
+        const int rr = e / NB, c = e % NB;
+        if (rr < nb && c < nb) A[(int64_t)(j0 + rr) * n + j0 + c] = S[rr][c];
+    }
+    if (!ok) atomicOr(bad, 1u);
+}
+
+// L21 = A21 U11^{-1} for the rows [j0 + nb, n): each thread solves one row x U11 = a,
+// x_c = (a_c - sum_{k<c} x_k U_kc) / U_cc, with U11 in LDS (identity past nb).
+__global__ void __launch_bounds__(64)
+    k_trsm_rows_nopiv(double* __restrict__ A, int n, int j0, int nb, unsigned* __restrict__ bad)
+{
+    __shared__ double Us[NB][NB + 1];
+    for (int q = threadIdx.x; q < NB * NB; q += blockDim.x) {
+        const int r = q / NB, c = q % NB;
+        Us[r][c] = (r < nb && c < nb) ? A[(int64_t)(j0 + r) * n + j0 + c] : (r == c ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    const int row = j0 + nb + blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= n) return;
+    double* a = A + (int64_t)row * n + j0;
+    double x[NB];
+#pragma unroll
+    for (int c = 0; c < NB; c++) x[c] = c < nb ? a[c] : 0.0;
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < NB; c++) {
+        double v = x[c];
+#pragma unroll
+        for (int k = 0; k < c; k++) v -= x[k] * Us[k][c];
+        x[c] = v / Us[c][c];
+        ok = ok && (c >= nb || fabs(x[c]) <= 1.0);
+    }
+#pragma unroll
+    for (int c = 0; c < NB; c++)
+        if (c < nb) a[c] = x[c];
+    if (!ok) atomicOr(bad, 1u);
+}
+
+// Returns false (A partly overwritten) when some multiplier has |l| > 1 or is not finite.
+bool lu_nopiv(double* A, int n, hipStream_t s)
+{
+    DBuf<unsigned> bad;
+    bad.alloc(1);
+    TD_HIP(hipMemsetAsync(bad.p, 0, sizeof(unsigned), s));
+    for (int j0 = 0; j0 < n; j0 += NB) {
+        const int nb = std::min(NB, n - j0);
+        hipLaunchKernelGGL(k_diag_nopiv, dim3(1), dim3(256), 0, s, A, n, j0, nb, bad.p);
+        TD_HIP(hipGetLastError());
+        const int rest = n - j0 - nb;
+        if (rest <= 0) break;
+        hipLaunchKernelGGL(k_trsm_rows_nopiv, dim3((rest + 63) / 64), dim3(64), 0, s, A, n, j0,
+                           nb, bad.p);
+        TD_HIP(hipGetLastError());
+        const int tpb = 128;
+        hipLaunchKernelGGL(k_trsm_lower_unit, dim3((rest + tpb - 1) / tpb), dim3(tpb), 0, s, A,
+                           n, j0, nb, A, n, j0, j0 + nb, rest);
+        TD_HIP(hipGetLastError());
+        gemm_sub(rest, rest, nb, A + (int64_t)(j0 + nb) * n + j0, n,
+                 A + (int64_t)j0 * n + j0 + nb, n, A + (int64_t)(j0 + nb) * n + j0 + nb, n, s);
+    }
+    unsigned h_bad = 0;
+    TD_HIP(hipMemcpyAsync(&h_bad, bad.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    TD_HIP(hipStreamSynchronize(s));
+    return h_bad == 0;
+}
+
+__global__ void k_iota(int* __restrict__ piv, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) piv[i] = i;
+}
+
 }  // namespace
 
 void lu_factor(double* A, int n, int* piv, hipStream_t s)
+{
+    {
+        // without interchanges first (see lu_nopiv); the copy restores A when a check fails
+        DBuf<double> keep;
+        keep.alloc((size_t)n * n);
+        TD_HIP(hipMemcpyAsync(keep.p, A, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        if (lu_nopiv(A, n, s)) {
+            hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, piv, n);
+            TD_HIP(hipGetLastError());
+            return;
+        }
+        TD_HIP(hipMemcpyAsync(A, keep.p, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    lu_factor_pivoted(A, n, piv, s);
+}
+
+void lu_factor_pivoted(double* A, int n, int* piv, hipStream_t s)
 {
     DBuf<int> status;
     status.alloc(1);
@@ -546,6 +689,42 @@ void lu_inverse(const double* LU, int n, const int* piv, double* Ainv, hipStream
     TD_HIP(hipGetLastError());
     solve_inplace(LU, n, Ainv, n, s);
     TD_HIP(hipStreamSynchronize(s));
+}
+
+void dense_solve(int n, const double* hA, int nrhs, double* hB, bool inverse, bool pivoted)
+{
+    hipStream_t s = nullptr;
+    DBuf<double> A, B;
+    DBuf<int> piv;
+    A.upload(hA, (size_t)n * n, s);
+    B.upload(hB, (size_t)n * nrhs, s);
+    piv.alloc(n);
+    if (pivoted)
+        lu_factor_pivoted(A.p, n, piv.p, s);
+    else
+        lu_factor(A.p, n, piv.p, s);
+    if (!inverse) {
+        lu_solve(A.p, n, piv.p, B.p, nrhs, s);
+    } else {
+        DBuf<double> Ai, X;
+        Ai.alloc((size_t)n * n);
+        X.alloc((size_t)n * nrhs);
+        lu_inverse(A.p, n, piv.p, Ai.p, s);
+        // X = Ai B, column by column of B through the GEMV (diagnostic, not fast)
+        DBuf<double> col, out;
+        col.alloc(n);
+        out.alloc(n);
+        std::vector<double> hc(n), ho(n);
+        for (int r = 0; r < nrhs; r++) {
+            for (int i = 0; i < n; i++) hc[i] = hB[(size_t)i * nrhs + r];
+            TD_HIP(hipMemcpy(col.p, hc.data(), n * sizeof(double), hipMemcpyHostToDevice));
+            gemv(Ai.p, n, col.p, out.p, 0.0, s);
+            TD_HIP(hipMemcpy(ho.data(), out.p, n * sizeof(double), hipMemcpyDeviceToHost));
+            for (int i = 0; i < n; i++) hB[(size_t)i * nrhs + r] = ho[i];
+        }
+        return;
+    }
+    TD_HIP(hipMemcpy(hB, B.p, (size_t)n * nrhs * sizeof(double), hipMemcpyDeviceToHost));
 }
 
 void gemv(const double* A, int n, const double* x, double* y, double beta, hipStream_t s)

@@ -44,52 +44,51 @@ __device__ __forceinline__ int row_of_chunk(const int64_t* row_chunk, int nnd, i
     return lo;
 }
 
-// FP64 node-pair store -> split-BF16 sub-tiles.  One 64-thread block per store chunk.  Element
-// (r, c) of a block lives in sub-tile (r / 16, c / 32) at lane (r % 16) + 16 ((c % 32) / 8),
-// element c % 8 (16x16x32 A-operand order).
-__global__ void __launch_bounds__(kChunk)
-    k_tp_fill(const double* __restrict__ vals, const uint32_t* __restrict__ idx,
-              const int64_t* __restrict__ row_chunk, const int64_t* __restrict__ row_valid,
-              int nnd, const int* __restrict__ pos, int nT, const int64_t* __restrict__ pair_off,
-              const int* __restrict__ pair_lo, const uint32_t* __restrict__ sub_off,
-              const uint16_t* __restrict__ mask, unsigned short* __restrict__ tp)
+// FP64 node-pair store -> split-BF16 sub-tiles, gathered: one wavefront per stored sub-tile
+// (row tile I, column tile J, lag k, sub-tile u = 3 m + kt).  Lane L holds block row 16 m + L % 16
+// and the 8 block columns 32 kt + 8 (L / 16) .. + 7 (16x16x32 A-operand order); each element is
+// looked up through the node pair's lag range and first entry, and the lane writes its 16 bytes
+// of each of the three terms.  Block row 3 ii + p, column 6 jj + 3 part + q holds G[p][q]
+// (part 0) or -H[p][q] (part 1) of node pair (perm[16 I + ii], perm[16 J + jj]).
+__global__ void __launch_bounds__(256)
+    k_tp_gather(const double* __restrict__ vals, const int2* __restrict__ hull,
+                const int64_t* __restrict__ off, const int* __restrict__ perm, int nnd,
+                const uint2* __restrict__ sub_id, int64_t nsub, bf16x8* __restrict__ tp)
 {
-    const int64_t c = blockIdx.x;
-    const int i = row_of_chunk(row_chunk, nnd, c);
-    const int lane = threadIdx.x;
-    if ((c - row_chunk[i]) * kChunk + lane >= row_valid[i]) return;
-    const uint32_t id = idx[c * kChunk + lane];
-    const int j = (int)(id >> kKBits), k = (int)(id & kMaxLag);
-    const int pi = pos[i], pj = pos[j];
-    const int I = pi / kTile, ii = pi % kTile, J = pj / kTile, jj = pj % kTile;
-    const int64_t blk = pair_off[(int64_t)I * nT + J] + (k - pair_lo[(int64_t)I * nT + J]);
-    const uint32_t first = sub_off[blk];
-    const unsigned msk = mask[blk];
-    double v[kVals];
-    const double* s = vals + c * (kVals * kChunk) + lane;
-#pragma unroll
-    for (int t = 0; t < kVals; t++) v[t] = s[t * kChunk];
+    const int64_t sidx = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (sidx >= nsub) return;
+    const int lane = threadIdx.x & 63;
+    const uint2 id = sub_id[sidx];
+    const int I = (int)(id.x >> 16), J = (int)(id.x & 0xFFFFu);
+    const int k = (int)(id.y >> 4), u = (int)(id.y & 15u);
+    const int m = u / 3, kt = u % 3;
+    const int r = 16 * m + (lane & 15);
+    const int i = perm[I * kTile + r / 3], p = r % 3;
     const int sym[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-    auto put = [&](int r, int col, const unsigned short* b) {
-        const int bit = (r >> 4) * 3 + (col >> 5);
-        const int q = __popc(msk & ((1u << bit) - 1u));
-        unsigned short* S = tp + ((int64_t)first + q) * kSubElems;
-        const int kk = col & 31;
-        const int o = ((r & 15) + 16 * (kk >> 3)) * 8 + (kk & 7);
+    bf16x8 out[3];
 #pragma unroll
-        for (int t = 0; t < 3; t++) S[t * 512 + o] = b[t];
-    };
-#pragma unroll
-    for (int p = 0; p < 3; p++)
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            unsigned short g[3], h[3];
-            split3(v[sym[p][q]], g);
-            split3(-v[6 + 3 * p + q], h);
-            const int r = 3 * ii + p;
-            put(r, 6 * jj + q, g);
-            put(r, 6 * jj + 3 + q, h);
+    for (int e = 0; e < 8; e++) {
+        const int c = 32 * kt + 8 * (lane >> 4) + e;
+        const int j = perm[J * kTile + c / 6];
+        const int part = (c % 6) / 3, q = c % 3;
+        double v = 0.0;
+        if (i >= 0 && j >= 0) {
+            const int64_t pr = (int64_t)i * nnd + j;
+            const int2 h = hull[pr];
+            if (h.x <= k && k <= h.y) {
+                const int64_t en = off[pr] + (k - h.x);
+                const int comp = part == 0 ? sym[p][q] : 6 + 3 * p + q;
+                v = vals[((en / kChunk) * kVals + comp) * kChunk + en % kChunk];
+                if (part) v = -v;
+            }
         }
+        unsigned short b[3];
+        split3(v, b);
+#pragma unroll
+        for (int t = 0; t < 3; t++) out[t][e] = (short)b[t];
+    }
+#pragma unroll
+    for (int t = 0; t < 3; t++) tp[(sidx * 3 + t) * 64 + lane] = out[t];
 }
 
 // Split-BF16 copy of ring slot `slot` (see zsplit_elem).
@@ -218,6 +217,7 @@ void tp_build(Context& c)
     std::vector<int64_t> off(npair, -1);
     std::vector<uint32_t> sub_off;
     std::vector<uint16_t> msk;
+    std::vector<uint2> sub_id;     // (I << 16 | J, k << 4 | u) of each stored sub-tile
     int64_t nb = 0, nsub = 0, nz = 0;
     for (size_t q = 0; q < npair; q++) {
         for (int u = 0; u < 9; u++) {
@@ -226,10 +226,14 @@ void tp_build(Context& c)
         }
         if (phi[q] < plo[q]) continue;
         off[q] = nb;
+        const uint32_t IJ = ((uint32_t)(q / nT) << 16) | (uint32_t)(q % nT);
         for (int k = plo[q]; k <= phi[q]; k++) {
             uint16_t mk = 0;
             for (int u = 0; u < 9; u++)
-                if (slo[q * 9 + u] <= k && k <= shi[q * 9 + u]) mk |= (uint16_t)(1u << u);
+                if (slo[q * 9 + u] <= k && k <= shi[q * 9 + u]) {
+                    mk |= (uint16_t)(1u << u);
+                    sub_id.push_back(make_uint2(IJ, ((uint32_t)k << 4) | (uint32_t)u));
+                }
             sub_off.push_back((uint32_t)nsub);
             msk.push_back(mk);
             nsub += __builtin_popcount(mk);
@@ -295,18 +299,20 @@ void tp_build(Context& c)
     T.sub_off.upload(sub_off.data(), sub_off.size(), c.stream);
     T.mask.upload(msk.data(), msk.size(), c.stream);
     T.ent.upload(ent.data(), ent.size(), c.stream);
-    DBuf<int64_t> d_off;
-    DBuf<int> d_plo;
-    d_off.upload(off.data(), off.size(), c.stream);
-    d_plo.upload(plo.data(), plo.size(), c.stream);
     T.vals.alloc((size_t)std::max<int64_t>(nsub, 1) * kSubElems);
-    TD_HIP(hipMemsetAsync(T.vals.p, 0, T.vals.bytes(), c.stream));
-    if (c.store.n_chunks > 0)
-        hipLaunchKernelGGL(k_tp_fill, dim3((unsigned)c.store.n_chunks), dim3(kChunk), 0, c.stream,
-                           c.store.vals.p, c.store.idx.p, c.store.row_chunk.p, c.store.row_valid.p,
-                           nnd, T.d_pos.p, nT, d_off.p, d_plo.p, T.sub_off.p, T.mask.p, T.vals.p);
+    DBuf<uint2> d_sub_id;   // lives until the stream is synchronised below
+    if (nsub > 0) {
+        if (!c.store.d_hull.p || !c.store.d_off.p)
+            throw std::logic_error("tile-pair copy needs the store's node-pair tables");
+        d_sub_id.upload(sub_id.data(), sub_id.size(), c.stream);
+        hipLaunchKernelGGL(k_tp_gather, dim3((unsigned)((nsub + 3) / 4)), dim3(256), 0, c.stream,
+                           c.store.vals.p, c.store.d_hull.p, c.store.d_off.p, T.d_perm.p, nnd,
+                           d_sub_id.p, nsub, reinterpret_cast<bf16x8*>(T.vals.p));
+    }
     TD_HIP(hipGetLastError());
     TD_HIP(hipStreamSynchronize(c.stream));
+    c.store.d_hull.reset();
+    c.store.d_off.reset();
     T.ready = true;
 }
 

@@ -14,7 +14,6 @@ constexpr int kTile = 16;                 // nodes per tile of the mixed-precisi
 constexpr int kRows = 3 * kTile;          // 48
 constexpr int kCols = 6 * kTile;          // 96
 constexpr int kSubElems = 3 * 512;        // bf16 per stored sub-tile: 3 terms x 64 lanes x 8
-constexpr int kGroup = 4;                 // row tiles per convolution workgroup (one per wave)
 constexpr int kApplyWaves = 4;            // wavefronts per step-solve workgroup
 
 // f_r(t): src/SolverT.cpp:379-394 returns 1; a per-RHS amplitude and a linear rise over t_rise
@@ -72,10 +71,10 @@ __device__ __forceinline__ void zsplit_elem(const double* __restrict__ ring, int
 
 // One wavefront's share of a convolution chunk: the steps [e0, e1) of its group's merged list,
 // for row tile `w` of the group, at step s; writes its 48 x (16 NT) FP64 partial sums to `out`
-// ([3][NT][64][4]) unless the row tile does not exist.  All four wavefronts of the workgroup
+// ([3][NT][64][4]) unless the row tile does not exist.  All wavefronts of the workgroup
 // call it together (it has a barrier per step): the history k-tiles of a step are loaded from
 // HBM once per workgroup into one of two LDS buffers, one step ahead of their use, and shared by
-// the four row tiles; each wavefront streams its own stored sub-tiles.
+// the group's row tiles; each wavefront streams its own stored sub-tiles.
 template <int NT>
 __device__ __forceinline__ void conv_chunk(const bf16x8* __restrict__ tp,
                                            const TPEntry* __restrict__ ent,
@@ -97,15 +96,15 @@ __device__ __forceinline__ void conv_chunk(const bf16x8* __restrict__ tp,
             for (int v = 0; v < 4; v++) acc[m][n][v] = 0.0;
     // wavefront w stages lines w, w + 4, ... of a block; line L = (term * 3 + k-tile) * NT + n
     bf16x8 zn[kPer];
-    auto zload = [&](const TPEntry& en) {
-        const int J = (int)(en.id >> kKBits), k = (int)(en.id & kMaxLag);
+    auto zload = [&](uint32_t id, unsigned kmask) {
+        const int J = (int)(id >> kKBits), k = (int)(id & kMaxLag);
         int slot = (s - k) % Rring;
         if (slot < 0) slot += Rring;
         const bf16x8* Zg = zs + ((int64_t)slot * nT + J) * (kLines * 64) + lane;
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
             const int L = q * kGroup + w;
-            if (L < kLines && ((en.kmask >> ((L / NT) % 3)) & 1u)) zn[q] = Zg[L * 64];
+            if (L < kLines && ((kmask >> ((L / NT) % 3)) & 1u)) zn[q] = Zg[L * 64];
         }
     };
     auto zstore = [&](int buf, unsigned kmask) {
@@ -115,17 +114,17 @@ __device__ __forceinline__ void conv_chunk(const bf16x8* __restrict__ tp,
             if (L < kLines && ((kmask >> ((L / NT) % 3)) & 1u)) zl[buf][L * 64 + lane] = zn[q];
         }
     };
+    // w is the same in every lane of a wavefront: the entry fields are scalar loads
+    w = __builtin_amdgcn_readfirstlane(w);
     if (e0 < e1) {
-        const TPEntry en = ent[e0];
-        zload(en);
-        zstore(0, en.kmask);
+        zload(ent[e0].id, ent[e0].kmask);
+        zstore(0, ent[e0].kmask);
     }
     __syncthreads();
     for (int64_t e = e0; e < e1; e++) {
         const int cur = (int)((e - e0) & 1);
-        const TPEntry en = ent[e];
-        const unsigned msk = w == 0 ? en.msk[0] : w == 1 ? en.msk[1] : w == 2 ? en.msk[2] : en.msk[3];
-        const uint32_t first = w == 0 ? en.sub[0] : w == 1 ? en.sub[1] : w == 2 ? en.sub[2] : en.sub[3];
+        const unsigned msk = ent[e].msk[w];
+        const uint32_t first = ent[e].sub[w];
         // this wavefront's stored sub-tiles of the step (read once: non-temporal, so they do not
         // evict history blocks), then the next step's history lines, then the products
         const bf16x8* A = tp + (int64_t)first * (kSubElems / 8) + lane;
@@ -142,9 +141,8 @@ __device__ __forceinline__ void conv_chunk(const bf16x8* __restrict__ tp,
         const bool more = e + 1 < e1;
         unsigned knext = 0;
         if (more) {
-            const TPEntry nx = ent[e + 1];
-            knext = nx.kmask;
-            zload(nx);
+            knext = ent[e + 1].kmask;
+            zload(ent[e + 1].id, knext);
         }
         if (msk) {
             const bf16x8* Z = zl[cur] + lane;

@@ -255,16 +255,16 @@ __device__ __forceinline__ double cubic_b1_int(double r, double cp, double cs, d
     return B;
 }
 
-// Stokes_CubicBSpline (src/GreenFunctionT.cpp:522-585) at time t, written as the 15 block
-// values of one point in the global block layout (G[p][q] = U[q][p], H[p][q] = T[q][p]).
-__device__ __forceinline__ void bspline_blocks(double t, double bw, double r, const double* rh,
-                                               const double* n, const Material& m, double* out)
+// The Stokes displacement and traction of a pulsed point load, written as the 15 block values of
+// one point in the global block layout (G[p][q] = U[q][p], H[p][q] = T[q][p]), from the load's
+// value GC and rate DGC at the two retarded times t - r/c_p, t - r/c_s and the integral I1 of the
+// load against the near-field term: the common tail of Stokes_CubicBSpline
+// (src/GreenFunctionT.cpp:522-585) and Stokes_RampedHeaviside (:449-519).
+__device__ __forceinline__ void pulse_blocks(double GC1, double GC2, double DGC1, double DGC2,
+                                             double I1, double r, const double* rh,
+                                             const double* n, const Material& m, double* out)
 {
     const double cp = m.cp, cs = m.cs;
-    double GC1, GC2, DGC1, DGC2;
-    cubic_b1(t - r / cp, bw, GC1, DGC1);
-    cubic_b1(t - r / cs, bw, GC2, DGC2);
-    const double I1 = cubic_b1_int(r, cp, cs, t, bw);
     const double DI2 = GC1 / (cp * cp) - GC2 / (cs * cs);
     const double DI3 = GC2;
     const double k = cs / cp;
@@ -298,6 +298,54 @@ __device__ __forceinline__ void bspline_blocks(double t, double bw, double r, co
             out[6 + 3 * p + q] =
                 (s1 * I1 + s2 * TI2 + s3 * TI3 + s4 * TI4 + s5 * TI5) / (4.0 * kPi);
         }
+}
+
+// Stokes_CubicBSpline (src/GreenFunctionT.cpp:522-585) at time t: spline pulse of width bw.
+__device__ __forceinline__ void bspline_blocks(double t, double bw, double r, const double* rh,
+                                               const double* n, const Material& m, double* out)
+{
+    double GC1, GC2, DGC1, DGC2;
+    cubic_b1(t - r / m.cp, bw, GC1, DGC1);
+    cubic_b1(t - r / m.cs, bw, GC2, DGC2);
+    pulse_blocks(GC1, GC2, DGC1, DGC2, cubic_b1_int(r, m.cp, m.cs, t, bw), r, rh, n, m, out);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Ramped-Heaviside temporal kernel (src/GreenFunctionT.cpp:449-519, 589-622, 661-680, 705-731):
+// the load g(t) = min(max(k t, 0), 1) with slope k = 2/dt (SetTime, :111-117).  No Compute case
+// of the reference reaches these functions; here they are kernel 6, in the element loop of
+// TDElementRampHeaviLinearT with the 40-point time rule.
+// ---------------------------------------------------------------------------------------------
+__host__ __device__ inline double ramp_load(double t, double k)
+{
+    // src/GreenFunctionT.cpp:722
+    return k * t * (heav(t) - heav(t - 1 / k)) + heav(t - 1 / k);
+}
+
+// One wave's part of I1 in Stokes_RampedHeaviside (src/GreenFunctionT.cpp:466-469): the ramp
+// k (t - tau) for t - r/c > tau > max(0, ...) and the constant 1 after it, against the
+// near-field time function (tau^2 / r^2 - 1 / c^2).
+__device__ __forceinline__ double ramp_i1(double t, double k, double r, double c)
+{
+    const double tr = t / r, ic = 1 / c;
+    const double ramp = k * ((tr * tr - ic * ic) * t / 2 - (tr * tr * tr - ic * ic * ic) * r / 3);
+    const double tk = t - 1 / k, tkr = tk / r;
+    const double tail = (1 - k * t) * (tkr * tkr - ic * ic) / 2 + k * r * (tkr * tkr * tkr - ic * ic * ic) / 3;
+    return ramp * heav(t - r / c) + tail * heav(tk - r / c);
+}
+
+// Stokes_RampedHeaviside (src/GreenFunctionT.cpp:449-519) at time t.
+__device__ __forceinline__ void ramp_blocks(double t, double k, double r, const double* rh,
+                                            const double* n, const Material& m, double* out)
+{
+    const double cp = m.cp, cs = m.cs;
+    const double HC1 = heav(t - r / cp), HC2 = heav(t - r / cs);
+    const double HKC1 = heav(t - 1 / k - r / cp), HKC2 = heav(t - 1 / k - r / cs);
+    const double GC1 = k * (t - r / cp) * (HC1 - HKC1) + HKC1;
+    const double GC2 = k * (t - r / cs) * (HC2 - HKC2) + HKC2;
+    const double DGC1 = k * (HC1 - HKC1), DGC2 = k * (HC2 - HKC2);
+    const double I1 = ramp_i1(t, k, r, cp) - ramp_i1(t, k, r, cs);
+    pulse_blocks(GC1, GC2, DGC1, DGC2, I1, r, rh, n, m, out);
 }
 
 // Kelvin traction * dt/2 (src/GreenFunctionT.cpp:205-231), returned transposed as the C-term

@@ -102,14 +102,21 @@ struct LagStore {
     std::vector<int64_t> h_row_chunk, h_row_valid;
     std::vector<int2> h_hull;     // [nnd*nnd] lag range of each node pair
     int64_t nonzero_entries = 0;  // before padding
+    // node-pair tables on the device, kept from the build of a mixed-precision store until the
+    // tile-pair copy has been gathered from it: lag range and first entry of each pair
+    DBuf<int2> d_hull;            // [nnd*nnd]
+    DBuf<int64_t> d_off;          // [nnd*nnd]
 };
 
+// Row tiles per group of the mixed-precision convolution: one wavefront each in a workgroup.
+constexpr int kGroup = 4;
+
 // One step of a tile group's merged list: column tile J and lag k (id = J << 10 | k); for each
-// of the group's 4 row tiles the first stored sub-tile of its block at (J, k) and the block's
+// of the group's row tiles the first stored sub-tile of its block at (J, k) and the block's
 // sub-tile mask (0: no block); the k-tiles any of them uses.
 struct TPEntry {
-    uint32_t sub[4];
-    uint16_t msk[4];
+    uint32_t sub[kGroup];
+    uint16_t msk[kGroup];
     uint32_t id;
     uint32_t kmask;
 };
@@ -272,7 +279,16 @@ double stream_copy(size_t bytes, int launches);
 
 // lu.hip: in-place LU with partial pivoting of a dense row-major n x n device matrix, and
 // solves with it.  piv is a device int array of length n.
+// Gauss-Legendre rule of the reference for nq in {6, 10, 20, 40} (capi.hip); false otherwise.
+bool gauss_table(int nq, const double** x, const double** w);
+
+// P A = L U in place (row-major, piv as LAPACK's getrf): without interchanges when partial
+// pivoting would make none, else with them; lu_factor_pivoted always takes the pivoted path.
 void lu_factor(double* A, int n, int* piv, hipStream_t s);
+void lu_factor_pivoted(double* A, int n, int* piv, hipStream_t s);
+// Host-to-host X = A^{-1} B with lu_factor + lu_solve, or through lu_inverse (diagnostic);
+// pivoted: factor with lu_factor_pivoted.
+void dense_solve(int n, const double* A, int nrhs, double* B, bool inverse, bool pivoted);
 void lu_solve(const double* LU, int n, const int* piv, double* B, int ncols, hipStream_t s);
 void lu_inverse(const double* LU, int n, const int* piv, double* Ainv, hipStream_t s);
 

@@ -13,9 +13,11 @@ PKG = Path(__file__).resolve().parents[1]
 # TDBEM_LIBRARY selects another build of the same library (same C ABI)
 LIB_PATH = Path(os.environ.get("TDBEM_LIBRARY", str(PKG / "libtdbem.so")))
 
+ERR_ARG, ERR_HIP, ERR_STATE, ERR_NUMERIC = -1, -2, -3, -4
 KERNEL_HEAVISIDE = 2
 KERNEL_STEP = 4
 KERNEL_BSPLINE = 5
+KERNEL_RAMP = 6
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 
@@ -64,6 +66,7 @@ def lib():
         L.tdbem_mfma_selftest.argtypes = [C.POINTER(C.c_float)] * 3
         L.tdbem_stream_copy.argtypes = [C.c_size_t, C.c_int, dp]
         L.tdbem_mfma_f64_selftest.argtypes = [dp, dp, dp]
+        L.tdbem_dense_solve.argtypes = [C.c_int, dp, C.c_int, dp, C.c_int]
         L.tdbem_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
         L.tdbem_batch_tune.argtypes = [vp, dp]
         L.tdbem_batch_set_params.argtypes = [vp, C.c_int, C.c_int]
@@ -272,6 +275,21 @@ def mfma_f64_selftest(A, B):
     if rc:
         raise TdbemError(rc, lib().tdbem_last_error(None).decode())
     return D
+
+
+def dense_solve(A, B, inverse=False, pivoted=False):
+    """X = A^{-1} B on the GPU with the step-0 solver: the march's LU and triangular solves
+    (inverse=False) or the explicit inverse the march applies (inverse=True); pivoted=True forces
+    partial pivoting.  B is n or n x nrhs."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0]
+    X = np.array(B, dtype=np.float64, order="C", copy=True)
+    nrhs = 1 if X.ndim == 1 else X.shape[1]
+    mode = (1 if inverse else 0) | (2 if pivoted else 0)
+    rc = lib().tdbem_dense_solve(C.c_int(n), _d(A), C.c_int(nrhs), _d(X), C.c_int(mode))
+    if rc:
+        raise TdbemError(rc, lib().tdbem_last_error(None).decode())
+    return X
 
 
 class Batch:

@@ -2,7 +2,7 @@
 (oracle/_ref/libtdbem_ref.so, built from /root/reference/src by `make -C oracle ref`).
 
 Run in the development container, where the reference tree is mounted:
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [green_ramp] [elements]
 Each fixture holds the inputs and the reference's outputs; no reference source is copied.
 """
 import sys
@@ -43,9 +43,41 @@ def element_cases(seed=11):
     return rows
 
 
-def main():
+def green_ramp_cases(seed=17, count=120):
+    """The ramped-Heaviside functions of GreenFunctionT at random points and times: the 40-point
+    time integrals (DU1|DU2|DT1|DT2|K1|K2), Stokes_RampedHeaviside at a time between steps
+    (U|T), and the Int_RampedHeaviside_Linear weights (g1|g2)."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for _ in range(count):
+        E, nu, rho = rng.uniform(0.5, 3), rng.uniform(0.0, 0.45), rng.uniform(0.5, 2)
+        d = rng.normal(size=3) * rng.uniform(0.05, 3)
+        nrm = rng.normal(size=3)
+        dt = rng.uniform(0.01, 0.5)
+        t = dt * rng.integers(1, 40)
+        tp = t + dt * rng.uniform(-1, 1)
+        lin = O.green(6, E, nu, rho, d, nrm, t, dt, use_ref=True)
+        U, T, g = O.ramp_point(E, nu, rho, d, nrm, tp, dt, use_ref=True)
+        rows.append((np.array([E, nu, rho, t, tp, dt]), d, nrm,
+                     np.concatenate([x.ravel() for x in lin] + [U.ravel(), T.ravel(), g])))
+    return rows
+
+
+def main(names):
     if O.ref() is None:
         raise SystemExit("oracle/_ref/libtdbem_ref.so is not built (make -C oracle ref)")
+    if "green_ramp" in names:
+        rows = green_ramp_cases()
+        np.savez_compressed(HERE / "green_ramp.npz", scalars=np.array([r[0] for r in rows]),
+                            direction=np.array([r[1] for r in rows]),
+                            normal=np.array([r[2] for r in rows]),
+                            out=np.array([r[3] for r in rows]))
+    if "elements" in names or "assembly" in names:
+        write_elements_and_assembly()
+    print("golden fixtures written to", HERE)
+
+
+def write_elements_and_assembly():
     rows = element_cases()
     np.savez_compressed(
         HERE / "elements.npz",
@@ -69,8 +101,8 @@ def main():
         np.savez_compressed(HERE / f"assemble_{name}.npz", coords=p.coords, ien=p.ien, E=p.E,
                             nu=p.nu, rho=p.rho, dt=p.dt, if_ext=p.if_ext, upper=upper,
                             steps=np.array(steps, np.int32), GH=np.array(mats))
-    print("golden fixtures written to", HERE)
 
 
 if __name__ == "__main__":
-    main()
+    # python tests/golden/make_golden.py [green_ramp] [elements]   (default: all)
+    main(sys.argv[1:] or ["green_ramp", "elements"])

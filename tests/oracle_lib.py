@@ -92,7 +92,13 @@ def green(kernel, E, nu, rho, direction, normal, t, dt, use_ref=False):
     n = _f64(normal)
     if use_ref:
         out = np.zeros(54)
-        ref().ref_green(I(kernel), D(E), D(nu), D(rho), _ptr(d), _ptr(n), D(t), D(dt), _ptr(out))
+        if kernel == 6:
+            pt, g = np.zeros(18), np.zeros(2)
+            ref().ref_green_ramp(D(E), D(nu), D(rho), _ptr(d), _ptr(n), D(t), D(dt), _ptr(out),
+                                 _ptr(pt), _ptr(g))
+        else:
+            ref().ref_green(I(kernel), D(E), D(nu), D(rho), _ptr(d), _ptr(n), D(t), D(dt),
+                            _ptr(out))
         return [out[9 * i:9 * i + 9].reshape(3, 3) for i in range(6)]
     m = material(E, nu, rho)
     o = [np.zeros(9) for _ in range(6)]
@@ -108,7 +114,29 @@ def green(kernel, E, nu, rho, direction, normal, t, dt, use_ref=False):
     elif kernel == 5:
         lib.or_stokes_bspline_linear(C.byref(m), _ptr(d), _ptr(n), D(t), D(dt), *[_ptr(x) for x in o[:4]])
         lib.or_kelvin_bspline_linear(C.byref(m), _ptr(d), _ptr(n), D(t), D(dt), _ptr(o[4]), _ptr(o[5]))
+    elif kernel == 6:
+        lib.or_stokes_ramp_linear(C.byref(m), _ptr(d), _ptr(n), D(t), D(dt), *[_ptr(x) for x in o[:4]])
+        lib.or_kelvin_ramp_linear(C.byref(m), _ptr(d), _ptr(n), D(t), D(dt), _ptr(o[4]), _ptr(o[5]))
+    else:
+        raise ValueError(f"kernel {kernel}")
     return [x.reshape(3, 3) for x in o]
+
+
+def ramp_point(E, nu, rho, direction, normal, t, dt, use_ref=False):
+    """Stokes_RampedHeaviside at time t (slope 2/dt): U, T (3x3 each), and the
+    Int_RampedHeaviside_Linear weights (g1, g2) at the same t."""
+    d = _f64(direction)
+    n = _f64(normal)
+    if use_ref:
+        out, pt, g = np.zeros(54), np.zeros(18), np.zeros(2)
+        ref().ref_green_ramp(D(E), D(nu), D(rho), _ptr(d), _ptr(n), D(t), D(dt), _ptr(out),
+                             _ptr(pt), _ptr(g))
+        return pt[:9].reshape(3, 3), pt[9:].reshape(3, 3), g
+    m = material(E, nu, rho)
+    U, T, g = np.zeros(9), np.zeros(9), np.zeros(2)
+    oracle().or_stokes_ramp_point(C.byref(m), _ptr(d), _ptr(n), D(t), D(dt), _ptr(U), _ptr(T))
+    oracle().or_ramp_weights(D(t), D(dt), _ptr(g))
+    return U.reshape(3, 3), T.reshape(3, 3), g
 
 
 def shape_eval(etype, X, e1, e2, use_ref=False):

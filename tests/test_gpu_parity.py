@@ -86,6 +86,25 @@ def test_assemble_step_bspline_matches_oracle(gpu, case, steps):
         assert err <= TOL, f"{case} {name}: rel err {err:.3e}"
 
 
+@pytest.mark.parametrize("case,steps", [("plane1e", [1, 2, 3, 6]),
+                                        ("bar", [1, 2, 5]),
+                                        ("bar8", [1, 3])])
+def test_assemble_step_ramp_matches_oracle(gpu, case, steps):
+    """The ramped-Heaviside kernel (Int_Stokes_RampedHeaviside_Linear and
+    Int_Kelvin_RampedHeaviside_Linear in the TDElementRampHeaviLinearT element loop): Stokes part by
+    the 40-point time rule at every point, Kelvin part into the C-term with the ramp's own
+    phi_1 / phi_2 weights every step."""
+    p = {"plane1e": helpers.plane_1e, "bar": lambda: helpers.small_bar(h=1.0),
+         "bar8": helpers.small_quadratic_bar}[case]()
+    _announce(f"assemble_step ramp {case} nnd={p.nnd} nel={p.nel}")
+    with _ctx(p, kernel=api.KERNEL_RAMP) as ctx:
+        got = [ctx.assemble_step(n) for n in steps]
+    ref = [_oracle_step(p, n, kernel=api.KERNEL_RAMP) for n in steps]
+    for m, name in enumerate(("G1", "G2", "H1", "H2")):
+        err = helpers.rel_fro_set([g[m] for g in got], [r[m] for r in ref])
+        assert err <= TOL, f"{case} {name}: rel err {err:.3e}"
+
+
 @pytest.mark.parametrize("case", ["bar", "sphere", "plane1e"])
 def test_lag_store_matches_oracle(gpu, case):
     p = {"bar": helpers.small_bar, "sphere": helpers.small_sphere,

@@ -153,3 +153,65 @@ def test_bspline_kelvin_weights_integrate_the_spline():
                             points=[t - bw * f for f in (0, .25, .5, .75, 1) if 0 < t - bw * f < dt])[0]
         # the reference integrates with 20 Gauss points; compare to the exact integral loosely
         assert np.abs(K1 - KT * g1).max() <= 2e-3 * np.abs(KT).max() * max(abs(g1), 1e-3)
+
+
+def test_ramp_kernel_zero_before_arrival_and_static_after():
+    """Ramped-Heaviside load of slope 2/dt: nothing before the P wave arrives; once the S wave
+    has passed the end of the ramp (t - dt/2 >= r/c_s) the response is the static Kelvin
+    displacement and traction."""
+    cp, cs, mu = speeds()
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        d = rng.normal(size=3)
+        n = rng.normal(size=3)
+        r = np.linalg.norm(d)
+        dt = rng.uniform(0.01, 0.3)
+        U, T, g = O.ramp_point(E, NU, RHO, d, n, r / cp * rng.uniform(0.2, 0.999), dt)
+        assert np.all(U == 0) and np.all(T == 0)
+        t = r / cs + dt / 2 + dt * rng.uniform(0.0, 6.0)
+        U, T, g = O.ramp_point(E, NU, RHO, d, n, t, dt)
+        UK = kelvin_u(d, mu, NU)
+        TK = O.green(4, E, NU, RHO, d, n, t, dt)[4] * 2 / dt      # Kelvin_Linear = T^K dt/2
+        np.testing.assert_allclose(U, UK, rtol=1e-9, atol=1e-11 * np.abs(UK).max())
+        np.testing.assert_allclose(T, TK, rtol=1e-8, atol=1e-10 * np.abs(TK).max())
+
+
+def test_ramp_displacement_is_the_ramp_convolution_of_the_heaviside_response():
+    """U_ramp(t) = k * integral over [0, 1/k] of U_Heaviside(t - s) ds (k = 2/dt): Duhamel's
+    principle for the load g(t) = min(max(k t, 0), 1), by brute-force quadrature of the
+    textbook Heaviside-load displacement."""
+    cp, cs, mu = speeds()
+    rng = np.random.default_rng(6)
+    for _ in range(10):
+        d = rng.normal(size=3)
+        r = np.linalg.norm(d)
+        dt = rng.uniform(0.05, 0.4)
+        k = 2 / dt
+        t = r / cp + rng.uniform(-0.2, 4.0) * dt
+        U, _, _ = O.ramp_point(E, NU, RHO, d, [0, 0, 1], t, dt)
+        brk = [x for x in (t - r / cp, t - r / cs) if 0 < x < 1 / k]
+        ref = np.zeros((3, 3))
+        for i in range(3):
+            for j in range(3):
+                ref[i, j] = k * integrate.quad(
+                    lambda s: stokes_heaviside_u(d, t - s, cp, cs, RHO)[i, j], 0, 1 / k,
+                    points=brk or None, epsabs=0, epsrel=1e-13, limit=200)[0]
+        scale = max(np.abs(ref).max(), 1e-30)
+        assert np.abs(U - ref).max() <= 1e-10 * scale, (t, np.abs(U - ref).max() / scale)
+
+
+def test_ramp_kelvin_weights_are_the_hat_integrals_of_the_ramp():
+    """Int_RampedHeaviside_Linear: g1, g2 = integral over the step of phi_1, phi_2 times the
+    ramp load at t - tau (40-point rule), against adaptive quadrature.  The rule is exact for the
+    piecewise-polynomial integrand once no kink of the load lies inside the step, and within a
+    few 1e-5 dt when one does (the reference's own quadrature error there)."""
+    for dt in (0.1, 0.37):
+        k = 2 / dt
+        for t in np.arange(0.25, 4.0, 0.25) * dt:
+            _, _, g = O.ramp_point(E, NU, RHO, [1, 0, 0], [0, 0, 1], t, dt)
+            load = lambda s: min(max(k * (t - s), 0.0), 1.0)
+            kinks = [x for x in (t, t - 1 / k) if 0 < x < dt]
+            e1 = integrate.quad(lambda s: (dt - s) / dt * load(s), 0, dt, points=kinks or None)[0]
+            e2 = integrate.quad(lambda s: s / dt * load(s), 0, dt, points=kinks or None)[0]
+            tol = 1e-4 * dt if kinks else 1e-13 * dt
+            assert abs(g[0] - e1) <= tol and abs(g[1] - e2) <= tol, (t / dt, g, e1, e2)

@@ -43,11 +43,25 @@ def test_oracle_matches_golden_assembly(name):
             assert helpers.rel_fro(a, b) < 1e-13
 
 
+def test_oracle_matches_golden_ramp_functions():
+    """The ramped-Heaviside functions (src/GreenFunctionT.cpp:449-519, 589-622, 661-680, 705-731),
+    which no element of the reference calls, against the reference's own compiled functions."""
+    g = np.load(GOLD / "green_ramp.npz")
+    assert len(g["out"]) >= 100
+    for sc, d, n, ref in zip(g["scalars"], g["direction"], g["normal"], g["out"]):
+        E, nu, rho, t, tp, dt = sc
+        lin = O.green(6, E, nu, rho, d, n, t, dt)
+        U, T, w = O.ramp_point(E, nu, rho, d, n, tp, dt)
+        got = np.concatenate([x.ravel() for x in lin] + [U.ravel(), T.ravel(), w])
+        for a, b in ((got[:54], ref[:54]), (got[54:72], ref[54:72]), (got[72:], ref[72:])):
+            assert np.abs(a - b).max() <= 1e-13 * max(np.abs(b).max(), 1e-300)
+
+
 needs_ref = pytest.mark.skipif(O.ref() is None, reason="oracle/_ref not built (no reference tree)")
 
 
 @needs_ref
-@pytest.mark.parametrize("kernel", [2, 4, 5])
+@pytest.mark.parametrize("kernel", [2, 4, 5, 6])
 def test_green_functions_match_reference(kernel):
     rng = np.random.default_rng(kernel)
     for _ in range(200):
