@@ -38,8 +38,8 @@ TP_BLOCK_META = 6                        # per block: first sub-tile (4 B) and m
 FP64_ENTRY_BYTES = 15 * 8 + 4            # 15 values + index per (i, j, k)
 # HBM bytes per launch (read + written) from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE)
 # on this workload, scaled by the counters of a 1 GiB k_stream_copy (factors 2.0 and 1.0);
-# profiles/k_conv_mixed_pmc_sphere1536.md and profiles/k_conv_pmc_sphere1536.md hold the runs.
-MEASURED_TRAFFIC = {"k_conv_mixed": 11.779e9 + 99.3e6, "k_conv": 7.683e9 + 0.1e6}
+# profiles/k_conv_traffic_pmc_sphere1536.md holds the runs.
+MEASURED_TRAFFIC = {"k_conv_mixed": 10.311e9 + 99.3e6, "k_conv": 7.658e9 + 0.1e6}
 
 
 def parse():

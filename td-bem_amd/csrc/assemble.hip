@@ -901,8 +901,8 @@ void assemble_step_dense(Context& c, int step, int lower, int upper, double* dG1
 namespace {
 
 struct ShellPlan {
-    std::vector<int2> hull;          // [nnd*nnd]
-    std::vector<int64_t> off;        // [nnd*nnd] entry offsets
+    int2* hull = nullptr;            // [nnd*nnd] (the context's pinned copy)
+    int64_t* off = nullptr;          // [nnd*nnd] entry offsets (pinned)
     std::vector<int64_t> row_chunk;  // [nnd+1]
     std::vector<int64_t> row_valid;  // [nnd]
     int64_t nonzero = 0;
@@ -917,28 +917,44 @@ void plan_shells(Context& c, ShellPlan& P)
     hipLaunchKernelGGL(k_node_hull, dim3(grid_for(nn)), dim3(kBlock), 0, c.stream, c.pair_k.p,
                        c.adj_off.p, c.adj.p, c.nnd, c.nel, hull.p);
     TD_HIP(hipGetLastError());
-    P.hull.resize(nn);
-    TD_HIP(hipMemcpyAsync(P.hull.data(), hull.p, nn * sizeof(int2), hipMemcpyDeviceToHost,
+    c.pin_hull.reserve(nn);
+    P.hull = c.pin_hull.p;
+    TD_HIP(hipMemcpyAsync(P.hull, hull.p, nn * sizeof(int2), hipMemcpyDeviceToHost,
                           c.stream));
     TD_HIP(hipStreamSynchronize(c.stream));
-    P.off.resize(nn);
+    c.pin_off.reserve(nn);
+    P.off = c.pin_off.p;
     P.row_chunk.assign(c.nnd + 1, 0);
     P.row_valid.assign(c.nnd, 0);
-    int64_t chunk = 0;
-    for (int i = 0; i < c.nnd; i++) {
-        int64_t o = chunk * kChunk;
-        const int64_t row0 = o;
-        for (int j = 0; j < c.nnd; j++) {
-            const int2 h = P.hull[(int64_t)i * c.nnd + j];
-            P.off[(int64_t)i * c.nnd + j] = o;
-            if (h.y >= h.x) o += h.y - h.x + 1;
+    // entries per row node, then each row's first chunk, then the entry offset of every pair
+    const int nnd = c.nnd;
+    parallel_for(nnd, [&](int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; i++) {
+            int64_t n = 0;
+            for (int j = 0; j < nnd; j++) {
+                const int2 h = P.hull[i * nnd + j];
+                if (h.y >= h.x) n += h.y - h.x + 1;
+            }
+            P.row_valid[i] = n;
         }
-        P.row_valid[i] = o - row0;
-        P.nonzero += o - row0;
+    });
+    int64_t chunk = 0;
+    for (int i = 0; i < nnd; i++) {
         P.row_chunk[i] = chunk;
-        chunk += (o - row0 + kChunk - 1) / kChunk;
+        P.nonzero += P.row_valid[i];
+        chunk += (P.row_valid[i] + kChunk - 1) / kChunk;
     }
-    P.row_chunk[c.nnd] = chunk;
+    P.row_chunk[nnd] = chunk;
+    parallel_for(nnd, [&](int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; i++) {
+            int64_t o = P.row_chunk[i] * kChunk;
+            for (int j = 0; j < nnd; j++) {
+                const int2 h = P.hull[i * nnd + j];
+                P.off[i * nnd + j] = o;
+                if (h.y >= h.x) o += h.y - h.x + 1;
+            }
+        }
+    });
 }
 
 template <int ENND>
@@ -947,48 +963,82 @@ void fill_store(Context& c, const ShellPlan& P)
     LagStore& S = c.store;
     // per-pair slot layout on the host (pair_k copied back once)
     const int64_t np = (int64_t)c.nnd * c.nel;
-    std::vector<int2> pk(np);
-    TD_HIP(hipMemcpyAsync(pk.data(), c.pair_k.p, np * sizeof(int2), hipMemcpyDeviceToHost,
+    c.pin_pk.reserve(np);
+    const int2* pk = c.pin_pk.p;
+    TD_HIP(hipMemcpyAsync(c.pin_pk.p, c.pair_k.p, np * sizeof(int2), hipMemcpyDeviceToHost,
                           c.stream));
     TD_HIP(hipStreamSynchronize(c.stream));
     const int64_t slot_bytes = (int64_t)ENND * kVals * sizeof(double);
-    DBuf<double> scratch;
-    DBuf<int> slot_pair;
-    DBuf<int64_t> pair_slot;
     const double ext = c.if_ext ? 0.5 * c.dt : 0.0;
     KernelTimer lag_timer(c.stream, c.profile), gather_timer(c.stream, c.profile);
     if (c.profile) {
         c.counters.alloc(1);
         TD_HIP(hipMemsetAsync(c.counters.p, 0, sizeof(unsigned long long), c.stream));
     }
-    int i0 = 0;
-    while (i0 < c.nnd) {
-        // grow the batch while its slots fit the scratch budget
+    // slots (lags of in-shell element pairs) per row node, then row batches whose slots fit the
+    // scratch budget
+    std::vector<int64_t> rs(c.nnd);
+    parallel_for(c.nnd, [&](int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; i++) {
+            int64_t n = 0;
+            for (int e = 0; e < c.nel; e++) {
+                const int2 k = pk[i * c.nel + e];
+                if (k.y >= k.x) n += k.y - k.x + 1;
+            }
+            rs[i] = n;
+        }
+    });
+    struct Batch { int i0, i1; int64_t slots; };
+    std::vector<Batch> batches;
+    int64_t max_slots = 1;
+    for (int i0 = 0; i0 < c.nnd;) {
         int i1 = i0;
         int64_t slots = 0;
-        while (i1 < c.nnd) {
-            int64_t rs = 0;
-            for (int e = 0; e < c.nel; e++) {
-                const int2 k = pk[(int64_t)i1 * c.nel + e];
-                if (k.y >= k.x) rs += k.y - k.x + 1;
-            }
-            if (i1 > i0 && (slots + rs) * slot_bytes > kScratchBytes) break;
-            slots += rs;
-            i1++;
-        }
+        while (i1 < c.nnd && (i1 == i0 || (slots + rs[i1]) * slot_bytes <= kScratchBytes))
+            slots += rs[i1++];
+        batches.push_back({i0, i1, slots});
+        max_slots = std::max(max_slots, slots);
+        i0 = i1;
+    }
+    DBuf<double> scratch;
+    DBuf<int> slot_pair;
+    DBuf<int64_t> pair_slot;
+    scratch.reserve((size_t)max_slots * ENND * kVals);
+    // The host builds batch b + 1's slot lists while the device works on batch b: two host list
+    // sets, each reused only after the event that follows its upload; the device buffers are
+    // rewritten in stream order after the previous batch's kernels.
+    std::vector<int64_t> h_pair_slot[2];
+    std::vector<int> h_slot_pair[2];
+    hipEvent_t uploaded[2];
+    TD_HIP(hipEventCreateWithFlags(&uploaded[0], hipEventDisableTiming));
+    TD_HIP(hipEventCreateWithFlags(&uploaded[1], hipEventDisableTiming));
+    for (size_t b = 0; b < batches.size(); b++) {
+        const int set = (int)(b & 1);
+        const int i0 = batches[b].i0, i1 = batches[b].i1;
+        const int64_t slots = batches[b].slots;
         const int nb = i1 - i0;
-        std::vector<int64_t> h_pair_slot((size_t)nb * c.nel);
-        std::vector<int> h_slot_pair(slots);
-        int64_t s = 0;
-        for (int p = 0; p < nb * c.nel; p++) {
-            h_pair_slot[p] = s;
-            const int2 k = pk[(int64_t)i0 * c.nel + p];
-            if (k.y >= k.x)
-                for (int t = k.x; t <= k.y; t++) h_slot_pair[s++] = p;
-        }
-        scratch.alloc(std::max<int64_t>(slots, 1) * ENND * kVals);
-        slot_pair.upload(h_slot_pair.data(), h_slot_pair.size(), c.stream);
-        pair_slot.upload(h_pair_slot.data(), h_pair_slot.size(), c.stream);
+        if (b >= 2) TD_HIP(hipEventSynchronize(uploaded[set]));
+        auto& hp = h_pair_slot[set];
+        auto& hs = h_slot_pair[set];
+        hp.resize((size_t)nb * c.nel);
+        hs.resize((size_t)slots);
+        std::vector<int64_t> row_start(nb + 1, 0);
+        for (int r = 0; r < nb; r++) row_start[r + 1] = row_start[r] + rs[i0 + r];
+        parallel_for(nb, [&](int64_t lo, int64_t hi) {
+            for (int64_t r = lo; r < hi; r++) {
+                int64_t sl = row_start[r];
+                for (int e = 0; e < c.nel; e++) {
+                    const int64_t p = r * c.nel + e;
+                    hp[p] = sl;
+                    const int2 k = pk[(int64_t)i0 * c.nel + p];
+                    if (k.y >= k.x)
+                        for (int t = k.x; t <= k.y; t++) hs[sl++] = (int)p;
+                }
+            }
+        });
+        slot_pair.upload(hs.data(), hs.size(), c.stream);
+        pair_slot.upload(hp.data(), hp.size(), c.stream);
+        TD_HIP(hipEventRecord(uploaded[set], c.stream));
         if (slots > 0) {
             lag_timer.begin();
             hipLaunchKernelGGL(k_pair_lags<ENND>, dim3(grid_for(slots)), dim3(kBlock), 0,
@@ -1009,10 +1059,10 @@ void fill_store(Context& c, const ShellPlan& P)
             TD_HIP(hipGetLastError());
             gather_timer.end();
         }
-        // host vectors of this batch are reused next iteration: wait for the uploads to land
-        TD_HIP(hipStreamSynchronize(c.stream));
-        i0 = i1;
     }
+    TD_HIP(hipStreamSynchronize(c.stream));
+    TD_HIP(hipEventDestroy(uploaded[0]));
+    TD_HIP(hipEventDestroy(uploaded[1]));
     if (c.profile) {
         c.tm.lag_launches += lag_timer.count();
         c.tm.lag_kernel_ms += lag_timer.collect();
@@ -1050,12 +1100,11 @@ void build_lag_store(Context& c)
     S.idx.alloc(std::max<int64_t>(S.n_entries, 1));
     TD_HIP(hipMemsetAsync(S.idx.p, 0, S.idx.bytes(), c.stream));
     S.h_row_valid = P.row_valid;
-    S.h_hull = P.hull;
     S.row_chunk.upload(P.row_chunk.data(), P.row_chunk.size(), c.stream);
     S.row_valid.upload(P.row_valid.data(), P.row_valid.size(), c.stream);
-    S.d_off.upload(P.off.data(), P.off.size(), c.stream);
-    S.d_hull.upload(P.hull.data(), P.hull.size(), c.stream);
     const int64_t nn = (int64_t)c.nnd * c.nnd;
+    S.d_off.upload(P.off, nn, c.stream);
+    S.d_hull.upload(P.hull, nn, c.stream);
     hipLaunchKernelGGL(k_fill_idx, dim3(grid_for(nn)), dim3(kBlock), 0, c.stream, S.d_hull.p,
                        S.d_off.p, c.nnd, S.idx.p);
     TD_HIP(hipGetLastError());
@@ -1068,6 +1117,11 @@ void build_lag_store(Context& c)
         // only the mixed path's tile-pair copy reads them again
         S.d_off.reset();
         S.d_hull.reset();
+    }
+    // the pinned planning copies stay for the next build unless they are large
+    if ((int64_t)c.nnd * c.nnd * 24 > kPinnedKeepBytes) {
+        c.pin_off.reserve(0);
+        c.pin_pk.reserve(0);
     }
     c.store_ready = true;
     c.tm.store_ms +=

@@ -190,97 +190,127 @@ void tp_build(Context& c)
     // 6 jj .. 6 jj + 5 (k-tiles (6 jj) / 32 and (6 jj + 5) / 32)
     const size_t npair = (size_t)nT * nT;
     std::vector<int> slo(npair * 9, 1 << 30), shi(npair * 9, -1);
-    const auto& H = c.store.h_hull;
-    for (int pi = 0; pi < nT * kTile; pi++) {
-        const int i = T.perm[pi];
-        if (i < 0) continue;
-        const int I = pi / kTile, ii = pi % kTile;
-        const int m0 = (3 * ii) >> 4, m1 = (3 * ii + 2) >> 4;
-        for (int pj = 0; pj < nT * kTile; pj++) {
-            const int j = T.perm[pj];
-            if (j < 0) continue;
-            const int2 h = H[(size_t)i * nnd + j];
-            if (h.y < h.x) continue;
-            const int J = pj / kTile, jj = pj % kTile;
-            const int k0 = (6 * jj) >> 5, k1 = (6 * jj + 5) >> 5;
-            int* lo = &slo[((size_t)I * nT + J) * 9];
-            int* hi = &shi[((size_t)I * nT + J) * 9];
-            for (int m = m0; m <= m1; m++)
-                for (int kt = k0; kt <= k1; kt++) {
-                    lo[3 * m + kt] = std::min(lo[3 * m + kt], h.x);
-                    hi[3 * m + kt] = std::max(hi[3 * m + kt], h.y);
-                }
+    const HostBuf<int2>& H = c.pin_hull;   // node-pair lag ranges of the store just built
+    // row tiles in parallel: each writes only the sub-tile ranges of its own tile pairs (I, J)
+    parallel_for(nT, [&](int64_t Ilo, int64_t Ihi) {
+        for (int64_t pi = Ilo * kTile; pi < Ihi * kTile; pi++) {
+            const int i = T.perm[pi];
+            if (i < 0) continue;
+            const int I = (int)(pi / kTile), ii = (int)(pi % kTile);
+            const int m0 = (3 * ii) >> 4, m1 = (3 * ii + 2) >> 4;
+            for (int pj = 0; pj < nT * kTile; pj++) {
+                const int j = T.perm[pj];
+                if (j < 0) continue;
+                const int2 h = H[(size_t)i * nnd + j];
+                if (h.y < h.x) continue;
+                const int J = pj / kTile, jj = pj % kTile;
+                const int k0 = (6 * jj) >> 5, k1 = (6 * jj + 5) >> 5;
+                int* lo = &slo[((size_t)I * nT + J) * 9];
+                int* hi = &shi[((size_t)I * nT + J) * 9];
+                for (int m = m0; m <= m1; m++)
+                    for (int kt = k0; kt <= k1; kt++) {
+                        lo[3 * m + kt] = std::min(lo[3 * m + kt], h.x);
+                        hi[3 * m + kt] = std::max(hi[3 * m + kt], h.y);
+                    }
+            }
         }
-    }
-    // blocks: every lag of each tile-pair hull; sub-tiles stored where the lag is in range
+    });
+    // blocks: every lag of each tile-pair hull; sub-tiles stored where the lag is in range.
+    // Counted per tile pair in parallel, offsets by a prefix sum, then filled in parallel.
     std::vector<int> plo(npair, 1 << 30), phi(npair, -1);
-    std::vector<int64_t> off(npair, -1);
-    std::vector<uint32_t> sub_off;
-    std::vector<uint16_t> msk;
-    std::vector<uint2> sub_id;     // (I << 16 | J, k << 4 | u) of each stored sub-tile
-    int64_t nb = 0, nsub = 0, nz = 0;
+    std::vector<int64_t> off(npair, -1), qsub(npair + 1, 0), qblk(npair + 1, 0);
+    auto block_mask = [&](size_t q, int k) {
+        uint16_t mk = 0;
+        for (int u = 0; u < 9; u++)
+            if (slo[q * 9 + u] <= k && k <= shi[q * 9 + u]) mk |= (uint16_t)(1u << u);
+        return mk;
+    };
+    parallel_for((int64_t)npair, [&](int64_t lo, int64_t hi) {
+        for (int64_t q = lo; q < hi; q++) {
+            for (int u = 0; u < 9; u++) {
+                plo[q] = std::min(plo[q], slo[q * 9 + u]);
+                phi[q] = std::max(phi[q], shi[q * 9 + u]);
+            }
+            int64_t ns = 0;
+            for (int k = plo[q]; k <= phi[q]; k++) ns += __builtin_popcount(block_mask(q, k));
+            qsub[q + 1] = ns;
+            qblk[q + 1] = phi[q] >= plo[q] ? phi[q] - plo[q] + 1 : 0;
+        }
+    }, 256);
     for (size_t q = 0; q < npair; q++) {
-        for (int u = 0; u < 9; u++) {
-            plo[q] = std::min(plo[q], slo[q * 9 + u]);
-            phi[q] = std::max(phi[q], shi[q * 9 + u]);
-        }
-        if (phi[q] < plo[q]) continue;
-        off[q] = nb;
-        const uint32_t IJ = ((uint32_t)(q / nT) << 16) | (uint32_t)(q % nT);
-        for (int k = plo[q]; k <= phi[q]; k++) {
-            uint16_t mk = 0;
-            for (int u = 0; u < 9; u++)
-                if (slo[q * 9 + u] <= k && k <= shi[q * 9 + u]) {
-                    mk |= (uint16_t)(1u << u);
-                    sub_id.push_back(make_uint2(IJ, ((uint32_t)k << 4) | (uint32_t)u));
-                }
-            sub_off.push_back((uint32_t)nsub);
-            msk.push_back(mk);
-            nsub += __builtin_popcount(mk);
-            nz += mk != 0;
-            nb++;
-        }
+        qsub[q + 1] += qsub[q];
+        qblk[q + 1] += qblk[q];
     }
+    const int64_t nb = qblk[npair], nsub = qsub[npair];
     if (nsub >= (int64_t)UINT32_MAX || nb >= (int64_t)INT32_MAX)
         throw std::runtime_error("tile store too large");
+    std::vector<uint32_t> sub_off(nb);
+    std::vector<uint16_t> msk(nb);
+    std::vector<uint2> sub_id(nsub);   // (I << 16 | J, k << 4 | u) of each stored sub-tile
+    std::vector<int64_t> qnz(npair, 0);
+    parallel_for((int64_t)npair, [&](int64_t lo, int64_t hi) {
+        for (int64_t q = lo; q < hi; q++) {
+            if (phi[q] < plo[q]) continue;
+            off[q] = qblk[q];
+            const uint32_t IJ = ((uint32_t)(q / nT) << 16) | (uint32_t)(q % nT);
+            int64_t b = qblk[q], sub = qsub[q];
+            for (int k = plo[q]; k <= phi[q]; k++, b++) {
+                const uint16_t mk = block_mask(q, k);
+                sub_off[b] = (uint32_t)sub;
+                msk[b] = mk;
+                for (int u = 0; u < 9; u++)
+                    if ((mk >> u) & 1u) sub_id[sub++] = make_uint2(IJ, ((uint32_t)k << 4) | (uint32_t)u);
+                qnz[q] += mk != 0;
+            }
+        }
+    }, 256);
+    int64_t nz = 0;
+    for (int64_t x : qnz) nz += x;
     T.nblocks = nz;
     T.nsub = nsub;
-    // merged (J, k) lists of the groups of 4 row tiles
+    // merged (J, k) lists of the groups of kGroup row tiles, built per group in parallel
+    std::vector<std::vector<TPEntry>> gent(T.nG);
+    parallel_for(T.nG, [&](int64_t glo, int64_t ghi) {
+        for (int64_t g = glo; g < ghi; g++) {
+            auto& list = gent[g];
+            for (int J = 0; J < nT; J++) {
+                int klo = 1 << 30, khi = -1;
+                for (int w = 0; w < kGroup; w++) {
+                    const int64_t I = g * kGroup + w;
+                    if (I >= nT) continue;
+                    const size_t q = (size_t)I * nT + J;
+                    klo = std::min(klo, plo[q]);
+                    khi = std::max(khi, phi[q]);
+                }
+                for (int k = klo; k <= khi; k++) {
+                    TPEntry e;
+                    e.id = ((uint32_t)J << kKBits) | (uint32_t)k;
+                    e.kmask = 0;
+                    bool any = false;
+                    for (int w = 0; w < kGroup; w++) {
+                        e.sub[w] = 0;
+                        e.msk[w] = 0;
+                        const int64_t I = g * kGroup + w;
+                        if (I >= nT) continue;
+                        const size_t q = (size_t)I * nT + J;
+                        if (off[q] < 0 || k < plo[q] || k > phi[q]) continue;
+                        const int64_t b = off[q] + (k - plo[q]);
+                        if (!msk[b]) continue;
+                        e.sub[w] = sub_off[b];
+                        e.msk[w] = msk[b];
+                        for (int m = 0; m < 3; m++) e.kmask |= (msk[b] >> (3 * m)) & 7u;
+                        any = true;
+                    }
+                    if (any) list.push_back(e);
+                }
+            }
+        }
+    });
     std::vector<TPEntry> ent;
     std::vector<int64_t> goff(T.nG + 1, 0);
     for (int g = 0; g < T.nG; g++) {
         goff[g] = (int64_t)ent.size();
-        for (int J = 0; J < nT; J++) {
-            int klo = 1 << 30, khi = -1;
-            for (int w = 0; w < kGroup; w++) {
-                const int I = g * kGroup + w;
-                if (I >= nT) continue;
-                const size_t q = (size_t)I * nT + J;
-                klo = std::min(klo, plo[q]);
-                khi = std::max(khi, phi[q]);
-            }
-            for (int k = klo; k <= khi; k++) {
-                TPEntry e;
-                e.id = ((uint32_t)J << kKBits) | (uint32_t)k;
-                e.kmask = 0;
-                bool any = false;
-                for (int w = 0; w < kGroup; w++) {
-                    e.sub[w] = 0;
-                    e.msk[w] = 0;
-                    const int I = g * kGroup + w;
-                    if (I >= nT) continue;
-                    const size_t q = (size_t)I * nT + J;
-                    if (off[q] < 0 || k < plo[q] || k > phi[q]) continue;
-                    const int64_t b = off[q] + (k - plo[q]);
-                    if (!msk[b]) continue;
-                    e.sub[w] = sub_off[b];
-                    e.msk[w] = msk[b];
-                    for (int m = 0; m < 3; m++) e.kmask |= (msk[b] >> (3 * m)) & 7u;
-                    any = true;
-                }
-                if (any) ent.push_back(e);
-            }
-        }
+        ent.insert(ent.end(), gent[g].begin(), gent[g].end());
     }
     goff[T.nG] = (int64_t)ent.size();
     T.nentries = (int64_t)ent.size();
@@ -313,6 +343,7 @@ void tp_build(Context& c)
     TD_HIP(hipStreamSynchronize(c.stream));
     c.store.d_hull.reset();
     c.store.d_off.reset();
+    if ((int64_t)nnd * nnd * 24 > kPinnedKeepBytes) c.pin_hull.reserve(0);
     T.ready = true;
 }
 

@@ -240,21 +240,33 @@ __device__ __forceinline__ void apply_wave(const double* __restrict__ Af, int n,
 #pragma unroll
     for (int q = 0; q < NT; q++) acc[q] = f64x4{0.0, 0.0, 0.0, 0.0};
     const double* a = Af + ((int64_t)T * C) * 512 + lane * 8;
-    for (int c = c0; c < c1; c++) {
-        double av[8];
+    const double* x = Xf + lane * 8;
+    // two fragments in flight: the loads of fragment c + 1 are issued before the products of c
+    double av[2][8], xv[2][NT][8];
+    auto load = [&](int b, int c) {
 #pragma unroll
-        for (int e = 0; e < 8; e++) av[e] = a[(int64_t)c * 512 + e];
+        for (int e = 0; e < 8; e++) av[b][e] = a[(int64_t)c * 512 + e];
 #pragma unroll
-        for (int q = 0; q < NT; q++) {
-            const double* x = Xf + ((int64_t)q * C + c) * 512 + lane * 8;
-            double xv[8];
+        for (int q = 0; q < NT; q++)
 #pragma unroll
-            for (int e = 0; e < 8; e++) xv[e] = x[e];
+            for (int e = 0; e < 8; e++) xv[b][q][e] = x[((int64_t)q * C + c) * 512 + e];
+    };
+    auto mul = [&](int b) {
+#pragma unroll
+        for (int q = 0; q < NT; q++)
 #pragma unroll
             for (int e = 0; e < 8; e++)
-                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[e], xv[e], acc[q], 0, 0, 0);
-        }
+                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[b][e], xv[b][q][e], acc[q], 0, 0, 0);
+    };
+    if (c0 < c1) load(0, c0);
+    int c = c0;
+    for (; c + 1 < c1; c += 2) {
+        load(1, c + 1);
+        mul(0);
+        if (c + 2 < c1) load(0, c + 2);
+        mul(1);
     }
+    if (c < c1) mul(0);
     // D fragment of the f64 16x16x4 product: lane = rhs + 16 * (row % 4), element row / 4
     const int Np = frag_rows(n);
 #pragma unroll

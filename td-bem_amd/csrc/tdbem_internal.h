@@ -4,8 +4,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "tdbem_device.h"
@@ -21,6 +23,7 @@ template <typename T>
 struct DBuf {
     T* p = nullptr;
     size_t n = 0;
+    size_t cap = 0;
     DBuf() = default;
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
@@ -29,7 +32,7 @@ struct DBuf {
     {
         if (p) (void)hipFree(p);
         p = nullptr;
-        n = 0;
+        n = cap = 0;
     }
     void alloc(size_t count)
     {
@@ -37,15 +40,78 @@ struct DBuf {
         reset();
         if (count == 0) return;
         TD_HIP(hipMalloc(&p, count * sizeof(T)));
-        n = count;
+        n = cap = count;
+    }
+    // n = count, keeping the allocation when it is large enough (no hipFree / hipMalloc, which
+    // synchronise the device, in loops that resize a buffer)
+    void reserve(size_t count)
+    {
+        if (p && count <= cap) {
+            n = count;
+            return;
+        }
+        alloc(count);
     }
     void upload(const T* h, size_t count, hipStream_t s)
     {
-        alloc(count);
+        reserve(count);
         if (count) TD_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
     }
     size_t bytes() const { return n * sizeof(T); }
 };
+
+// Pinned host copies of the shell planning (24 B per node pair) larger than this are released
+// after use instead of being kept for the next build.
+constexpr int64_t kPinnedKeepBytes = 256LL << 20;
+
+// Pinned host buffer (hipHostMalloc) that keeps its allocation across resizes: large transfers
+// between the shell planning on the host and the device run at full bus speed.
+template <typename T>
+struct HostBuf {
+    T* p = nullptr;
+    size_t n = 0, cap = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    ~HostBuf()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+    // reserve(0) releases the allocation
+    void reserve(size_t count)
+    {
+        if (p && count <= cap && count > 0) {
+            n = count;
+            return;
+        }
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = cap = 0;
+        if (count == 0) return;
+        TD_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), count * sizeof(T), hipHostMallocDefault));
+        n = cap = count;
+    }
+    T& operator[](size_t i) { return p[i]; }
+    const T& operator[](size_t i) const { return p[i]; }
+};
+
+// f(lo, hi) over [0, n) split into contiguous ranges on up to 16 host threads, at least `grain`
+// items each (host-side planning loops over rows of node pairs).
+template <typename F>
+void parallel_for(int64_t n, F&& f, int64_t grain = 1)
+{
+    const int64_t nt = std::min<int64_t>(
+        {16, (int64_t)std::max(1u, std::thread::hardware_concurrency()),
+         std::max<int64_t>(n / std::max<int64_t>(grain, 1), 1)});
+    if (nt <= 1) {
+        f((int64_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(nt);
+    for (int64_t t = 0; t < nt; t++) th.emplace_back([&f, t, n, nt] { f(n * t / nt, n * (t + 1) / nt); });
+    for (auto& x : th) x.join();
+}
 
 // Brackets kernel launches with HIP events on one stream; collect() waits and sums the times.
 struct KernelTimer {
@@ -100,7 +166,6 @@ struct LagStore {
     DBuf<int64_t> row_chunk;      // [nnd + 1] chunk offsets per row node
     DBuf<int64_t> row_valid;      // [nnd] entries before padding, per row node
     std::vector<int64_t> h_row_chunk, h_row_valid;
-    std::vector<int2> h_hull;     // [nnd*nnd] lag range of each node pair
     int64_t nonzero_entries = 0;  // before padding
     // node-pair tables on the device, kept from the build of a mixed-precision store until the
     // tile-pair copy has been gathered from it: lag range and first entry of each pair
@@ -168,6 +233,11 @@ struct Context {
     DBuf<double> cterm;    // [nnd][9] sum over elements of the Kelvin C-term
     bool cterm_ready = false;
     DBuf<int2> pair_k;     // [nnd*nel] (k_lo, k_hi), k_hi < k_lo when the pair is empty
+    // host copies of the shell planning (pinned, kept between builds): lag range and first store
+    // entry of every node pair [nnd*nnd] (the tile-pair copy reads pin_hull), pair_k [nnd*nel]
+    HostBuf<int2> pin_hull;
+    HostBuf<int64_t> pin_off;
+    HostBuf<int2> pin_pk;
     bool ranges_ready = false;
 
     LagStore store;
