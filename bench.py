@@ -132,8 +132,11 @@ def copy_calibration(api):
     read + 1 GiB written per launch) whose FETCH_SIZE / WRITE_SIZE counters scale the counters of
     the kernels above to bytes."""
     ms = api.stream_copy(1 << 30, 3)
+    rd = api.stream_read(8 << 30, 3)
     return {"kernel": "k_stream_copy", "bytes_read": 1 << 30, "bytes_written": 1 << 30,
-            "launches": 3, "avg_ms": round(ms, 4), "GBs": round(2 * (1 << 30) / ms / 1e6, 1)}
+            "launches": 3, "avg_ms": round(ms, 4), "GBs": round(2 * (1 << 30) / ms / 1e6, 1),
+            "read_only": {"kernel": "k_stream_read", "bytes": 8 << 30, "launches": 3,
+                          "avg_ms": round(rd, 4), "GBs": round((8 << 30) / rd / 1e6, 1)}}
 
 
 def serve_leg(api, configs, serve, nreq, rate):

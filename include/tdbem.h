@@ -129,6 +129,12 @@ int tdbem_mfma_f64_selftest(const double* A, const double* B, double* D);
  * what they read for it.  No counterpart in the reference. */
 int tdbem_stream_copy(size_t bytes, int launches, double* avg_ms);
 
+/* Diagnostic: a read-only stream of `bytes` bytes (a multiple of 16) with non-temporal loads,
+ * `launches` times; *avg_ms = mean launch time from HIP events.  bytes / avg_ms is the read
+ * bandwidth a streaming kernel reaches on this GPU, the practical ceiling of the HBM-bound
+ * convolutions.  No counterpart in the reference. */
+int tdbem_stream_read(size_t bytes, int launches, double* avg_ms);
+
 /* Diagnostic: the step-0 dense solver on its own.  A is n x n row-major (host), B is n x nrhs
  * row-major (host), overwritten with X = A^{-1} B.  mode 0: the LU of the march (no row
  * interchanges when partial pivoting would make none, else partial pivoting) and two triangular

@@ -346,6 +346,7 @@ void tp_conv(Context& c, const unsigned short* zs, int nrhs, int Rring, int s, d
 void tp_reduce(Context& c, const double* partial, int nrhs, double* hist, int N);
 void mfma_selftest(const float* A, const float* B, float* D);
 double stream_copy(size_t bytes, int launches);
+double stream_read(size_t bytes, int launches);
 
 // lu.hip: in-place LU with partial pivoting of a dense row-major n x n device matrix, and
 // solves with it.  piv is a device int array of length n.

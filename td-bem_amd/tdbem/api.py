@@ -65,6 +65,7 @@ def lib():
         L.tdbem_set_precision.argtypes = [vp, C.c_int]
         L.tdbem_mfma_selftest.argtypes = [C.POINTER(C.c_float)] * 3
         L.tdbem_stream_copy.argtypes = [C.c_size_t, C.c_int, dp]
+        L.tdbem_stream_read.argtypes = [C.c_size_t, C.c_int, dp]
         L.tdbem_mfma_f64_selftest.argtypes = [dp, dp, dp]
         L.tdbem_dense_solve.argtypes = [C.c_int, dp, C.c_int, dp, C.c_int]
         L.tdbem_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
@@ -260,6 +261,16 @@ def stream_copy(nbytes, launches=3):
     in ms.  Its byte counters calibrate those of the other kernels."""
     ms = C.c_double(0.0)
     rc = lib().tdbem_stream_copy(C.c_size_t(nbytes), C.c_int(launches), C.byref(ms))
+    if rc:
+        raise TdbemError(rc, lib().tdbem_last_error(None).decode())
+    return ms.value
+
+
+def stream_read(nbytes, launches=3):
+    """Read-only non-temporal stream over nbytes (k_stream_read); returns the mean launch time in
+    ms: nbytes / time is the practical read bandwidth of this GPU."""
+    ms = C.c_double(0.0)
+    rc = lib().tdbem_stream_read(C.c_size_t(nbytes), C.c_int(launches), C.byref(ms))
     if rc:
         raise TdbemError(rc, lib().tdbem_last_error(None).decode())
     return ms.value
