@@ -1315,3 +1315,210 @@ int or_march_fused(const or_march_cfg* c, double* const* MG, double* const* MH, 
     free(zt);
     return rc;
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Two-region solver SolverMRT (src/SolverMRT.cpp), restated literally: the global matrices   */
+/* assembled from the regions' lag matrices through the interior / interface index maps       */
+/* (InterfaceIndexing :223-568, Assemble_GH_R1/R2 :685-850), the load of SetLoad (:935-1026)  */
+/* and the marching of Solve_Direct (:1035-1099) or Solve_Averaging (:1101-1249).             */
+/* ------------------------------------------------------------------------------------------ */
+double or_mrt_load_factor(const or_mrt_cfg* c, double t)
+{
+    /* src/SolverMRT.cpp:937-939 (p0 = 5.46366e9, decay 2000); kind 1: amp * min(1, t/t_rise) */
+    if (c->load_kind == 0) return 5.46366e9 * t * exp(-2000 * t);
+    if (c->t_rise > 0) return c->amp * (t < c->t_rise ? t / c->t_rise : 1.0);
+    return c->amp;
+}
+
+/* Global index maps.  Unknowns (columns of H): interior dofs of region 1, of region 2, interface
+ * displacements, interface tractions (both in region 1's interface order); columns of G: the
+ * interior dofs of region 1 then region 2.  Rows: region 1's dofs then region 2's.
+ * hcol[r][d]: H column of region r's dof d; gcol[r][d]: G column (interior dofs) or -1;
+ * tcol[r][d]: traction column of an interface dof or -1.  Returns -1 on an interface mismatch. */
+static int mrt_maps(const or_mrt_cfg* c, int* hcol[2], int* gcol[2], int* tcol[2])
+{
+    const int nz = c->n_iface, NZ = 3 * nz;
+    const int nint1 = 3 * (c->nnd[0] - nz), nint2 = 3 * (c->nnd[1] - nz);
+    int r, i, d;
+    int* slot2 = malloc(sizeof(int) * (nz > 0 ? nz : 1));
+    /* region 2's interface nodes matched to region 1's by coordinates (:440-477) */
+    for (i = 0; i < nz; i++) {
+        const double* x2 = c->coords[1] + 3 * c->iface[1][i];
+        int match = -1, j;
+        for (j = 0; j < nz; j++) {
+            const double* x1 = c->coords[0] + 3 * c->iface[0][j];
+            double dx = x1[0] - x2[0], dy = x1[1] - x2[1], dz = x1[2] - x2[2];
+            if (sqrt(dx * dx + dy * dy + dz * dz) < 1e-10) {
+                match = j;
+                break;
+            }
+        }
+        if (match < 0) {
+            free(slot2);
+            return -1; /* "Interface mismatch, interface nodes must match one-to-one" */
+        }
+        slot2[i] = match;
+    }
+    for (r = 0; r < 2; r++) {
+        const int nnd = c->nnd[r];
+        int* lab = calloc(nnd, sizeof(int));
+        int k = 0;
+        for (i = 0; i < nz; i++) lab[c->iface[r][i]] = 1 + i;
+        for (i = 0; i < nnd; i++)
+            for (d = 0; d < 3; d++) {
+                const int dof = 3 * i + d;
+                if (!lab[i]) {
+                    /* interior nodes in ascending order (:235-246, 359-370) */
+                    hcol[r][dof] = (r == 0 ? 0 : nint1) + k;
+                    gcol[r][dof] = (r == 0 ? 0 : nint1) + k;
+                    tcol[r][dof] = -1;
+                    k++;
+                } else {
+                    const int zi = lab[i] - 1, slot = r == 0 ? zi : slot2[zi];
+                    hcol[r][dof] = nint1 + nint2 + 3 * slot + d;
+                    gcol[r][dof] = -1;
+                    tcol[r][dof] = nint1 + nint2 + NZ + 3 * slot + d;
+                }
+            }
+        free(lab);
+    }
+    free(slot2);
+    return 0;
+}
+
+int or_mrt_march(const or_mrt_cfg* c, double* const* MG1, double* const* MH1, double* const* MG2,
+                 double* const* MH2, double* X)
+{
+    const int nz = c->n_iface, K = c->K;
+    const int N[2] = {3 * c->nnd[0], 3 * c->nnd[1]};
+    const int NH = N[0] + N[1], NG = NH - 6 * nz;
+    double* const* MG[2] = {MG1, MG2};
+    double* const* MH[2] = {MH1, MH2};
+    int *hcol[2], *gcol[2], *tcol[2];
+    double **Gg, **Hg, *Gc, *Hc, *load, *T, *rhs, *Gt, *Ht;
+    int *piv, r, i, d, k, n, rc = 0;
+    for (r = 0; r < 2; r++) {
+        hcol[r] = malloc(sizeof(int) * N[r]);
+        gcol[r] = malloc(sizeof(int) * N[r]);
+        tcol[r] = malloc(sizeof(int) * N[r]);
+    }
+    if (mrt_maps(c, hcol, gcol, tcol)) {
+        for (r = 0; r < 2; r++) { free(hcol[r]); free(gcol[r]); free(tcol[r]); }
+        return -1;
+    }
+    /* global lag matrices (Assemble_GH_R1: region 1's interface G enters H negated; R2: as is) */
+    Gg = malloc(sizeof(double*) * (K + 1));
+    Hg = malloc(sizeof(double*) * (K + 1));
+    for (k = 0; k <= K; k++) {
+        Gg[k] = calloc((long)NH * NG, sizeof(double));
+        Hg[k] = calloc((long)NH * NH, sizeof(double));
+        for (r = 0; r < 2; r++) {
+            const int row0 = r == 0 ? 0 : N[0];
+            for (i = 0; i < N[r]; i++)
+                for (d = 0; d < N[r]; d++) {
+                    const double g = MG[r][k][(long)i * N[r] + d], h = MH[r][k][(long)i * N[r] + d];
+                    const long row = row0 + i;
+                    Hg[k][row * NH + hcol[r][d]] = h;
+                    if (gcol[r][d] >= 0) Gg[k][row * NG + gcol[r][d]] = g;
+                    else Hg[k][row * NH + tcol[r][d]] = r == 0 ? -g : g;
+                }
+        }
+    }
+    Gc = malloc(sizeof(double) * (long)NH * NG);
+    Hc = malloc(sizeof(double) * (long)NH * NH);
+    Gt = malloc(sizeof(double) * (long)NH * NG);
+    Ht = malloc(sizeof(double) * (long)NH * NH);
+    T = calloc((long)(c->numstep + 1) * NG, sizeof(double));
+    load = malloc(sizeof(double) * NG);
+    rhs = malloc(sizeof(double) * NH);
+    piv = malloc(sizeof(int) * NH);
+    memset(X, 0, sizeof(double) * (long)(c->numstep + 1) * NH);
+    for (n = 1; n <= c->numstep && rc == 0; n++) {
+        /* SetLoad at fCurrStep = n: interior FBC values times f(n dt) (:935-1026) */
+        const double f = or_mrt_load_factor(c, n * c->dt);
+        int s = c->scheme == 0 ? n : n - 1, m, fi;
+        memset(load, 0, sizeof(double) * NG);
+        for (r = 0; r < 2; r++)
+            for (i = 0; i < c->n_fbc[r]; i++) {
+                const int dof = c->fbc_dof[r][i];
+                if (gcol[r][dof] >= 0) load[gcol[r][dof]] = c->fbc_val[r][i] * f;
+            }
+        if (c->scheme == 1 && n == 1) continue;   /* Solve_Averaging: nothing at step 1 */
+        memset(rhs, 0, sizeof(double) * NH);
+        if (c->scheme == 0 || n == 2) {
+            memcpy(Gc, Gg[0], sizeof(double) * (long)NH * NG);
+            memcpy(Hc, Hg[0], sizeof(double) * (long)NH * NH);
+        } else {
+            for (i = 0; i < NH * NG; i++) Gc[i] = 4 * Gg[0][i] + (K >= 1 ? Gg[1][i] : 0);
+            for (i = 0; i < NH * NH; i++) Hc[i] = 4 * Hg[0][i] + (K >= 1 ? Hg[1][i] : 0);
+        }
+        for (i = 0; i < NH; i++) {
+            double acc = 0;
+            for (m = 0; m < NG; m++) acc += Gc[(long)i * NG + m] * load[m];
+            rhs[i] += acc;
+        }
+#define MRT_TERM(w, lag, step)                                                                   \
+    do {                                                                                        \
+        const double* Gk = Gg[lag];                                                             \
+        const double* Hk = Hg[lag];                                                             \
+        const double* tv = T + (long)(step) * NG;                                               \
+        const double* xv = X + (long)(step) * NH;                                               \
+        for (i = 0; i < NH; i++) {                                                              \
+            double acc = 0;                                                                     \
+            for (m = 0; m < NG; m++) acc += Gk[(long)i * NG + m] * tv[m];                       \
+            for (m = 0; m < NH; m++) acc -= Hk[(long)i * NH + m] * xv[m];                       \
+            rhs[i] += (w) * acc;                                                                \
+        }                                                                                       \
+    } while (0)
+        if (c->scheme == 0) {
+            /* :1059-1076 */
+            for (fi = 1; fi <= (n < K ? n : K); fi++) MRT_TERM(1.0, fi, n - fi);
+        } else if (n >= 3) {
+            /* :1150-1230, t_0 = x_0 = 0 makes the bgn terms vanish */
+            for (i = 0; i < NH * NG; i++) Gt[i] = (K >= 2 ? Gg[2][i] : 0) + 2 * (K >= 1 ? Gg[1][i] : 0);
+            for (i = 0; i < NH * NH; i++) Ht[i] = (K >= 2 ? Hg[2][i] : 0) + 2 * (K >= 1 ? Hg[1][i] : 0);
+            {
+                const double* tv = T + (long)(s - 1) * NG;
+                const double* xv = X + (long)(s - 1) * NH;
+                for (i = 0; i < NH; i++) {
+                    double acc = 0;
+                    for (m = 0; m < NG; m++) acc += Gt[(long)i * NG + m] * tv[m];
+                    for (m = 0; m < NH; m++) acc -= Ht[(long)i * NH + m] * xv[m];
+                    rhs[i] += acc;
+                }
+            }
+            for (fi = 3; fi <= (s < K ? s : K); fi++) MRT_TERM(1.0, fi, s + 1 - fi);
+            for (fi = 2; fi <= (s - 1 < K ? s - 1 : K); fi++) MRT_TERM(2.0, fi, s - fi);
+            for (fi = 1; fi <= (s - 2 < K ? s - 2 : K); fi++) MRT_TERM(1.0, fi, s - 1 - fi);
+        }
+#undef MRT_TERM
+        memcpy(Ht, Hc, sizeof(double) * (long)NH * NH);
+        if (or_lu_factor(NH, Ht, piv)) rc = -2;
+        else {
+            or_lu_solve(NH, Ht, piv, rhs);
+            memcpy(X + (long)s * NH, rhs, sizeof(double) * NH);
+            memcpy(T + (long)s * NG, load, sizeof(double) * NG);
+        }
+    }
+    for (k = 0; k <= K; k++) { free(Gg[k]); free(Hg[k]); }
+    free(Gg); free(Hg); free(Gc); free(Hc); free(Gt); free(Ht); free(T); free(load); free(rhs);
+    free(piv);
+    for (r = 0; r < 2; r++) { free(hcol[r]); free(gcol[r]); free(tcol[r]); }
+    return rc;
+}
+
+int or_mrt_extract(const or_mrt_cfg* c, int region, const double* x, double* u)
+{
+    /* ExtractField (:1251-1301): region displacements from the global unknowns */
+    int* hcol[2], *gcol[2], *tcol[2], r, d, rc;
+    for (r = 0; r < 2; r++) {
+        hcol[r] = malloc(sizeof(int) * 3 * c->nnd[r]);
+        gcol[r] = malloc(sizeof(int) * 3 * c->nnd[r]);
+        tcol[r] = malloc(sizeof(int) * 3 * c->nnd[r]);
+    }
+    rc = mrt_maps(c, hcol, gcol, tcol);
+    if (!rc)
+        for (d = 0; d < 3 * c->nnd[region]; d++) u[d] = x[hcol[region][d]];
+    for (r = 0; r < 2; r++) { free(hcol[r]); free(gcol[r]); free(tcol[r]); }
+    return rc;
+}

@@ -132,6 +132,34 @@ int or_march_literal(const or_march_cfg* c, double* const* MG, double* const* MH
 int or_march_fused(const or_march_cfg* c, double* const* MG, double* const* MH, double* U,
                    double* T);
 
+/* Two-region solver (SolverMRT, src/SolverMRT.cpp).  Region r: nnd[r] nodes with coordinates
+ * coords[r] (for matching the interface nodes), n_iface interface nodes iface[r] (0-based, same
+ * count in both regions; region 2's are matched to region 1's by position), traction loads
+ * fbc_dof/val[r] (interior dofs only count; SolverMRT ignores displacement conditions).
+ * scheme 0: Solve_Direct (what SolverMRT::Solve runs, :1028-1033); 1: Solve_Averaging.
+ * load_kind 0: f(t) = 5.46366e9 t exp(-2000 t) of SetLoad (:937-939); 1: amp * min(1, t/t_rise).
+ * The load of step n is f(n dt) in both schemes, as SetLoad forms it from fCurrStep. */
+typedef struct {
+    int nnd[2];
+    const double* coords[2];
+    int n_iface;
+    const int* iface[2];
+    int n_fbc[2];
+    const int* fbc_dof[2];
+    const double* fbc_val[2];
+    int K, numstep, scheme, load_kind;
+    double dt, amp, t_rise;
+} or_mrt_cfg;
+double or_mrt_load_factor(const or_mrt_cfg* c, double t);
+/* X: [numstep+1][3 (nnd[0] + nnd[1])] global unknowns fDis_DB_Global (interior displacements of
+ * region 1, of region 2, interface displacements, interface tractions), x_0 = 0.  MG1/MH1 (MG2/MH2)
+ * are region 1's (2's) K+1 lag matrices of or_build_lags.  Returns 0, -1 (interface mismatch),
+ * -2 (singular matrix). */
+int or_mrt_march(const or_mrt_cfg* c, double* const* MG1, double* const* MH1, double* const* MG2,
+                 double* const* MH2, double* X);
+/* ExtractField (:1251-1301): region `region`'s displacements u[3 nnd] from global unknowns x. */
+int or_mrt_extract(const or_mrt_cfg* c, int region, const double* x, double* u);
+
 /* Dense LU with partial pivoting (the KSPSolve stand-in), exposed for tests. */
 int or_lu_factor(int n, double* A, int* piv);
 void or_lu_solve(int n, const double* LU, const int* piv, double* b);

@@ -51,6 +51,7 @@ def oracle():
             raise RuntimeError(f"{ORACLE_SO} is not built (run __graft_entry__.build())")
         lib = C.CDLL(str(ORACLE_SO))
         lib.or_load_factor.restype = D
+        lib.or_mrt_load_factor.restype = D
         _oracle = lib
     return _oracle
 
@@ -220,3 +221,54 @@ def march(MG, MH, numstep, dt, ubc_dof, ubc_val, fbc_dof, fbc_val, a1=0.25, a2=0
     if rc:
         raise RuntimeError(f"march failed rc={rc}")
     return U, T
+
+
+class MrtCfg(C.Structure):
+    _fields_ = [("nnd", I * 2), ("coords", dp * 2), ("n_iface", I), ("iface", C.POINTER(C.c_int) * 2),
+                ("n_fbc", I * 2), ("fbc_dof", C.POINTER(C.c_int) * 2), ("fbc_val", dp * 2),
+                ("K", I), ("numstep", I), ("scheme", I), ("load_kind", I),
+                ("dt", D), ("amp", D), ("t_rise", D)]
+
+
+def mrt_march(regions, lags, numstep, dt, K, scheme=0, load_kind=0, amp=1.0, t_rise=0.0):
+    """SolverMRT restated (or_mrt_march): regions = two dicts with coords, iface (0-based node
+    ids), fbc_dof, fbc_val; lags = [(MG1, MH1), (MG2, MH2)] with K + 1 lags each.  Returns the
+    global unknowns X [numstep+1][3 (nnd1 + nnd2)] and the region displacements [2][numstep+1][3 nnd]."""
+    keep = []
+
+    def arr(a, t):
+        a = np.ascontiguousarray(a, dtype=t)
+        keep.append(a)
+        return a
+    cfg = MrtCfg()
+    for r, reg in enumerate(regions):
+        co = arr(reg["coords"], np.float64)
+        cfg.nnd[r] = co.shape[0]
+        cfg.coords[r] = _ptr(co)
+        cfg.iface[r] = _ptr(arr(reg["iface"], np.int32), C.c_int)
+        fd = arr(reg["fbc_dof"], np.int32)
+        cfg.n_fbc[r] = len(fd)
+        cfg.fbc_dof[r] = _ptr(fd, C.c_int)
+        cfg.fbc_val[r] = _ptr(arr(reg["fbc_val"], np.float64))
+    cfg.n_iface = len(regions[0]["iface"])
+    cfg.K, cfg.numstep, cfg.scheme, cfg.load_kind = K, numstep, scheme, load_kind
+    cfg.dt, cfg.amp, cfg.t_rise = dt, amp, t_rise
+    ptrs = []
+    for MG, MH in lags:
+        MG = arr(MG, np.float64); MH = arr(MH, np.float64)
+        ptrs.append(((dp * (K + 1))(*[_ptr(MG[k]) for k in range(K + 1)]),
+                     (dp * (K + 1))(*[_ptr(MH[k]) for k in range(K + 1)])))
+    NH = 3 * (cfg.nnd[0] + cfg.nnd[1])
+    X = np.zeros((numstep + 1, NH))
+    rc = oracle().or_mrt_march(C.byref(cfg), ptrs[0][0], ptrs[0][1], ptrs[1][0], ptrs[1][1], _ptr(X))
+    if rc == -1:
+        raise ValueError("Interface mismatch, interface nodes must match one-to-one")
+    if rc:
+        raise RuntimeError(f"mrt march failed rc={rc}")
+    U = []
+    for r in range(2):
+        u = np.zeros((numstep + 1, 3 * cfg.nnd[r]))
+        for s in range(numstep + 1):
+            oracle().or_mrt_extract(C.byref(cfg), I(r), _ptr(X[s]), _ptr(u[s]))
+        U.append(u)
+    return X, U
