@@ -165,6 +165,54 @@ int tdbem_batch_set_params(tdbem_batch* b, int rounds, int solve_waves);
 int tdbem_batch_run(tdbem_batch* b, int steps_per_graph, int max_replays, int* launches);
 int tdbem_batch_destroy(tdbem_batch* b);
 
+/* SolverMRT::SetMaxStep (src/SolverMRT.cpp:1318-1321), for any context: the lag store holds
+ * M_0 .. M_K (K capped at numstep), M_K without the part the step after K would add, as the
+ * reference's lag database of fMaxStep + 1 matrices does (UpdateGHLinear, :850-873).  A change
+ * drops a built store.  tdbem_create sets K to SolverT::ComputeMaxStep's value. */
+int tdbem_set_max_step(tdbem_context* ctx, int max_step);
+
+/* Two-region solver: SolverMRT (src/SolverMRT.cpp) over two region contexts (FP64, Step kernel,
+ * the same dt and numstep), which must outlive it and are not marched on their own meanwhile.
+ *   tdbem_mrt_create  — SolverMRT::SolverMRT with LocalIndexing / InterfaceIndexing (:22-60,
+ *                       :160-568) and SetMaxStep(max_step) (the reference's default is 10).
+ *                       iface1/iface2: the n_iface interface node ids (0-based) of each region;
+ *                       region 2's are matched to region 1's by coordinates (1e-10), and
+ *                       TDBEM_ERR_ARG "Interface mismatch, interface nodes must match
+ *                       one-to-one" is returned when they do not.
+ *   tdbem_mrt_init    — builds the region lag stores if needed and the global step matrix
+ *                       (Assemble_GH_R1/R2, :685-850) and its inverse.  scheme:
+ *                       TDBEM_MRT_DIRECT (Solve_Direct, :1035-1099, what SolverMRT::Solve runs)
+ *                       or TDBEM_MRT_AVERAGING (Solve_Averaging, :1101-1249).  load_kind:
+ *                       TDBEM_MRT_LOAD_PULSE, f(t) = 5.46366e9 t exp(-2000 t) of SetLoad
+ *                       (:935-939; amp, t_rise unused), or TDBEM_MRT_LOAD_RAMP, f(t) =
+ *                       amp min(1, t / t_rise) (amp when t_rise <= 0).  The loads are the
+ *                       regions' traction conditions (fbc_dof per region, 0-based dofs) times
+ *                       f(fCurrStep dt); conditions on interface dofs are ignored, as are
+ *                       displacement conditions (SetLoad reads only the FBC lists).
+ *   tdbem_mrt_run     — the next nsteps steps (UpdateGHLinear + SetLoad + Solve per step).
+ *                       Solve_Direct solves x_1 .. x_numstep; Solve_Averaging solves x_s at
+ *                       fCurrStep = s + 1, so x_1 .. x_{numstep-1}, with x_numstep left 0.
+ *   tdbem_mrt_get_state — step s (0 .. numstep): x [n] the global unknowns fDis_DB_Global
+ *                       (n = tdbem_mrt_num_unknowns = 3 (nnd1 + nnd2)); u [n] and t [n] the
+ *                       regions' displacements (ExtractField, :1251-1301) and tractions, region
+ *                       1's 3 nnd1 dofs then region 2's.  Any of x, u, t may be NULL.
+ * Errors are read with tdbem_last_error(NULL). */
+enum { TDBEM_MRT_DIRECT = 0, TDBEM_MRT_AVERAGING = 1 };
+enum { TDBEM_MRT_LOAD_PULSE = 0, TDBEM_MRT_LOAD_RAMP = 1 };
+typedef struct tdbem_mrt tdbem_mrt;
+int tdbem_mrt_create(tdbem_context* region1, tdbem_context* region2, int max_step, int n_iface,
+                     const int* iface1, const int* iface2, tdbem_mrt** out);
+int tdbem_mrt_init(tdbem_mrt* m, int scheme, int load_kind, double amp, double t_rise,
+                   int n_fbc1, const int* fbc_dof1, const double* fbc_val1, int n_fbc2,
+                   const int* fbc_dof2, const double* fbc_val2);
+int tdbem_mrt_run(tdbem_mrt* m, int nsteps);
+int tdbem_mrt_step_index(const tdbem_mrt* m);
+int tdbem_mrt_num_unknowns(const tdbem_mrt* m);
+int tdbem_mrt_get_state(tdbem_mrt* m, int s, double* x, double* u, double* t);
+/* host wall clock (ms) of tdbem_mrt_init and tdbem_mrt_run, summed (no reference counterpart) */
+int tdbem_mrt_get_timings(const tdbem_mrt* m, double* init_ms, double* run_ms);
+int tdbem_mrt_destroy(tdbem_mrt* m);
+
 /* Instrumentation (no counterpart in the reference, which has no timers): when on, the lag
  * kernel, the store gather, the history convolution and the step solve are bracketed with HIP
  * events on the context stream.  tdbem_get_timings fills out[TDBEM_NUM_TIMINGS] in this order:

@@ -18,6 +18,11 @@ struct tdbem_batch {
     ~tdbem_batch() { tdbem::batch_destroy(b); }
 };
 
+struct tdbem_mrt {
+    tdbem::Mrt* m = nullptr;
+    ~tdbem_mrt() { tdbem::mrt_destroy(m); }
+};
+
 namespace tdbem {
 
 void check_hip(hipError_t e, const char* what)
@@ -147,23 +152,28 @@ __global__ void __launch_bounds__(256)
         dst[i] = src[i];
 }
 
-// Read-only stream: 8 non-temporal 16 B loads in flight per thread, grid-stride, folded into
-// one word that is written only if it matches a value the data never holds (so the loads stay).
+// Read-only stream: each workgroup reads one contiguous span of kReadSpan 16 B words with 4
+// non-temporal loads in flight per thread, folded into one word that is written only if it
+// matches a value the data never holds (so the loads stay).  Contiguous spans per workgroup
+// stream faster than a grid-stride walk on this part (about 7.1 against 6.5-7.0 TB/s).
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+constexpr int64_t kReadSpan = 16384;
 __global__ void __launch_bounds__(256)
     k_stream_read(const u32x4* __restrict__ src, int64_t n16, unsigned* __restrict__ sink)
 {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t b0 = (int64_t)blockIdx.x * kReadSpan;
+    const int64_t b1 = b0 + kReadSpan < n16 ? b0 + kReadSpan : n16;
     u32x4 acc = {0u, 0u, 0u, 0u};
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i + 7 * stride < n16; i += 8 * stride) {
-        u32x4 v[8];
+    for (int64_t i = b0 + threadIdx.x; i < b1; i += 4 * 256) {
+        u32x4 v[4];
 #pragma unroll
-        for (int u = 0; u < 8; u++) v[u] = __builtin_nontemporal_load(&src[i + u * stride]);
+        for (int u = 0; u < 4; u++) {
+            const int64_t j = i + u * 256;
+            v[u] = j < b1 ? __builtin_nontemporal_load(&src[j]) : u32x4{0u, 0u, 0u, 0u};
+        }
 #pragma unroll
-        for (int u = 0; u < 8; u++) acc ^= v[u];
+        for (int u = 0; u < 4; u++) acc ^= v[u];
     }
-    for (; i < n16; i += stride) acc ^= src[i];
     const unsigned x = acc.x ^ acc.y ^ acc.z ^ acc.w;
     if (x == 0x7fc01234u) sink[0] = x;
 }
@@ -185,7 +195,8 @@ double stream_read(size_t bytes, int launches)
         KernelTimer t(s, true);
         for (int l = 0; l < launches; l++) {
             t.begin();
-            hipLaunchKernelGGL(k_stream_read, dim3(16384), dim3(256), 0, s, a.p, n16, sink.p);
+            hipLaunchKernelGGL(k_stream_read, dim3((unsigned)((n16 + kReadSpan - 1) / kReadSpan)),
+                               dim3(256), 0, s, a.p, n16, sink.p);
             t.end();
         }
         TD_HIP(hipGetLastError());
@@ -620,6 +631,66 @@ int tdbem_batch_run(tdbem_batch* b, int steps_per_graph, int max_replays, int* l
 int tdbem_batch_destroy(tdbem_batch* b)
 {
     delete b;
+    return TDBEM_OK;
+}
+
+int tdbem_set_max_step(tdbem_context* ctx, int max_step)
+{
+    return guarded(ctx, [&](Context& c) { set_max_step(c, max_step); });
+}
+
+int tdbem_mrt_create(tdbem_context* region1, tdbem_context* region2, int max_step, int n_iface,
+                     const int* iface1, const int* iface2, tdbem_mrt** out)
+{
+    if (!region1 || !region2 || !out) return TDBEM_ERR_ARG;
+    *out = nullptr;
+    return batch_guarded([&] {
+        std::unique_ptr<tdbem_mrt> m(new tdbem_mrt());
+        m->m = mrt_create(&region1->c, &region2->c, max_step, n_iface, iface1, iface2);
+        *out = m.release();
+    });
+}
+
+int tdbem_mrt_init(tdbem_mrt* m, int scheme, int load_kind, double amp, double t_rise,
+                   int n_fbc1, const int* fbc_dof1, const double* fbc_val1, int n_fbc2,
+                   const int* fbc_dof2, const double* fbc_val2)
+{
+    if (!m) return TDBEM_ERR_ARG;
+    return batch_guarded([&] {
+        const int n[2] = {n_fbc1, n_fbc2};
+        const int* dof[2] = {fbc_dof1, fbc_dof2};
+        const double* val[2] = {fbc_val1, fbc_val2};
+        if (n_fbc1 < 0 || n_fbc2 < 0) throw std::invalid_argument("negative FBC count");
+        mrt_init(*m->m, scheme, load_kind, amp, t_rise, n, dof, val);
+    });
+}
+
+int tdbem_mrt_run(tdbem_mrt* m, int nsteps)
+{
+    if (!m) return TDBEM_ERR_ARG;
+    return batch_guarded([&] { mrt_run(*m->m, nsteps); });
+}
+
+int tdbem_mrt_step_index(const tdbem_mrt* m) { return m ? mrt_step_count(*m->m) : TDBEM_ERR_ARG; }
+
+int tdbem_mrt_num_unknowns(const tdbem_mrt* m) { return m ? mrt_unknowns(*m->m) : TDBEM_ERR_ARG; }
+
+int tdbem_mrt_get_state(tdbem_mrt* m, int s, double* x, double* u, double* t)
+{
+    if (!m) return TDBEM_ERR_ARG;
+    return batch_guarded([&] { mrt_state(*m->m, s, x, u, t); });
+}
+
+int tdbem_mrt_get_timings(const tdbem_mrt* m, double* init_ms, double* run_ms)
+{
+    if (!m || !init_ms || !run_ms) return TDBEM_ERR_ARG;
+    mrt_timings(*m->m, init_ms, run_ms);
+    return TDBEM_OK;
+}
+
+int tdbem_mrt_destroy(tdbem_mrt* m)
+{
+    delete m;
     return TDBEM_OK;
 }
 

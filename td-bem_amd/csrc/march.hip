@@ -188,16 +188,25 @@ __global__ void k_finalize(const double* __restrict__ parts, int nparts, int Np,
                       a2, a3, U, T, ring, t);
 }
 
-template <int NRHS>
-void launch_conv(Context& c, March& M)
+}  // namespace
+
+void conv_fp64(Context& c, const double* ring, int R, int s, int nrhs, double* hist,
+               hipStream_t st)
 {
-    hipLaunchKernelGGL(k_conv<NRHS>, dim3(c.nnd), dim3(kConvThreads), 0, c.stream,
-                       c.store.vals.p, c.store.idx.p, c.store.row_chunk.p, M.N, M.ring.p, M.R,
-                       M.s, M.hist.p);
+    const int N = 3 * c.nnd;
+    const dim3 grid(c.nnd), block(kConvThreads);
+    const double *v = c.store.vals.p;
+    const uint32_t* idx = c.store.idx.p;
+    const int64_t* rc = c.store.row_chunk.p;
+    switch (nrhs) {
+    case 1: hipLaunchKernelGGL(k_conv<1>, grid, block, 0, st, v, idx, rc, N, ring, R, s, hist); break;
+    case 2: hipLaunchKernelGGL(k_conv<2>, grid, block, 0, st, v, idx, rc, N, ring, R, s, hist); break;
+    case 4: hipLaunchKernelGGL(k_conv<4>, grid, block, 0, st, v, idx, rc, N, ring, R, s, hist); break;
+    case 8: hipLaunchKernelGGL(k_conv<8>, grid, block, 0, st, v, idx, rc, N, ring, R, s, hist); break;
+    default: throw std::runtime_error("nrhs must be 1, 2, 4 or 8");
+    }
     TD_HIP(hipGetLastError());
 }
-
-}  // namespace
 
 void march_free(Context& c)
 {
@@ -419,13 +428,7 @@ void march_run(Context& c, int nsteps, bool wait)
                 reduce_timer.end();
             } else {
                 conv_timer.begin();
-                switch (M.nrhs) {
-                case 1: launch_conv<1>(c, M); break;
-                case 2: launch_conv<2>(c, M); break;
-                case 4: launch_conv<4>(c, M); break;
-                case 8: launch_conv<8>(c, M); break;
-                default: throw std::runtime_error("nrhs must be 1, 2, 4 or 8");
-                }
+                conv_fp64(c, M.ring.p, M.R, s, M.nrhs, M.hist.p, c.stream);
                 conv_timer.end();
             }
             hipLaunchKernelGGL(k_rhs, dim3(blocks(tot)), dim3(256), 0, c.stream, M.gF.p, M.gU.p,

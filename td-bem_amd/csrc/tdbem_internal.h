@@ -281,6 +281,24 @@ int march_step_count(const Context& c);
 void march_history(Context& c, int rhs, double* U, double* T);
 void march_state(Context& c, int rhs, int s, double* u, double* t);
 void march_free(Context& c);
+// k_conv: hist[r][3i+p] = sum over the FP64 lag store of G z^t - H z^u, z from ring slot
+// (s - k) mod R of the layout [R][nrhs][2][3 nnd]; nrhs in {1, 2, 4, 8}
+void conv_fp64(Context& c, const double* ring, int R, int s, int nrhs, double* hist,
+               hipStream_t st);
+
+// mrt.hip: the two-region solver (SolverMRT) over two FP64 region contexts
+struct Mrt;
+// SetMaxStep: the lag store holds M_0 .. M_K (K capped at numstep); a change drops the store
+void set_max_step(Context& c, int K);
+Mrt* mrt_create(Context* c1, Context* c2, int max_step, int nz, const int* if1, const int* if2);
+void mrt_destroy(Mrt* M);
+void mrt_init(Mrt& M, int scheme, int load_kind, double amp, double t_rise, const int* n_fbc,
+              const int* const* fbc_dof, const double* const* fbc_val);
+void mrt_run(Mrt& M, int nsteps);
+int mrt_unknowns(const Mrt& M);
+int mrt_step_count(const Mrt& M);
+void mrt_state(Mrt& M, int s, double* x, double* u, double* t);
+void mrt_timings(const Mrt& M, double* init_ms, double* run_ms);
 
 // One scenario of a batched march, as the batched kernels see it (batch.hip): the march state
 // and the mixed-precision store of one context.

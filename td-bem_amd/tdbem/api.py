@@ -20,6 +20,8 @@ KERNEL_BSPLINE = 5
 KERNEL_RAMP = 6
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+MRT_DIRECT, MRT_AVERAGING = 0, 1
+MRT_LOAD_PULSE, MRT_LOAD_RAMP = 0, 1
 
 _lib = None
 
@@ -77,6 +79,16 @@ def lib():
         L.tdbem_set_profiling.argtypes = [vp, C.c_int]
         L.tdbem_reset_timings.argtypes = [vp]
         L.tdbem_get_timings.argtypes = [vp, dp]
+        L.tdbem_set_max_step.argtypes = [vp, C.c_int]
+        L.tdbem_mrt_create.argtypes = [vp, vp, C.c_int, C.c_int, ip, ip, C.POINTER(vp)]
+        L.tdbem_mrt_init.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, ip,
+                                     dp, C.c_int, ip, dp]
+        L.tdbem_mrt_run.argtypes = [vp, C.c_int]
+        L.tdbem_mrt_step_index.argtypes = [vp]
+        L.tdbem_mrt_num_unknowns.argtypes = [vp]
+        L.tdbem_mrt_get_state.argtypes = [vp, C.c_int, dp, dp, dp]
+        L.tdbem_mrt_get_timings.argtypes = [vp, dp, dp]
+        L.tdbem_mrt_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -137,6 +149,10 @@ class Context:
     @property
     def max_step(self):
         return lib().tdbem_max_step(self._h)
+
+    def set_max_step(self, K):
+        """SetMaxStep: keep lags 0..K (a change drops a built lag store)."""
+        self._check(lib().tdbem_set_max_step(self._h, int(K)))
 
     def assemble_step(self, step, lower=0, upper=None):
         upper = self.nnd - 1 if upper is None else upper
@@ -352,3 +368,81 @@ class Batch:
 
     def __exit__(self, *a):
         self.close()
+
+
+class TwoRegionSolver:
+    """SolverMRT (src/SolverMRT.cpp) over two FP64 region contexts (tdbem_mrt_*).
+
+    iface1 / iface2: interface node ids (0-based) of each region; region 2's are matched to
+    region 1's by coordinates.  max_step is SetMaxStep's value (the reference's default is 10).
+    The region contexts must stay alive while the solver exists."""
+
+    def __init__(self, region1, region2, iface1, iface2, max_step=10):
+        self.regions = (region1, region2)
+        i1 = np.ascontiguousarray(iface1, dtype=np.int32)
+        i2 = np.ascontiguousarray(iface2, dtype=np.int32)
+        if len(i1) != len(i2):
+            raise ValueError("Interface mismatch, interface nodes must match one-to-one")
+        h = C.c_void_p()
+        rc = lib().tdbem_mrt_create(region1._h, region2._h, int(max_step), len(i1), _i(i1),
+                                    _i(i2), C.byref(h))
+        if rc:
+            raise TdbemError(rc, lib().tdbem_last_error(None).decode())
+        self._h = h
+        self.n = lib().tdbem_mrt_num_unknowns(self._h)
+        self.numstep = region1.numstep
+
+    def _check(self, rc):
+        if rc:
+            raise TdbemError(rc, lib().tdbem_last_error(None).decode())
+
+    def init(self, scheme=MRT_DIRECT, load_kind=MRT_LOAD_PULSE, amp=1.0, t_rise=0.0,
+             fbc1=((), ()), fbc2=((), ())):
+        """fbc1 / fbc2: (dofs, values) traction conditions of each region."""
+        arrs = []
+        for dof, val in (fbc1, fbc2):
+            d = np.ascontiguousarray(dof, dtype=np.int32)
+            v = np.ascontiguousarray(val, dtype=np.float64)
+            if len(d) != len(v):
+                raise ValueError("FBC dofs and values differ in length")
+            arrs += [d, v]
+        self._fbc = arrs
+        self._check(lib().tdbem_mrt_init(self._h, int(scheme), int(load_kind), float(amp),
+                                         float(t_rise), len(arrs[0]), _i(arrs[0]), _d(arrs[1]),
+                                         len(arrs[2]), _i(arrs[2]), _d(arrs[3])))
+        self.scheme = scheme
+
+    def run(self, nsteps=None):
+        self._check(lib().tdbem_mrt_run(self._h, int(nsteps if nsteps is not None else
+                                                     self.numstep)))
+        return lib().tdbem_mrt_step_index(self._h)
+
+    def state(self, s):
+        """(x, u, t) of step s: global unknowns, region displacements and tractions."""
+        x, u, t = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
+        self._check(lib().tdbem_mrt_get_state(self._h, int(s), _d(x), _d(u), _d(t)))
+        return x, u, t
+
+    def history(self):
+        """X, U, T [numstep + 1][n]."""
+        X = np.zeros((self.numstep + 1, self.n))
+        U, T = np.zeros_like(X), np.zeros_like(X)
+        for s in range(self.numstep + 1):
+            X[s], U[s], T[s] = self.state(s)
+        return X, U, T
+
+    def timings(self):
+        a, b = C.c_double(), C.c_double()
+        self._check(lib().tdbem_mrt_get_timings(self._h, C.byref(a), C.byref(b)))
+        return {"init_ms": a.value, "run_ms": b.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().tdbem_mrt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -81,6 +81,25 @@ def bar(h=0.25, length=4.0, numstep=100, beta=0.6, E=1.0, nu=0.25, rho=1.0):
                    ubc, np.zeros(len(ubc)), fbc, np.ones(len(fbc)), meta={"h": h})
 
 
+def split_bar(h=0.5, length=4.0, numstep=40, beta=0.6, E=1.0, nu=0.25, rho=1.0):
+    """The free 1x1xL bar cut at z = L/2 into two regions for the two-region solver: region 1
+    [0,1]^2 x [0,L/2], region 2 above it, interface z = L/2 (region 1's top face, region 2's
+    bottom face), unit traction t_z on z = L.  Returns (region1, region2, iface1, iface2)."""
+    dt = _dt(E, nu, rho, h, beta)
+    m1 = meshgen.box_surface(1.0, 1.0, length / 2, h)
+    m2 = meshgen.box_surface(1.0, 1.0, length / 2, h)
+    c2 = m2.coords.copy()
+    c2[:, 2] += length / 2
+    top = m2.tags["ztop"]
+    fbc = np.sort(3 * top + 2).astype(np.int32)
+    none_i, none_v = np.zeros(0, np.int32), np.zeros(0)
+    r1 = Problem("split_bar_lower", m1.coords, m1.ien, E, nu, rho, dt, numstep, 0, none_i, none_v,
+                 none_i, none_v, meta={"h": h})
+    r2 = Problem("split_bar_upper", c2, m2.ien, E, nu, rho, dt, numstep, 0, none_i, none_v, fbc,
+                 np.ones(len(fbc)), meta={"h": h})
+    return r1, r2, m1.tags["ztop"].astype(np.int32), m2.tags["z0"].astype(np.int32)
+
+
 def _pressure_fbc(coords, center, p, nodes=None):
     """Traction t = p * (x - c)/|x - c| on every dof of `nodes` (a pressure p in a cavity pushes
     the wall outward; the element normals point into the cavity)."""

@@ -2,8 +2,10 @@
 (oracle/_ref/libtdbem_ref.so, built from /root/reference/src by `make -C oracle ref`).
 
 Run in the development container, where the reference tree is mounted:
-    python tests/golden/make_golden.py [green_ramp] [elements]
-Each fixture holds the inputs and the reference's outputs; no reference source is copied.
+    python tests/golden/make_golden.py [green_ramp] [elements] [layer_decks]
+Each fixture holds the inputs and the reference's outputs (layer_decks: the data of two input
+decks, for the two-region solver, which needs PETSc in the reference); no reference source is
+copied.
 """
 import sys
 from pathlib import Path
@@ -63,7 +65,24 @@ def green_ramp_cases(seed=17, count=120):
     return rows
 
 
+def write_layer_decks():
+    """geometry/Layer_U_test0.txt and Layer_B_test0.txt, the two regions of the reference's
+    two-region (SolverMRT) decks, as parsed by InputT::ReadInput: the deck data only."""
+    from tdbem import deck
+    ref_root = Path("/root/reference/geometry")
+    out = {}
+    for r, name in enumerate(("Layer_U_test0", "Layer_B_test0")):
+        d = deck.read_deck(ref_root / f"{name}.txt")
+        out.update({f"coords{r}": d.coords, f"ien{r}": d.ien, f"iface{r}": d.interface,
+                    f"fbc_dof{r}": d.fbc_dof, f"fbc_val{r}": d.fbc_val,
+                    f"scalars{r}": np.array([d.E, d.nu, d.density, d.dt, d.numstep, d.nq,
+                                             d.etype, d.if_ext, d.interpolation], float)})
+    np.savez_compressed(HERE / "layer_decks.npz", **out)
+
+
 def main(names):
+    if "layer_decks" in names:
+        write_layer_decks()
     if O.ref() is None:
         raise SystemExit("oracle/_ref/libtdbem_ref.so is not built (make -C oracle ref)")
     if "green_ramp" in names:
@@ -104,5 +123,5 @@ def write_elements_and_assembly():
 
 
 if __name__ == "__main__":
-    # python tests/golden/make_golden.py [green_ramp] [elements]   (default: all)
-    main(sys.argv[1:] or ["green_ramp", "elements"])
+    # python tests/golden/make_golden.py [green_ramp] [elements] [layer_decks]  (default: all)
+    main(sys.argv[1:] or ["green_ramp", "elements", "layer_decks"])
