@@ -473,4 +473,169 @@ void SolverStepLinearT::WriteGH(const char* name_prefix, int step, int num_row, 
     std::fclose(f);
 }
 
+// ------------------------------------------------------------------------------------------
+// SolverMRT
+// ------------------------------------------------------------------------------------------
+namespace {
+
+void check_mrt(int rc)
+{
+    if (rc != TDBEM_OK) {
+        static thread_local std::string msg;
+        msg = tdbem_last_error(nullptr);
+        throw msg.c_str();
+    }
+}
+
+}  // namespace
+
+SolverMRT::SolverMRT(ModelManagerT* model_1, ModelManagerT* model_2) : fModel{model_1, model_2}
+{
+    // src/SolverMRT.cpp:22-60: dt and numstep come from model 1, fMaxStep starts at 10
+    if (model_1->GetNumInterface() != model_2->GetNumInterface())
+        fail("Interface mismatch, interface nodes must match one-to-one");
+    SetTimeControl(model_1->GetDT(), model_1->GetNumStep());
+}
+
+SolverMRT::~SolverMRT()
+{
+    tdbem_mrt_destroy(fMrt);
+    tdbem_destroy(fCtx[0]);
+    tdbem_destroy(fCtx[1]);
+}
+
+void SolverMRT::SetTimeControl(double dt, int num_steps)
+{
+    fDt = dt;
+    fNumStep = num_steps;
+    tdbem_mrt_destroy(fMrt);
+    fMrt = nullptr;
+    for (int r = 0; r < 2; r++) {
+        tdbem_destroy(fCtx[r]);
+        fCtx[r] = nullptr;
+        double E, nu, rho;
+        fModel[r]->GetMaterialConstants(E, nu, rho);
+        check_mrt(tdbem_create(fModel[r]->GetNND(), fModel[r]->GetCoords(), fModel[r]->GetNEL(),
+                               fModel[r]->GetIEN(), fModel[r]->GetElementType(),
+                               fModel[r]->GetNQ(), E, nu, rho, fModel[r]->GetIfExt(), dt,
+                               num_steps, TDBEM_KERNEL_STEP, &fCtx[r]));
+    }
+    check_mrt(tdbem_mrt_create(fCtx[0], fCtx[1], fMaxStep, fModel[0]->GetNumInterface(),
+                               fModel[0]->GetInterface(), fModel[1]->GetInterface(), &fMrt));
+    fScheme = -1;
+}
+
+void SolverMRT::SetMaxStep(int max_step)
+{
+    if (fScheme >= 0) fail("SetMaxStep after the first solve");
+    fMaxStep = max_step;
+    SetTimeControl(fDt, fNumStep);
+}
+
+void SolverMRT::GetNodeLowHigh(int* low_1, int* high_1, int* low_2, int* high_2) const
+{
+    *low_1 = 0;
+    *high_1 = fModel[0]->GetNND() - 1;
+    *low_2 = 0;
+    *high_2 = fModel[1]->GetNND() - 1;
+}
+
+void SolverMRT::GetDofLowHigh(int* low_1, int* high_1, int* low_2, int* high_2) const
+{
+    *low_1 = 0;
+    *high_1 = 3 * fModel[0]->GetNND() - 1;
+    *low_2 = 0;
+    *high_2 = 3 * fModel[1]->GetNND() - 1;
+}
+
+void SolverMRT::Advance(int scheme, int target)
+{
+    if (fScheme < 0) {
+        int nu[2], nf[2];
+        const int *ud[2], *fd[2];
+        const double *uv[2], *fv[2];
+        for (int r = 0; r < 2; r++)
+            fModel[r]->BoundaryCondition_Global(nu[r], &ud[r], &uv[r], nf[r], &fd[r], &fv[r]);
+        check_mrt(tdbem_mrt_init(fMrt, scheme, TDBEM_MRT_LOAD_PULSE, 1.0, 0.0, nf[0], fd[0], fv[0],
+                                 nf[1], fd[1], fv[1]));
+        fScheme = scheme;
+    } else if (fScheme != scheme) {
+        fail("Solve_Direct and Solve_Averaging cannot be mixed in one run");
+    }
+    while (tdbem_mrt_step_index(fMrt) < target) check_mrt(tdbem_mrt_run(fMrt, 1));
+}
+
+// src/SolverMRT.cpp:1035-1099: step fCurrStep solves x_{fCurrStep}
+void SolverMRT::Solve_Direct() { Advance(TDBEM_MRT_DIRECT, fCurrStep); }
+
+// src/SolverMRT.cpp:1101-1249: step fCurrStep solves x_{fCurrStep-1}
+void SolverMRT::Solve_Averaging() { Advance(TDBEM_MRT_AVERAGING, fCurrStep - 1); }
+
+void SolverMRT::GetField(int region, int step, std::vector<double>& u)
+{
+    if (region < 0 || region > 1) fail("region must be 0 or 1");
+    const int n = tdbem_mrt_num_unknowns(fMrt);
+    std::vector<double> all(n, 0.0);
+    if (fScheme >= 0) check_mrt(tdbem_mrt_get_state(fMrt, step, nullptr, all.data(), nullptr));
+    const int n0 = 3 * fModel[0]->GetNND();
+    if (region == 0)
+        u.assign(all.begin(), all.begin() + n0);
+    else
+        u.assign(all.begin() + n0, all.end());
+}
+
+void SolverMRT::RecordResult(const char* path, int record_dof, bool vtk)
+{
+    // src/SolverMRT.cpp:1325-1383: region 2's dof of step fCurrStep-1 at time dt (fCurrStep-1)
+    std::ofstream f;
+    if (fCurrStep == 1)
+        f.open(path, std::ios::out | std::ios::trunc);
+    else
+        f.open(path, std::ios::out | std::ios::app);
+    std::vector<double> u2;
+    GetField(1, fCurrStep - 1, u2);
+    if (record_dof < 0 || record_dof >= (int)u2.size()) fail("record dof out of range");
+    f << std::setprecision(6);
+    f << std::setw(10) << fDt * (fCurrStep - 1) << std::setw(15) << u2[record_dof] << std::endl;
+    if (vtk) {
+        OutPutVTK("BEM_1st_", 0);
+        OutPutVTK("BEM_2nd_", 1);
+    }
+}
+
+void SolverMRT::OutPutVTK(std::string filename, int region)
+{
+    // src/SolverMRT.cpp:1386-1466: legacy VTK, quads (cell type 9), displacement of fCurrStep-1
+    ModelManagerT* m = fModel[region];
+    std::vector<double> u;
+    GetField(region, fCurrStep - 1, u);
+    filename += std::to_string(fCurrStep) + ".vtk";
+    std::ofstream f(filename, std::ios::out | std::ios::trunc);
+    const int nnd = m->GetNND(), nel = m->GetNEL();
+    const int* ien = m->GetIEN();
+    const double* X = m->GetCoords();
+    f << "# vtk DataFile Version 3.0 " << std::endl;
+    f << "Boundary element result" << std::endl;
+    f << "ASCII" << std::endl;
+    f << "DATASET UNSTRUCTURED_GRID " << std::endl;
+    f << "POINTS " << nnd << " float" << std::endl;
+    f << std::setprecision(6);
+    for (int n = 0; n < nnd; n++)
+        f << std::setw(15) << X[3 * n] << std::setw(15) << X[3 * n + 1] << std::setw(15)
+          << X[3 * n + 2] << std::endl;
+    f << "CELLS  " << nel << " " << nel * 5 << std::endl;
+    const int ennd = m->GetENND();
+    for (int e = 0; e < nel; e++)
+        f << std::setw(5) << 4 << std::setw(5) << ien[ennd * e] << std::setw(5)
+          << ien[ennd * e + 1] << std::setw(5) << ien[ennd * e + 2] << std::setw(5)
+          << ien[ennd * e + 3] << std::endl;
+    f << "CELL_TYPES " << nel << std::endl;
+    for (int e = 0; e < nel; e++) f << 9 << std::endl;
+    f << "POINT_DATA " << nnd << std::endl;
+    f << "VECTORS " << "Displacement " << "double" << std::endl;
+    for (int n = 0; n < nnd; n++)
+        f << std::setw(15) << u[3 * n] << std::setw(15) << u[3 * n + 1] << std::setw(15)
+          << u[3 * n + 2] << std::endl;
+}
+
 }  // namespace TD_BEM

@@ -74,6 +74,8 @@ public:
     {
         fInput.GetMaterialConstants(E, nu, rho);
     }
+    int GetNumInterface() const { return fInput.GetNumInterface(); }
+    const int* GetInterface() const { return fInput.GetInterface(); }
     void Distributed_BC(int low, int high);
     void BoundaryCondition_Local(int& UBC_num, const int** UBC_DOFs, const double** UBC_Values,
                                  int& FBC_num, const int** FBC_DOFs,
@@ -189,6 +191,48 @@ private:
     int fMaxStep = 0, fNumStep = 0, fCurrStep = 0, fN = 0;
     double fDt = 0;
     bool fInit = false;
+};
+
+// src/SolverMRT.h:24-134: two regions (model_1, model_2) coupled on their interface nodes, each
+// on its own GPU context; lag stores, the global step matrix and the march live on the GPU
+// (tdbem_mrt_*).  Solve runs Solve_Direct as the reference's does; Solve_Averaging is the other
+// scheme (one scheme per run).  The loads are each model's FBC list times the pulse of SetLoad.
+// RecordResult appends the time and the region-2 dof the reference records (46*3-1, its "coarse
+// mesh" line) of step fCurrStep-1 to displacement.txt and writes the OutPutVTK files
+// BEM_1st_<n>.vtk and BEM_2nd_<n>.vtk.
+class SolverMRT {
+public:
+    SolverMRT(ModelManagerT* model_1, ModelManagerT* model_2);
+    ~SolverMRT();
+    void SetTimeControl(double dt, int num_steps);
+    void SetCurrStep(int curr_step) { fCurrStep = curr_step; }
+    void SetMaxStep(int max_step);
+    int GetMaxStep() const { return fMaxStep; }
+    void GetNodeLowHigh(int* low_1, int* high_1, int* low_2, int* high_2) const;
+    void GetDofLowHigh(int* low_1, int* high_1, int* low_2, int* high_2) const;
+    void UpdateGHLinear(const double*, const double*, const double*, const double*,
+                        const double*, const double*, const double*, const double*)
+    {
+        // the lag stores are assembled on the GPU in one pass by the first solve
+    }
+    void SetLoad() {}
+    void Solve() { Solve_Direct(); }
+    void Solve_Direct();
+    void Solve_Averaging();
+    void ExtractField() {}
+    void RecordResult(const char* path = "displacement.txt", int record_dof = 46 * 3 - 1,
+                      bool vtk = true);
+    void OutPutVTK(std::string filename, int region);
+    // displacements u [3 nnd_r] of region r (0 or 1) at step s (0 .. numstep)
+    void GetField(int region, int step, std::vector<double>& u);
+
+private:
+    void Advance(int scheme, int target);
+    ModelManagerT* fModel[2];
+    tdbem_context* fCtx[2] = {nullptr, nullptr};
+    tdbem_mrt* fMrt = nullptr;
+    int fMaxStep = 10, fNumStep = 0, fCurrStep = 0, fScheme = -1;
+    double fDt = 0;
 };
 
 }  // namespace TD_BEM

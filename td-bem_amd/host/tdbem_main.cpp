@@ -1,14 +1,21 @@
 // tdbem_main.cpp — command-line driver with the step loop of src/main.cpp:22-149 on the GPU.
 //
 //   tdbem_main input.txt [--write-gh] [--history out.bin]
+//   tdbem_main region1.txt --mrt region2.txt [--averaging] [--max-step K] [--no-vtk]
 //
 // Reads the deck and alpha.txt (a1 a2) from the working directory like the reference, marches
 // numstep steps, and writes displacement.txt in the reference's format.  --write-gh also writes
 // the FE-BE matrices G1_step<k>.txt, G2_step<k>.txt, H1_step<k>.txt, H2_step<k>.txt for every
 // assembled step (the calls commented out at main.cpp:113-116, with their G1/G2 naming: file G1
 // holds fG_2 = phi_2 part, file G2 holds fG_1).  --history writes the full-precision u and t
-// histories as raw doubles [2][numstep][3 nnd].  Unlike the reference, errors exit non-zero.
+// histories as raw doubles [2][numstep][3 nnd].  --mrt runs the two-region solver SolverMRT
+// (src/SolverMRT.cpp; the reference ships no driver for it) on the two decks with the step loop
+// UpdateGHLinear / SetLoad / Solve / RecordResult: Solve_Direct as SolverMRT::Solve does, or
+// Solve_Averaging with --averaging; fMaxStep 10 unless --max-step; displacement.txt gets the
+// reference's recorded region-2 dof and BEM_1st_<n>.vtk / BEM_2nd_<n>.vtk the region fields
+// (not with --no-vtk).  Unlike the reference, errors exit non-zero.
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iostream>
@@ -24,17 +31,53 @@ int main(int argc, char** argv)
         std::cerr << "Input file not specified, try ./TD_BEM input.txt" << std::endl;
         return 2;
     }
-    bool write_gh = false;
+    bool write_gh = false, averaging = false, vtk = true;
     const char* history = nullptr;
+    const char* mrt = nullptr;
+    int max_step = 10;
     for (int a = 2; a < argc; a++) {
         if (!std::strcmp(argv[a], "--write-gh"))
             write_gh = true;
         else if (!std::strcmp(argv[a], "--history") && a + 1 < argc)
             history = argv[++a];
+        else if (!std::strcmp(argv[a], "--mrt") && a + 1 < argc)
+            mrt = argv[++a];
+        else if (!std::strcmp(argv[a], "--averaging"))
+            averaging = true;
+        else if (!std::strcmp(argv[a], "--no-vtk"))
+            vtk = false;
+        else if (!std::strcmp(argv[a], "--max-step") && a + 1 < argc)
+            max_step = std::atoi(argv[++a]);
         else {
             std::cerr << "unknown option " << argv[a] << std::endl;
             return 2;
         }
+    }
+    if (mrt) {
+        try {
+            ModelManagerT model_1, model_2;
+            model_1.ReadInput(argv[1]);
+            model_2.ReadInput(mrt);
+            SolverMRT solver(&model_1, &model_2);
+            if (max_step != 10) solver.SetMaxStep(max_step);
+            const int num_step = model_1.GetNumStep();
+            for (int curr_step = 1; curr_step <= num_step; curr_step++) {
+                solver.SetCurrStep(curr_step);
+                solver.UpdateGHLinear(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                      nullptr, nullptr);
+                solver.SetLoad();
+                if (averaging)
+                    solver.Solve_Averaging();
+                else
+                    solver.Solve();
+                solver.RecordResult("displacement.txt", 46 * 3 - 1, vtk);
+            }
+        } catch (const char* msg) {
+            std::cerr << msg << std::endl;
+            return 1;
+        }
+        std::cout << "The computation terminated properly" << std::endl;
+        return 0;
     }
     try {
         ModelManagerT model;

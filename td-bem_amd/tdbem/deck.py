@@ -186,8 +186,9 @@ def _num(v):
 
 def write_deck(path, mesh_coords, ien, *, E, nu, density, dt, numstep, nq=6, if_ext=0,
                ubc_dof=(), ubc_val=(), fbc_dof=(), fbc_val=(), etype=1, interpolation=4,
-               formulation=1):
-    """Write a deck in the reference format (full-precision numbers, 1-based ids)."""
+               formulation=1, interface=None):
+    """Write a deck in the reference format (full-precision numbers, 1-based ids); interface:
+    0-based interface node ids of a two-region deck (num_interface / interface blocks)."""
     ubc_dof = np.asarray(ubc_dof, dtype=np.int64)
     fbc_dof = np.asarray(fbc_dof, dtype=np.int64)
     out = [f"formulation={formulation};", f"numstep={numstep};", f"dt={_num(dt)};",
@@ -208,5 +209,8 @@ def write_deck(path, mesh_coords, ien, *, E, nu, density, dt, numstep, nq=6, if_
     out += ["", "UBC_values="] + lst(ubc_val, _num)
     out += ["", f"FBC_num={len(fbc_dof)};", "FBC_dof="] + lst(fbc_dof + 1, str)
     out += ["", "FBC_values="] + lst(fbc_val, _num)
+    if interface is not None:
+        iface = np.asarray(interface, dtype=np.int64)
+        out += ["", f"num_interface={len(iface)}", "", "interface="] + lst(iface + 1, str)
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")

@@ -153,3 +153,15 @@ def test_serendipity_conversion():
     assert m.ien.shape == (b.nel, 8)
     mid = m.coords[m.ien[:, 4]]
     np.testing.assert_allclose(mid, 0.5 * (m.coords[m.ien[:, 0]] + m.coords[m.ien[:, 1]]))
+
+
+def test_write_deck_round_trips_the_interface(tmp_path):
+    z = np.load(Path(__file__).resolve().parent / "golden" / "layer_decks.npz")
+    f = tmp_path / "region.txt"
+    deck.write_deck(f, z["coords0"], z["ien0"], E=1e6, nu=0.0, density=1.0, dt=5e-3, numstep=10,
+                    fbc_dof=z["fbc_dof0"], fbc_val=z["fbc_val0"], formulation=3,
+                    interface=z["iface0"])
+    d = deck.read_deck(f)
+    np.testing.assert_array_equal(d.interface, z["iface0"])
+    np.testing.assert_array_equal(d.coords, z["coords0"])
+    np.testing.assert_array_equal(d.fbc_dof, z["fbc_dof0"])

@@ -103,10 +103,13 @@ def check(got, ref, per_step=False):
         assert helpers.rel_fro(got, ref) <= TOL
 
 
+@pytest.mark.parametrize("order", ["UB", "BU"])
 @pytest.mark.parametrize("scheme", [api.MRT_DIRECT, api.MRT_AVERAGING])
-def test_layer_decks(scheme):
+def test_layer_decks(scheme, order):
     regs = layer_regions()
-    print(f"layer decks, scheme {scheme}", flush=True)
+    if order == "BU":   # Layer_B as region 1: the order SolverMRT::RecordResult's dof implies
+        regs = regs[::-1]
+    print(f"layer decks {order}, scheme {scheme}", flush=True)
     X, U, T = run_gpu(regs, 10, scheme, api.MRT_LOAD_PULSE)
     Xr, Ur = run_oracle(regs, 10, scheme, 0)
     assert np.abs(Xr).max() > 0
@@ -144,3 +147,36 @@ def test_interface_mismatch_is_an_argument_error():
         api.TwoRegionSolver(ctx[0], ctx[1], regs[0]["iface"], regs[1]["iface"])
     assert e.value.code == api.ERR_ARG
     assert "Interface mismatch" in e.value.msg
+
+
+def test_driver_two_region_run(tmp_path):
+    """tdbem_main B.txt --mrt U.txt: the step loop of the host SolverMRT class; displacement.txt
+    holds (dt (n-1), region 2's dof 46*3-1 at step n-1) per step as SolverMRT::RecordResult
+    writes it, and BEM_2nd_<n>.vtk the region-2 field of step n-1."""
+    import subprocess
+    from tdbem import deck
+    regs = layer_regions()[::-1]
+    paths = []
+    for r, name in zip(regs, ("B.txt", "U.txt")):
+        deck.write_deck(tmp_path / name, r["coords"], r["ien"], E=r["E"], nu=r["nu"],
+                        density=r["rho"], dt=r["dt"], numstep=r["numstep"], nq=r["nq"],
+                        if_ext=r["if_ext"], fbc_dof=r["fbc_dof"], fbc_val=r["fbc_val"],
+                        formulation=3, interface=r["iface"])
+        paths.append(str(tmp_path / name))
+    exe = Path(api.PKG) / "tdbem_main"
+    out = subprocess.run([str(exe), paths[0], "--mrt", paths[1]], cwd=tmp_path,
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    rows = np.loadtxt(tmp_path / "displacement.txt")
+    Xr, Ur = run_oracle(regs, 10, 0, 0)
+    n0 = 3 * len(regs[0]["coords"])
+    numstep = regs[0]["numstep"]
+    assert rows.shape == (numstep, 2)
+    np.testing.assert_allclose(rows[:, 0], regs[0]["dt"] * np.arange(numstep), rtol=1e-5)
+    ref = Ur[:numstep, n0 + 46 * 3 - 1]
+    np.testing.assert_allclose(rows[:, 1], ref, rtol=2e-5, atol=1e-6 * np.abs(ref).max())
+    vtk = (tmp_path / f"BEM_2nd_{numstep}.vtk").read_text().split("\n")
+    k = vtk.index("VECTORS Displacement double")
+    field = np.array([[float(v) for v in line.split()] for line in vtk[k + 1:k + 1 + len(regs[1]["coords"])]])
+    np.testing.assert_allclose(field.ravel(), Ur[numstep - 1, n0:], rtol=2e-5,
+                               atol=1e-6 * np.abs(Ur[numstep - 1, n0:]).max())

@@ -230,3 +230,28 @@ def test_split_bar_behaves_like_the_uncut_bar():
         rel = np.abs(got[m] - ref[m]) / np.abs(ref[m])
         print("split bar: max rel", rel.max(), "rms rel", np.sqrt(np.mean(rel ** 2)))
         assert rel.max() < 0.1 and np.sqrt(np.mean(rel ** 2)) < 0.04
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_region_order_does_not_matter(scheme):
+    """Swapping the regions permutes the unknowns and flips the sign of the interface traction
+    (tau is always region 1's traction); the displacements stay the same."""
+    K, numstep, dt = 4, 8, 3e-4
+    regions, lags = _random_case(21 + scheme)
+    X, U = O.mrt_march(regions, lags, numstep, dt, K, scheme=scheme, load_kind=1, amp=1.0,
+                       t_rise=2.5 * dt)
+    Xs, Us = O.mrt_march(regions[::-1], lags[::-1], numstep, dt, K, scheme=scheme, load_kind=1,
+                         amp=1.0, t_rise=2.5 * dt)
+    scale = max(np.abs(U[0]).max(), np.abs(U[1]).max())
+    np.testing.assert_allclose(Us[0], U[1], rtol=0, atol=1e-11 * scale)
+    np.testing.assert_allclose(Us[1], U[0], rtol=0, atol=1e-11 * scale)
+    # tau in region 2's interface order, as region 1 of the swapped run sees it
+    i1, i2 = regions[0]["iface"], regions[1]["iface"]
+    c1, c2 = regions[0]["coords"], regions[1]["coords"]
+    nz = len(i1)
+    tau = X[:, -3 * nz:].reshape(-1, nz, 3)
+    taus = Xs[:, -3 * nz:].reshape(-1, nz, 3)
+    for q, n in enumerate(i2):
+        sl = int(np.argmin(np.linalg.norm(c1[i1] - c2[n], axis=1)))
+        np.testing.assert_allclose(taus[:, q], -tau[:, sl], rtol=0,
+                                   atol=1e-11 * np.abs(tau).max())
