@@ -473,6 +473,22 @@ void SolverStepLinearT::WriteGH(const char* name_prefix, int step, int num_row, 
     std::fclose(f);
 }
 
+void SolverStepLinearT::WriteGH_Binary(const char* name_prefix, int step, int num_row,
+                                       int num_column, const double* data)
+{
+    // full precision: "<prefix><step>.bin" = int64 rows, int64 columns, then the row-major
+    // doubles (little-endian, as the host stores them)
+    std::ostringstream name;
+    name << name_prefix << step << ".bin";
+    std::FILE* f = std::fopen(name.str().c_str(), "wb");
+    if (!f) fail("SolverT::WriteGH, cannot open the output file");
+    const long long dims[2] = {num_row, num_column};
+    bool ok = std::fwrite(dims, sizeof(dims), 1, f) == 1;
+    for (int i = 0; i < num_row && ok; i++)
+        ok = std::fwrite(data + (size_t)i * fN, sizeof(double), num_column, f) == (size_t)num_column;
+    if (std::fclose(f) != 0 || !ok) fail("SolverT::WriteGH, cannot write the output file");
+}
+
 // ------------------------------------------------------------------------------------------
 // SolverMRT
 // ------------------------------------------------------------------------------------------

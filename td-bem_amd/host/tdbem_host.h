@@ -184,6 +184,10 @@ public:
     // src/SolverT.cpp:670-732 with one process: "<prefix><step>.txt", %15.5e rows, no header.
     void WriteGH(const char* name_prefix, int step, int num_row, int num_column,
                  const double* data, int comm_size = 1, int comm_rank = 0);
+    // The same matrices at full precision (no reference counterpart): "<prefix><step>.bin",
+    // two int64 (rows, columns) then the row-major doubles.
+    void WriteGH_Binary(const char* name_prefix, int step, int num_row, int num_column,
+                        const double* data);
 
 private:
     ModelManagerT* fModel;

@@ -1,13 +1,14 @@
 // tdbem_main.cpp — command-line driver with the step loop of src/main.cpp:22-149 on the GPU.
 //
-//   tdbem_main input.txt [--write-gh] [--history out.bin]
+//   tdbem_main input.txt [--write-gh] [--write-gh-binary] [--history out.bin]
 //   tdbem_main region1.txt --mrt region2.txt [--averaging] [--max-step K] [--no-vtk]
 //
 // Reads the deck and alpha.txt (a1 a2) from the working directory like the reference, marches
 // numstep steps, and writes displacement.txt in the reference's format.  --write-gh also writes
 // the FE-BE matrices G1_step<k>.txt, G2_step<k>.txt, H1_step<k>.txt, H2_step<k>.txt for every
 // assembled step (the calls commented out at main.cpp:113-116, with their G1/G2 naming: file G1
-// holds fG_2 = phi_2 part, file G2 holds fG_1).  --history writes the full-precision u and t
+// holds fG_2 = phi_2 part, file G2 holds fG_1); --write-gh-binary writes the same matrices at full
+// precision as G1_step<k>.bin ... (int64 rows, int64 columns, row-major doubles).  --history writes the full-precision u and t
 // histories as raw doubles [2][numstep][3 nnd].  --mrt runs the two-region solver SolverMRT
 // (src/SolverMRT.cpp; the reference ships no driver for it) on the two decks with the step loop
 // UpdateGHLinear / SetLoad / Solve / RecordResult: Solve_Direct as SolverMRT::Solve does, or
@@ -31,13 +32,15 @@ int main(int argc, char** argv)
         std::cerr << "Input file not specified, try ./TD_BEM input.txt" << std::endl;
         return 2;
     }
-    bool write_gh = false, averaging = false, vtk = true;
+    bool write_gh = false, write_bin = false, averaging = false, vtk = true;
     const char* history = nullptr;
     const char* mrt = nullptr;
     int max_step = 10;
     for (int a = 2; a < argc; a++) {
         if (!std::strcmp(argv[a], "--write-gh"))
             write_gh = true;
+        else if (!std::strcmp(argv[a], "--write-gh-binary"))
+            write_bin = true;
         else if (!std::strcmp(argv[a], "--history") && a + 1 < argc)
             history = argv[++a];
         else if (!std::strcmp(argv[a], "--mrt") && a + 1 < argc)
@@ -97,7 +100,7 @@ int main(int argc, char** argv)
                   << "alpha_1=" << a1 << "  alpha_2=" << a2 << "\n"
                   << "***********************************" << std::endl;
 
-        ElastoDynamicsT* ED = write_gh ? new ElastoDynamicsT(&model) : nullptr;
+        ElastoDynamicsT* ED = (write_gh || write_bin) ? new ElastoDynamicsT(&model) : nullptr;
         const int max_step = solver.GetMaxStep();
         for (int curr_step = 1; curr_step <= num_step; curr_step++) {
             if (ED && curr_step <= max_step) {
@@ -105,10 +108,18 @@ int main(int argc, char** argv)
                 double *G_0, *G_1, *H_0, *H_1;
                 ED->GetGHLinear(&G_0, &G_1, &H_0, &H_1);
                 const int rows = dof_high - dof_low + 1, cols = 3 * model.GetNND();
-                solver.WriteGH("G1_step", curr_step - 1, rows, cols, G_1);
-                solver.WriteGH("G2_step", curr_step - 1, rows, cols, G_0);
-                solver.WriteGH("H1_step", curr_step - 1, rows, cols, H_1);
-                solver.WriteGH("H2_step", curr_step - 1, rows, cols, H_0);
+                if (write_gh) {
+                    solver.WriteGH("G1_step", curr_step - 1, rows, cols, G_1);
+                    solver.WriteGH("G2_step", curr_step - 1, rows, cols, G_0);
+                    solver.WriteGH("H1_step", curr_step - 1, rows, cols, H_1);
+                    solver.WriteGH("H2_step", curr_step - 1, rows, cols, H_0);
+                }
+                if (write_bin) {
+                    solver.WriteGH_Binary("G1_step", curr_step - 1, rows, cols, G_1);
+                    solver.WriteGH_Binary("G2_step", curr_step - 1, rows, cols, G_0);
+                    solver.WriteGH_Binary("H1_step", curr_step - 1, rows, cols, H_1);
+                    solver.WriteGH_Binary("H2_step", curr_step - 1, rows, cols, H_0);
+                }
             }
             solver.SetCurrStep(curr_step);
             solver.UpdateGHLinear(nullptr, nullptr, nullptr, nullptr);
