@@ -15,8 +15,8 @@ and fp64_8rhs for the FP64 path at 8 right-hand sides, its best rate.  The mixed
 8 right-hand sides of ||u_mixed - u_fp64||_F / ||u_fp64||_F (and the same for t).
 
 roofline: the mixed path's dominant kernel k_conv_mixed, HBM-bound: algorithmic bytes per launch =
-stored 16 x 32 split-BF16 sub-tiles x 3072 B + merged-list entries x 24 B + blocks x 6 B + the
-split-BF16 history read once, over its mean duration from HIP events on the context stream;
+stored 16 x 32 FP32 sub-tiles x 2048 B + merged-list entries x 32 B + the split-BF16 history
+read once, over its mean duration from HIP events on the context stream;
 plain_fp64.roofline gives k_conv the same way (stored entries x 124 B).
 Printed as one JSON line; --gpus N > 1 exits (one GPU per box in this build).
 """
@@ -32,9 +32,8 @@ sys.path.insert(0, str(ROOT / "td-bem_amd"))
 import numpy as np  # noqa: E402
 
 MI355X_HBM_GBS = 8000.0
-TP_SUBTILE_BYTES = 3 * 16 * 32 * 2       # split-BF16 16x32 sub-tile
-TP_ENTRY_BYTES = 24                      # merged-list step (4 block ids, J/k, k-tile mask)
-TP_BLOCK_META = 6                        # per block: first sub-tile (4 B) and mask (2 B)
+TP_SUBTILE_BYTES = 16 * 32 * 4           # FP32 16x32 sub-tile
+TP_ENTRY_BYTES = 32                      # merged-list step: 4 first sub-tiles, 4 masks, J/k, k-tile mask
 FP64_ENTRY_BYTES = 15 * 8 + 4            # 15 values + index per (i, j, k)
 # HBM bytes per launch (read + written) from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE)
 # on this workload, scaled by the counters of a 1 GiB k_stream_copy (factors 2.0 and 1.0);
@@ -244,11 +243,11 @@ def main():
             mixed_hist = [ctx.history(r) for r in range(8)]
         nconv = max(tm["conv_launches"], 1)
         conv_ms = tm["conv_ms"] / nconv
-        # per launch: every stored split-BF16 sub-tile, the merged lists and block metadata, and
-        # the split-BF16 history read once ((K + 1) slots x 96 columns per tile x R x 6 B)
+        # per launch: every stored FP32 sub-tile, the merged lists, and the split-BF16 history
+        # read once ((K + 1) slots x 96 columns per tile x R x 6 B)
         nT = -(-p.nnd // 16)
         mixed_bytes = (tm["tile_subtiles"] * TP_SUBTILE_BYTES + tm["tile_entries"] * TP_ENTRY_BYTES
-                       + tm["tile_blocks"] * TP_BLOCK_META + (K + 1) * nT * 96 * R * 6)
+                       + (K + 1) * nT * 96 * R * 6)
         out.update({
             "value": round(args.steps * (p.numstep - 1) * R / wall, 2),
             "ms_per_step": round(wall / args.steps * 1e3, 3),

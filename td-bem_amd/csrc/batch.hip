@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(64 * kGroup, NT <= 2 ? 2 : 1)
     if (!act[cw.w]) return;
     const BatchScen& d = D[cw.w];
     const int w = threadIdx.x >> 6;
-    conv_chunk<NT>(reinterpret_cast<const bf16x8*>(d.tp), d.ent,
+    conv_chunk<NT>(reinterpret_cast<const f32x4*>(d.tp), d.ent,
                    reinterpret_cast<const bf16x8*>(d.zs), d.nT, d.R, sdev[cw.w], cw.y, cw.z, w,
                    cw.x * kGroup + w < d.nT,
                    partial + (((int64_t)blockIdx.x * kGroup + w) * 3 * NT) * 256);

@@ -6,19 +6,21 @@
 // (48 dofs).  For each pair of tiles (I, J) and each lag k in the union of their node-pair shells
 // the 48 x 96 block A[(3 ii + p)][(6 jj + 3 part + q)] (part 0 = G[p][q], times z^t; part 1 =
 // -H[p][q], times z^u) is cut into 3 x 3 sub-tiles of 16 x 32; only sub-tiles that some node pair
-// reaches at lag k are stored.  Each value is split as a1 + a2 + a3 (bf16, 24 significant bits)
-// and laid out in MFMA operand order, so a wavefront loads each 16 x 32 operand as one 1 KiB line.
-// The history ring (FP64) is mirrored as split-BF16 fragments per (slot, column tile).
+// reaches at lag k are stored, in FP32, laid out in MFMA A-operand order so that a wavefront
+// loads each 16 x 32 operand as two contiguous 1 KiB lines (2 KiB per sub-tile).  The history
+// ring (FP64) is mirrored as split-BF16 fragments (three terms, RNE) per (slot, column tile).
 //
 // Work.  A workgroup takes 4 row tiles (one per wavefront, neighbours in the bisection order)
 // and its wavefronts walk the same merged (J, k) list of the four, each skipping the steps where
 // its row tile has no block.  Walking in step, they fetch the history fragment of
 // (J, slot of s - k) close together in time, so it comes from HBM once and from L1/L2 after.
 //
-// Per sub-tile the products a1 z1 + a1 z2 + a2 z1 + a1 z3 + a2 z2 + a3 z1 (all terms down to
-// 2^-24 relative) run on v_mfma_f32_16x16x32_bf16; the leading product and the sum of the
-// corrections leave FP32 for FP64 accumulators after every sub-tile, so rounding does not grow
-// with the number of blocks and lags summed.
+// Per sub-tile each FP32 value is split exactly in registers as a1 + a2 + a3 (bf16, 8 + 8 + 8
+// significant bits) and the products a1 z1 + a1 z2 + a2 z1 + a2 z2 + a3 z1 + a1 z3 + a3 z2 (the
+// dropped a2 z3 + a3 z3 stay below 2^-23 of |a| |z|) run on v_mfma_f32_16x16x32_bf16; the leading
+// product and the sum of the corrections leave FP32 for FP64 accumulators after every sub-tile,
+// so rounding does not grow with the number of blocks and lags summed.  Storing FP32 instead of
+// the three bf16 terms moves 2 KiB instead of 3 KiB per sub-tile through HBM, the bound.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -44,16 +46,16 @@ __device__ __forceinline__ int row_of_chunk(const int64_t* row_chunk, int nnd, i
     return lo;
 }
 
-// FP64 node-pair store -> split-BF16 sub-tiles, gathered: one wavefront per stored sub-tile
+// FP64 node-pair store -> FP32 sub-tiles, gathered: one wavefront per stored sub-tile
 // (row tile I, column tile J, lag k, sub-tile u = 3 m + kt).  Lane L holds block row 16 m + L % 16
 // and the 8 block columns 32 kt + 8 (L / 16) .. + 7 (16x16x32 A-operand order); each element is
-// looked up through the node pair's lag range and first entry, and the lane writes its 16 bytes
-// of each of the three terms.  Block row 3 ii + p, column 6 jj + 3 part + q holds G[p][q]
+// looked up through the node pair's lag range and first entry, and the lane writes columns
+// 0..3 of its eight to the sub-tile's first 1 KiB line and 4..7 to the second.  Block row 3 ii + p, column 6 jj + 3 part + q holds G[p][q]
 // (part 0) or -H[p][q] (part 1) of node pair (perm[16 I + ii], perm[16 J + jj]).
 __global__ void __launch_bounds__(256)
     k_tp_gather(const double* __restrict__ vals, const int2* __restrict__ hull,
                 const int64_t* __restrict__ off, const int* __restrict__ perm, int nnd,
-                const uint2* __restrict__ sub_id, int64_t nsub, bf16x8* __restrict__ tp)
+                const uint2* __restrict__ sub_id, int64_t nsub, f32x4* __restrict__ tp)
 {
     const int64_t sidx = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (sidx >= nsub) return;
@@ -65,7 +67,7 @@ __global__ void __launch_bounds__(256)
     const int r = 16 * m + (lane & 15);
     const int i = perm[I * kTile + r / 3], p = r % 3;
     const int sym[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-    bf16x8 out[3];
+    f32x4 out[2];
 #pragma unroll
     for (int e = 0; e < 8; e++) {
         const int c = 32 * kt + 8 * (lane >> 4) + e;
@@ -82,13 +84,10 @@ __global__ void __launch_bounds__(256)
                 if (part) v = -v;
             }
         }
-        unsigned short b[3];
-        split3(v, b);
-#pragma unroll
-        for (int t = 0; t < 3; t++) out[t][e] = (short)b[t];
+        out[e >> 2][e & 3] = (float)v;
     }
 #pragma unroll
-    for (int t = 0; t < 3; t++) tp[(sidx * 3 + t) * 64 + lane] = out[t];
+    for (int h = 0; h < 2; h++) tp[(sidx * 2 + h) * 64 + lane] = out[h];
 }
 
 // Split-BF16 copy of ring slot `slot` (see zsplit_elem).
@@ -104,7 +103,7 @@ __global__ void k_zsplit(const double* __restrict__ ring, int N, int nrhs, int n
 // One workgroup per chunk of a group's merged list, one wavefront per row tile (conv_chunk).
 template <int NT>
 __global__ void __launch_bounds__(64 * kGroup, NT <= 2 ? 2 : 1)
-    k_conv_mixed(const bf16x8* __restrict__ tp, const TPEntry* __restrict__ ent,
+    k_conv_mixed(const f32x4* __restrict__ tp, const TPEntry* __restrict__ ent,
                  const int4* __restrict__ chunk, const bf16x8* __restrict__ zs, int nT, int Rring,
                  int s, double* __restrict__ partial)
 {
@@ -329,7 +328,7 @@ void tp_build(Context& c)
     T.sub_off.upload(sub_off.data(), sub_off.size(), c.stream);
     T.mask.upload(msk.data(), msk.size(), c.stream);
     T.ent.upload(ent.data(), ent.size(), c.stream);
-    T.vals.alloc((size_t)std::max<int64_t>(nsub, 1) * kSubElems);
+    T.vals.alloc((size_t)std::max<int64_t>(nsub, 1) * kSubFloats);
     DBuf<uint2> d_sub_id;   // lives until the stream is synchronised below
     if (nsub > 0) {
         if (!c.store.d_hull.p || !c.store.d_off.p)
@@ -337,7 +336,7 @@ void tp_build(Context& c)
         d_sub_id.upload(sub_id.data(), sub_id.size(), c.stream);
         hipLaunchKernelGGL(k_tp_gather, dim3((unsigned)((nsub + 3) / 4)), dim3(256), 0, c.stream,
                            c.store.vals.p, c.store.d_hull.p, c.store.d_off.p, T.d_perm.p, nnd,
-                           d_sub_id.p, nsub, reinterpret_cast<bf16x8*>(T.vals.p));
+                           d_sub_id.p, nsub, reinterpret_cast<f32x4*>(T.vals.p));
     }
     TD_HIP(hipGetLastError());
     TD_HIP(hipStreamSynchronize(c.stream));
@@ -435,7 +434,7 @@ void tp_conv(Context& c, const unsigned short* zs, int nrhs, int Rring, int s, d
     if (T.plan_nrhs != nrhs) throw std::logic_error("mixed convolution not planned for nrhs");
     const int NT = nrhs / 16;
     const dim3 grid((unsigned)T.nchunks), block(64 * kGroup);
-    const bf16x8* tpv = reinterpret_cast<const bf16x8*>(T.vals.p);
+    const f32x4* tpv = reinterpret_cast<const f32x4*>(T.vals.p);
     const bf16x8* z = reinterpret_cast<const bf16x8*>(zs);
     switch (NT) {
     case 1: hipLaunchKernelGGL(k_conv_mixed<1>, grid, block, 0, c.stream, tpv, T.ent.p, T.chunk.p, z, T.nT, Rring, s, partial); break;

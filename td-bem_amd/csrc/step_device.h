@@ -13,7 +13,7 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 constexpr int kTile = 16;                 // nodes per tile of the mixed-precision store
 constexpr int kRows = 3 * kTile;          // 48
 constexpr int kCols = 6 * kTile;          // 96
-constexpr int kSubElems = 3 * 512;        // bf16 per stored sub-tile: 3 terms x 64 lanes x 8
+constexpr int kSubFloats = 512;           // FP32 per stored sub-tile: [2][64 lanes][4]
 constexpr int kApplyWaves = 4;            // wavefronts per step-solve workgroup
 
 // f_r(t): src/SolverT.cpp:379-394 returns 1; a per-RHS amplitude and a linear rise over t_rise
@@ -45,6 +45,25 @@ __device__ __forceinline__ void split3(double v, unsigned short* b)
     b[2] = bf16_rne((float)r2);
 }
 
+// x = hi + mid + lo exactly for an FP32 x: hi keeps the leading 8 significant bits (truncated),
+// mid the next 8 of the remainder x - hi, lo the rest (at most 8 bits, so it is a bf16 exactly).
+// Eight values, as two f32x4 in 16x16x32 A-operand order, become three bf16x8 operands.
+__device__ __forceinline__ void split_f32x8(const f32x4& a0, const f32x4& a1, bf16x8& hi,
+                                            bf16x8& mid, bf16x8& lo)
+{
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const float x = e < 4 ? a0[e] : a1[e - 4];
+        const unsigned xb = __float_as_uint(x);
+        const float r1 = x - __uint_as_float(xb & 0xFFFF0000u);
+        const unsigned rb = __float_as_uint(r1);
+        const float r2 = r1 - __uint_as_float(rb & 0xFFFF0000u);
+        hi[e] = (short)(xb >> 16);
+        mid[e] = (short)(rb >> 16);
+        lo[e] = (short)(__float_as_uint(r2) >> 16);
+    }
+}
+
 // Element t (< nT * 96 * nrhs) of the split-BF16 copy of ring slot `slot`, in B-operand order
 // (lane = rhs + 16 * (k / 8)), laid out [slot][J][term][k-tile][rhs tile][64 lanes][8].  The ring
 // holds [slot][rhs][part][N] with part 0 = u, part 1 = t; block column part 0 multiplies z^t.
@@ -74,9 +93,10 @@ __device__ __forceinline__ void zsplit_elem(const double* __restrict__ ring, int
 // ([3][NT][64][4]) unless the row tile does not exist.  All wavefronts of the workgroup
 // call it together (it has a barrier per step): the history k-tiles of a step are loaded from
 // HBM once per workgroup into one of two LDS buffers, one step ahead of their use, and shared by
-// the group's row tiles; each wavefront streams its own stored sub-tiles.
+// the group's row tiles; each wavefront streams its own stored FP32 sub-tiles and splits each
+// into three bf16 operands in registers (split_f32x8) just before its products.
 template <int NT>
-__device__ __forceinline__ void conv_chunk(const bf16x8* __restrict__ tp,
+__device__ __forceinline__ void conv_chunk(const f32x4* __restrict__ tp,
                                            const TPEntry* __restrict__ ent,
                                            const bf16x8* __restrict__ zs, int nT, int Rring, int s,
                                            int64_t e0, int64_t e1, int w, bool write,
@@ -127,15 +147,15 @@ __device__ __forceinline__ void conv_chunk(const bf16x8* __restrict__ tp,
         const uint32_t first = ent[e].sub[w];
         // this wavefront's stored sub-tiles of the step (read once: non-temporal, so they do not
         // evict history blocks), then the next step's history lines, then the products
-        const bf16x8* A = tp + (int64_t)first * (kSubElems / 8) + lane;
-        bf16x8 a[9][3];
+        const f32x4* A = tp + (int64_t)first * (kSubFloats / 4) + lane;
+        f32x4 a[9][2];
         int q = 0;
 #pragma unroll
         for (int u = 0; u < 9; u++)
             if ((msk >> u) & 1u) {
 #pragma unroll
-                for (int t = 0; t < 3; t++)
-                    a[u][t] = __builtin_nontemporal_load(&A[(q * 3 + t) * 64]);
+                for (int h = 0; h < 2; h++)
+                    a[u][h] = __builtin_nontemporal_load(&A[(q * 2 + h) * 64]);
                 q++;
             }
         const bool more = e + 1 < e1;
@@ -159,19 +179,23 @@ __device__ __forceinline__ void conv_chunk(const bf16x8* __restrict__ tp,
                 for (int m = 0; m < 3; m++) {
                     const int u = 3 * m + kt;
                     if (!((msk >> u) & 1u)) continue;
-                    // the leading product a1 z1 and the five corrections sum in separate FP32
-                    // registers; both move to FP64 after each sub-tile
+                    bf16x8 a1, a2, a3;
+                    split_f32x8(a[u][0], a[u][1], a1, a2, a3);
+                    // the leading product a1 z1 and the six corrections (smallest first) sum in
+                    // separate FP32 registers; both move to FP64 after each sub-tile.  Dropped:
+                    // a2 z3 and a3 z3, below 2^-23 of |a| |z|.
 #pragma unroll
                     for (int n = 0; n < NT; n++) {
                         const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
                         const f32x4 dm =
-                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][0], z[0][n], zero, 0, 0, 0);
+                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, z[0][n], zero, 0, 0, 0);
                         f32x4 dc =
-                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][2], z[0][n], zero, 0, 0, 0);
-                        dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][1], z[1][n], dc, 0, 0, 0);
-                        dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][0], z[2][n], dc, 0, 0, 0);
-                        dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][1], z[0][n], dc, 0, 0, 0);
-                        dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][0], z[1][n], dc, 0, 0, 0);
+                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, z[1][n], zero, 0, 0, 0);
+                        dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, z[2][n], dc, 0, 0, 0);
+                        dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, z[0][n], dc, 0, 0, 0);
+                        dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, z[1][n], dc, 0, 0, 0);
+                        dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, z[0][n], dc, 0, 0, 0);
+                        dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, z[1][n], dc, 0, 0, 0);
 #pragma unroll
                         for (int v = 0; v < 4; v++) acc[m][n][v] += (double)dm[v] + (double)dc[v];
                     }

@@ -188,7 +188,7 @@ struct TPEntry {
 
 // Tile-pair store of the mixed-precision convolution (mixed.hip): nodes ordered by recursive
 // coordinate bisection into tiles of 16; per (row tile, column tile, lag) a 48 x 96 block of
-// which only the non-zero 16 x 32 sub-tiles are stored, split-BF16, in MFMA operand order.
+// which only the non-zero 16 x 32 sub-tiles are stored, FP32, in MFMA operand order.
 struct TPStore {
     int nT = 0, nG = 0;              // row tiles, groups of 4 row tiles
     std::vector<int> perm, pos;      // tile position -> node (-1 = padding); node -> position
@@ -196,7 +196,7 @@ struct TPStore {
     int64_t nblocks = 0;             // blocks with a non-zero sub-tile
     int64_t nsub = 0;                // stored sub-tiles
     int64_t nentries = 0;            // merged-list entries over all groups
-    DBuf<unsigned short> vals;       // [nsub][3 terms][64][8] bf16
+    DBuf<float> vals;                // [nsub][2][64][4] FP32, 16x16x32 A-operand order
     DBuf<uint32_t> sub_off;          // per block (all lags of each tile-pair hull): first sub-tile
     DBuf<uint16_t> mask;             // per block: bit 3 m + kt set when sub-tile (m, kt) is stored
     DBuf<TPEntry> ent;               // merged lists, group after group
