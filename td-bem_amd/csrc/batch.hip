@@ -82,14 +82,15 @@ __global__ void __launch_bounds__(64 * kGroup, NT <= 2 ? 2 : 1)
     k_b_conv(const BatchScen* __restrict__ D, const int4* __restrict__ chunk,
              const int* __restrict__ sdev, const int* __restrict__ act, double* __restrict__ partial)
 {
-    const int4 cw = chunk[blockIdx.x];
+    const int c = xcd_chunk(blockIdx.x, gridDim.x);
+    const int4 cw = chunk[c];
     if (!act[cw.w]) return;
     const BatchScen& d = D[cw.w];
     const int w = threadIdx.x >> 6;
     conv_chunk<NT>(reinterpret_cast<const f32x4*>(d.tp), d.ent,
                    reinterpret_cast<const bf16x8*>(d.zs), d.nT, d.R, sdev[cw.w], cw.y, cw.z, w,
                    cw.x * kGroup + w < d.nT,
-                   partial + (((int64_t)blockIdx.x * kGroup + w) * 3 * NT) * 256);
+                   partial + (((int64_t)c * kGroup + w) * 3 * NT) * 256);
 }
 
 __global__ void k_b_reduce(const BatchScen* __restrict__ D, const int* __restrict__ off, int S,

@@ -107,11 +107,12 @@ __global__ void __launch_bounds__(64 * kGroup, NT <= 2 ? 2 : 1)
                  const int4* __restrict__ chunk, const bf16x8* __restrict__ zs, int nT, int Rring,
                  int s, double* __restrict__ partial)
 {
-    const int4 cw = chunk[blockIdx.x];
+    const int c = xcd_chunk(blockIdx.x, gridDim.x);
+    const int4 cw = chunk[c];
     const int w = threadIdx.x >> 6;
     // partial sums of chunk c: [c][wave][3][NT][64][4]
     conv_chunk<NT>(tp, ent, zs, nT, Rring, s, cw.y, cw.z, w, cw.x * kGroup + w < nT,
-                   partial + (((int64_t)blockIdx.x * kGroup + w) * 3 * NT) * 256);
+                   partial + (((int64_t)c * kGroup + w) * 3 * NT) * 256);
 }
 
 // hist[r][dof] = sum over the chunks of a row tile's group (see reduce_elem).
@@ -325,8 +326,6 @@ void tp_build(Context& c)
     T.plan_nrhs = 0;
     T.d_perm.upload(T.perm.data(), T.perm.size(), c.stream);
     T.d_pos.upload(T.pos.data(), T.pos.size(), c.stream);
-    T.sub_off.upload(sub_off.data(), sub_off.size(), c.stream);
-    T.mask.upload(msk.data(), msk.size(), c.stream);
     T.ent.upload(ent.data(), ent.size(), c.stream);
     T.vals.alloc((size_t)std::max<int64_t>(nsub, 1) * kSubFloats);
     DBuf<uint2> d_sub_id;   // lives until the stream is synchronised below

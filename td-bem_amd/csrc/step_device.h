@@ -45,6 +45,17 @@ __device__ __forceinline__ void split3(double v, unsigned short* b)
     b[2] = bf16_rne((float)r2);
 }
 
+// Workgroup b of a grid of nb is dispatched to XCD b % 8 (round robin over the eight XCDs of an
+// MI355X, each with its own L2).  Its logical chunk: XCD x takes the x-th contiguous eighth of
+// the chunks, in order, so the workgroups resident together on one XCD walk neighbouring row-tile
+// groups and share their history blocks in that XCD's L2.  A bijection of [0, nb).
+__device__ __forceinline__ int xcd_chunk(int b, int nb)
+{
+    constexpr int kXcd = 8;
+    const int q = nb / kXcd, r = nb % kXcd, x = b % kXcd, i = b / kXcd;
+    return x * q + min(x, r) + i;
+}
+
 // x = hi + mid + lo exactly for an FP32 x: hi keeps the leading 8 significant bits (truncated),
 // mid the next 8 of the remainder x - hi, lo the rest (at most 8 bits, so it is a bf16 exactly).
 // Eight values, as two f32x4 in 16x16x32 A-operand order, become three bf16x8 operands.
@@ -136,40 +147,50 @@ __device__ __forceinline__ void conv_chunk(const f32x4* __restrict__ tp,
     };
     // w is the same in every lane of a wavefront: the entry fields are scalar loads
     w = __builtin_amdgcn_readfirstlane(w);
+    // this wavefront's stored sub-tiles of an entry (read once: non-temporal, so they do not
+    // evict history blocks) go to a[u] by sub-tile position u; the next entry's sub-tiles of
+    // k-tile kt are requested as soon as the products of k-tile kt have consumed a[kt], a[3+kt],
+    // a[6+kt], so their latency overlaps the rest of the entry and the barrier
+    f32x4 a[9][2];
+    auto aload = [&](unsigned mk, uint32_t first, int kt) {
+        const f32x4* A = tp + (int64_t)first * (kSubFloats / 4) + lane;
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            const int u = 3 * m + kt;
+            if ((mk >> u) & 1u) {
+                const int q = __builtin_popcount(mk & ((1u << u) - 1u));
+#pragma unroll
+                for (int h = 0; h < 2; h++)
+                    a[u][h] = __builtin_nontemporal_load(&A[(q * 2 + h) * 64]);
+            }
+        }
+    };
     if (e0 < e1) {
         zload(ent[e0].id, ent[e0].kmask);
         zstore(0, ent[e0].kmask);
+        const unsigned m0 = ent[e0].msk[w];
+        const uint32_t f0 = ent[e0].sub[w];
+#pragma unroll
+        for (int kt = 0; kt < 3; kt++) aload(m0, f0, kt);
     }
     __syncthreads();
     for (int64_t e = e0; e < e1; e++) {
         const int cur = (int)((e - e0) & 1);
         const unsigned msk = ent[e].msk[w];
-        const uint32_t first = ent[e].sub[w];
-        // this wavefront's stored sub-tiles of the step (read once: non-temporal, so they do not
-        // evict history blocks), then the next step's history lines, then the products
-        const f32x4* A = tp + (int64_t)first * (kSubFloats / 4) + lane;
-        f32x4 a[9][2];
-        int q = 0;
-#pragma unroll
-        for (int u = 0; u < 9; u++)
-            if ((msk >> u) & 1u) {
-#pragma unroll
-                for (int h = 0; h < 2; h++)
-                    a[u][h] = __builtin_nontemporal_load(&A[(q * 2 + h) * 64]);
-                q++;
-            }
         const bool more = e + 1 < e1;
-        unsigned knext = 0;
+        unsigned knext = 0, mnext = 0;
+        uint32_t fnext = 0;
         if (more) {
             knext = ent[e + 1].kmask;
+            mnext = ent[e + 1].msk[w];
+            fnext = ent[e + 1].sub[w];
             zload(ent[e + 1].id, knext);
         }
-        if (msk) {
-            const bf16x8* Z = zl[cur] + lane;
+        const bf16x8* Z = zl[cur] + lane;
 #pragma unroll
-            for (int kt = 0; kt < 3; kt++) {
-                const unsigned rows = (msk >> kt) & 0x49u;   // bits kt, 3 + kt, 6 + kt
-                if (!rows) continue;
+        for (int kt = 0; kt < 3; kt++) {
+            const unsigned rows = (msk >> kt) & 0x49u;   // bits kt, 3 + kt, 6 + kt
+            if (rows) {
                 bf16x8 z[3][NT];
 #pragma unroll
                 for (int t = 0; t < 3; t++)
@@ -201,6 +222,7 @@ __device__ __forceinline__ void conv_chunk(const f32x4* __restrict__ tp,
                     }
                 }
             }
+            if (more) aload(mnext, fnext, kt);
         }
         if (more) zstore(cur ^ 1, knext);
         __syncthreads();

@@ -197,8 +197,6 @@ struct TPStore {
     int64_t nsub = 0;                // stored sub-tiles
     int64_t nentries = 0;            // merged-list entries over all groups
     DBuf<float> vals;                // [nsub][2][64][4] FP32, 16x16x32 A-operand order
-    DBuf<uint32_t> sub_off;          // per block (all lags of each tile-pair hull): first sub-tile
-    DBuf<uint16_t> mask;             // per block: bit 3 m + kt set when sub-tile (m, kt) is stored
     DBuf<TPEntry> ent;               // merged lists, group after group
     std::vector<int64_t> h_grp_off;  // [nG + 1]
     std::vector<float> h_work;       // per entry: relative cost (bytes moved)
