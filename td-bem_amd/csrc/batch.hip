@@ -1,7 +1,7 @@
 // batch.hip — many independent time-marching scenarios (uneven DOF and step counts) stepped
 // together.  Each scenario is a context with a mixed-precision march; one step of every scenario
 // is seven launches in total, whatever the number of scenarios:
-//   advance + load factors, split-BF16 history copy, MFMA convolution, reduction, right-hand
+//   advance + load factors, FP32 history copy, MFMA convolution, reduction, right-hand
 //   sides, step solve, finalize,
 // each launch covering all scenarios through a table of per-scenario descriptors and a device
 // step counter per scenario (scenarios past their last step drop out).  A run captures
@@ -60,8 +60,8 @@ __global__ void k_b_step(const BatchScen* __restrict__ D, int* __restrict__ sdev
     if (a && r < d.nrhs) d.fac[r] = load_factor(d.amp[r], d.rise[r], s_new * d.dt);
 }
 
-// The three ring slots s - 2 .. s changed by the previous step, in split-BF16 operand order.
-__global__ void k_b_zsplit(const BatchScen* __restrict__ D, const int* __restrict__ off, int S,
+// The three ring slots s - 2 .. s changed by the previous step, in FP32 operand order.
+__global__ void k_b_zcopy(const BatchScen* __restrict__ D, const int* __restrict__ off, int S,
                            const int* __restrict__ sdev, const int* __restrict__ act)
 {
     const int sc = scen_of(off, S, blockIdx.x);
@@ -73,7 +73,7 @@ __global__ void k_b_zsplit(const BatchScen* __restrict__ D, const int* __restric
     const int m = (int)(t / per);
     int slot = (sdev[sc] - 2 + m) % d.R;
     if (slot < 0) slot += d.R;
-    zsplit_elem(d.ring, d.N, d.nrhs, d.nT, d.perm, slot, d.nrhs / 16, d.zs, t - m * per);
+    zcopy_elem(d.ring, d.N, d.nrhs, d.nT, d.perm, slot, d.nrhs / 16, d.zs, t - m * per);
 }
 
 // One workgroup per chunk (chunk.w = scenario), one wavefront per row tile of its group.
@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(64 * kGroup, NT <= 2 ? 2 : 1)
     const BatchScen& d = D[cw.w];
     const int w = threadIdx.x >> 6;
     conv_chunk<NT>(reinterpret_cast<const f32x4*>(d.tp), d.ent,
-                   reinterpret_cast<const bf16x8*>(d.zs), d.nT, d.R, sdev[cw.w], cw.y, cw.z, w,
+                   reinterpret_cast<const f32x4*>(d.zs), d.nT, d.R, sdev[cw.w], cw.y, cw.z, w,
                    cw.x * kGroup + w < d.nT,
                    partial + (((int64_t)c * kGroup + w) * 3 * NT) * 256);
 }
@@ -254,7 +254,7 @@ void launch_apply(Batch& B)
 void launch_step(Batch& B)
 {
     hipLaunchKernelGGL(k_b_step, dim3(B.S), dim3(64), 0, B.stream, B.desc.p, B.sdev.p, B.act.p);
-    hipLaunchKernelGGL(k_b_zsplit, dim3(B.nb_z), dim3(kThreads), 0, B.stream, B.desc.p,
+    hipLaunchKernelGGL(k_b_zcopy, dim3(B.nb_z), dim3(kThreads), 0, B.stream, B.desc.p,
                        B.off_z.p, B.S, B.sdev.p, B.act.p);
     launch_conv(B);
     hipLaunchKernelGGL(k_b_reduce, dim3(B.nb_r), dim3(kThreads), 0, B.stream, B.desc.p,

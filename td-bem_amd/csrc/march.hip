@@ -41,7 +41,7 @@ struct March {
     DBuf<double> fac;       // [nrhs] load factor of the current step
     DBuf<double> d_amp, d_rise;
     int precision = 0;             // 1: split-BF16 MFMA convolution
-    DBuf<unsigned short> zs;       // split-BF16 mirror of the ring (mixed)
+    DBuf<float> zs;                // FP32 mirror of the ring in MFMA operand order (mixed)
     DBuf<double> partial;          // per-wave partial sums of the mixed convolution
     int64_t zslot = 0;
 };
@@ -417,8 +417,8 @@ void march_run(Context& c, int nsteps, bool wait)
                 // the previous finalize changed ring slots s-2, s-1 and s
                 for (int m = s - 2; m <= s; m++) {
                     const int sl = ((m % M.R) + M.R) % M.R;
-                    // k_zsplit adds the slot offset itself: pass the ring base
-                    tp_zsplit(c, M.ring.p, N, M.nrhs, sl, M.zs.p);
+                    // k_zcopy adds the slot offset itself: pass the ring base
+                    tp_zcopy(c, M.ring.p, N, M.nrhs, sl, M.zs.p);
                 }
                 conv_timer.begin();
                 tp_conv(c, M.zs.p, M.nrhs, M.R, s, M.partial.p);

@@ -8,7 +8,8 @@
 // -H[p][q], times z^u) is cut into 3 x 3 sub-tiles of 16 x 32; only sub-tiles that some node pair
 // reaches at lag k are stored, in FP32, laid out in MFMA A-operand order so that a wavefront
 // loads each 16 x 32 operand as two contiguous 1 KiB lines (2 KiB per sub-tile).  The history
-// ring (FP64) is mirrored as split-BF16 fragments (three terms, RNE) per (slot, column tile).
+// ring (FP64) is mirrored in FP32, in MFMA B-operand order, per (slot, column tile); a workgroup
+// splits each history block it stages into LDS into three bf16 terms.
 //
 // Work.  A workgroup takes 4 row tiles (one per wavefront, neighbours in the bisection order)
 // and its wavefronts walk the same merged (J, k) list of the four, each skipping the steps where
@@ -16,8 +17,8 @@
 // (J, slot of s - k) close together in time, so it comes from HBM once and from L1/L2 after.
 //
 // Per sub-tile each FP32 value is split exactly in registers as a1 + a2 + a3 (bf16, 8 + 8 + 8
-// significant bits) and the products a1 z1 + a1 z2 + a2 z1 + a2 z2 + a3 z1 + a1 z3 + a3 z2 (the
-// dropped a2 z3 + a3 z3 stay below 2^-23 of |a| |z|) run on v_mfma_f32_16x16x32_bf16; the leading
+// significant bits; z likewise) and the products of all term pairs but a3 z3 (below 2^-28 of
+// |a| |z|) run on v_mfma_f32_16x16x32_bf16; the leading
 // product and the sum of the corrections leave FP32 for FP64 accumulators after every sub-tile,
 // so rounding does not grow with the number of blocks and lags summed.  Storing FP32 instead of
 // the three bf16 terms moves 2 KiB instead of 3 KiB per sub-tile through HBM, the bound.
@@ -90,21 +91,20 @@ __global__ void __launch_bounds__(256)
     for (int h = 0; h < 2; h++) tp[(sidx * 2 + h) * 64 + lane] = out[h];
 }
 
-// Split-BF16 copy of ring slot `slot` (see zsplit_elem).
-__global__ void k_zsplit(const double* __restrict__ ring, int N, int nrhs, int nT,
-                         const int* __restrict__ perm, int slot, int NT,
-                         unsigned short* __restrict__ zs)
+// FP32 copy of ring slot `slot` (see zcopy_elem).
+__global__ void k_zcopy(const double* __restrict__ ring, int N, int nrhs, int nT,
+                        const int* __restrict__ perm, int slot, int NT, float* __restrict__ zs)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nT * kCols * nrhs) return;
-    zsplit_elem(ring, N, nrhs, nT, perm, slot, NT, zs, t);
+    zcopy_elem(ring, N, nrhs, nT, perm, slot, NT, zs, t);
 }
 
 // One workgroup per chunk of a group's merged list, one wavefront per row tile (conv_chunk).
 template <int NT>
 __global__ void __launch_bounds__(64 * kGroup, NT <= 2 ? 2 : 1)
     k_conv_mixed(const f32x4* __restrict__ tp, const TPEntry* __restrict__ ent,
-                 const int4* __restrict__ chunk, const bf16x8* __restrict__ zs, int nT, int Rring,
+                 const int4* __restrict__ chunk, const f32x4* __restrict__ zs, int nT, int Rring,
                  int s, double* __restrict__ partial)
 {
     const int c = xcd_chunk(blockIdx.x, gridDim.x);
@@ -345,16 +345,16 @@ void tp_build(Context& c)
     T.ready = true;
 }
 
-void tp_zsplit(Context& c, const double* ring, int N, int nrhs, int slot, unsigned short* zs)
+void tp_zcopy(Context& c, const double* ring, int N, int nrhs, int slot, float* zs)
 {
     const TPStore& T = c.tp;
     const int64_t n = (int64_t)T.nT * kCols * nrhs;
-    hipLaunchKernelGGL(k_zsplit, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, ring,
+    hipLaunchKernelGGL(k_zcopy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, ring,
                        N, nrhs, T.nT, T.d_perm.p, slot, nrhs / 16, zs);
     TD_HIP(hipGetLastError());
 }
 
-int64_t tp_zslot_elems(const Context& c, int nrhs) { return (int64_t)c.tp.nT * 9 * (nrhs / 16) * 512; }
+int64_t tp_zslot_elems(const Context& c, int nrhs) { return (int64_t)c.tp.nT * 3 * (nrhs / 16) * 512; }
 
 int tp_resident_groups(int nrhs, int device)
 {
@@ -427,14 +427,14 @@ void tp_plan(Context& c, int nrhs, int rounds)
     T.plan_rounds = rounds;
 }
 
-void tp_conv(Context& c, const unsigned short* zs, int nrhs, int Rring, int s, double* partial)
+void tp_conv(Context& c, const float* zs, int nrhs, int Rring, int s, double* partial)
 {
     const TPStore& T = c.tp;
     if (T.plan_nrhs != nrhs) throw std::logic_error("mixed convolution not planned for nrhs");
     const int NT = nrhs / 16;
     const dim3 grid((unsigned)T.nchunks), block(64 * kGroup);
     const f32x4* tpv = reinterpret_cast<const f32x4*>(T.vals.p);
-    const bf16x8* z = reinterpret_cast<const bf16x8*>(zs);
+    const f32x4* z = reinterpret_cast<const f32x4*>(zs);
     switch (NT) {
     case 1: hipLaunchKernelGGL(k_conv_mixed<1>, grid, block, 0, c.stream, tpv, T.ent.p, T.chunk.p, z, T.nT, Rring, s, partial); break;
     case 2: hipLaunchKernelGGL(k_conv_mixed<2>, grid, block, 0, c.stream, tpv, T.ent.p, T.chunk.p, z, T.nT, Rring, s, partial); break;

@@ -30,21 +30,6 @@ __device__ __forceinline__ unsigned short bf16_rne(float f)
     return (unsigned short)(u >> 16);
 }
 
-__device__ __forceinline__ float bf16_to_float(unsigned short b)
-{
-    return __uint_as_float(((unsigned int)b) << 16);
-}
-
-// v = b1 + b2 + b3 + O(2^-24 |v|), each a bf16; residuals formed exactly in FP64.
-__device__ __forceinline__ void split3(double v, unsigned short* b)
-{
-    b[0] = bf16_rne((float)v);
-    const double r1 = v - (double)bf16_to_float(b[0]);
-    b[1] = bf16_rne((float)r1);
-    const double r2 = r1 - (double)bf16_to_float(b[1]);
-    b[2] = bf16_rne((float)r2);
-}
-
 // Workgroup b of a grid of nb is dispatched to XCD b % 8 (round robin over the eight XCDs of an
 // MI355X, each with its own L2).  Its logical chunk: XCD x takes the x-th contiguous eighth of
 // the chunks, in order, so the workgroups resident together on one XCD walk neighbouring row-tile
@@ -75,12 +60,13 @@ __device__ __forceinline__ void split_f32x8(const f32x4& a0, const f32x4& a1, bf
     }
 }
 
-// Element t (< nT * 96 * nrhs) of the split-BF16 copy of ring slot `slot`, in B-operand order
-// (lane = rhs + 16 * (k / 8)), laid out [slot][J][term][k-tile][rhs tile][64 lanes][8].  The ring
-// holds [slot][rhs][part][N] with part 0 = u, part 1 = t; block column part 0 multiplies z^t.
-__device__ __forceinline__ void zsplit_elem(const double* __restrict__ ring, int N, int nrhs,
-                                            int nT, const int* __restrict__ perm, int slot,
-                                            int NT, unsigned short* __restrict__ zs, int64_t t)
+// Element t (< nT * 96 * nrhs) of the FP32 copy of ring slot `slot`, in B-operand order
+// (lane = rhs + 16 * (k / 8), element k % 8), laid out [slot][J][k-tile][rhs tile][2][64 lanes][4]
+// (elements 0-3 of all lanes, then 4-7).  The ring holds [slot][rhs][part][N] with part 0 = u,
+// part 1 = t; block column part 0 multiplies z^t.
+__device__ __forceinline__ void zcopy_elem(const double* __restrict__ ring, int N, int nrhs,
+                                           int nT, const int* __restrict__ perm, int slot,
+                                           int NT, float* __restrict__ zs, int64_t t)
 {
     const int r = (int)(t % nrhs);
     const int c = (int)((t / nrhs) % kCols);
@@ -90,13 +76,10 @@ __device__ __forceinline__ void zsplit_elem(const double* __restrict__ ring, int
     const int rpart = 1 - part;
     const double v =
         node < 0 ? 0.0 : ring[((int64_t)(slot * nrhs + r) * 2 + rpart) * N + 3 * node + q];
-    unsigned short b[3];
-    split3(v, b);
     const int kt = c >> 5, kk = c & 31, nt = r >> 4, n = r & 15;
     const int lane = n + 16 * (kk >> 3), e = kk & 7;
-    unsigned short* base = zs + (((int64_t)slot * nT + J) * 3) * 3 * NT * 512;
-#pragma unroll
-    for (int tm = 0; tm < 3; tm++) base[((tm * 3 + kt) * NT + nt) * 512 + lane * 8 + e] = b[tm];
+    zs[((((int64_t)slot * nT + J) * 3 + kt) * NT + nt) * 512 + (e >> 2) * 256 + lane * 4 + (e & 3)] =
+        (float)v;
 }
 
 // One wavefront's share of a convolution chunk: the steps [e0, e1) of its group's merged list,
@@ -109,13 +92,15 @@ __device__ __forceinline__ void zsplit_elem(const double* __restrict__ ring, int
 template <int NT>
 __device__ __forceinline__ void conv_chunk(const f32x4* __restrict__ tp,
                                            const TPEntry* __restrict__ ent,
-                                           const bf16x8* __restrict__ zs, int nT, int Rring, int s,
+                                           const f32x4* __restrict__ zs, int nT, int Rring, int s,
                                            int64_t e0, int64_t e1, int w, bool write,
                                            double* __restrict__ out)
 {
-    // one (slot, column tile) history block: [term][k-tile][rhs tile] lines of 64 x 16 B
+    // one (slot, column tile) history block: in HBM, [k-tile][rhs tile] units of 64 lanes x 8
+    // FP32; in LDS, split exactly into [term][k-tile][rhs tile] lines of 64 x 16 B (bf16x8)
     constexpr int kLines = 9 * NT;
-    constexpr int kPer = (kLines + kGroup - 1) / kGroup;   // lines per wavefront
+    constexpr int kUnits = 3 * NT;
+    constexpr int kPer = (kUnits + kGroup - 1) / kGroup;   // units per wavefront
     __shared__ bf16x8 zl[2][kLines * 64];
     const int lane = threadIdx.x & 63;
     double acc[3][NT][4];
@@ -125,24 +110,34 @@ __device__ __forceinline__ void conv_chunk(const f32x4* __restrict__ tp,
         for (int n = 0; n < NT; n++)
 #pragma unroll
             for (int v = 0; v < 4; v++) acc[m][n][v] = 0.0;
-    // wavefront w stages lines w, w + 4, ... of a block; line L = (term * 3 + k-tile) * NT + n
-    bf16x8 zn[kPer];
+    // wavefront w stages units w, w + 4, ... of a block; unit U = k-tile * NT + rhs tile
+    f32x4 zn[kPer][2];
     auto zload = [&](uint32_t id, unsigned kmask) {
         const int J = (int)(id >> kKBits), k = (int)(id & kMaxLag);
         int slot = (s - k) % Rring;
         if (slot < 0) slot += Rring;
-        const bf16x8* Zg = zs + ((int64_t)slot * nT + J) * (kLines * 64) + lane;
+        const f32x4* Zg = zs + ((int64_t)slot * nT + J) * (kUnits * 128) + lane;
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
-            const int L = q * kGroup + w;
-            if (L < kLines && ((kmask >> ((L / NT) % 3)) & 1u)) zn[q] = Zg[L * 64];
+            const int U = q * kGroup + w;
+            if (U < kUnits && ((kmask >> (U / NT)) & 1u)) {
+                zn[q][0] = Zg[(U * 2) * 64];
+                zn[q][1] = Zg[(U * 2 + 1) * 64];
+            }
         }
     };
     auto zstore = [&](int buf, unsigned kmask) {
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
-            const int L = q * kGroup + w;
-            if (L < kLines && ((kmask >> ((L / NT) % 3)) & 1u)) zl[buf][L * 64 + lane] = zn[q];
+            const int U = q * kGroup + w;
+            if (U < kUnits && ((kmask >> (U / NT)) & 1u)) {
+                bf16x8 t0, t1, t2;
+                split_f32x8(zn[q][0], zn[q][1], t0, t1, t2);
+                const int kt = U / NT, nt = U % NT;
+                zl[buf][((0 * 3 + kt) * NT + nt) * 64 + lane] = t0;
+                zl[buf][((1 * 3 + kt) * NT + nt) * 64 + lane] = t1;
+                zl[buf][((2 * 3 + kt) * NT + nt) * 64 + lane] = t2;
+            }
         }
     };
     // w is the same in every lane of a wavefront: the entry fields are scalar loads
@@ -202,9 +197,9 @@ __device__ __forceinline__ void conv_chunk(const f32x4* __restrict__ tp,
                     if (!((msk >> u) & 1u)) continue;
                     bf16x8 a1, a2, a3;
                     split_f32x8(a[u][0], a[u][1], a1, a2, a3);
-                    // the leading product a1 z1 and the six corrections (smallest first) sum in
-                    // separate FP32 registers; both move to FP64 after each sub-tile.  Dropped:
-                    // a2 z3 and a3 z3, below 2^-23 of |a| |z|.
+                    // the leading product a1 z1 and the seven corrections (smallest first) sum
+                    // in separate FP32 registers; both move to FP64 after each sub-tile.
+                    // Dropped: a3 z3, below 2^-28 of |a| |z|.
 #pragma unroll
                     for (int n = 0; n < NT; n++) {
                         const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
@@ -212,6 +207,7 @@ __device__ __forceinline__ void conv_chunk(const f32x4* __restrict__ tp,
                             __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, z[0][n], zero, 0, 0, 0);
                         f32x4 dc =
                             __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, z[1][n], zero, 0, 0, 0);
+                        dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, z[2][n], dc, 0, 0, 0);
                         dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, z[2][n], dc, 0, 0, 0);
                         dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, z[0][n], dc, 0, 0, 0);
                         dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, z[1][n], dc, 0, 0, 0);

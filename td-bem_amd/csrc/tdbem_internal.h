@@ -323,7 +323,7 @@ struct BatchScen {
     // mixed-precision store (mixed.hip)
     const void* tp;
     const TPEntry* ent;
-    unsigned short* zs;
+    float* zs;
     const int* perm;
     int nT, gch0;           // gch0: this scenario's first entry in the batch's group-chunk table
 };
@@ -344,7 +344,8 @@ int batch_run(Batch& b, int steps_per_graph, int max_replays);
 
 // mixed.hip
 void tp_build(Context& c);
-void tp_zsplit(Context& c, const double* ring, int N, int nrhs, int slot, unsigned short* zs);
+// FP32 copy of ring slot `slot` in the B-operand order of the mixed convolution
+void tp_zcopy(Context& c, const double* ring, int N, int nrhs, int slot, float* zs);
 int64_t tp_zslot_elems(const Context& c, int nrhs);
 // plans the chunks of k_conv_mixed for nrhs right-hand sides (after tp_build)
 void tp_plan(Context& c, int nrhs, int rounds = 4);
@@ -357,7 +358,7 @@ int tp_resident_groups(int nrhs, int device);
 double tp_work(const Context& c);
 int64_t tp_partial_elems(const Context& c, int nrhs);
 // k_conv_mixed: per-wave FP64 partial sums of the history at step s
-void tp_conv(Context& c, const unsigned short* zs, int nrhs, int Rring, int s, double* partial);
+void tp_conv(Context& c, const float* zs, int nrhs, int Rring, int s, double* partial);
 // fixed-order sum of the partials into hist[nrhs][N]
 void tp_reduce(Context& c, const double* partial, int nrhs, double* hist, int N);
 void mfma_selftest(const float* A, const float* B, float* D);
