@@ -234,23 +234,25 @@ __device__ __forceinline__ void conv_chunk(const f32x4* __restrict__ tp,
 
 // Element t (< nT * 48 * nrhs) of hist[r][dof] = the fixed-order sum over the chunks of the row
 // tile's group; partial of chunk c, wave w: [c][w][3][NT][64][4], 16x16 output fragment
-// lane = col + 16 * (row / 4), element row % 4.  grp_chunk: first chunk of each group.
+// lane = col + 16 * (row / 4), element row % 4.  t runs through a row tile's partials in that
+// order, so the reads of consecutive threads are contiguous.  grp_chunk: first chunk of each
+// group.
 __device__ __forceinline__ void reduce_elem(const double* __restrict__ partial,
                                             const int* __restrict__ grp_chunk, int NT,
                                             const int* __restrict__ perm, int N, int nrhs,
                                             double* __restrict__ hist, int64_t t)
 {
-    const int r = (int)(t % nrhs);
-    const int row = (int)((t / nrhs) % kRows);
-    const int I = (int)(t / ((int64_t)nrhs * kRows));
+    const int per = kRows * nrhs;   // = 3 NT 256
+    const int I = (int)(t / per), o = (int)(t % per);
+    const int v = o & 3, lane = (o >> 2) & 63, mn = o >> 8;
+    const int m = mn / NT, nt = mn % NT;
+    const int row = 16 * m + 4 * (lane >> 4) + v, r = 16 * nt + (lane & 15);
     const int node = perm[I * kTile + row / 3];
     if (node < 0) return;
-    const int m = row >> 4, i = row & 15, nt = r >> 4, n = r & 15;
-    const int lane = n + 16 * (i >> 2), v = i & 3;
     const int g = I / kGroup, w = I % kGroup;
     double sum = 0.0;
     for (int c = grp_chunk[g]; c < grp_chunk[g + 1]; c++)
-        sum += partial[(((((int64_t)c * kGroup + w) * 3 + m) * NT + nt) * 64 + lane) * 4 + v];
+        sum += partial[((int64_t)c * kGroup + w) * per + o];
     hist[(int64_t)r * N + 3 * node + row % 3] = sum;
 }
 
