@@ -15,7 +15,7 @@ and fp64_8rhs for the FP64 path at 8 right-hand sides, its best rate.  The mixed
 8 right-hand sides of ||u_mixed - u_fp64||_F / ||u_fp64||_F (and the same for t).
 
 roofline: the mixed path's dominant kernel k_conv_mixed, HBM-bound: algorithmic bytes per launch =
-stored 16 x 32 FP32 sub-tiles x 2048 B + merged-list entries x 32 B + the split-BF16 history
+stored 16 x 32 FP32 sub-tiles x 2048 B + merged-list entries x 32 B + the FP32 history copy
 read once, over its mean duration from HIP events on the context stream;
 plain_fp64.roofline gives k_conv the same way (stored entries x 124 B).
 Printed as one JSON line; --gpus N > 1 exits (one GPU per box in this build).
@@ -38,7 +38,7 @@ FP64_ENTRY_BYTES = 15 * 8 + 4            # 15 values + index per (i, j, k)
 # HBM bytes per launch (read + written) from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE)
 # on this workload, scaled by the counters of a 1 GiB k_stream_copy (factors 2.0 and 1.0);
 # profiles/k_conv_traffic_pmc_sphere1536.md holds the runs.
-MEASURED_TRAFFIC = {"k_conv_mixed": 10.311e9 + 99.3e6, "k_conv": 7.658e9 + 0.1e6}
+MEASURED_TRAFFIC = {"k_conv_mixed": 6.928e9 + 99.3e6, "k_conv": 7.656e9 + 0.1e6}
 
 
 def parse():
@@ -243,11 +243,11 @@ def main():
             mixed_hist = [ctx.history(r) for r in range(8)]
         nconv = max(tm["conv_launches"], 1)
         conv_ms = tm["conv_ms"] / nconv
-        # per launch: every stored FP32 sub-tile, the merged lists, and the split-BF16 history
-        # read once ((K + 1) slots x 96 columns per tile x R x 6 B)
+        # per launch: every stored FP32 sub-tile, the merged lists, and the FP32 history copy
+        # read once ((K + 1) slots x 96 columns per tile x R x 4 B)
         nT = -(-p.nnd // 16)
         mixed_bytes = (tm["tile_subtiles"] * TP_SUBTILE_BYTES + tm["tile_entries"] * TP_ENTRY_BYTES
-                       + (K + 1) * nT * 96 * R * 6)
+                       + (K + 1) * nT * 96 * R * 4)
         out.update({
             "value": round(args.steps * (p.numstep - 1) * R / wall, 2),
             "ms_per_step": round(wall / args.steps * 1e3, 3),
