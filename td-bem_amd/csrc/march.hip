@@ -33,7 +33,7 @@ struct March {
     int nsplit = 1, Np = 0;
     DBuf<double> gF, gU;    // G~ y split into the FBC part (scaled by f) and the UBC part
     DBuf<double> wF, wU;    // A_2^{-1} (G~_2 y_F), A_2^{-1} (G~_2 y_U) for the first solve
-    DBuf<double> ring;      // [R][nrhs][2][N], part 0 = u, part 1 = t
+    DBuf<double> ring;      // [R][2][N][nrhs], part 0 = u, part 1 = t (node-major)
     DBuf<double> hist;      // [nrhs][N]
     DBuf<double> U, T;      // [numstep][nrhs][N]
     DBuf<int> is_ubc, is_fbc;
@@ -123,12 +123,13 @@ __global__ void __launch_bounds__(kConvThreads)
                      g22 = v[320];
         const double h00 = v[384], h01 = v[448], h02 = v[512], h10 = v[576], h11 = v[640],
                      h12 = v[704], h20 = v[768], h21 = v[832], h22 = v[896];
+        // node-major ring: the 3 NRHS values of node j's u (then t) are contiguous
+        const double* zu = ring + ((int64_t)slot * 2 * N + 3 * j) * NRHS;
+        const double* zt = zu + (int64_t)N * NRHS;
 #pragma unroll
         for (int r = 0; r < NRHS; r++) {
-            const double* zu = ring + ((int64_t)(slot * NRHS + r) * 2) * N + 3 * j;
-            const double* zt = zu + N;
-            const double u0 = zu[0], u1 = zu[1], u2 = zu[2];
-            const double t0 = zt[0], t1 = zt[1], t2 = zt[2];
+            const double u0 = zu[r], u1 = zu[NRHS + r], u2 = zu[2 * NRHS + r];
+            const double t0 = zt[r], t1 = zt[NRHS + r], t2 = zt[2 * NRHS + r];
             acc[r][0] += g00 * t0 + g01 * t1 + g02 * t2 - (h00 * u0 + h01 * u1 + h02 * u2);
             acc[r][1] += g01 * t0 + g11 * t1 + g12 * t2 - (h10 * u0 + h11 * u1 + h12 * u2);
             acc[r][2] += g02 * t0 + g12 * t1 + g22 * t2 - (h20 * u0 + h21 * u1 + h22 * u2);

@@ -62,7 +62,7 @@ __device__ __forceinline__ void split_f32x8(const f32x4& a0, const f32x4& a1, bf
 
 // Element t (< nT * 96 * nrhs) of the FP32 copy of ring slot `slot`, in B-operand order
 // (lane = rhs + 16 * (k / 8), element k % 8), laid out [slot][J][k-tile][rhs tile][2][64 lanes][4]
-// (elements 0-3 of all lanes, then 4-7).  The ring holds [slot][rhs][part][N] with part 0 = u,
+// (elements 0-3 of all lanes, then 4-7).  The ring holds [slot][part][N][rhs] with part 0 = u,
 // part 1 = t; block column part 0 multiplies z^t.
 __device__ __forceinline__ void zcopy_elem(const double* __restrict__ ring, int N, int nrhs,
                                            int nT, const int* __restrict__ perm, int slot,
@@ -75,7 +75,7 @@ __device__ __forceinline__ void zcopy_elem(const double* __restrict__ ring, int 
     const int part = (c % 6) / 3, q = c % 3;
     const int rpart = 1 - part;
     const double v =
-        node < 0 ? 0.0 : ring[((int64_t)(slot * nrhs + r) * 2 + rpart) * N + 3 * node + q];
+        node < 0 ? 0.0 : ring[(((int64_t)slot * 2 + rpart) * N + 3 * node + q) * nrhs + r];
     const int kt = c >> 5, kk = c & 31, nt = r >> 4, n = r & 15;
     const int lane = n + 16 * (kk >> 3), e = kk & 7;
     zs[((((int64_t)slot * nT + J) * 3 + kt) * NT + nt) * 512 + (e >> 2) * 256 + lane * 4 + (e & 3)] =
@@ -352,7 +352,7 @@ __device__ __forceinline__ void finalize_elem(const double* __restrict__ parts, 
     auto at = [&](int m, int part) -> double* {
         int sl = m % R;
         if (sl < 0) sl += R;
-        return ring + ((int64_t)(sl * nrhs + r) * 2 + part) * N + d;
+        return ring + (((int64_t)sl * 2 + part) * N + d) * nrhs + r;
     };
     *at(s - 1, 0) += a1 * u;            // y_{s-1} complete
     *at(s - 1, 1) += a1 * tr;

@@ -280,7 +280,7 @@ void march_history(Context& c, int rhs, double* U, double* T);
 void march_state(Context& c, int rhs, int s, double* u, double* t);
 void march_free(Context& c);
 // k_conv: hist[r][3i+p] = sum over the FP64 lag store of G z^t - H z^u, z from ring slot
-// (s - k) mod R of the layout [R][nrhs][2][3 nnd]; nrhs in {1, 2, 4, 8}
+// (s - k) mod R of the node-major layout [R][2][3 nnd][nrhs]; nrhs in {1, 2, 4, 8}
 void conv_fp64(Context& c, const double* ring, int R, int s, int nrhs, double* hist,
                hipStream_t st);
 
