@@ -114,15 +114,18 @@ __global__ void __launch_bounds__(kConvThreads)
     for (int r = 0; r < NRHS; r++) acc[r][0] = acc[r][1] = acc[r][2] = 0.0;
     const int64_t c1 = row_chunk[i + 1];
     for (int64_t c = row_chunk[i] + wave; c < c1; c += kConvThreads / 64) {
-        const uint32_t id = idx[c * kChunk + lane];
+        // the store is read once per step: non-temporal loads keep the history ring in cache
+        const uint32_t id = __builtin_nontemporal_load(&idx[c * kChunk + lane]);
         const int j = (int)(id >> kKBits), k = (int)(id & kMaxLag);
         int slot = (s - k) % R;
         if (slot < 0) slot += R;
         const double* v = vals + c * (kVals * kChunk) + lane;
-        const double g00 = v[0], g01 = v[64], g02 = v[128], g11 = v[192], g12 = v[256],
-                     g22 = v[320];
-        const double h00 = v[384], h01 = v[448], h02 = v[512], h10 = v[576], h11 = v[640],
-                     h12 = v[704], h20 = v[768], h21 = v[832], h22 = v[896];
+        double g[kVals];
+#pragma unroll
+        for (int q = 0; q < kVals; q++) g[q] = __builtin_nontemporal_load(&v[64 * q]);
+        const double g00 = g[0], g01 = g[1], g02 = g[2], g11 = g[3], g12 = g[4], g22 = g[5];
+        const double h00 = g[6], h01 = g[7], h02 = g[8], h10 = g[9], h11 = g[10], h12 = g[11],
+                     h20 = g[12], h21 = g[13], h22 = g[14];
         // node-major ring: the 3 NRHS values of node j's u (then t) are contiguous
         const double* zu = ring + ((int64_t)slot * 2 * N + 3 * j) * NRHS;
         const double* zt = zu + (int64_t)N * NRHS;
