@@ -113,29 +113,53 @@ __global__ void __launch_bounds__(kConvThreads)
 #pragma unroll
     for (int r = 0; r < NRHS; r++) acc[r][0] = acc[r][1] = acc[r][2] = 0.0;
     const int64_t c1 = row_chunk[i + 1];
-    for (int64_t c = row_chunk[i] + wave; c < c1; c += kConvThreads / 64) {
+    // U chunks per wavefront and iteration: their loads are all issued before the products
+    constexpr int U = NRHS == 1 ? 2 : 1;
+    constexpr int W = kConvThreads / 64;
+    for (int64_t c = row_chunk[i] + wave; c < c1; c += U * W) {
         // the store is read once per step: non-temporal loads keep the history ring in cache
-        const uint32_t id = __builtin_nontemporal_load(&idx[c * kChunk + lane]);
-        const int j = (int)(id >> kKBits), k = (int)(id & kMaxLag);
-        int slot = (s - k) % R;
-        if (slot < 0) slot += R;
-        const double* v = vals + c * (kVals * kChunk) + lane;
-        double g[kVals];
+        double g[U][kVals], zv[U][2][3 * NRHS];
 #pragma unroll
-        for (int q = 0; q < kVals; q++) g[q] = __builtin_nontemporal_load(&v[64 * q]);
-        const double g00 = g[0], g01 = g[1], g02 = g[2], g11 = g[3], g12 = g[4], g22 = g[5];
-        const double h00 = g[6], h01 = g[7], h02 = g[8], h10 = g[9], h11 = g[10], h12 = g[11],
-                     h20 = g[12], h21 = g[13], h22 = g[14];
-        // node-major ring: the 3 NRHS values of node j's u (then t) are contiguous
-        const double* zu = ring + ((int64_t)slot * 2 * N + 3 * j) * NRHS;
-        const double* zt = zu + (int64_t)N * NRHS;
+        for (int q = 0; q < U; q++) {
+            const int64_t cq = c + q * W;
+            if (cq >= c1) {
 #pragma unroll
-        for (int r = 0; r < NRHS; r++) {
-            const double u0 = zu[r], u1 = zu[NRHS + r], u2 = zu[2 * NRHS + r];
-            const double t0 = zt[r], t1 = zt[NRHS + r], t2 = zt[2 * NRHS + r];
-            acc[r][0] += g00 * t0 + g01 * t1 + g02 * t2 - (h00 * u0 + h01 * u1 + h02 * u2);
-            acc[r][1] += g01 * t0 + g11 * t1 + g12 * t2 - (h10 * u0 + h11 * u1 + h12 * u2);
-            acc[r][2] += g02 * t0 + g12 * t1 + g22 * t2 - (h20 * u0 + h21 * u1 + h22 * u2);
+                for (int v = 0; v < kVals; v++) g[q][v] = 0.0;
+#pragma unroll
+                for (int v = 0; v < 3 * NRHS; v++) zv[q][0][v] = zv[q][1][v] = 0.0;
+                continue;
+            }
+            const uint32_t id = __builtin_nontemporal_load(&idx[cq * kChunk + lane]);
+            const int j = (int)(id >> kKBits), k = (int)(id & kMaxLag);
+            int slot = (s - k) % R;
+            if (slot < 0) slot += R;
+            const double* v = vals + cq * (kVals * kChunk) + lane;
+#pragma unroll
+            for (int w = 0; w < kVals; w++) g[q][w] = __builtin_nontemporal_load(&v[64 * w]);
+            // node-major ring: the 3 NRHS values of node j's u (then t) are contiguous
+            const double* zu = ring + ((int64_t)slot * 2 * N + 3 * j) * NRHS;
+            const double* zt = zu + (int64_t)N * NRHS;
+#pragma unroll
+            for (int w = 0; w < 3 * NRHS; w++) {
+                zv[q][0][w] = zu[w];
+                zv[q][1][w] = zt[w];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            const double g00 = g[q][0], g01 = g[q][1], g02 = g[q][2], g11 = g[q][3],
+                         g12 = g[q][4], g22 = g[q][5];
+            const double h00 = g[q][6], h01 = g[q][7], h02 = g[q][8], h10 = g[q][9],
+                         h11 = g[q][10], h12 = g[q][11], h20 = g[q][12], h21 = g[q][13],
+                         h22 = g[q][14];
+#pragma unroll
+            for (int r = 0; r < NRHS; r++) {
+                const double u0 = zv[q][0][r], u1 = zv[q][0][NRHS + r], u2 = zv[q][0][2 * NRHS + r];
+                const double t0 = zv[q][1][r], t1 = zv[q][1][NRHS + r], t2 = zv[q][1][2 * NRHS + r];
+                acc[r][0] += g00 * t0 + g01 * t1 + g02 * t2 - (h00 * u0 + h01 * u1 + h02 * u2);
+                acc[r][1] += g01 * t0 + g11 * t1 + g12 * t2 - (h10 * u0 + h11 * u1 + h12 * u2);
+                acc[r][2] += g02 * t0 + g12 * t1 + g22 * t2 - (h20 * u0 + h21 * u1 + h22 * u2);
+            }
         }
     }
 #pragma unroll
