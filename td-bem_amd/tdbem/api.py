@@ -319,6 +319,45 @@ def dense_solve(A, B, inverse=False, pivoted=False):
     return X
 
 
+def prepare_contexts(problems, loads, precision=PRECISION_MIXED, workers=8):
+    """One context per problem with its set-up done (lag store, step-0 factorisations and
+    inverse, march_init with the problem's boundary data), `workers` set-ups at a time: each
+    context has its own HIP stream and the C ABI calls release the GIL, so the set-ups of many
+    scenarios overlap on the device and on the host.  loads: per problem (amp, t_rise) arrays of
+    the right-hand sides.  Returns the contexts in problem order; on an error the contexts
+    already made are closed and the first error is raised."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(i):
+        p = problems[i]
+        amp, rise = loads[i]
+        ctx = Context.from_problem(p)
+        try:
+            ctx.set_precision(precision)
+            ctx.build_lag_store()
+            ctx.march_init(p.a1, p.a2, p.ubc_dof, p.ubc_val, p.fbc_dof, p.fbc_val, amp=amp,
+                           t_rise=rise)
+        except BaseException:
+            ctx.close()
+            raise
+        return ctx
+
+    with ThreadPoolExecutor(max_workers=max(1, int(workers))) as ex:
+        futs = [ex.submit(one, i) for i in range(len(problems))]
+    ctxs, err = [], None
+    for f in futs:
+        try:
+            ctxs.append(f.result())
+        except BaseException as e:  # noqa: BLE001 - re-raised below after cleanup
+            if err is None:
+                err = e
+    if err is not None:
+        for c in ctxs:
+            c.close()
+        raise err
+    return ctxs
+
+
 class Batch:
     """Scenarios marched together (tdbem_batch_*): each context after a mixed-precision
     march_init with the same number of right-hand sides; results stay in the contexts."""

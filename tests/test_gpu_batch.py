@@ -112,3 +112,24 @@ def test_batch_rejects_mixed_right_hand_side_counts(gpu):
         api.Batch([a, c])
     a.close()
     c.close()
+
+
+def test_concurrent_set_up_matches_sequential(gpu):
+    """api.prepare_contexts runs the set-ups of several scenarios on host threads at once (each
+    context on its own stream); the marches give the same histories as set-ups done one by
+    one."""
+    probs = _scenarios()
+    print("[gpu case] concurrent set-up", flush=True)
+    ref = _alone(probs)
+    loads = [_loads(p, 100 + i) for i, p in enumerate(probs)]
+    ctxs = api.prepare_contexts(probs, loads, precision=api.PRECISION_MIXED, workers=4)
+    try:
+        for ctx, r in zip(ctxs, ref):
+            ctx.march_run()
+            for k in range(NRHS):
+                U, T = ctx.history(k)
+                np.testing.assert_array_equal(U, r[k][0])
+                np.testing.assert_array_equal(T, r[k][1])
+    finally:
+        for c in ctxs:
+            c.close()
