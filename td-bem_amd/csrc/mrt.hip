@@ -161,6 +161,9 @@ Mrt* mrt_create(Context* c1, Context* c2, int max_step, int nz, const int* if1, 
 {
     if (!c1 || !c2 || c1 == c2) throw std::invalid_argument("two distinct region contexts needed");
     if (nz < 0 || (nz > 0 && (!if1 || !if2))) throw std::invalid_argument("bad interface arrays");
+    if (max_step < 0) throw std::invalid_argument("max step must be >= 0");
+    if (c1->device != c2->device)
+        throw std::invalid_argument("the region contexts must be on the same GPU");
     for (Context* c : {c1, c2}) {
         if (c->kernel != 4)
             throw std::invalid_argument("the two-region solver runs the Step kernel");
