@@ -442,17 +442,22 @@ std::vector<int> host_permutation(const int* dpiv, int n, hipStream_t s)
     return perm;
 }
 
-void solve_inplace(const double* LU, int n, double* B, int ncols, hipStream_t s)
+// B := U^{-1} L^{-1} B for B of n x ncols.  identity: B is the n x n identity, so L^{-1} B is
+// unit lower triangular and the forward pass of row block b touches only its first
+// j0 + nb columns (a third of the work of the full forward pass).
+void solve_inplace(const double* LU, int n, double* B, int ncols, hipStream_t s,
+                   bool identity = false)
 {
     const int nblk = (n + NB - 1) / NB;
     const int tpb = 128;
     for (int b = 0; b < nblk; b++) {
         const int j0 = b * NB, nb = std::min(NB, n - j0);
-        hipLaunchKernelGGL(k_trsm_lower_unit, dim3((ncols + tpb - 1) / tpb), dim3(tpb), 0, s, LU,
-                           n, j0, nb, B, ncols, j0, 0, ncols);
+        const int cols = identity ? j0 + nb : ncols;
+        hipLaunchKernelGGL(k_trsm_lower_unit, dim3((cols + tpb - 1) / tpb), dim3(tpb), 0, s, LU,
+                           n, j0, nb, B, ncols, j0, 0, cols);
         TD_HIP(hipGetLastError());
         const int rest = n - j0 - nb;
-        gemm_sub(rest, ncols, nb, LU + (int64_t)(j0 + nb) * n + j0, n, B + (int64_t)j0 * ncols,
+        gemm_sub(rest, cols, nb, LU + (int64_t)(j0 + nb) * n + j0, n, B + (int64_t)j0 * ncols,
                  ncols, B + (int64_t)(j0 + nb) * ncols, ncols, s);
     }
     for (int b = nblk - 1; b >= 0; b--) {
@@ -687,7 +692,9 @@ void lu_inverse(const double* LU, int n, const int* piv, double* Ainv, hipStream
     hipLaunchKernelGGL(k_perm_identity, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
                        dperm.p, n, Ainv);
     TD_HIP(hipGetLastError());
-    solve_inplace(LU, n, Ainv, n, s);
+    bool identity = true;
+    for (int i = 0; i < n && identity; i++) identity = perm[i] == i;
+    solve_inplace(LU, n, Ainv, n, s, identity);
     TD_HIP(hipStreamSynchronize(s));
 }
 
