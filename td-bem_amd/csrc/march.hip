@@ -93,6 +93,34 @@ __global__ void k_unpack2(const double* __restrict__ in, int N, double* __restri
     b[d] = in[2 * d + 1];
 }
 
+// Xf = r (nrhs right-hand sides [nrhs][N]) in the fragment order of the step solve
+__global__ void k_frag_pack(const double* __restrict__ r, int N, int nrhs, double* __restrict__ Xf)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)N * nrhs) return;
+    const int k = (int)(t / N), d = (int)(t % N);
+    Xf[frag_index(N, k, d)] = r[t];
+}
+
+// x += scale x the sum of the column-split partials of the step solve ([nsplit][nrhs][Np])
+__global__ void k_parts_add(const double* __restrict__ parts, int nsplit, int Np, int N,
+                            int nrhs, double scale, double* __restrict__ x)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)N * nrhs) return;
+    const int k = (int)(t / N), d = (int)(t % N);
+    double v = 0.0;
+    for (int p = 0; p < nsplit; p++) v += parts[((int64_t)p * nrhs + k) * Np + d];
+    x[t] += scale * v;
+}
+
+__global__ void k_sub(const double* __restrict__ a, const double* __restrict__ b, int64_t n,
+                      double* __restrict__ out)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = a[t] - b[t];
+}
+
 __global__ void k_negate(double* __restrict__ v, int N)
 {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
@@ -216,6 +244,61 @@ __global__ void k_finalize(const double* __restrict__ parts, int nparts, int Np,
                       a2, a3, U, T, ring, t);
 }
 
+// x = A2^{-1} b for the two right-hand sides ([2][N]) of the first solve, by iterative
+// refinement with the inverse of the n >= 3 matrix A = c0 (A2 + (a1 / c0) M_1) (column-exchanged
+// alike), c0 = 2 a1 + a3: x <- x + c0 A^{-1} (b - A2 x) contracts with the spectral radius of
+// (A / c0)^{-1} (a1 / c0) M_1 (about 0.2 for the sphere and 0.5 for the bar at a1 = a2 = 1/4;
+// 0 when a1 = 0).  Stops at a residual of 1e-15 of |A2 x| + |b|; returns false when it stalls
+// above 1e-14 or runs out of iterations.
+bool refine_first(const double* A2, int N, const double* Af, double c0, const double* b,
+                  double* x, hipStream_t s)
+{
+    const int Np = frag_rows(N), nsplit = apply_splits(N);
+    const int64_t n2 = 2LL * N;
+    DBuf<double> Xf, parts, r, ax;
+    Xf.alloc((size_t)frag_elems(N, 2));
+    parts.alloc((size_t)nsplit * 2 * Np);
+    r.alloc(n2);
+    ax.alloc(n2);
+    TD_HIP(hipMemsetAsync(Xf.p, 0, Xf.bytes(), s));   // padding stays zero
+    TD_HIP(hipMemsetAsync(x, 0, n2 * sizeof(double), s));
+    TD_HIP(hipMemcpyAsync(r.p, b, n2 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    std::vector<double> hb(n2), hr(n2), hax(n2);
+    TD_HIP(hipMemcpyAsync(hb.data(), b, n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    TD_HIP(hipStreamSynchronize(s));
+    auto norm = [&](const std::vector<double>& v, int k) {
+        double q = 0.0;
+        for (int d = 0; d < N; d++) q += v[(size_t)k * N + d] * v[(size_t)k * N + d];
+        return std::sqrt(q);
+    };
+    const double bn[2] = {norm(hb, 0), norm(hb, 1)};
+    double prev = 1e300;
+    for (int it = 0; it < 200; it++) {
+        hipLaunchKernelGGL(k_frag_pack, dim3(blocks(n2)), dim3(256), 0, s, r.p, N, 2, Xf.p);
+        apply_frag(Af, N, Xf.p, 2, parts.p, s);
+        hipLaunchKernelGGL(k_parts_add, dim3(blocks(n2)), dim3(256), 0, s, parts.p, nsplit, Np, N,
+                           2, c0, x);
+        gemv(A2, N, x, ax.p, 0.0, s);
+        gemv(A2, N, x + N, ax.p + N, 0.0, s);
+        hipLaunchKernelGGL(k_sub, dim3(blocks(n2)), dim3(256), 0, s, b, ax.p, n2, r.p);
+        TD_HIP(hipGetLastError());
+        TD_HIP(hipMemcpyAsync(hr.data(), r.p, n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        TD_HIP(hipMemcpyAsync(hax.data(), ax.p, n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        TD_HIP(hipStreamSynchronize(s));
+        double rel = 0.0;
+        for (int k = 0; k < 2; k++) {
+            const double scale = norm(hax, k) + bn[k];
+            const double rn = norm(hr, k);
+            if (!std::isfinite(rn)) return false;
+            if (scale > 0) rel = std::max(rel, rn / scale);
+        }
+        if (rel <= 1e-15) return true;
+        if (it >= 4 && rel > 0.7 * prev) return rel <= 1e-14;
+        prev = rel;
+    }
+    return false;
+}
+
 }  // namespace
 
 void conv_fp64(Context& c, const double* ring, int R, int s, int nrhs, double* hist,
@@ -316,25 +399,6 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
     M.wF.alloc(N);
     M.wU.alloc(N);
 
-    // n == 2: A_2 = M_0 exchanged; gF2 = G_0 y_F, gU2 = -H_0 y_U
-    hipLaunchKernelGGL(k_combine, dim3(blocks(NN)), dim3(256), 0, c.stream, G0.p, H0.p,
-                       (const double*)nullptr, (const double*)nullptr, 1.0, 0.0, M.is_ubc.p, N,
-                       A.p, Gc.p, Hc.p);
-    TD_HIP(hipGetLastError());
-    gemv(Gc.p, N, dyF.p, M.gF.p, 0.0, c.stream);
-    gemv(Hc.p, N, dyU.p, M.gU.p, 0.0, c.stream);
-    hipLaunchKernelGGL(k_negate, dim3(blocks(N)), dim3(256), 0, c.stream, M.gU.p, N);
-    lu_factor(A.p, N, piv.p, c.stream);
-    {
-        DBuf<double> B;
-        B.alloc((size_t)2 * N);
-        hipLaunchKernelGGL(k_pack2, dim3(blocks(N)), dim3(256), 0, c.stream, M.gF.p, M.gU.p, N,
-                           B.p);
-        lu_solve(A.p, N, piv.p, B.p, 2, c.stream);
-        hipLaunchKernelGGL(k_unpack2, dim3(blocks(N)), dim3(256), 0, c.stream, B.p, N, M.wF.p,
-                           M.wU.p);
-        TD_HIP(hipStreamSynchronize(c.stream));
-    }
     // n >= 3: (2 a1 + a3) M_0 + a1 M_1 (src/SolverStepLinearT.cpp:384-393)
     hipLaunchKernelGGL(k_combine, dim3(blocks(NN)), dim3(256), 0, c.stream, G0.p, H0.p, G1.p,
                        H1.p, 2 * a1 + M.a3, a1, M.is_ubc.p, N, A.p, Gc.p, Hc.p);
@@ -348,6 +412,37 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
         lu_inverse(A.p, N, piv.p, Gc.p, c.stream);
         M.Af.alloc((size_t)frag_elems(N, N));
         to_frag(Gc.p, N, M.Af.p, c.stream);
+    }
+    // n == 2 (src/SolverStepLinearT.cpp:355-378): A_2 = M_0 exchanged (into G1; G1, H1 and Hc
+    // are free now), gF2 = G_0 y_F, gU2 = -H_0 y_U; solved by refinement with the inverse above,
+    // or by its own factorisation when that does not converge
+    hipLaunchKernelGGL(k_combine, dim3(blocks(NN)), dim3(256), 0, c.stream, G0.p, H0.p,
+                       (const double*)nullptr, (const double*)nullptr, 1.0, 0.0, M.is_ubc.p, N,
+                       G1.p, H1.p, Hc.p);
+    TD_HIP(hipGetLastError());
+    {
+        DBuf<double> b2, x2;
+        b2.alloc((size_t)2 * N);
+        x2.alloc((size_t)2 * N);
+        gemv(H1.p, N, dyF.p, b2.p, 0.0, c.stream);
+        gemv(Hc.p, N, dyU.p, b2.p + N, 0.0, c.stream);
+        hipLaunchKernelGGL(k_negate, dim3(blocks(N)), dim3(256), 0, c.stream, b2.p + N, N);
+        if (refine_first(G1.p, N, M.Af.p, 2 * a1 + M.a3, b2.p, x2.p, c.stream)) {
+            TD_HIP(hipMemcpyAsync(M.wF.p, x2.p, N * sizeof(double), hipMemcpyDeviceToDevice,
+                                  c.stream));
+            TD_HIP(hipMemcpyAsync(M.wU.p, x2.p + N, N * sizeof(double), hipMemcpyDeviceToDevice,
+                                  c.stream));
+        } else {
+            DBuf<double> B;
+            B.alloc((size_t)2 * N);
+            hipLaunchKernelGGL(k_pack2, dim3(blocks(N)), dim3(256), 0, c.stream, b2.p, b2.p + N,
+                               N, B.p);
+            lu_factor(G1.p, N, piv.p, c.stream);
+            lu_solve(G1.p, N, piv.p, B.p, 2, c.stream);
+            hipLaunchKernelGGL(k_unpack2, dim3(blocks(N)), dim3(256), 0, c.stream, B.p, N,
+                               M.wF.p, M.wU.p);
+        }
+        TD_HIP(hipStreamSynchronize(c.stream));
     }
 
     M.precision = c.precision;
