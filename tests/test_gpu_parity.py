@@ -132,7 +132,8 @@ def test_lag_store_matches_oracle(gpu, case):
 
 
 @pytest.mark.parametrize("case,a1,a2", [("bar", 0.25, 0.25), ("bar", 0.0, 0.0),
-                                        ("sphere", 0.25, 0.25), ("bar", 0.4, 0.1)])
+                                        ("sphere", 0.25, 0.25), ("bar", 0.4, 0.1),
+                                        ("bar", 0.1, 0.6), ("bar", 1.0, 0.0)])
 def test_march_matches_oracle(gpu, case, a1, a2):
     p = {"bar": helpers.small_bar, "sphere": helpers.small_sphere}[case](numstep=40)
     _announce(f"march {case} a1={a1} a2={a2}")
