@@ -141,7 +141,8 @@ def copy_calibration(api):
 def serve_leg(api, configs, serve, nreq, rate):
     """Continuous-batching serving of mixed-precision solves (bar and sphere geometries of 72 to
     864 elements, 100 to 300 steps, 16 or 32 load cases, Poisson arrivals), against the same
-    requests solved one after another on the plain FP64 path with one load case each."""
+    requests solved one after another on the plain FP64 path with one load case each; then the
+    same requests at 2x, 4x and 8x the arrival rate (rate_sweep)."""
     probs = configs.serving_mix(nreq, seed=5)
     reqs = serve.poisson_requests(probs, rate=rate, seed=5)
     res, summary = serve.Server(max_active=8).run(reqs)
@@ -161,6 +162,14 @@ def serve_leg(api, configs, serve, nreq, rate):
                                           "time_steps_per_s": round(steps / serial, 2)}})
     summary["speedup_vs_plain_fp64_serial"] = round(summary["time_steps_per_s"] /
                                                     (steps / serial), 2)
+    # the same requests at higher arrival rates: throughput up to the device's, latency after
+    sweep = []
+    for r in (2 * rate, 4 * rate, 8 * rate):
+        _, s_r = serve.Server(max_active=8).run(serve.poisson_requests(probs, rate=r, seed=5))
+        sweep.append({"arrival_rate_per_s": r, "time_steps_per_s": s_r["time_steps_per_s"],
+                      "latency_p50_s": s_r["latency_p50_s"], "latency_p99_s": s_r["latency_p99_s"],
+                      "makespan_s": s_r["makespan_s"], "failed": s_r["failed"]})
+    summary["rate_sweep"] = sweep
     return summary
 
 
