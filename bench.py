@@ -38,7 +38,7 @@ FP64_ENTRY_BYTES = 15 * 8 + 4            # 15 values + index per (i, j, k)
 # HBM bytes per launch (read + written) from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE)
 # on this workload, scaled by the counters of a 1 GiB k_stream_copy (factors 2.0 and 1.0);
 # profiles/k_conv_traffic_pmc_sphere1536.md holds the runs.
-MEASURED_TRAFFIC = {"k_conv_mixed": 6.928e9 + 99.3e6, "k_conv": 7.656e9 + 0.1e6}
+MEASURED_TRAFFIC = {"k_conv_mixed": 6.928e9 + 99.3e6, "k_conv": 7.444e9 + 0.1e6}
 
 
 def parse():
