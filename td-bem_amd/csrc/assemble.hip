@@ -545,7 +545,7 @@ __device__ __forceinline__ void integrate_pair_pulse(const double* coords, const
 }
 
 template <int ENND>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, ENND == 4 ? 2 : 1)
     k_pair_lags(const double* __restrict__ coords, const int* __restrict__ ien,
                 const double* __restrict__ geo, const double* __restrict__ gauss, int nq, int nel,
                 double dt, Material m, int Kc, int i0, int64_t nslots,
