@@ -358,7 +358,51 @@ __global__ void k_fill_idx(const int2* __restrict__ hull, const int64_t* __restr
 // (pair, part) gives phi_1 (part 0) or phi_2 (part 1) of step n; for the Heaviside kernel
 // (src/GreenFunctionT.cpp:126-129) there is no (t - dt) level.
 // ---------------------------------------------------------------------------------------------
+// FP64 integration of a pair over the points of for_each_point (regular rule for sing < 0,
+// Duffy triangles / sub-elements for a singular node), accumulated into out[ENND][15].
 template <int ENND>
+__device__ __forceinline__ int integrate_fp64(const double* X, int sing, const double* coll,
+                                              const double* geo_e, const double* gauss, int nq,
+                                              const TimeCombo& tc, double dt, const Material& m,
+                                              double* out)
+{
+    int active = 0;
+    const double inv_dt = 1.0 / dt;
+    double acc[ENND][kVals];
+#pragma unroll
+    for (int a = 0; a < ENND; a++)
+#pragma unroll
+        for (int v = 0; v < kVals; v++) acc[a][v] = 0.0;
+    for_each_point<ENND>(X, sing, geo_e, gauss, nq,
+                         [&](const double* x, const double* n, double wJ, const double* N) {
+                             double d[3] = {x[0] - coll[0], x[1] - coll[1], x[2] - coll[2]};
+                             const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                             double S[6];
+                             if (!eval_s(tc, r, dt, inv_dt, m, S)) return;
+                             active++;
+                             const double ir = 1.0 / r;
+                             d[0] *= ir;
+                             d[1] *= ir;
+                             d[2] *= ir;
+                             double blk[kVals];
+                             blocks_from_s<double>(S, d, n, r, ir, ir * ir, m, blk);
+#pragma unroll
+                             for (int a = 0; a < ENND; a++) {
+                                 const double wN = wJ * N[a];
+#pragma unroll
+                                 for (int v = 0; v < kVals; v++) acc[a][v] += wN * blk[v];
+                             }
+                         });
+#pragma unroll
+    for (int a = 0; a < ENND; a++)
+#pragma unroll
+        for (int v = 0; v < kVals; v++) out[a * kVals + v] = acc[a][v];
+    return active;
+}
+
+// REGULAR: the caller guarantees that no node of e lies at the collocation point, so the
+// singular rules are not compiled in (the regular path then fits two waves per SIMD).
+template <int ENND, bool REGULAR = false>
 __device__ __forceinline__ int integrate_pair(const double* coords, const int* ien,
                                               const double* geo, const double* gauss, int nq,
                                               int i, int e, const TimeCombo& tc, double dt,
@@ -368,17 +412,17 @@ __device__ __forceinline__ int integrate_pair(const double* coords, const int* i
     const double coll[3] = {coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]};
     double X[3 * ENND];
     load_element<ENND>(coords, ien, e, X);
-    const int sing = singular_node<ENND>(X, coll);
+    const int sing = REGULAR ? -1 : singular_node<ENND>(X, coll);
+    const double* geo_e = geo + (size_t)e * nq * nq * (7 + ENND);
     double acc[ENND][kVals];
 #pragma unroll
     for (int a = 0; a < ENND; a++)
 #pragma unroll
         for (int v = 0; v < kVals; v++) acc[a][v] = 0.0;
-    const double inv_dt = 1.0 / dt;
     if (fp32 && sing < 0 && !near_singular<ENND>(X, coll)) {
         // regular pair, mixed precision: distance and retarded times in FP64, the time
         // integrals and the 15 block values in FP32, sums over the points in FP64
-        for_each_point<ENND>(X, sing, geo + (size_t)e * nq * nq * (7 + ENND), gauss, nq,
+        for_each_point<ENND>(X, -1, geo_e, gauss, nq,
                              [&](const double* x, const double* n, double wJ, const double* N) {
                                  const double d0 = x[0] - coll[0], d1 = x[1] - coll[1],
                                               d2 = x[2] - coll[2];
@@ -408,31 +452,7 @@ __device__ __forceinline__ int integrate_pair(const double* coords, const int* i
             for (int v = 0; v < kVals; v++) out[a * kVals + v] = acc[a][v];
         return active;
     }
-    for_each_point<ENND>(X, sing, geo + (size_t)e * nq * nq * (7 + ENND), gauss, nq,
-                         [&](const double* x, const double* n, double wJ, const double* N) {
-                             double d[3] = {x[0] - coll[0], x[1] - coll[1], x[2] - coll[2]};
-                             const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-                             double S[6];
-                             if (!eval_s(tc, r, dt, inv_dt, m, S)) return;
-                             active++;
-                             const double ir = 1.0 / r;
-                             d[0] *= ir;
-                             d[1] *= ir;
-                             d[2] *= ir;
-                             double blk[kVals];
-                             blocks_from_s<double>(S, d, n, r, ir, ir * ir, m, blk);
-#pragma unroll
-                             for (int a = 0; a < ENND; a++) {
-                                 const double wN = wJ * N[a];
-#pragma unroll
-                                 for (int v = 0; v < kVals; v++) acc[a][v] += wN * blk[v];
-                             }
-                         });
-#pragma unroll
-    for (int a = 0; a < ENND; a++)
-#pragma unroll
-        for (int v = 0; v < kVals; v++) out[a * kVals + v] = acc[a][v];
-    return active;
+    return integrate_fp64<ENND>(X, sing, coll, geo_e, gauss, nq, tc, dt, m, out);
 }
 
 // The reference's 20-point time rule (src/GreenFunctionT.cpp:1042-1164, 9 digits), for the
@@ -544,18 +564,33 @@ __device__ __forceinline__ void integrate_pair_pulse(const double* coords, const
         for (int v = 0; v < kVals; v++) out[a * kVals + v] = acc[a][v];
 }
 
-template <int ENND>
-__global__ void __launch_bounds__(kBlock, ENND == 4 ? 2 : 1)
+// Lag slots of one row batch.  slot_pair[s] is the pair of slot s, complemented (~p) for the
+// pairs whose element has a node within 1e-3 of the collocation point: the SINGULAR launch runs
+// over those slots (list sing_slots, nsing entries) with the full integrator, the regular launch
+// over all the others with the regular-only one.
+template <int ENND, bool SINGULAR>
+__global__ void __launch_bounds__(kBlock, (ENND == 4 && !SINGULAR) ? 2 : 1)
     k_pair_lags(const double* __restrict__ coords, const int* __restrict__ ien,
                 const double* __restrict__ geo, const double* __restrict__ gauss, int nq, int nel,
                 double dt, Material m, int Kc, int i0, int64_t nslots,
                 const int* __restrict__ slot_pair, const int64_t* __restrict__ pair_slot,
-                const int2* __restrict__ pair_k, double* __restrict__ scratch,
+                const int2* __restrict__ pair_k, const int64_t* __restrict__ sing_slots,
+                int64_t nsing, double* __restrict__ scratch,
                 unsigned long long* __restrict__ counter, int fp32)
 {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslots) return;
-    const int p = slot_pair[s];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t s;
+    int p;
+    if (SINGULAR) {
+        if (t >= nsing) return;
+        s = sing_slots[t];
+        p = ~slot_pair[s];
+    } else {
+        if (t >= nslots) return;
+        s = t;
+        p = slot_pair[s];
+        if (p < 0) return;
+    }
     const int i = i0 + p / nel, e = p % nel;
     const int k = pair_k[(int64_t)i * nel + e].x + (int)(s - pair_slot[p]);
     TimeCombo tc;
@@ -570,8 +605,9 @@ __global__ void __launch_bounds__(kBlock, ENND == 4 ? 2 : 1)
     tc.cI[2] = 0.0;
     tc.cJ[2] = hasJ;
     tc.static_ok = true;
-    const int active = integrate_pair<ENND>(coords, ien, geo, gauss, nq, i, e, tc, dt, m,
-                                            scratch + s * (ENND * kVals), fp32 != 0);
+    const int active = integrate_pair<ENND, !SINGULAR>(coords, ien, geo, gauss, nq, i, e, tc, dt,
+                                                       m, scratch + s * (ENND * kVals),
+                                                       fp32 != 0);
     if (counter) atomicAdd(counter, (unsigned long long)active);
 }
 
@@ -1007,8 +1043,9 @@ void fill_store(Context& c, const ShellPlan& P)
     // The host builds batch b + 1's slot lists while the device works on batch b: two host list
     // sets, each reused only after the event that follows its upload; the device buffers are
     // rewritten in stream order after the previous batch's kernels.
-    std::vector<int64_t> h_pair_slot[2];
+    std::vector<int64_t> h_pair_slot[2], h_sing[2];
     std::vector<int> h_slot_pair[2];
+    DBuf<int64_t> sing_slots;
     hipEvent_t uploaded[2];
     TD_HIP(hipEventCreateWithFlags(&uploaded[0], hipEventDisableTiming));
     TD_HIP(hipEventCreateWithFlags(&uploaded[1], hipEventDisableTiming));
@@ -1024,29 +1061,53 @@ void fill_store(Context& c, const ShellPlan& P)
         hs.resize((size_t)slots);
         std::vector<int64_t> row_start(nb + 1, 0);
         for (int r = 0; r < nb; r++) row_start[r + 1] = row_start[r] + rs[i0 + r];
+        auto& hsing = h_sing[set];
         parallel_for(nb, [&](int64_t lo, int64_t hi) {
             for (int64_t r = lo; r < hi; r++) {
                 int64_t sl = row_start[r];
+                const double* xc = &c.h_coords[3 * (i0 + r)];
                 for (int e = 0; e < c.nel; e++) {
                     const int64_t p = r * c.nel + e;
                     hp[p] = sl;
                     const int2 k = pk[(int64_t)i0 * c.nel + p];
-                    if (k.y >= k.x)
-                        for (int t = k.x; t <= k.y; t++) hs[sl++] = (int)p;
+                    if (k.y < k.x) continue;
+                    // the device's singular test is a squared distance below 1e-9; a pair goes
+                    // to the singular launch well before that (squared distance below 1e-6)
+                    bool near = false;
+                    for (int a = 0; a < ENND; a++) {
+                        const double* xa = &c.h_coords[3 * (int64_t)c.h_ien[(int64_t)e * ENND + a]];
+                        const double dx = xa[0] - xc[0], dy = xa[1] - xc[1], dz = xa[2] - xc[2];
+                        near = near || dx * dx + dy * dy + dz * dz < 1e-6;
+                    }
+                    for (int t = k.x; t <= k.y; t++) hs[sl++] = near ? ~(int)p : (int)p;
                 }
             }
         });
+        hsing.clear();
+        for (int64_t sl = 0; sl < slots; sl++)
+            if (hs[sl] < 0) hsing.push_back(sl);
         slot_pair.upload(hs.data(), hs.size(), c.stream);
         pair_slot.upload(hp.data(), hp.size(), c.stream);
+        if (!hsing.empty()) sing_slots.upload(hsing.data(), hsing.size(), c.stream);
         TD_HIP(hipEventRecord(uploaded[set], c.stream));
         if (slots > 0) {
             lag_timer.begin();
-            hipLaunchKernelGGL(k_pair_lags<ENND>, dim3(grid_for(slots)), dim3(kBlock), 0,
+            hipLaunchKernelGGL((k_pair_lags<ENND, false>), dim3(grid_for(slots)), dim3(kBlock), 0,
                                c.stream, c.coords.p, c.ien.p, c.geo.p, c.gauss.p, c.nq, c.nel,
                                c.dt, c.mat, c.Kc, i0, slots, slot_pair.p, pair_slot.p,
-                               c.pair_k.p, scratch.p, c.profile ? c.counters.p : nullptr,
-                               c.precision == 1 ? 1 : 0);
+                               c.pair_k.p, sing_slots.p, (int64_t)0, scratch.p,
+                               c.profile ? c.counters.p : nullptr, c.precision == 1 ? 1 : 0);
             TD_HIP(hipGetLastError());
+            const int64_t nsing = (int64_t)hsing.size();
+            if (nsing > 0) {
+                hipLaunchKernelGGL((k_pair_lags<ENND, true>), dim3(grid_for(nsing)),
+                                   dim3(kBlock), 0, c.stream, c.coords.p, c.ien.p, c.geo.p,
+                                   c.gauss.p, c.nq, c.nel, c.dt, c.mat, c.Kc, i0, slots,
+                                   slot_pair.p, pair_slot.p, c.pair_k.p, sing_slots.p, nsing,
+                                   scratch.p, c.profile ? c.counters.p : nullptr,
+                                   c.precision == 1 ? 1 : 0);
+                TD_HIP(hipGetLastError());
+            }
             lag_timer.end();
         }
         const int64_t ch0 = P.row_chunk[i0], ch1 = P.row_chunk[i1];
