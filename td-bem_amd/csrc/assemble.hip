@@ -364,7 +364,7 @@ template <int ENND>
 __device__ __forceinline__ int integrate_fp64(const double* X, int sing, const double* coll,
                                               const double* geo_e, const double* gauss, int nq,
                                               const TimeCombo& tc, double dt, const Material& m,
-                                              double* out)
+                                              double* out, int64_t ostride)
 {
     int active = 0;
     const double inv_dt = 1.0 / dt;
@@ -396,17 +396,18 @@ __device__ __forceinline__ int integrate_fp64(const double* X, int sing, const d
 #pragma unroll
     for (int a = 0; a < ENND; a++)
 #pragma unroll
-        for (int v = 0; v < kVals; v++) out[a * kVals + v] = acc[a][v];
+        for (int v = 0; v < kVals; v++) out[(a * kVals + v) * ostride] = acc[a][v];
     return active;
 }
 
-// REGULAR: the caller guarantees that no node of e lies at the collocation point, so the
+// Value v of node a goes to out[(a * 15 + v) * ostride].  REGULAR: the caller guarantees that no node of e lies at the collocation point, so the
 // singular rules are not compiled in (the regular path then fits two waves per SIMD).
 template <int ENND, bool REGULAR = false>
 __device__ __forceinline__ int integrate_pair(const double* coords, const int* ien,
                                               const double* geo, const double* gauss, int nq,
                                               int i, int e, const TimeCombo& tc, double dt,
-                                              const Material& m, double* out, bool fp32 = false)
+                                              const Material& m, double* out, bool fp32 = false,
+                                              int64_t ostride = 1)
 {
     int active = 0;
     const double coll[3] = {coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]};
@@ -449,10 +450,10 @@ __device__ __forceinline__ int integrate_pair(const double* coords, const int* i
 #pragma unroll
         for (int a = 0; a < ENND; a++)
 #pragma unroll
-            for (int v = 0; v < kVals; v++) out[a * kVals + v] = acc[a][v];
+            for (int v = 0; v < kVals; v++) out[(a * kVals + v) * ostride] = acc[a][v];
         return active;
     }
-    return integrate_fp64<ENND>(X, sing, coll, geo_e, gauss, nq, tc, dt, m, out);
+    return integrate_fp64<ENND>(X, sing, coll, geo_e, gauss, nq, tc, dt, m, out, ostride);
 }
 
 // The reference's 20-point time rule (src/GreenFunctionT.cpp:1042-1164, 9 digits), for the
@@ -564,7 +565,9 @@ __device__ __forceinline__ void integrate_pair_pulse(const double* coords, const
         for (int v = 0; v < kVals; v++) out[a * kVals + v] = acc[a][v];
 }
 
-// Lag slots of one row batch.  slot_pair[s] is the pair of slot s, complemented (~p) for the
+// Lag slots of one row batch, value v of element node a of slot s at scratch[(a * 15 + v) *
+// nslots + s] (consecutive slots of a pair are consecutive lags, so the gather's lanes read
+// runs of consecutive addresses).  slot_pair[s] is the pair of slot s, complemented (~p) for the
 // pairs whose element has a node within 1e-3 of the collocation point: the SINGULAR launch runs
 // over those slots (list sing_slots, nsing entries) with the full integrator, the regular launch
 // over all the others with the regular-only one.
@@ -606,8 +609,7 @@ __global__ void __launch_bounds__(kBlock, (ENND == 4 && !SINGULAR) ? 2 : 1)
     tc.cJ[2] = hasJ;
     tc.static_ok = true;
     const int active = integrate_pair<ENND, !SINGULAR>(coords, ien, geo, gauss, nq, i, e, tc, dt,
-                                                       m, scratch + s * (ENND * kVals),
-                                                       fp32 != 0);
+                                                       m, scratch + s, fp32 != 0, nslots);
     if (counter) atomicAdd(counter, (unsigned long long)active);
 }
 
@@ -728,7 +730,7 @@ __global__ void __launch_bounds__(kChunk)
                    const int* __restrict__ adj, const double* __restrict__ cterm,
                    const int64_t* __restrict__ row_chunk, const int64_t* __restrict__ row_valid,
                    const uint32_t* __restrict__ idx, int nnd, int nel, int i0, double ext,
-                   int64_t chunk0, double* __restrict__ vals)
+                   int64_t chunk0, int64_t nslots, double* __restrict__ vals)
 {
     const int64_t c = chunk0 + blockIdx.x;
     const int i = row_of_chunk(row_chunk, nnd, c);
@@ -745,9 +747,9 @@ __global__ void __launch_bounds__(kChunk)
             const int2 pk = pair_k[(int64_t)i * nel + e];
             if (k < pk.x || k > pk.y) continue;
             const int64_t p = (int64_t)(i - i0) * nel + e;
-            const double* s = scratch + (pair_slot[p] + (k - pk.x)) * (ENND * kVals) + loc * kVals;
+            const double* s = scratch + (pair_slot[p] + (k - pk.x)) + loc * kVals * nslots;
 #pragma unroll
-            for (int t = 0; t < kVals; t++) v[t] += s[t];
+            for (int t = 0; t < kVals; t++) v[t] += s[t * nslots];
         }
         if (j == i && k <= 1) {
             // C-term and the exterior-problem term go to fH_1 and fH_2 of step 1, i.e. into
@@ -1116,7 +1118,7 @@ void fill_store(Context& c, const ShellPlan& P)
             hipLaunchKernelGGL(k_store_gather<ENND>, dim3((unsigned)(ch1 - ch0)), dim3(kChunk), 0,
                                c.stream, scratch.p, pair_slot.p, c.pair_k.p, c.adj_off.p, c.adj.p,
                                c.cterm.p, S.row_chunk.p, S.row_valid.p, S.idx.p, c.nnd, c.nel,
-                               i0, ext, ch0, S.vals.p);
+                               i0, ext, ch0, slots, S.vals.p);
             TD_HIP(hipGetLastError());
             gather_timer.end();
         }
