@@ -213,9 +213,11 @@ int tdbem_mrt_get_state(tdbem_mrt* m, int s, double* x, double* u, double* t);
 int tdbem_mrt_get_timings(const tdbem_mrt* m, double* init_ms, double* run_ms);
 int tdbem_mrt_destroy(tdbem_mrt* m);
 
-/* Instrumentation (no counterpart in the reference, which has no timers): when on, the lag
- * kernel, the store gather, the history convolution and the step solve are bracketed with HIP
- * events on the context stream.  tdbem_get_timings fills out[TDBEM_NUM_TIMINGS] in this order:
+/* Instrumentation (no counterpart in the reference, which has no timers): level
+ * TDBEM_PROFILE_ALL (any nonzero level other than TDBEM_PROFILE_CONV) brackets the lag kernel,
+ * the store gather, the history convolution, the reduction and the step solve with HIP events on
+ * the context stream and counts the lag kernel's active points; TDBEM_PROFILE_CONV brackets only
+ * the history convolution (two events per step instead of six), 0 turns it off.  tdbem_get_timings fills out[TDBEM_NUM_TIMINGS] in this order:
  * store_ms, init_ms, march_ms (host wall clock of build_lag_store / march_init / march_run),
  * lag_kernel_ms, gather_ms, conv_ms, solve_ms (summed event times), lag_launches,
  * conv_launches, active_points (quadrature-point x lag evaluations done by the lag kernel),
@@ -225,7 +227,9 @@ int tdbem_mrt_destroy(tdbem_mrt* m);
  * sub-tile), tile_entries (steps of the merged per-group lists).  conv_ms covers k_conv or
  * k_conv_mixed alone. */
 #define TDBEM_NUM_TIMINGS 16
-int tdbem_set_profiling(tdbem_context* ctx, int on);
+#define TDBEM_PROFILE_ALL 1
+#define TDBEM_PROFILE_CONV 2
+int tdbem_set_profiling(tdbem_context* ctx, int level);
 int tdbem_reset_timings(tdbem_context* ctx);
 int tdbem_get_timings(tdbem_context* ctx, double* out);
 

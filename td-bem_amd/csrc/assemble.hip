@@ -1008,8 +1008,9 @@ void fill_store(Context& c, const ShellPlan& P)
     TD_HIP(hipStreamSynchronize(c.stream));
     const int64_t slot_bytes = (int64_t)ENND * kVals * sizeof(double);
     const double ext = c.if_ext ? 0.5 * c.dt : 0.0;
-    KernelTimer lag_timer(c.stream, c.profile), gather_timer(c.stream, c.profile);
-    if (c.profile) {
+    const bool prof = c.profile == 1;
+    KernelTimer lag_timer(c.stream, prof), gather_timer(c.stream, prof);
+    if (prof) {
         c.counters.alloc(1);
         TD_HIP(hipMemsetAsync(c.counters.p, 0, sizeof(unsigned long long), c.stream));
     }
@@ -1098,7 +1099,7 @@ void fill_store(Context& c, const ShellPlan& P)
                                c.stream, c.coords.p, c.ien.p, c.geo.p, c.gauss.p, c.nq, c.nel,
                                c.dt, c.mat, c.Kc, i0, slots, slot_pair.p, pair_slot.p,
                                c.pair_k.p, sing_slots.p, (int64_t)0, scratch.p,
-                               c.profile ? c.counters.p : nullptr, c.precision == 1 ? 1 : 0);
+                               prof ? c.counters.p : nullptr, c.precision == 1 ? 1 : 0);
             TD_HIP(hipGetLastError());
             const int64_t nsing = (int64_t)hsing.size();
             if (nsing > 0) {
@@ -1106,7 +1107,7 @@ void fill_store(Context& c, const ShellPlan& P)
                                    dim3(kBlock), 0, c.stream, c.coords.p, c.ien.p, c.geo.p,
                                    c.gauss.p, c.nq, c.nel, c.dt, c.mat, c.Kc, i0, slots,
                                    slot_pair.p, pair_slot.p, c.pair_k.p, sing_slots.p, nsing,
-                                   scratch.p, c.profile ? c.counters.p : nullptr,
+                                   scratch.p, prof ? c.counters.p : nullptr,
                                    c.precision == 1 ? 1 : 0);
                 TD_HIP(hipGetLastError());
             }
@@ -1126,7 +1127,7 @@ void fill_store(Context& c, const ShellPlan& P)
     TD_HIP(hipStreamSynchronize(c.stream));
     TD_HIP(hipEventDestroy(uploaded[0]));
     TD_HIP(hipEventDestroy(uploaded[1]));
-    if (c.profile) {
+    if (prof) {
         c.tm.lag_launches += lag_timer.count();
         c.tm.lag_kernel_ms += lag_timer.collect();
         c.tm.gather_ms += gather_timer.collect();

@@ -566,9 +566,11 @@ int tdbem_stream_copy(size_t bytes, int launches, double* avg_ms)
     }
 }
 
-int tdbem_set_profiling(tdbem_context* ctx, int on)
+int tdbem_set_profiling(tdbem_context* ctx, int level)
 {
-    return guarded(ctx, [&](Context& c) { c.profile = on != 0; });
+    return guarded(ctx, [&](Context& c) {
+        c.profile = level == 0 ? 0 : (level == TDBEM_PROFILE_CONV ? 2 : 1);
+    });
 }
 
 int tdbem_reset_timings(tdbem_context* ctx)

@@ -525,8 +525,13 @@ void march_run(Context& c, int nsteps, bool wait)
     const int64_t tot = (int64_t)N * M.nrhs;
     const auto t0 = std::chrono::steady_clock::now();
     // Without a wait the steps are only queued; per-kernel timing needs the wait.
-    KernelTimer conv_timer(c.stream, c.profile && wait), solve_timer(c.stream, c.profile && wait),
-        reduce_timer(c.stream, c.profile && wait && M.precision == 1);
+    KernelTimer conv_timer(c.stream, c.profile && wait),
+        solve_timer(c.stream, c.profile == 1 && wait),
+        reduce_timer(c.stream, c.profile == 1 && wait && M.precision == 1);
+    const size_t steps = (size_t)std::max(0, std::min(nsteps, c.numstep - 1 - M.s));
+    conv_timer.reserve(steps);
+    solve_timer.reserve(steps);
+    reduce_timer.reserve(steps);
     for (int it = 0; it < nsteps; it++) {
         if (M.s + 1 >= c.numstep) break;  // histories hold x_0 .. x_{numstep-1}
         const int s = ++M.s;

@@ -118,33 +118,51 @@ struct KernelTimer {
     hipStream_t s = nullptr;
     bool on = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    size_t used = 0;   // pairs recorded since the last collect; the rest are kept for reuse
     KernelTimer(hipStream_t stream, bool enabled) : s(stream), on(enabled) {}
-    ~KernelTimer() { collect(); }
-    void begin()
+    ~KernelTimer()
     {
-        if (!on) return;
-        hipEvent_t a, b;
-        TD_HIP(hipEventCreate(&a));
-        TD_HIP(hipEventCreate(&b));
-        TD_HIP(hipEventRecord(a, s));
-        ev.emplace_back(a, b);
-    }
-    void end()
-    {
-        if (on && !ev.empty()) TD_HIP(hipEventRecord(ev.back().second, s));
-    }
-    int64_t count() const { return (int64_t)ev.size(); }
-    double collect()
-    {
-        double ms = 0;
-        if (!ev.empty()) (void)hipEventSynchronize(ev.back().second);
+        collect();
         for (auto& p : ev) {
-            float t = 0;
-            if (hipEventElapsedTime(&t, p.first, p.second) == hipSuccess) ms += t;
             (void)hipEventDestroy(p.first);
             (void)hipEventDestroy(p.second);
         }
-        ev.clear();
+    }
+    // creates the events of n launches up front, outside the launch sequence
+    void reserve(size_t n)
+    {
+        while (on && ev.size() < n) {
+            hipEvent_t a, b;
+            TD_HIP(hipEventCreate(&a));
+            TD_HIP(hipEventCreate(&b));
+            ev.emplace_back(a, b);
+        }
+    }
+    void begin()
+    {
+        if (!on) return;
+        if (used == ev.size()) {
+            hipEvent_t a, b;
+            TD_HIP(hipEventCreate(&a));
+            TD_HIP(hipEventCreate(&b));
+            ev.emplace_back(a, b);
+        }
+        TD_HIP(hipEventRecord(ev[used++].first, s));
+    }
+    void end()
+    {
+        if (on && used > 0) TD_HIP(hipEventRecord(ev[used - 1].second, s));
+    }
+    int64_t count() const { return (int64_t)used; }
+    double collect()
+    {
+        double ms = 0;
+        if (used > 0) (void)hipEventSynchronize(ev[used - 1].second);
+        for (size_t i = 0; i < used; i++) {
+            float t = 0;
+            if (hipEventElapsedTime(&t, ev[i].first, ev[i].second) == hipSuccess) ms += t;
+        }
+        used = 0;
         return ms;
     }
 };
@@ -248,7 +266,7 @@ struct Context {
     struct March* march = nullptr;
 
     // Optional timing of the hot kernels with HIP events on the context stream.
-    bool profile = false;
+    int profile = 0;   // 0 off, 1 every timed kernel, 2 the history convolution only
     struct Timings {
         double store_ms = 0, init_ms = 0, march_ms = 0;  // host wall clock, synchronised
         double lag_kernel_ms = 0, gather_ms = 0, conv_ms = 0, solve_ms = 0, reduce_ms = 0;

@@ -22,6 +22,7 @@ PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 MRT_DIRECT, MRT_AVERAGING = 0, 1
 MRT_LOAD_PULSE, MRT_LOAD_RAMP = 0, 1
+PROFILE_ALL, PROFILE_CONV = 1, 2
 
 _lib = None
 
@@ -234,8 +235,11 @@ class Context:
                    "solve_ms", "lag_launches", "conv_launches", "active_points", "store_nonzero",
                    "store_padded", "tile_subtiles", "reduce_ms", "tile_blocks", "tile_entries")
 
-    def set_profiling(self, on=True):
-        self._check(lib().tdbem_set_profiling(self._h, int(bool(on))))
+    def set_profiling(self, on=True, conv_only=False):
+        """Event timing of the kernels (tdbem_set_profiling); conv_only times the history
+        convolution alone (two events per step, no store-build timers or point counts)."""
+        level = 0 if not on else (PROFILE_CONV if conv_only else PROFILE_ALL)
+        self._check(lib().tdbem_set_profiling(self._h, level))
 
     def reset_timings(self):
         self._check(lib().tdbem_reset_timings(self._h))
