@@ -223,7 +223,7 @@ int tdbem_mrt_destroy(tdbem_mrt* m);
  * conv_launches, active_points (quadrature-point x lag evaluations done by the lag kernel),
  * store_nonzero (stored (i, j, k) blocks before padding), store_padded, tile_subtiles (16 x 32
  * split-BF16 sub-tiles of the mixed-precision store, 0 until built), reduce_ms (the mixed path's
- * reduction of partial sums, summed event times), tile_blocks (48 x 96 blocks with a stored
+ * reduction of partial sums into the step solve's right-hand side, summed event times), tile_blocks (48 x 96 blocks with a stored
  * sub-tile), tile_entries (steps of the merged per-group lists).  conv_ms covers k_conv or
  * k_conv_mixed alone. */
 #define TDBEM_NUM_TIMINGS 16

@@ -1,8 +1,8 @@
 // batch.hip — many independent time-marching scenarios (uneven DOF and step counts) stepped
 // together.  Each scenario is a context with a mixed-precision march; one step of every scenario
-// is seven launches in total, whatever the number of scenarios:
-//   advance + load factors, FP32 history copy, MFMA convolution, reduction, right-hand
-//   sides, step solve, finalize,
+// is six launches in total, whatever the number of scenarios:
+//   advance + load factors, FP32 history copy, MFMA convolution, reduction into the
+//   right-hand sides, step solve, finalize,
 // each launch covering all scenarios through a table of per-scenario descriptors and a device
 // step counter per scenario (scenarios past their last step drop out).  A run captures
 // `steps_per_graph` steps of this sequence into one hipGraph and replays it, so one graph launch
@@ -93,26 +93,18 @@ __global__ void __launch_bounds__(64 * kGroup, NT <= 2 ? 2 : 1)
                    partial + (((int64_t)c * kGroup + w) * 3 * NT) * 256);
 }
 
-__global__ void k_b_reduce(const BatchScen* __restrict__ D, const int* __restrict__ off, int S,
-                           const int* __restrict__ act, const double* __restrict__ partial,
-                           const int* __restrict__ gch)
+// the reduction of the partial sums, written as the step solve's right-hand side
+__global__ void k_b_reduce_rhs(const BatchScen* __restrict__ D, const int* __restrict__ off,
+                               int S, const int* __restrict__ act,
+                               const double* __restrict__ partial, const int* __restrict__ gch)
 {
     const int sc = scen_of(off, S, blockIdx.x);
     if (!act[sc]) return;
     const BatchScen& d = D[sc];
     const int64_t t = (int64_t)(blockIdx.x - off[sc]) * kThreads + threadIdx.x;
     if (t >= (int64_t)d.nT * kRows * d.nrhs) return;
-    reduce_elem(partial, gch + d.gch0, d.nrhs / 16, d.perm, d.N, d.nrhs, d.hist, t);
-}
-
-__global__ void k_b_rhs(const BatchScen* __restrict__ D, const int* __restrict__ off, int S,
-                        const int* __restrict__ act)
-{
-    const int sc = scen_of(off, S, blockIdx.x);
-    if (!act[sc]) return;
-    const BatchScen& d = D[sc];
-    const int64_t t = (int64_t)(blockIdx.x - off[sc]) * kThreads + threadIdx.x;
-    if (t < (int64_t)d.N * d.nrhs) rhs_elem(d.gF, d.gU, d.hist, d.fac, d.N, d.Xf, t);
+    reduce_rhs_elem(partial, gch + d.gch0, d.nrhs / 16, d.perm, d.N, d.nrhs, d.gF, d.gU, d.fac,
+                    d.Xf, t);
 }
 
 template <int NT>
@@ -257,10 +249,8 @@ void launch_step(Batch& B)
     hipLaunchKernelGGL(k_b_zcopy, dim3(B.nb_z), dim3(kThreads), 0, B.stream, B.desc.p,
                        B.off_z.p, B.S, B.sdev.p, B.act.p);
     launch_conv(B);
-    hipLaunchKernelGGL(k_b_reduce, dim3(B.nb_r), dim3(kThreads), 0, B.stream, B.desc.p,
+    hipLaunchKernelGGL(k_b_reduce_rhs, dim3(B.nb_r), dim3(kThreads), 0, B.stream, B.desc.p,
                        B.off_r.p, B.S, B.act.p, B.partial.p, B.gch.p);
-    hipLaunchKernelGGL(k_b_rhs, dim3(B.nb_rhs), dim3(kThreads), 0, B.stream, B.desc.p,
-                       B.off_rhs.p, B.S, B.act.p);
     launch_apply(B);
     hipLaunchKernelGGL(k_b_finalize, dim3(B.nb_rhs), dim3(kThreads), 0, B.stream, B.desc.p,
                        B.off_rhs.p, B.S, B.sdev.p, B.act.p);

@@ -456,7 +456,7 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
     }
     M.ring.alloc((size_t)M.R * nrhs * 2 * N);
     TD_HIP(hipMemsetAsync(M.ring.p, 0, M.ring.bytes(), c.stream));
-    M.hist.alloc((size_t)nrhs * N);
+    if (M.precision != 1) M.hist.alloc((size_t)nrhs * N);   // the mixed path sums into Xf
     M.Np = frag_rows(N);
     M.nsplit = apply_splits(N);
     M.Xf.alloc((size_t)frag_elems(N, nrhs));
@@ -484,7 +484,6 @@ void march_describe(Context& c, BatchScen& d)
         throw std::logic_error("batch: every context needs a mixed-precision march_init");
     March& M = *c.march;
     d.ring = M.ring.p;
-    d.hist = M.hist.p;
     d.gF = M.gF.p;
     d.gU = M.gU.p;
     d.Xf = M.Xf.p;
@@ -535,32 +534,32 @@ void march_run(Context& c, int nsteps, bool wait)
     for (int it = 0; it < nsteps; it++) {
         if (M.s + 1 >= c.numstep) break;  // histories hold x_0 .. x_{numstep-1}
         const int s = ++M.s;
-        hipLaunchKernelGGL(k_fac, dim3(1), dim3(64), 0, c.stream, M.d_amp.p, M.d_rise.p,
-                           s * c.dt, M.nrhs, M.fac.p);
+        const bool mixed = M.precision == 1 && s >= 2;
+        if (!mixed)
+            hipLaunchKernelGGL(k_fac, dim3(1), dim3(64), 0, c.stream, M.d_amp.p, M.d_rise.p,
+                               s * c.dt, M.nrhs, M.fac.p);
         if (s == 1) {
             hipLaunchKernelGGL(k_first, dim3(blocks(tot)), dim3(256), 0, c.stream, M.wF.p, M.wU.p,
                                M.fac.p, N, M.Np, M.nrhs, M.parts.p);
         } else {
-            if (M.precision == 1) {
-                // the previous finalize changed ring slots s-2, s-1 and s
-                for (int m = s - 2; m <= s; m++) {
-                    const int sl = ((m % M.R) + M.R) % M.R;
-                    // k_zcopy adds the slot offset itself: pass the ring base
-                    tp_zcopy(c, M.ring.p, N, M.nrhs, sl, M.zs.p);
-                }
+            if (mixed) {
+                // per step: FP32 history copy + load factors, convolution, reduction into the
+                // right-hand side, step solve, finalize
+                tp_prepare_step(c, M.ring.p, N, M.nrhs, s, M.R, M.zs.p, M.d_amp.p, M.d_rise.p,
+                                s * c.dt, M.fac.p);
                 conv_timer.begin();
                 tp_conv(c, M.zs.p, M.nrhs, M.R, s, M.partial.p);
                 conv_timer.end();
                 reduce_timer.begin();
-                tp_reduce(c, M.partial.p, M.nrhs, M.hist.p, N);
+                tp_reduce_rhs(c, M.partial.p, M.nrhs, N, M.gF.p, M.gU.p, M.fac.p, M.Xf.p);
                 reduce_timer.end();
             } else {
                 conv_timer.begin();
                 conv_fp64(c, M.ring.p, M.R, s, M.nrhs, M.hist.p, c.stream);
                 conv_timer.end();
+                hipLaunchKernelGGL(k_rhs, dim3(blocks(tot)), dim3(256), 0, c.stream, M.gF.p,
+                                   M.gU.p, M.hist.p, M.fac.p, N, M.nrhs, M.Xf.p);
             }
-            hipLaunchKernelGGL(k_rhs, dim3(blocks(tot)), dim3(256), 0, c.stream, M.gF.p, M.gU.p,
-                               M.hist.p, M.fac.p, N, M.nrhs, M.Xf.p);
             solve_timer.begin();
             apply_frag(M.Af.p, N, M.Xf.p, M.nrhs, M.parts.p, c.stream);
             solve_timer.end();

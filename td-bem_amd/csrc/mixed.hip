@@ -91,13 +91,23 @@ __global__ void __launch_bounds__(256)
     for (int h = 0; h < 2; h++) tp[(sidx * 2 + h) * 64 + lane] = out[h];
 }
 
-// FP32 copy of ring slot `slot` (see zcopy_elem).
-__global__ void k_zcopy(const double* __restrict__ ring, int N, int nrhs, int nT,
-                        const int* __restrict__ perm, int slot, int NT, float* __restrict__ zs)
+// FP32 copies (see zcopy_elem) of the ring slots of steps s - 2, s - 1, s (the ones the
+// previous step's finalize changed), and the load factors f_r(t) of the step in block 0.
+__global__ void k_zcopy3(const double* __restrict__ ring, int N, int nrhs, int nT,
+                         const int* __restrict__ perm, int s, int R, int NT,
+                         float* __restrict__ zs, const double* __restrict__ amp,
+                         const double* __restrict__ rise, double t_load,
+                         double* __restrict__ fac)
 {
+    if (blockIdx.x == 0 && (int)threadIdx.x < nrhs)
+        fac[threadIdx.x] = load_factor(amp[threadIdx.x], rise[threadIdx.x], t_load);
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nT * kCols * nrhs) return;
-    zcopy_elem(ring, N, nrhs, nT, perm, slot, NT, zs, t);
+    const int64_t per = (int64_t)nT * kCols * nrhs;
+    if (t >= 3 * per) return;
+    const int m = (int)(t / per);
+    int slot = (s - 2 + m) % R;
+    if (slot < 0) slot += R;
+    zcopy_elem(ring, N, nrhs, nT, perm, slot, NT, zs, t - m * per);
 }
 
 // One workgroup per chunk of a group's merged list, one wavefront per row tile (conv_chunk).
@@ -115,15 +125,16 @@ __global__ void __launch_bounds__(64 * kGroup, NT <= 2 ? 2 : 1)
                    partial + (((int64_t)c * kGroup + w) * 3 * NT) * 256);
 }
 
-// hist[r][dof] = sum over the chunks of a row tile's group (see reduce_elem).
-__global__ void k_tp_reduce(const double* __restrict__ partial, int nT,
-                            const int* __restrict__ grp_chunk, int NT,
-                            const int* __restrict__ perm, int N, int nrhs,
-                            double* __restrict__ hist)
+// Step solve right-hand side from the partial sums of the history (see reduce_rhs_elem).
+__global__ void k_tp_reduce_rhs(const double* __restrict__ partial, int nT,
+                                const int* __restrict__ grp_chunk, int NT,
+                                const int* __restrict__ perm, int N, int nrhs,
+                                const double* __restrict__ gF, const double* __restrict__ gU,
+                                const double* __restrict__ fac, double* __restrict__ Xf)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nT * kRows * nrhs) return;
-    reduce_elem(partial, grp_chunk, NT, perm, N, nrhs, hist, t);
+    reduce_rhs_elem(partial, grp_chunk, NT, perm, N, nrhs, gF, gU, fac, Xf, t);
 }
 
 // One 16x16x32 bf16 product with the lane layouts assumed above, for the layout self-test.
@@ -345,12 +356,13 @@ void tp_build(Context& c)
     T.ready = true;
 }
 
-void tp_zcopy(Context& c, const double* ring, int N, int nrhs, int slot, float* zs)
+void tp_prepare_step(Context& c, const double* ring, int N, int nrhs, int s, int R, float* zs,
+                     const double* amp, const double* rise, double t_load, double* fac)
 {
     const TPStore& T = c.tp;
-    const int64_t n = (int64_t)T.nT * kCols * nrhs;
-    hipLaunchKernelGGL(k_zcopy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, ring,
-                       N, nrhs, T.nT, T.d_perm.p, slot, nrhs / 16, zs);
+    const int64_t n = 3LL * T.nT * kCols * nrhs;
+    hipLaunchKernelGGL(k_zcopy3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, ring,
+                       N, nrhs, T.nT, T.d_perm.p, s, R, nrhs / 16, zs, amp, rise, t_load, fac);
     TD_HIP(hipGetLastError());
 }
 
@@ -445,13 +457,15 @@ void tp_conv(Context& c, const float* zs, int nrhs, int Rring, int s, double* pa
     TD_HIP(hipGetLastError());
 }
 
-void tp_reduce(Context& c, const double* partial, int nrhs, double* hist, int N)
+void tp_reduce_rhs(Context& c, const double* partial, int nrhs, int N, const double* gF,
+                   const double* gU, const double* fac, double* Xf)
 {
     const TPStore& T = c.tp;
     const int NT = nrhs / 16;
     const int64_t n = (int64_t)T.nT * kRows * nrhs;
-    hipLaunchKernelGGL(k_tp_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream,
-                       partial, T.nT, T.grp_chunk.p, NT, T.d_perm.p, N, nrhs, hist);
+    hipLaunchKernelGGL(k_tp_reduce_rhs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       c.stream, partial, T.nT, T.grp_chunk.p, NT, T.d_perm.p, N, nrhs, gF, gU,
+                       fac, Xf);
     TD_HIP(hipGetLastError());
 }
 

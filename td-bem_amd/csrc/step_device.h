@@ -236,27 +236,46 @@ __device__ __forceinline__ void conv_chunk(const f32x4* __restrict__ tp,
 // tile's group; partial of chunk c, wave w: [c][w][3][NT][64][4], 16x16 output fragment
 // lane = col + 16 * (row / 4), element row % 4.  t runs through a row tile's partials in that
 // order, so the reads of consecutive threads are contiguous.  grp_chunk: first chunk of each
-// group.
-__device__ __forceinline__ void reduce_elem(const double* __restrict__ partial,
-                                            const int* __restrict__ grp_chunk, int NT,
-                                            const int* __restrict__ perm, int N, int nrhs,
-                                            double* __restrict__ hist, int64_t t)
+// group.  Returns false for the padding rows of the last tile (no dof).
+__device__ __forceinline__ bool reduce_sum(const double* __restrict__ partial,
+                                           const int* __restrict__ grp_chunk, int NT,
+                                           const int* __restrict__ perm, int nrhs, int64_t t,
+                                           int& r, int& dof, double& sum)
 {
     const int per = kRows * nrhs;   // = 3 NT 256
     const int I = (int)(t / per), o = (int)(t % per);
     const int v = o & 3, lane = (o >> 2) & 63, mn = o >> 8;
     const int m = mn / NT, nt = mn % NT;
-    const int row = 16 * m + 4 * (lane >> 4) + v, r = 16 * nt + (lane & 15);
+    const int row = 16 * m + 4 * (lane >> 4) + v;
+    r = 16 * nt + (lane & 15);
     const int node = perm[I * kTile + row / 3];
-    if (node < 0) return;
+    if (node < 0) return false;
+    dof = 3 * node + row % 3;
     const int g = I / kGroup, w = I % kGroup;
-    double sum = 0.0;
+    double acc = 0.0;
     for (int c = grp_chunk[g]; c < grp_chunk[g + 1]; c++)
-        sum += partial[((int64_t)c * kGroup + w) * per + o];
-    hist[(int64_t)r * N + 3 * node + row % 3] = sum;
+        acc += partial[((int64_t)c * kGroup + w) * per + o];
+    sum = acc;
+    return true;
+}
+
+__device__ __forceinline__ void reduce_elem(const double* __restrict__ partial,
+                                            const int* __restrict__ grp_chunk, int NT,
+                                            const int* __restrict__ perm, int N, int nrhs,
+                                            double* __restrict__ hist, int64_t t)
+{
+    int r, d;
+    double sum;
+    if (reduce_sum(partial, grp_chunk, NT, perm, nrhs, t, r, d, sum))
+        hist[(int64_t)r * N + d] = sum;
 }
 
 // Element t (< N * nrhs): r = f_r gF + gU + hist, written in the fragment order of the step solve.
+__device__ __forceinline__ double rhs_value(double fac, double gF, double gU, double hist)
+{
+    return fac * gF + gU + hist;
+}
+
 __device__ __forceinline__ void rhs_elem(const double* __restrict__ gF,
                                          const double* __restrict__ gU,
                                          const double* __restrict__ hist,
@@ -264,7 +283,23 @@ __device__ __forceinline__ void rhs_elem(const double* __restrict__ gF,
                                          double* __restrict__ Xf, int64_t t)
 {
     const int r = (int)(t / N), d = (int)(t % N);
-    Xf[frag_index(N, r, d)] = fac[r] * gF[d] + gU[d] + hist[t];
+    Xf[frag_index(N, r, d)] = rhs_value(fac[r], gF[d], gU[d], hist[t]);
+}
+
+// reduce_elem and rhs_elem in one pass: the summed history goes straight into the step solve's
+// right-hand side (every dof of every rhs is one element t of the reduction).
+__device__ __forceinline__ void reduce_rhs_elem(const double* __restrict__ partial,
+                                                const int* __restrict__ grp_chunk, int NT,
+                                                const int* __restrict__ perm, int N, int nrhs,
+                                                const double* __restrict__ gF,
+                                                const double* __restrict__ gU,
+                                                const double* __restrict__ fac,
+                                                double* __restrict__ Xf, int64_t t)
+{
+    int r, d;
+    double sum;
+    if (reduce_sum(partial, grp_chunk, NT, perm, nrhs, t, r, d, sum))
+        Xf[frag_index(N, r, d)] = rhs_value(fac[r], gF[d], gU[d], sum);
 }
 
 // Wavefront g (< frag_rows(n) / 16 * nsplit) of parts[p][r][i] = sum over the columns j of

@@ -321,7 +321,6 @@ void mrt_timings(const Mrt& M, double* init_ms, double* run_ms);
 struct BatchScen {
     // march (march.hip)
     double* ring;
-    double* hist;
     const double* gF;
     const double* gU;
     double* Xf;
@@ -362,8 +361,10 @@ int batch_run(Batch& b, int steps_per_graph, int max_replays);
 
 // mixed.hip
 void tp_build(Context& c);
-// FP32 copy of ring slot `slot` in the B-operand order of the mixed convolution
-void tp_zcopy(Context& c, const double* ring, int N, int nrhs, int slot, float* zs);
+// start of step s of the mixed march (one launch): FP32 copies of ring slots s - 2 .. s in the
+// B-operand order of the mixed convolution, and the load factors fac[r] = f_r(t_load)
+void tp_prepare_step(Context& c, const double* ring, int N, int nrhs, int s, int R, float* zs,
+                     const double* amp, const double* rise, double t_load, double* fac);
 int64_t tp_zslot_elems(const Context& c, int nrhs);
 // plans the chunks of k_conv_mixed for nrhs right-hand sides (after tp_build)
 void tp_plan(Context& c, int nrhs, int rounds = 4);
@@ -377,8 +378,10 @@ double tp_work(const Context& c);
 int64_t tp_partial_elems(const Context& c, int nrhs);
 // k_conv_mixed: per-wave FP64 partial sums of the history at step s
 void tp_conv(Context& c, const float* zs, int nrhs, int Rring, int s, double* partial);
-// fixed-order sum of the partials into hist[nrhs][N]
-void tp_reduce(Context& c, const double* partial, int nrhs, double* hist, int N);
+// fixed-order sum of the partials, written as the step solve's right-hand side
+// Xf = f_r gF + gU + (history sum) in fragment order
+void tp_reduce_rhs(Context& c, const double* partial, int nrhs, int N, const double* gF,
+                   const double* gU, const double* fac, double* Xf);
 void mfma_selftest(const float* A, const float* B, float* D);
 double stream_copy(size_t bytes, int launches);
 double stream_read(size_t bytes, int launches);
