@@ -905,7 +905,7 @@ void assemble_step_dense(Context& c, int step, int lower, int upper, double* dG1
     const double ext = (c.if_ext && step == 1) ? 0.5 * c.dt : 0.0;
     const int64_t per_node = (int64_t)c.nel * 2 * c.ennd * kVals * sizeof(double);
     const int rows_per_batch = (int)std::max<int64_t>(1, kScratchBytes / per_node);
-    DBuf<double> scratch;
+    DBuf<double> scratch(c.stream);
     scratch.alloc((size_t)std::min(rows_per_batch, upper - lower + 1) * c.nel * 2 * c.ennd *
                   kVals);
     const int64_t ncol = 3LL * c.nnd;
@@ -950,7 +950,7 @@ void plan_shells(Context& c, ShellPlan& P)
 {
     compute_pair_ranges(c);
     const int64_t nn = (int64_t)c.nnd * c.nnd;
-    DBuf<int2> hull;
+    DBuf<int2> hull(c.stream);
     hull.alloc(nn);
     hipLaunchKernelGGL(k_node_hull, dim3(grid_for(nn)), dim3(kBlock), 0, c.stream, c.pair_k.p,
                        c.adj_off.p, c.adj.p, c.nnd, c.nel, hull.p);
@@ -1039,16 +1039,20 @@ void fill_store(Context& c, const ShellPlan& P)
         max_slots = std::max(max_slots, slots);
         i0 = i1;
     }
-    DBuf<double> scratch;
-    DBuf<int> slot_pair;
-    DBuf<int64_t> pair_slot;
+    DBuf<double> scratch(c.stream);
+    DBuf<int> slot_pair(c.stream);
+    DBuf<int64_t> pair_slot(c.stream);
     scratch.reserve((size_t)max_slots * ENND * kVals);
+    int max_rows = 1;
+    for (const Batch& bt : batches) max_rows = std::max(max_rows, bt.i1 - bt.i0);
+    slot_pair.reserve((size_t)max_slots);
+    pair_slot.reserve((size_t)max_rows * c.nel);
     // The host builds batch b + 1's slot lists while the device works on batch b: two host list
     // sets, each reused only after the event that follows its upload; the device buffers are
     // rewritten in stream order after the previous batch's kernels.
     std::vector<int64_t> h_pair_slot[2], h_sing[2];
     std::vector<int> h_slot_pair[2];
-    DBuf<int64_t> sing_slots;
+    DBuf<int64_t> sing_slots(c.stream);
     hipEvent_t uploaded[2];
     TD_HIP(hipEventCreateWithFlags(&uploaded[0], hipEventDisableTiming));
     TD_HIP(hipEventCreateWithFlags(&uploaded[1], hipEventDisableTiming));

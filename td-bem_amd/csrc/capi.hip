@@ -2,6 +2,8 @@
 #include <cmath>
 #include <memory>
 #include <cstring>
+#include <mutex>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -40,6 +42,21 @@ Context::~Context()
 namespace {
 
 thread_local std::string g_create_error;
+
+// Stream-ordered temporaries (DBuf with a stream) come from the device's default memory pool;
+// it keeps up to 16 GiB of freed memory between uses instead of returning it at every
+// synchronisation, so the next set-up reuses it without mapping memory again.
+void keep_pool_memory(int device)
+{
+    static std::mutex mu;
+    static std::set<int> done;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!done.insert(device).second) return;
+    hipMemPool_t pool = nullptr;
+    TD_HIP(hipDeviceGetDefaultMemPool(&pool, device));
+    uint64_t keep = 16ull << 30;
+    TD_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+}
 
 // Gauss tables of the shape-function quadrature (src/ShapeFunctionT.cpp:83-163).  The 10- and
 // 20-point rows carry the reference's 9 digits, and the 19th 20-point abscissa keeps the
@@ -286,6 +303,7 @@ int tdbem_create(int nnd, const double* coords, int nel, const int* ien, int ety
         Context& c = ctx->c;
         TD_HIP(hipGetDevice(&c.device));
         TD_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        keep_pool_memory(c.device);
         c.nnd = nnd;
         c.nel = nel;
         c.etype = etype;

@@ -587,7 +587,7 @@ __global__ void __launch_bounds__(64)
 // Returns false (A partly overwritten) when some multiplier has |l| > 1 or is not finite.
 bool lu_nopiv(double* A, int n, hipStream_t s)
 {
-    DBuf<unsigned> bad;
+    DBuf<unsigned> bad(s);
     bad.alloc(1);
     TD_HIP(hipMemsetAsync(bad.p, 0, sizeof(unsigned), s));
     for (int j0 = 0; j0 < n; j0 += NB) {
@@ -624,7 +624,7 @@ void lu_factor(double* A, int n, int* piv, hipStream_t s)
 {
     {
         // without interchanges first (see lu_nopiv); the copy restores A when a check fails
-        DBuf<double> keep;
+        DBuf<double> keep(s);
         keep.alloc((size_t)n * n);
         TD_HIP(hipMemcpyAsync(keep.p, A, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, s));
         if (lu_nopiv(A, n, s)) {
@@ -639,7 +639,7 @@ void lu_factor(double* A, int n, int* piv, hipStream_t s)
 
 void lu_factor_pivoted(double* A, int n, int* piv, hipStream_t s)
 {
-    DBuf<int> status;
+    DBuf<int> status(s);
     status.alloc(1);
     TD_HIP(hipMemsetAsync(status.p, 0, sizeof(int), s));
     for (int j0 = 0; j0 < n; j0 += NB) {
@@ -670,8 +670,8 @@ void lu_factor_pivoted(double* A, int n, int* piv, hipStream_t s)
 void lu_solve(const double* LU, int n, const int* piv, double* B, int ncols, hipStream_t s)
 {
     std::vector<int> perm = host_permutation(piv, n, s);
-    DBuf<int> dperm;
-    DBuf<double> tmp;
+    DBuf<int> dperm(s);
+    DBuf<double> tmp(s);
     dperm.upload(perm.data(), n, s);
     tmp.alloc((size_t)n * ncols);
     const int64_t tot = (int64_t)n * ncols;
@@ -686,7 +686,7 @@ void lu_solve(const double* LU, int n, const int* piv, double* B, int ncols, hip
 void lu_inverse(const double* LU, int n, const int* piv, double* Ainv, hipStream_t s)
 {
     std::vector<int> perm = host_permutation(piv, n, s);
-    DBuf<int> dperm;
+    DBuf<int> dperm(s);
     dperm.upload(perm.data(), n, s);
     const int64_t tot = (int64_t)n * n;
     hipLaunchKernelGGL(k_perm_identity, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,

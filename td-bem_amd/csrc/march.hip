@@ -255,7 +255,7 @@ bool refine_first(const double* A2, int N, const double* Af, double c0, const do
 {
     const int Np = frag_rows(N), nsplit = apply_splits(N);
     const int64_t n2 = 2LL * N;
-    DBuf<double> Xf, parts, r, ax;
+    DBuf<double> Xf(s), parts(s), r(s), ax(s);
     Xf.alloc((size_t)frag_elems(N, 2));
     parts.alloc((size_t)nsplit * 2 * Np);
     r.alloc(n2);
@@ -377,12 +377,13 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
     M.is_fbc.upload(isf.data(), N, c.stream);
     M.ubc_val.upload(uv.data(), N, c.stream);
     M.fbc_val.upload(fv.data(), N, c.stream);
-    DBuf<double> dyF, dyU;
+    DBuf<double> dyF(c.stream), dyU(c.stream);
     dyF.upload(yF.data(), N, c.stream);
     dyU.upload(yU.data(), N, c.stream);
 
     const int64_t NN = (int64_t)N * N;
-    DBuf<double> G0, H0, G1, H1, A, Gc, Hc;
+    DBuf<double> G0(c.stream), H0(c.stream), G1(c.stream), H1(c.stream), A(c.stream),
+        Gc(c.stream), Hc(c.stream);
     G0.alloc(NN);
     H0.alloc(NN);
     G1.alloc(NN);
@@ -392,7 +393,7 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
     Hc.alloc(NN);
     lag_dense(c, 0, G0.p, H0.p);
     lag_dense(c, 1, G1.p, H1.p);
-    DBuf<int> piv;
+    DBuf<int> piv(c.stream);
     piv.alloc(N);
     M.gF.alloc(N);
     M.gU.alloc(N);
@@ -421,7 +422,7 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
                        G1.p, H1.p, Hc.p);
     TD_HIP(hipGetLastError());
     {
-        DBuf<double> b2, x2;
+        DBuf<double> b2(c.stream), x2(c.stream);
         b2.alloc((size_t)2 * N);
         x2.alloc((size_t)2 * N);
         gemv(H1.p, N, dyF.p, b2.p, 0.0, c.stream);
@@ -433,7 +434,7 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
             TD_HIP(hipMemcpyAsync(M.wU.p, x2.p + N, N * sizeof(double), hipMemcpyDeviceToDevice,
                                   c.stream));
         } else {
-            DBuf<double> B;
+            DBuf<double> B(c.stream);
             B.alloc((size_t)2 * N);
             hipLaunchKernelGGL(k_pack2, dim3(blocks(N)), dim3(256), 0, c.stream, b2.p, b2.p + N,
                                N, B.p);

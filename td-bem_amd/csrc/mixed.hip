@@ -339,7 +339,7 @@ void tp_build(Context& c)
     T.d_pos.upload(T.pos.data(), T.pos.size(), c.stream);
     T.ent.upload(ent.data(), ent.size(), c.stream);
     T.vals.alloc((size_t)std::max<int64_t>(nsub, 1) * kSubFloats);
-    DBuf<uint2> d_sub_id;   // lives until the stream is synchronised below
+    DBuf<uint2> d_sub_id(c.stream);
     if (nsub > 0) {
         if (!c.store.d_hull.p || !c.store.d_off.p)
             throw std::logic_error("tile-pair copy needs the store's node-pair tables");

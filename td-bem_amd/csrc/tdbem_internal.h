@@ -18,19 +18,31 @@ namespace tdbem {
 void check_hip(hipError_t e, const char* what);
 #define TD_HIP(call) ::tdbem::check_hip((call), #call)
 
-// Device buffer owned by a context (freed by ~DBuf).
+// Device buffer owned by a context (freed by ~DBuf).  A buffer constructed with a stream is
+// stream-ordered: hipMallocAsync / hipFreeAsync on that stream (the device's memory pool, which
+// keep_pool_memory makes keep freed memory), for temporaries used on that stream only; hipFree
+// synchronises the whole device, so the temporaries of set-ups running on several streams at
+// once would otherwise wait for each other's kernels.
 template <typename T>
 struct DBuf {
     T* p = nullptr;
     size_t n = 0;
     size_t cap = 0;
+    hipStream_t ord = nullptr;
+    bool ordered = false;
     DBuf() = default;
+    explicit DBuf(hipStream_t s) : ord(s), ordered(true) {}
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
     ~DBuf() { reset(); }
     void reset()
     {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (ordered)
+                (void)hipFreeAsync(p, ord);
+            else
+                (void)hipFree(p);
+        }
         p = nullptr;
         n = cap = 0;
     }
@@ -39,7 +51,10 @@ struct DBuf {
         if (count == n && p) return;
         reset();
         if (count == 0) return;
-        TD_HIP(hipMalloc(&p, count * sizeof(T)));
+        if (ordered)
+            TD_HIP(hipMallocAsync((void**)&p, count * sizeof(T), ord));
+        else
+            TD_HIP(hipMalloc(&p, count * sizeof(T)));
         n = cap = count;
     }
     // n = count, keeping the allocation when it is large enough (no hipFree / hipMalloc, which
