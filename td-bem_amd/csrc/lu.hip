@@ -508,46 +508,47 @@ void panel_rec(double* A, int n, int c0, int w, int j0, int pnb, int* piv, int* 
 // every multiplier satisfies |l| <= 1.  The step-0 matrices of the march are of that kind (their
 // multipliers stay well below 1), so lu_factor first factors without interchanges, checking
 // |l| <= 1 for every multiplier, and falls back to the pivoted panels only when a check fails.
-// Per panel of NB columns: the NB x NB diagonal block in LDS (one wavefront), the rows below it
-// against U11 (thread per row), the block row right of it against L11, and the trailing GEMM.
+// Per panel of NB columns: the NB x NB diagonal block (one wavefront, rows in registers), the
+// rows below it against U11 (thread per row), the block row right of it against L11, and the
+// trailing GEMM.
 // ---------------------------------------------------------------------------------------------
 
-// Diagonal block A[j0 .. j0 + nb) x [j0 .. j0 + nb), unit-lower L and upper U in place, in LDS.
-// Four wavefronts: lane r of wavefront q updates row r in the columns 16 q .. 16 q + 15.
-__global__ void __launch_bounds__(256)
+// Diagonal block A[j0 .. j0 + nb) x [j0 .. j0 + nb), unit-lower L and upper U in place.  One
+// wavefront, lane r holding row r in registers (identity past nb); at step c lane c publishes
+// the pivot row through LDS and every lane below it eliminates column c.  The operations per
+// element are those of the textbook right-looking elimination, in the same order.
+__global__ void __launch_bounds__(64)
     k_diag_nopiv(double* __restrict__ A, int n, int j0, int nb, unsigned* __restrict__ bad)
 {
-    __shared__ double S[NB][NB + 1];
-    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
-        const int r = e / NB, c = e % NB;
-        S[r][c] = (r < nb && c < nb) ? A[(int64_t)(j0 + r) * n + j0 + c] : 0.0;
-    }
-    __syncthreads();
-    const int r = threadIdx.x & 63, q0 = (threadIdx.x >> 6) * 16;
-    bool ok = true;
-    for (int c = 0; c < nb; c++) {
-        double l = 0.0;
-        const bool act = r > c && r < nb;
-        if (act) {
-            l = S[r][c] / S[c][c];
-            ok = ok && fabs(l) <= 1.0;   // false for NaN as well
-        }
-        __syncthreads();                 // every S[r][c] read before the owner replaces it
-        if (act) {
+    __shared__ double P[NB];
+    const int r = threadIdx.x;
+    double* row = A + (int64_t)(j0 + r) * n + j0;
+    double a[NB];
 #pragma unroll
-            for (int jj = 0; jj < 16; jj++) {
-                const int j = q0 + jj;
-                if (j == c)
-                    S[r][j] = l;
-                else if (j > c)
-                    S[r][j] -= l * S[c][j];
+    for (int j = 0; j < NB; j++) a[j] = (r < nb && j < nb) ? row[j] : (r == j ? 1.0 : 0.0);
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < NB; c++) {
+        if (c < nb) {   // uniform
+            if (r == c) {
+#pragma unroll
+                for (int j = c; j < NB; j++) P[j] = a[j];
             }
+            __syncthreads();
+            if (r > c) {
+                const double l = a[c] / P[c];
+                ok = ok && fabs(l) <= 1.0;   // false for NaN as well
+                a[c] = l;
+#pragma unroll
+                for (int j = c + 1; j < NB; j++) a[j] -= l * P[j];
+            }
+            __syncthreads();   // P is rewritten by the next pivot lane
         }
-        __syncthreads();
     }
-    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
-        const int rr = e / NB, c = e % NB;
-        if (rr < nb && c < nb) A[(int64_t)(j0 + rr) * n + j0 + c] = S[rr][c];
+    if (r < nb) {
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+            if (j < nb) row[j] = a[j];
     }
     if (!ok) atomicOr(bad, 1u);
 }
@@ -592,7 +593,7 @@ bool lu_nopiv(double* A, int n, hipStream_t s)
     TD_HIP(hipMemsetAsync(bad.p, 0, sizeof(unsigned), s));
     for (int j0 = 0; j0 < n; j0 += NB) {
         const int nb = std::min(NB, n - j0);
-        hipLaunchKernelGGL(k_diag_nopiv, dim3(1), dim3(256), 0, s, A, n, j0, nb, bad.p);
+        hipLaunchKernelGGL(k_diag_nopiv, dim3(1), dim3(64), 0, s, A, n, j0, nb, bad.p);
         TD_HIP(hipGetLastError());
         const int rest = n - j0 - nb;
         if (rest <= 0) break;
