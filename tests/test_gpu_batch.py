@@ -1,7 +1,10 @@
 """Batched march (tdbem_batch_*): scenarios of uneven size and step count stepped together by one
 hipGraph of several steps.  Every graph replay, with eager work of another context and host
 synchronisation between replays, is checked against each scenario marched on its own, and the
-autotuner's choice leaves the results unchanged."""
+autotuner's choice leaves the results unchanged to 1e-12.  (The tunables change the order of FP64
+sums; the FP32 history copy can turn such last-bit differences into FP32 roundings of single
+values, which stay far below 1e-12 over these 18-37 steps and reach ~3e-8 after 600 sphere
+steps, against the mixed path's 3e-7 distance from FP64.)"""
 import numpy as np
 import pytest
 
