@@ -341,7 +341,7 @@ void batch_tune(Batch& B, double* report)
     for (int r : rounds_c) {
         B.rounds = r;
         batch_plan(B);
-        const double ms = time_launches(B, [&] { launch_conv(B); }, 3);
+        const double ms = time_launches(B, [&] { launch_conv(B); }, 2);
         if (ms < best_conv) {
             best_conv = ms;
             best_r = r;
@@ -351,7 +351,7 @@ void batch_tune(Batch& B, double* report)
     for (int w : waves_c) {
         B.apply_waves = w;
         batch_plan(B);
-        const double ms = time_launches(B, [&] { launch_apply(B); }, 3);
+        const double ms = time_launches(B, [&] { launch_apply(B); }, 2);
         if (ms < best_apply) {
             best_apply = ms;
             best_w = w;
