@@ -69,21 +69,29 @@ __global__ void __launch_bounds__(256)
     const int i = perm[I * kTile + r / 3], p = r % 3;
     const int sym[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
     f32x4 out[2];
+    // the lane's 8 columns belong to at most 3 column nodes: look each node pair up once
+    int last = -1;
+    int64_t en = -1;   // entry of (i, j, k), -1 when the pair has no lag k
 #pragma unroll
     for (int e = 0; e < 8; e++) {
         const int c = 32 * kt + 8 * (lane >> 4) + e;
-        const int j = perm[J * kTile + c / 6];
+        const int jj = c / 6;
         const int part = (c % 6) / 3, q = c % 3;
-        double v = 0.0;
-        if (i >= 0 && j >= 0) {
-            const int64_t pr = (int64_t)i * nnd + j;
-            const int2 h = hull[pr];
-            if (h.x <= k && k <= h.y) {
-                const int64_t en = off[pr] + (k - h.x);
-                const int comp = part == 0 ? sym[p][q] : 6 + 3 * p + q;
-                v = vals[((en / kChunk) * kVals + comp) * kChunk + en % kChunk];
-                if (part) v = -v;
+        if (jj != last) {
+            last = jj;
+            en = -1;
+            const int j = perm[J * kTile + jj];
+            if (i >= 0 && j >= 0) {
+                const int64_t pr = (int64_t)i * nnd + j;
+                const int2 h = hull[pr];
+                if (h.x <= k && k <= h.y) en = off[pr] + (k - h.x);
             }
+        }
+        double v = 0.0;
+        if (en >= 0) {
+            const int comp = part == 0 ? sym[p][q] : 6 + 3 * p + q;
+            v = vals[((en / kChunk) * kVals + comp) * kChunk + en % kChunk];
+            if (part) v = -v;
         }
         out[e >> 2][e & 3] = (float)v;
     }
