@@ -52,7 +52,9 @@ enum {
 int tdbem_create(int nnd, const double* coords, int nel, const int* ien, int etype, int nq,
                  double E, double nu, double rho, int if_ext, double dt, int numstep, int kernel,
                  tdbem_context** out);
-void tdbem_destroy(tdbem_context* ctx);
+/* Frees the context (TDBEM_OK for NULL).  A context that is a member of a batch is not freed:
+ * TDBEM_ERR_STATE until it has left the batch (tdbem_batch_remove / tdbem_batch_destroy). */
+int tdbem_destroy(tdbem_context* ctx);
 const char* tdbem_last_error(const tdbem_context* ctx);
 
 /* SolverT::GetMaxStep (src/SolverT.cpp:252-255). */
@@ -76,6 +78,10 @@ int tdbem_build_lag_store(tdbem_context* ctx);
 int64_t tdbem_lag_store_bytes(tdbem_context* ctx);
 /* Dense N x N (N = 3 nnd) lag k of G and H on the host: fG_DB[k], fH_DB[k]. */
 int tdbem_get_lag(tdbem_context* ctx, int k, double* G, double* H);
+/* The rows of collocation nodes lower..upper of lag k, [3 (upper - lower + 1)][N] each: one
+ * rank's row block of fG_DB[k], fH_DB[k] (rows fDofLow..fDofHigh, src/SolverT.cpp:128-140), as
+ * WriteGH writes them per rank (src/SolverT.cpp:670-732). */
+int tdbem_get_lag_rows(tdbem_context* ctx, int k, int lower, int upper, double* G, double* H);
 
 /* SolverStepLinearT::Initialize/SetLoad/Solve_Averaging set-up (src/SolverStepLinearT.cpp:30-56,
  * :314-393; src/SolverT.cpp:344-377): alpha (a1, a2), displacement (UBC) and traction (FBC)
@@ -107,8 +113,9 @@ int tdbem_get_state(tdbem_context* ctx, int rhs, int s, double* u, double* t);
  *   TDBEM_PRECISION_FP64 (default): the plain path, nrhs in {1, 2, 4, 8};
  *   TDBEM_PRECISION_MIXED_BF16X3: regular element pairs farther than two element diameters are
  *   integrated with FP32 time integrals and kernels (FP64 distances, FP64 sums); the lags are
- *   stored as three-term split BF16 16 x 32 sub-tiles, products on MFMA accumulated in FP32 per
- *   sub-tile and in FP64 across sub-tiles and lags, nrhs in {16, 32, 48, 64}.  Singular and
+ *   stored as FP32 16 x 32 sub-tiles (only those a node pair reaches), each value split exactly
+ *   into three BF16 terms in registers before its MFMA products, which are accumulated in FP32
+ *   per sub-tile and in FP64 across sub-tiles and lags, nrhs in {16, 32, 48, 64}.  Singular and
  *   near-singular pairs, the C-term, the step-0 LU/inverse and the step solve stay FP64.
  * Changing the precision discards a lag store built with the other one.  tdbem_assemble_step
  * is always FP64. */
@@ -150,16 +157,26 @@ int tdbem_synchronize(tdbem_context* ctx);
 
 /* Batched march (no counterpart in the reference, which runs one problem per process): many
  * independent contexts, each after a mixed-precision tdbem_march_init with the same nrhs, are
- * stepped together, one step of all of them per seven kernel launches, captured as a hipGraph of
+ * stepped together, one step of all of them per six kernel launches, captured as a hipGraph of
  * steps_per_graph steps that is replayed until every context reaches its own last step.  The
- * contexts must stay alive and must not be marched on their own while the batch exists; their
- * histories (tdbem_get_history) hold the results.  tdbem_batch_tune times the tunables (workgroup
- * rounds of the convolution, wavefronts of the step solve) on the batch and keeps the fastest;
- * report (may be NULL) receives {rounds, solve waves, convolution ms, solve ms}.
- * tdbem_batch_run returns the number of graph launches in *launches (may be NULL); max_replays
- * <= 0 runs to the end. */
+ * batch's members may change between runs (continuous batching): tdbem_batch_add admits a
+ * context (solving its first step on its own stream if it has not started), tdbem_batch_remove
+ * lets one leave; the next run or tune re-plans the work table and captures a new graph.  While
+ * a context is a member, the calls that would free or re-initialise what the batch reads
+ * (tdbem_destroy, tdbem_march_init / run / enqueue, tdbem_build_lag_store, tdbem_set_precision,
+ * tdbem_set_max_step, tdbem_mrt_create) return TDBEM_ERR_STATE; its histories
+ * (tdbem_get_history) hold the results.  tdbem_batch_create takes count >= 0 initial members.
+ * tdbem_batch_tune times the tunables (workgroup rounds of the convolution, wavefronts of the
+ * step solve) on the batch and keeps the fastest; report (may be NULL) receives {rounds, solve
+ * waves, convolution ms, solve ms}.  tdbem_batch_run returns the number of graph launches in
+ * *launches (may be NULL); max_replays <= 0 runs to the end; members that reach their last step
+ * stay members (idle) until removed. */
 typedef struct tdbem_batch tdbem_batch;
 int tdbem_batch_create(tdbem_context* const* ctxs, int count, tdbem_batch** out);
+int tdbem_batch_add(tdbem_batch* b, tdbem_context* ctx);
+int tdbem_batch_remove(tdbem_batch* b, tdbem_context* ctx);
+/* number of members */
+int tdbem_batch_size(const tdbem_batch* b);
 int tdbem_batch_tune(tdbem_batch* b, double* report);
 int tdbem_batch_set_params(tdbem_batch* b, int rounds, int solve_waves);
 int tdbem_batch_run(tdbem_batch* b, int steps_per_graph, int max_replays, int* launches);

@@ -1036,6 +1036,8 @@ int or_lu_factor(int n, double* A, int* piv)
                 A[(long)k * n + j] = A[(long)p * n + j];
                 A[(long)p * n + j] = tmp;
             }
+        /* rows are independent: each is updated by one thread in the same order as serially */
+#pragma omp parallel for private(j) schedule(static) if (n - k > 256)
         for (i = k + 1; i < n; i++) {
             double l = A[(long)i * n + k] / A[(long)k * n + k];
             double* ri = A + (long)i * n;
@@ -1081,6 +1083,8 @@ double or_load_factor(const or_march_cfg* c, double t)
 static void gemv_add(int N, double alpha, const double* A, const double* x, double* y)
 {
     int i, j;
+    /* one row per thread, each summed in the serial order */
+#pragma omp parallel for private(j) schedule(static) if (N > 256)
     for (i = 0; i < N; i++) {
         double s = 0;
         const double* row = A + (long)i * N;

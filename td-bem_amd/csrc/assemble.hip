@@ -20,7 +20,7 @@
 #include <cstring>
 #include <stdexcept>
 
-#include "tdbem_internal.h"
+#include "step_device.h"
 
 namespace tdbem {
 
@@ -151,57 +151,6 @@ __device__ __forceinline__ bool eval_s(const TimeCombo& tc, double r, double dt,
                               inv_cs * inv_cs, tc.cI[l], tc.cJ[l], S);
     }
     return true;
-}
-
-// eval_s with the time integrals in FP32 (accumulate_hl_f32); same zero tests, in FP64.
-__device__ __forceinline__ bool eval_s_f32(const TimeCombo& tc, double r, double dt,
-                                           const Material& m, float* S)
-{
-    const double inv_cp = 1.0 / m.cp, inv_cs = 1.0 / m.cs;
-    const double ta = r * inv_cp;
-    const double ts = r * inv_cs + dt;
-    bool any = false, all_static = true;
-#pragma unroll
-    for (int l = 0; l < 3; l++) {
-        if (tc.cI[l] == 0.0 && tc.cJ[l] == 0.0) continue;
-        if (tc.t[l] > ta) any = true;
-        if (tc.t[l] < ts) all_static = false;
-    }
-    if (!any || (tc.static_ok && all_static)) return false;
-    const float inv_r2 = (float)(1.0 / (r * r));
-    const float fdt = (float)dt, finv_dt = (float)(1.0 / dt);
-#pragma unroll
-    for (int l = 0; l < 6; l++) S[l] = 0.f;
-#pragma unroll
-    for (int l = 0; l < 3; l++) {
-        if ((tc.cI[l] == 0.0 && tc.cJ[l] == 0.0) || !(tc.t[l] > ta)) continue;
-        accumulate_hl_f32(tc.t[l], r, fdt, finv_dt, inv_r2, inv_cp, inv_cs, (float)tc.cI[l],
-                          (float)tc.cJ[l], S);
-    }
-    return true;
-}
-
-// A regular pair is integrated in FP32 (mixed precision) unless the collocation point lies
-// within two element diameters of the element's centroid ("near-singular"): there the kernels
-// vary fast across the element and FP64 is kept, as for singular pairs.
-template <int ENND>
-__device__ __forceinline__ bool near_singular(const double* X, const double* coll)
-{
-    double c[3] = {0.0, 0.0, 0.0}, d2 = 0.0;
-#pragma unroll
-    for (int a = 0; a < ENND; a++) {
-        c[0] += X[3 * a];
-        c[1] += X[3 * a + 1];
-        c[2] += X[3 * a + 2];
-#pragma unroll
-        for (int b = a + 1; b < ENND; b++) {
-            const double dx = X[3 * a] - X[3 * b], dy = X[3 * a + 1] - X[3 * b + 1],
-                         dz = X[3 * a + 2] - X[3 * b + 2];
-            d2 = fmax(d2, dx * dx + dy * dy + dz * dz);
-        }
-    }
-    const double dx = c[0] / ENND - coll[0], dy = c[1] / ENND - coll[1], dz = c[2] / ENND - coll[2];
-    return dx * dx + dy * dy + dz * dz < 4.0 * d2;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -400,59 +349,20 @@ __device__ __forceinline__ int integrate_fp64(const double* X, int sing, const d
     return active;
 }
 
-// Value v of node a goes to out[(a * 15 + v) * ostride].  REGULAR: the caller guarantees that no node of e lies at the collocation point, so the
-// singular rules are not compiled in (the regular path then fits two waves per SIMD).
+// Value v of node a goes to out[(a * 15 + v) * ostride].  REGULAR: the caller guarantees that
+// no node of e lies at the collocation point, so the singular rules are not compiled in (the
+// regular path then fits two waves per SIMD).
 template <int ENND, bool REGULAR = false>
 __device__ __forceinline__ int integrate_pair(const double* coords, const int* ien,
                                               const double* geo, const double* gauss, int nq,
                                               int i, int e, const TimeCombo& tc, double dt,
-                                              const Material& m, double* out, bool fp32 = false,
-                                              int64_t ostride = 1)
+                                              const Material& m, double* out, int64_t ostride = 1)
 {
-    int active = 0;
     const double coll[3] = {coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]};
     double X[3 * ENND];
     load_element<ENND>(coords, ien, e, X);
     const int sing = REGULAR ? -1 : singular_node<ENND>(X, coll);
     const double* geo_e = geo + (size_t)e * nq * nq * (7 + ENND);
-    double acc[ENND][kVals];
-#pragma unroll
-    for (int a = 0; a < ENND; a++)
-#pragma unroll
-        for (int v = 0; v < kVals; v++) acc[a][v] = 0.0;
-    if (fp32 && sing < 0 && !near_singular<ENND>(X, coll)) {
-        // regular pair, mixed precision: distance and retarded times in FP64, the time
-        // integrals and the 15 block values in FP32, sums over the points in FP64
-        for_each_point<ENND>(X, -1, geo_e, gauss, nq,
-                             [&](const double* x, const double* n, double wJ, const double* N) {
-                                 const double d0 = x[0] - coll[0], d1 = x[1] - coll[1],
-                                              d2 = x[2] - coll[2];
-                                 const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-                                 float S[6];
-                                 if (!eval_s_f32(tc, r, dt, m, S)) return;
-                                 active++;
-                                 const double ir = 1.0 / r;
-                                 const float rh[3] = {(float)(d0 * ir), (float)(d1 * ir),
-                                                      (float)(d2 * ir)};
-                                 const float nf[3] = {(float)n[0], (float)n[1], (float)n[2]};
-                                 const float irf = (float)ir;
-                                 float blk[kVals];
-                                 blocks_from_s<float>(S, rh, nf, (float)r, irf, irf * irf, m,
-                                                      blk);
-#pragma unroll
-                                 for (int a = 0; a < ENND; a++) {
-                                     const float wN = (float)(wJ * N[a]);
-#pragma unroll
-                                     for (int v = 0; v < kVals; v++)
-                                         acc[a][v] += (double)(wN * blk[v]);
-                                 }
-                             });
-#pragma unroll
-        for (int a = 0; a < ENND; a++)
-#pragma unroll
-            for (int v = 0; v < kVals; v++) out[(a * kVals + v) * ostride] = acc[a][v];
-        return active;
-    }
     return integrate_fp64<ENND>(X, sing, coll, geo_e, gauss, nq, tc, dt, m, out, ostride);
 }
 
@@ -579,7 +489,7 @@ __global__ void __launch_bounds__(kBlock, (ENND == 4 && !SINGULAR) ? 2 : 1)
                 const int* __restrict__ slot_pair, const int64_t* __restrict__ pair_slot,
                 const int2* __restrict__ pair_k, const int64_t* __restrict__ sing_slots,
                 int64_t nsing, double* __restrict__ scratch,
-                unsigned long long* __restrict__ counter, int fp32)
+                unsigned long long* __restrict__ counter, const int* __restrict__ rows)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t s;
@@ -594,7 +504,8 @@ __global__ void __launch_bounds__(kBlock, (ENND == 4 && !SINGULAR) ? 2 : 1)
         p = slot_pair[s];
         if (p < 0) return;
     }
-    const int i = i0 + p / nel, e = p % nel;
+    // rows: the batch's row nodes when they are not the contiguous range i0..
+    const int i = rows ? rows[p / nel] : i0 + p / nel, e = p % nel;
     const int k = pair_k[(int64_t)i * nel + e].x + (int)(s - pair_slot[p]);
     TimeCombo tc;
     tc.t[0] = (k - 1) * dt;
@@ -609,7 +520,7 @@ __global__ void __launch_bounds__(kBlock, (ENND == 4 && !SINGULAR) ? 2 : 1)
     tc.cJ[2] = hasJ;
     tc.static_ok = true;
     const int active = integrate_pair<ENND, !SINGULAR>(coords, ien, geo, gauss, nq, i, e, tc, dt,
-                                                       m, scratch + s, fp32 != 0, nslots);
+                                                       m, scratch + s, nslots);
     if (counter) atomicAdd(counter, (unsigned long long)active);
 }
 
@@ -766,13 +677,15 @@ __global__ void __launch_bounds__(kChunk)
     for (int t = 0; t < kVals; t++) o[t * kChunk + lane] = v[t];
 }
 
-// Dense N x N G and H of one lag k from the store.
+// Dense rows of G and H of one lag k from the store: the chunks from chunk0 on belong to row
+// nodes lower.., whose rows land at 3 (i - lower) of the [3 (upper - lower + 1)][3 nnd] block.
 __global__ void __launch_bounds__(kChunk)
     k_store_to_dense(const double* __restrict__ vals, const uint32_t* __restrict__ idx,
                      const int64_t* __restrict__ row_chunk, const int64_t* __restrict__ row_valid,
-                     int nnd, int k, double* __restrict__ G, double* __restrict__ H)
+                     int nnd, int k, int lower, int64_t chunk0, double* __restrict__ G,
+                     double* __restrict__ H)
 {
-    const int64_t c = blockIdx.x;
+    const int64_t c = chunk0 + blockIdx.x;
     const int i = row_of_chunk(row_chunk, nnd, c);
     const int lane = threadIdx.x;
     const int64_t in_row = (c - row_chunk[i]) * kChunk + lane;
@@ -785,8 +698,8 @@ __global__ void __launch_bounds__(kChunk)
 #pragma unroll
     for (int t = 0; t < kVals; t++) v[t] = s[t * kChunk];
     const int64_t ncol = 3LL * nnd;
-    put_block(G, ncol, i, j, v, true);
-    put_block(H, ncol, i, j, v + 6, false);
+    put_block(G, ncol, i - lower, j, v, true);
+    put_block(H, ncol, i - lower, j, v + 6, false);
 }
 
 template <int ENND>
@@ -938,6 +851,17 @@ void assemble_step_dense(Context& c, int step, int lower, int upper, double* dG1
 
 namespace {
 
+// Pair ranges clipped to lags <= kmax (the FP64 store of the mixed-precision build keeps the
+// lags of the step-0 matrices only).
+__global__ void k_clip_pairs(const int2* __restrict__ in, int64_t n, int kmax,
+                             int2* __restrict__ out)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int2 v = in[t];
+    out[t] = (v.y < v.x || v.x > kmax) ? make_int2(1, 0) : make_int2(v.x, min(v.y, kmax));
+}
+
 struct ShellPlan {
     int2* hull = nullptr;            // [nnd*nnd] (the context's pinned copy)
     int64_t* off = nullptr;          // [nnd*nnd] entry offsets (pinned)
@@ -946,13 +870,15 @@ struct ShellPlan {
     int64_t nonzero = 0;
 };
 
-void plan_shells(Context& c, ShellPlan& P)
+// Node-pair hulls (lag range of each (i, j), from the pair ranges dpk of the elements around j)
+// into the pinned c.pin_hull, entries per row node and their total; with offsets = true also the
+// store layout (row chunks, first entry of every pair in c.pin_off).
+void plan_shells(Context& c, ShellPlan& P, const int2* dpk, bool offsets = true)
 {
-    compute_pair_ranges(c);
     const int64_t nn = (int64_t)c.nnd * c.nnd;
     DBuf<int2> hull(c.stream);
     hull.alloc(nn);
-    hipLaunchKernelGGL(k_node_hull, dim3(grid_for(nn)), dim3(kBlock), 0, c.stream, c.pair_k.p,
+    hipLaunchKernelGGL(k_node_hull, dim3(grid_for(nn)), dim3(kBlock), 0, c.stream, dpk,
                        c.adj_off.p, c.adj.p, c.nnd, c.nel, hull.p);
     TD_HIP(hipGetLastError());
     c.pin_hull.reserve(nn);
@@ -960,10 +886,9 @@ void plan_shells(Context& c, ShellPlan& P)
     TD_HIP(hipMemcpyAsync(P.hull, hull.p, nn * sizeof(int2), hipMemcpyDeviceToHost,
                           c.stream));
     TD_HIP(hipStreamSynchronize(c.stream));
-    c.pin_off.reserve(nn);
-    P.off = c.pin_off.p;
     P.row_chunk.assign(c.nnd + 1, 0);
     P.row_valid.assign(c.nnd, 0);
+    P.nonzero = 0;
     // entries per row node, then each row's first chunk, then the entry offset of every pair
     const int nnd = c.nnd;
     parallel_for(nnd, [&](int64_t lo, int64_t hi) {
@@ -983,6 +908,9 @@ void plan_shells(Context& c, ShellPlan& P)
         chunk += (P.row_valid[i] + kChunk - 1) / kChunk;
     }
     P.row_chunk[nnd] = chunk;
+    if (!offsets) return;
+    c.pin_off.reserve(nn);
+    P.off = c.pin_off.p;
     parallel_for(nnd, [&](int64_t lo, int64_t hi) {
         for (int64_t i = lo; i < hi; i++) {
             int64_t o = P.row_chunk[i] * kChunk;
@@ -995,88 +923,104 @@ void plan_shells(Context& c, ShellPlan& P)
     });
 }
 
-template <int ENND>
-void fill_store(Context& c, const ShellPlan& P)
+// One pass of the lag kernel over the rows `order` (row node per position, -1 = none; empty =
+// every node in order), in batches of whole multiples of `align` positions whose slots fit the
+// scratch budget.  hpk / dpk: the pair ranges (host, device) the slots cover.  After the lag
+// kernel of a batch of positions [b0, b1), consume(b0, b1, scratch, nslots, pair_slot) launches
+// the gather from its slots; the slot buffers are reused by the next batch in stream order.
+template <int ENND, typename Consume>
+void slot_pass(Context& c, const std::vector<int>& order, int align, const int2* hpk,
+               const int2* dpk, Consume&& consume)
 {
-    LagStore& S = c.store;
-    // per-pair slot layout on the host (pair_k copied back once)
-    const int64_t np = (int64_t)c.nnd * c.nel;
-    c.pin_pk.reserve(np);
-    const int2* pk = c.pin_pk.p;
-    TD_HIP(hipMemcpyAsync(c.pin_pk.p, c.pair_k.p, np * sizeof(int2), hipMemcpyDeviceToHost,
-                          c.stream));
-    TD_HIP(hipStreamSynchronize(c.stream));
+    const bool ident = order.empty();
+    const int npos = ident ? c.nnd : (int)order.size();
+    auto node_at = [&](int pos) { return ident ? pos : order[pos]; };
     const int64_t slot_bytes = (int64_t)ENND * kVals * sizeof(double);
-    const double ext = c.if_ext ? 0.5 * c.dt : 0.0;
     const bool prof = c.profile == 1;
     KernelTimer lag_timer(c.stream, prof), gather_timer(c.stream, prof);
-    if (prof) {
+    if (prof && !c.counters.p) {
         c.counters.alloc(1);
         TD_HIP(hipMemsetAsync(c.counters.p, 0, sizeof(unsigned long long), c.stream));
     }
-    // slots (lags of in-shell element pairs) per row node, then row batches whose slots fit the
-    // scratch budget
-    std::vector<int64_t> rs(c.nnd);
-    parallel_for(c.nnd, [&](int64_t lo, int64_t hi) {
-        for (int64_t i = lo; i < hi; i++) {
+    // slots (lags of in-shell element pairs) per position, then batches of whole `align` groups
+    // of positions whose slots fit the scratch budget
+    std::vector<int64_t> rs(npos, 0);
+    parallel_for(npos, [&](int64_t lo, int64_t hi) {
+        for (int64_t q = lo; q < hi; q++) {
+            const int i = node_at((int)q);
+            if (i < 0) continue;
             int64_t n = 0;
             for (int e = 0; e < c.nel; e++) {
-                const int2 k = pk[i * c.nel + e];
+                const int2 k = hpk[(int64_t)i * c.nel + e];
                 if (k.y >= k.x) n += k.y - k.x + 1;
             }
-            rs[i] = n;
+            rs[q] = n;
         }
     });
-    struct Batch { int i0, i1; int64_t slots; };
+    struct Batch { int b0, b1; int64_t slots; };
     std::vector<Batch> batches;
     int64_t max_slots = 1;
-    for (int i0 = 0; i0 < c.nnd;) {
-        int i1 = i0;
+    for (int b0 = 0; b0 < npos;) {
+        int b1 = b0;
         int64_t slots = 0;
-        while (i1 < c.nnd && (i1 == i0 || (slots + rs[i1]) * slot_bytes <= kScratchBytes))
-            slots += rs[i1++];
-        batches.push_back({i0, i1, slots});
+        while (b1 < npos) {
+            const int g1 = std::min(npos, b1 + align);
+            int64_t add = 0;
+            for (int q = b1; q < g1; q++) add += rs[q];
+            if (b1 > b0 && (slots + add) * slot_bytes > kScratchBytes) break;
+            slots += add;
+            b1 = g1;
+        }
+        batches.push_back({b0, b1, slots});
         max_slots = std::max(max_slots, slots);
-        i0 = i1;
+        b0 = b1;
     }
     DBuf<double> scratch(c.stream);
-    DBuf<int> slot_pair(c.stream);
+    DBuf<int> slot_pair(c.stream), d_rows(c.stream);
     DBuf<int64_t> pair_slot(c.stream);
     scratch.reserve((size_t)max_slots * ENND * kVals);
     int max_rows = 1;
-    for (const Batch& bt : batches) max_rows = std::max(max_rows, bt.i1 - bt.i0);
+    for (const Batch& bt : batches) max_rows = std::max(max_rows, bt.b1 - bt.b0);
     slot_pair.reserve((size_t)max_slots);
     pair_slot.reserve((size_t)max_rows * c.nel);
+    if (!ident) d_rows.reserve((size_t)max_rows);
     // The host builds batch b + 1's slot lists while the device works on batch b: two host list
     // sets, each reused only after the event that follows its upload; the device buffers are
     // rewritten in stream order after the previous batch's kernels.
     std::vector<int64_t> h_pair_slot[2], h_sing[2];
-    std::vector<int> h_slot_pair[2];
+    std::vector<int> h_slot_pair[2], h_rows[2];
     DBuf<int64_t> sing_slots(c.stream);
     hipEvent_t uploaded[2];
     TD_HIP(hipEventCreateWithFlags(&uploaded[0], hipEventDisableTiming));
     TD_HIP(hipEventCreateWithFlags(&uploaded[1], hipEventDisableTiming));
     for (size_t b = 0; b < batches.size(); b++) {
         const int set = (int)(b & 1);
-        const int i0 = batches[b].i0, i1 = batches[b].i1;
+        const int b0 = batches[b].b0, b1 = batches[b].b1;
         const int64_t slots = batches[b].slots;
-        const int nb = i1 - i0;
+        const int nb = b1 - b0;
         if (b >= 2) TD_HIP(hipEventSynchronize(uploaded[set]));
         auto& hp = h_pair_slot[set];
         auto& hs = h_slot_pair[set];
-        hp.resize((size_t)nb * c.nel);
+        auto& hr = h_rows[set];
+        hp.assign((size_t)nb * c.nel, 0);
         hs.resize((size_t)slots);
+        hr.resize(nb);
         std::vector<int64_t> row_start(nb + 1, 0);
-        for (int r = 0; r < nb; r++) row_start[r + 1] = row_start[r] + rs[i0 + r];
+        for (int r = 0; r < nb; r++) {
+            row_start[r + 1] = row_start[r] + rs[b0 + r];
+            hr[r] = node_at(b0 + r);
+        }
         auto& hsing = h_sing[set];
         parallel_for(nb, [&](int64_t lo, int64_t hi) {
             for (int64_t r = lo; r < hi; r++) {
+                const int i = hr[r];
+                if (i < 0) continue;
                 int64_t sl = row_start[r];
-                const double* xc = &c.h_coords[3 * (i0 + r)];
+                const double* xc = &c.h_coords[3 * (int64_t)i];
                 for (int e = 0; e < c.nel; e++) {
                     const int64_t p = r * c.nel + e;
                     hp[p] = sl;
-                    const int2 k = pk[(int64_t)i0 * c.nel + p];
+                    const int2 k = hpk[(int64_t)i * c.nel + e];
                     if (k.y < k.x) continue;
                     // the device's singular test is a squared distance below 1e-9; a pair goes
                     // to the singular launch well before that (squared distance below 1e-6)
@@ -1095,38 +1039,32 @@ void fill_store(Context& c, const ShellPlan& P)
             if (hs[sl] < 0) hsing.push_back(sl);
         slot_pair.upload(hs.data(), hs.size(), c.stream);
         pair_slot.upload(hp.data(), hp.size(), c.stream);
+        if (!ident) d_rows.upload(hr.data(), hr.size(), c.stream);
         if (!hsing.empty()) sing_slots.upload(hsing.data(), hsing.size(), c.stream);
         TD_HIP(hipEventRecord(uploaded[set], c.stream));
+        const int* rows = ident ? nullptr : d_rows.p;
         if (slots > 0) {
             lag_timer.begin();
             hipLaunchKernelGGL((k_pair_lags<ENND, false>), dim3(grid_for(slots)), dim3(kBlock), 0,
                                c.stream, c.coords.p, c.ien.p, c.geo.p, c.gauss.p, c.nq, c.nel,
-                               c.dt, c.mat, c.Kc, i0, slots, slot_pair.p, pair_slot.p,
-                               c.pair_k.p, sing_slots.p, (int64_t)0, scratch.p,
-                               prof ? c.counters.p : nullptr, c.precision == 1 ? 1 : 0);
+                               c.dt, c.mat, c.Kc, b0, slots, slot_pair.p, pair_slot.p, dpk,
+                               sing_slots.p, (int64_t)0, scratch.p,
+                               prof ? c.counters.p : nullptr, rows);
             TD_HIP(hipGetLastError());
             const int64_t nsing = (int64_t)hsing.size();
             if (nsing > 0) {
                 hipLaunchKernelGGL((k_pair_lags<ENND, true>), dim3(grid_for(nsing)),
                                    dim3(kBlock), 0, c.stream, c.coords.p, c.ien.p, c.geo.p,
-                                   c.gauss.p, c.nq, c.nel, c.dt, c.mat, c.Kc, i0, slots,
-                                   slot_pair.p, pair_slot.p, c.pair_k.p, sing_slots.p, nsing,
-                                   scratch.p, prof ? c.counters.p : nullptr,
-                                   c.precision == 1 ? 1 : 0);
+                                   c.gauss.p, c.nq, c.nel, c.dt, c.mat, c.Kc, b0, slots,
+                                   slot_pair.p, pair_slot.p, dpk, sing_slots.p, nsing,
+                                   scratch.p, prof ? c.counters.p : nullptr, rows);
                 TD_HIP(hipGetLastError());
             }
             lag_timer.end();
         }
-        const int64_t ch0 = P.row_chunk[i0], ch1 = P.row_chunk[i1];
-        if (ch1 > ch0) {
-            gather_timer.begin();
-            hipLaunchKernelGGL(k_store_gather<ENND>, dim3((unsigned)(ch1 - ch0)), dim3(kChunk), 0,
-                               c.stream, scratch.p, pair_slot.p, c.pair_k.p, c.adj_off.p, c.adj.p,
-                               c.cterm.p, S.row_chunk.p, S.row_valid.p, S.idx.p, c.nnd, c.nel,
-                               i0, ext, ch0, slots, S.vals.p);
-            TD_HIP(hipGetLastError());
-            gather_timer.end();
-        }
+        gather_timer.begin();
+        consume(b0, b1, (const double*)scratch.p, slots, (const int64_t*)pair_slot.p);
+        gather_timer.end();
     }
     TD_HIP(hipStreamSynchronize(c.stream));
     TD_HIP(hipEventDestroy(uploaded[0]));
@@ -1135,18 +1073,127 @@ void fill_store(Context& c, const ShellPlan& P)
         c.tm.lag_launches += lag_timer.count();
         c.tm.lag_kernel_ms += lag_timer.collect();
         c.tm.gather_ms += gather_timer.collect();
-        unsigned long long n = 0;
-        TD_HIP(hipMemcpy(&n, c.counters.p, sizeof(n), hipMemcpyDeviceToHost));
-        c.tm.active_points += (int64_t)n;
     }
+}
+
+// The FP64 store of the rows' in-shell lags (pair ranges hpk / dpk, layout P): slots of
+// contiguous row ranges, gathered per (i, j, k) entry by k_store_gather.
+template <int ENND>
+void fill_store(Context& c, const ShellPlan& P, const int2* hpk, const int2* dpk)
+{
+    LagStore& S = c.store;
+    const double ext = c.if_ext ? 0.5 * c.dt : 0.0;
+    slot_pass<ENND>(c, std::vector<int>(), 1, hpk, dpk,
+                    [&](int i0, int i1, const double* scratch, int64_t nslots,
+                        const int64_t* pair_slot) {
+                        const int64_t ch0 = P.row_chunk[i0], ch1 = P.row_chunk[i1];
+                        if (ch1 <= ch0) return;
+                        hipLaunchKernelGGL(k_store_gather<ENND>, dim3((unsigned)(ch1 - ch0)),
+                                           dim3(kChunk), 0, c.stream, scratch, pair_slot, dpk,
+                                           c.adj_off.p, c.adj.p, c.cterm.p, S.row_chunk.p,
+                                           S.row_valid.p, S.idx.p, c.nnd, c.nel, i0, ext, ch0,
+                                           nslots, S.vals.p);
+                        TD_HIP(hipGetLastError());
+                    });
+}
+
+// Allocates the FP64 store of layout P (pair hulls in P.hull, offsets in P.off) and fills it.
+template <int ENND>
+void build_fp64_store(Context& c, const ShellPlan& P, const int2* hpk, const int2* dpk)
+{
+    LagStore& S = c.store;
+    S.n_chunks = P.row_chunk[c.nnd];
+    S.n_entries = S.n_chunks * kChunk;
+    S.nonzero_entries = P.nonzero;
+    S.h_row_chunk = P.row_chunk;
+    S.h_row_valid = P.row_valid;
+    S.vals.alloc(std::max<int64_t>(S.n_chunks, 1) * kVals * kChunk);
+    S.idx.alloc(std::max<int64_t>(S.n_entries, 1));
+    TD_HIP(hipMemsetAsync(S.idx.p, 0, S.idx.bytes(), c.stream));
+    S.row_chunk.upload(P.row_chunk.data(), P.row_chunk.size(), c.stream);
+    S.row_valid.upload(P.row_valid.data(), P.row_valid.size(), c.stream);
+    const int64_t nn = (int64_t)c.nnd * c.nnd;
+    {
+        DBuf<int2> d_hull(c.stream);
+        DBuf<int64_t> d_off(c.stream);
+        d_off.upload(P.off, nn, c.stream);
+        d_hull.upload(P.hull, nn, c.stream);
+        hipLaunchKernelGGL(k_fill_idx, dim3(grid_for(nn)), dim3(kBlock), 0, c.stream, d_hull.p,
+                           d_off.p, c.nnd, S.idx.p);
+        TD_HIP(hipGetLastError());
+    }
+    fill_store<ENND>(c, P, hpk, dpk);
+}
+
+// The pair ranges of c.pair_k on the host (pinned, kept between builds unless large).
+const int2* host_pair_ranges(Context& c)
+{
+    const int64_t np = (int64_t)c.nnd * c.nel;
+    c.pin_pk.reserve(np);
+    TD_HIP(hipMemcpyAsync(c.pin_pk.p, c.pair_k.p, np * sizeof(int2), hipMemcpyDeviceToHost,
+                          c.stream));
+    TD_HIP(hipStreamSynchronize(c.stream));
+    return c.pin_pk.p;
+}
+
+// Mixed-precision store: the tile store of every lag, filled straight from the lag kernel's
+// slots in row tiles of the bisection order (no FP64 copy of the whole store is ever held),
+// and an FP64 store of lags 0 and 1 only, from which march_init forms the step-0 matrices.
+template <int ENND>
+void build_mixed(Context& c)
+{
+    const int64_t np = (int64_t)c.nnd * c.nel;
+    if (!c.tp.planned) {
+        ShellPlan Pf;
+        plan_shells(c, Pf, c.pair_k.p, false);
+        c.pair_lags = Pf.nonzero;
+        tp_plan_structure(c);
+    }
+    const int2* hpk = host_pair_ranges(c);
+    // lags 0 and 1 in FP64
+    {
+        DBuf<int2> pk01(c.stream);
+        pk01.alloc(np);
+        hipLaunchKernelGGL(k_clip_pairs, dim3(grid_for(np)), dim3(kBlock), 0, c.stream,
+                           c.pair_k.p, np, 1, pk01.p);
+        TD_HIP(hipGetLastError());
+        std::vector<int2> hpk01(np);
+        for (int64_t t = 0; t < np; t++) {
+            const int2 v = hpk[t];
+            hpk01[t] = (v.y < v.x || v.x > 1) ? make_int2(1, 0) : make_int2(v.x, std::min(v.y, 1));
+        }
+        ShellPlan P01;
+        plan_shells(c, P01, pk01.p);
+        build_fp64_store<ENND>(c, P01, hpk01.data(), pk01.p);
+    }
+    // every lag into the tile store, row tiles in bisection order
+    tp_alloc(c);
+    const double ext = c.if_ext ? 0.5 * c.dt : 0.0;
+    slot_pass<ENND>(c, c.tp.perm, kTile, hpk, c.pair_k.p,
+                    [&](int b0, int b1, const double* scratch, int64_t nslots,
+                        const int64_t* pair_slot) {
+                        tp_gather_slots(c, b0 / kTile, b1 / kTile, scratch, nslots, pair_slot,
+                                        ext);
+                    });
+    tp_finish(c);
 }
 
 }  // namespace
 
 int64_t predict_store_bytes(Context& c)
 {
+    compute_pair_ranges(c);
+    if (c.precision == 1) {
+        if (!c.tp.planned) {
+            ShellPlan Pf;
+            plan_shells(c, Pf, c.pair_k.p, false);
+            c.pair_lags = Pf.nonzero;
+            tp_plan_structure(c);
+        }
+        return c.tp.nsub * (int64_t)kSubFloats * 4 + c.tp.nentries * (int64_t)sizeof(TPEntry);
+    }
     ShellPlan P;
-    plan_shells(c, P);
+    plan_shells(c, P, c.pair_k.p, false);
     return P.row_chunk[c.nnd] * kChunk * (int64_t)(kVals * sizeof(double) + sizeof(uint32_t));
 }
 
@@ -1154,58 +1201,64 @@ void build_lag_store(Context& c)
 {
     if (c.Kc > kMaxLag) throw std::runtime_error("lag count exceeds the store's 10-bit lag index");
     const auto t0 = std::chrono::steady_clock::now();
-    c.tp.ready = false;  // the tile-pair copy is rebuilt from the new store when needed
+    c.store_ready = false;
+    c.tp.ready = false;
     c.store_precision = c.precision;
+    if (c.profile == 1 && c.counters.p)
+        TD_HIP(hipMemsetAsync(c.counters.p, 0, sizeof(unsigned long long), c.stream));
     compute_cterm(c);
-    ShellPlan P;
-    plan_shells(c, P);
-    LagStore& S = c.store;
-    S.n_chunks = P.row_chunk[c.nnd];
-    S.n_entries = S.n_chunks * kChunk;
-    S.nonzero_entries = P.nonzero;
-    S.h_row_chunk = P.row_chunk;
-    S.vals.alloc(std::max<int64_t>(S.n_chunks, 1) * kVals * kChunk);
-    S.idx.alloc(std::max<int64_t>(S.n_entries, 1));
-    TD_HIP(hipMemsetAsync(S.idx.p, 0, S.idx.bytes(), c.stream));
-    S.h_row_valid = P.row_valid;
-    S.row_chunk.upload(P.row_chunk.data(), P.row_chunk.size(), c.stream);
-    S.row_valid.upload(P.row_valid.data(), P.row_valid.size(), c.stream);
-    const int64_t nn = (int64_t)c.nnd * c.nnd;
-    S.d_off.upload(P.off, nn, c.stream);
-    S.d_hull.upload(P.hull, nn, c.stream);
-    hipLaunchKernelGGL(k_fill_idx, dim3(grid_for(nn)), dim3(kBlock), 0, c.stream, S.d_hull.p,
-                       S.d_off.p, c.nnd, S.idx.p);
-    TD_HIP(hipGetLastError());
-    if (c.ennd == 4)
-        fill_store<4>(c, P);
-    else
-        fill_store<8>(c, P);
+    compute_pair_ranges(c);
+    if (c.precision == 1) {
+        if (c.ennd == 4)
+            build_mixed<4>(c);
+        else
+            build_mixed<8>(c);
+        c.store_lags = std::min(1, c.Kc);
+    } else {
+        // a tile store of an earlier mixed-precision build is released
+        c.tp.vals.reset();
+        c.tp.ent.reset();
+        c.tp.planned = false;
+        ShellPlan P;
+        plan_shells(c, P, c.pair_k.p);
+        c.pair_lags = P.nonzero;
+        const int2* hpk = host_pair_ranges(c);
+        if (c.ennd == 4)
+            build_fp64_store<4>(c, P, hpk, c.pair_k.p);
+        else
+            build_fp64_store<8>(c, P, hpk, c.pair_k.p);
+        c.store_lags = c.Kc;
+    }
     TD_HIP(hipStreamSynchronize(c.stream));
-    if (c.precision != 1) {
-        // only the mixed path's tile-pair copy reads them again
-        S.d_off.reset();
-        S.d_hull.reset();
+    if (c.profile == 1 && c.counters.p) {
+        unsigned long long n = 0;
+        TD_HIP(hipMemcpy(&n, c.counters.p, sizeof(n), hipMemcpyDeviceToHost));
+        c.tm.active_points += (int64_t)n;
     }
     // the pinned planning copies stay for the next build unless they are large
     if ((int64_t)c.nnd * c.nnd * 24 > kPinnedKeepBytes) {
         c.pin_off.reserve(0);
         c.pin_pk.reserve(0);
+        c.pin_hull.reserve(0);
     }
     c.store_ready = true;
     c.tm.store_ms +=
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-void lag_dense(Context& c, int k, double* dG, double* dH)
+void lag_dense(Context& c, int k, double* dG, double* dH) { lag_rows(c, k, 0, c.nnd - 1, dG, dH); }
+
+void lag_rows(Context& c, int k, int lower, int upper, double* dG, double* dH)
 {
     if (!c.store_ready) throw std::runtime_error("lag store not built");
-    const int64_t N = 3LL * c.nnd;
-    TD_HIP(hipMemsetAsync(dG, 0, N * N * sizeof(double), c.stream));
-    TD_HIP(hipMemsetAsync(dH, 0, N * N * sizeof(double), c.stream));
-    if (k < 0 || k > c.Kc || c.store.n_chunks == 0) return;
-    hipLaunchKernelGGL(k_store_to_dense, dim3((unsigned)c.store.n_chunks), dim3(kChunk), 0,
-                       c.stream, c.store.vals.p, c.store.idx.p, c.store.row_chunk.p,
-                       c.store.row_valid.p, c.nnd, k, dG, dH);
+    const int64_t n = 3LL * (upper - lower + 1) * 3LL * c.nnd;
+    TD_HIP(hipMemsetAsync(dG, 0, n * sizeof(double), c.stream));
+    TD_HIP(hipMemsetAsync(dH, 0, n * sizeof(double), c.stream));
+    const int64_t ch0 = c.store.h_row_chunk[lower], ch1 = c.store.h_row_chunk[upper + 1];
+    if (k < 0 || k > c.Kc || ch1 <= ch0) return;
+    hipLaunchKernelGGL(k_store_to_dense, dim3((unsigned)(ch1 - ch0)), dim3(kChunk), 0, c.stream,
+                       c.store.vals.p, c.store.idx.p, c.store.row_chunk.p, c.store.row_valid.p,
+                       c.nnd, k, lower, ch0, dG, dH);
     TD_HIP(hipGetLastError());
 }
 

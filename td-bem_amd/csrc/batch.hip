@@ -12,6 +12,7 @@
 // the batch itself.
 #include <algorithm>
 #include <chrono>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <vector>
@@ -137,13 +138,14 @@ int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 }  // namespace
 
 struct Batch {
-    std::vector<Context*> ctx;
-    int S = 0, nrhs = 0, device = 0;
+    std::vector<Context*> ctx;   // members, in the order they joined
+    int S = 0, nrhs = 0, device = -1;
     hipStream_t stream = nullptr;
+    bool dirty = true;           // membership changed since the last plan
     std::vector<BatchScen> h_desc;
     DBuf<BatchScen> desc;
     DBuf<int> sdev, act;
-    std::vector<std::unique_ptr<DBuf<double>>> parts;
+    std::map<Context*, std::unique_ptr<DBuf<double>>> parts;
     DBuf<int> off_z, off_r, off_rhs, off_a;
     int nb_z = 0, nb_r = 0, nb_rhs = 0, nb_a = 0;
     DBuf<int4> chunk;
@@ -158,7 +160,11 @@ struct Batch {
     ~Batch()
     {
         drop_graph();
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+        for (Context* c : ctx) c->batch_refs--;
     }
     void drop_graph()
     {
@@ -176,6 +182,7 @@ namespace {
 void batch_plan(Batch& B)
 {
     B.drop_graph();
+    if (B.S == 0) return;
     const int NT = B.nrhs / 16;
     const int target = B.rounds * tp_resident_groups(B.nrhs, B.device);
     std::vector<double> work(B.S);
@@ -192,9 +199,10 @@ void batch_plan(Batch& B)
         tp_chunks(c, share, sc, ch, gch);
         d.nsplit = apply_splits(d.N, B.apply_waves);
         const size_t need = (size_t)d.nsplit * d.nrhs * d.Np;
-        if (!B.parts[sc]) B.parts[sc].reset(new DBuf<double>());
-        B.parts[sc]->alloc(need);
-        d.parts = B.parts[sc]->p;
+        auto& pb = B.parts[&c];
+        if (!pb) pb.reset(new DBuf<double>());
+        pb->alloc(need);
+        d.parts = pb->p;
         oz[sc + 1] = oz[sc] + (int)ceil_div(3LL * d.nT * kCols * d.nrhs, kThreads);
         orr[sc + 1] = orr[sc] + (int)ceil_div((int64_t)d.nT * kRows * d.nrhs, kThreads);
         orhs[sc + 1] = orhs[sc] + (int)ceil_div((int64_t)d.N * d.nrhs, kThreads);
@@ -290,42 +298,82 @@ double time_launches(Batch& B, F&& launch, int reps)
 
 }  // namespace
 
-Batch* batch_create(const std::vector<Context*>& ctxs)
+namespace {
+
+// Descriptors, device step tables and the plan for the current members, after a change of
+// membership (a graph captured for the previous members is dropped by batch_plan).
+void batch_refresh(Batch& B)
 {
-    if (ctxs.empty()) throw std::invalid_argument("batch: no contexts");
-    std::unique_ptr<Batch> B(new Batch());
-    B->ctx = ctxs;
-    B->S = (int)ctxs.size();
-    B->device = ctxs[0]->device;
-    B->nrhs = 0;
-    for (Context* c : ctxs) {
-        if (!c) throw std::invalid_argument("batch: null context");
-        if (c->device != B->device) throw std::invalid_argument("batch: contexts on different GPUs");
-        BatchScen d{};
-        march_describe(*c, d);
-        if (B->nrhs == 0) B->nrhs = d.nrhs;
-        if (d.nrhs != B->nrhs) throw std::invalid_argument("batch: contexts differ in nrhs");
-        TD_HIP(hipStreamSynchronize(c->stream));
-    }
-    TD_HIP(hipSetDevice(B->device));
-    TD_HIP(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
-    // step 1 (the n == 2 solve) of scenarios that have not started, on their own streams
-    for (Context* c : ctxs)
-        if (march_step_count(*c) == 0) march_run(*c, 1);
-    B->h_desc.resize(B->S);
-    B->parts.resize(B->S);
-    for (int sc = 0; sc < B->S; sc++) {
-        Context& c = *ctxs[sc];
-        BatchScen& d = B->h_desc[sc];
+    if (!B.dirty) return;
+    B.S = (int)B.ctx.size();
+    B.h_desc.assign(B.S, BatchScen{});
+    for (int sc = 0; sc < B.S; sc++) {
+        Context& c = *B.ctx[sc];
+        BatchScen& d = B.h_desc[sc];
         march_describe(c, d);
         d.tp = c.tp.vals.p;
         d.ent = c.tp.ent.p;
         d.perm = c.tp.d_perm.p;
         d.nT = c.tp.nT;
     }
-    B->sdev.alloc(B->S);
-    B->act.alloc(B->S);
-    batch_plan(*B);
+    for (auto it = B.parts.begin(); it != B.parts.end();) {
+        if (std::find(B.ctx.begin(), B.ctx.end(), it->first) == B.ctx.end())
+            it = B.parts.erase(it);
+        else
+            ++it;
+    }
+    B.sdev.reserve(std::max(B.S, 1));
+    B.act.reserve(std::max(B.S, 1));
+    batch_plan(B);
+    B.dirty = false;
+}
+
+}  // namespace
+
+void batch_add(Batch& B, Context* c)
+{
+    if (!c) throw std::invalid_argument("batch: null context");
+    if (std::find(B.ctx.begin(), B.ctx.end(), c) != B.ctx.end())
+        throw std::invalid_argument("batch: the context is already in the batch");
+    if (c->device != B.device) throw std::invalid_argument("batch: contexts on different GPUs");
+    BatchScen d{};
+    march_describe(*c, d);
+    if (B.nrhs != 0 && d.nrhs != B.nrhs)
+        throw std::invalid_argument("batch: contexts differ in nrhs");
+    TD_HIP(hipStreamSynchronize(c->stream));
+    // step 1 (the n == 2 solve) of a scenario that has not started, on its own stream
+    if (march_step_count(*c) == 0) march_run(*c, 1);
+    B.nrhs = d.nrhs;
+    B.ctx.push_back(c);
+    c->batch_refs++;
+    B.dirty = true;
+}
+
+void batch_remove(Batch& B, Context* c)
+{
+    auto it = std::find(B.ctx.begin(), B.ctx.end(), c);
+    if (it == B.ctx.end()) throw std::invalid_argument("batch: the context is not in the batch");
+    TD_HIP(hipStreamSynchronize(B.stream));
+    B.ctx.erase(it);
+    c->batch_refs--;
+    B.dirty = true;
+}
+
+int batch_size(const Batch& B) { return (int)B.ctx.size(); }
+
+Batch* batch_create(const std::vector<Context*>& ctxs)
+{
+    std::unique_ptr<Batch> B(new Batch());
+    if (ctxs.empty()) {
+        TD_HIP(hipGetDevice(&B->device));
+    } else {
+        if (!ctxs[0]) throw std::invalid_argument("batch: null context");
+        B->device = ctxs[0]->device;
+    }
+    TD_HIP(hipSetDevice(B->device));
+    TD_HIP(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+    for (Context* c : ctxs) batch_add(*B, c);
+    batch_refresh(*B);
     return B.release();
 }
 
@@ -333,6 +381,11 @@ void batch_destroy(Batch* B) { delete B; }
 
 void batch_tune(Batch& B, double* report)
 {
+    batch_refresh(B);
+    if (B.S == 0) {
+        if (report) report[0] = report[1] = report[2] = report[3] = 0.0;
+        return;
+    }
     batch_sync_state(B);
     const int rounds_c[] = {1, 2, 4, 8};
     const int waves_c[] = {1024, 2048, 4096, 8192};
@@ -373,13 +426,15 @@ void batch_set_params(Batch& B, int rounds, int apply_waves)
         throw std::invalid_argument("batch: tunables out of range");
     B.rounds = rounds;
     B.apply_waves = apply_waves;
-    batch_plan(B);
+    if (!B.dirty) batch_plan(B);   // otherwise the next refresh plans with them
 }
 
 int batch_run(Batch& B, int steps_per_graph, int max_replays)
 {
     if (steps_per_graph < 1 || steps_per_graph > 4096)
         throw std::invalid_argument("batch: steps_per_graph out of range");
+    batch_refresh(B);
+    if (B.S == 0) return 0;
     batch_sync_state(B);
     int remaining = 0;
     for (Context* c : B.ctx)

@@ -134,6 +134,13 @@ int max_step(const std::vector<double>& X, int nnd, double E, double nu, double 
     return (int)std::ceil(max_dis / (Cs * dt)) + 1;
 }
 
+// Refuses a call that would free or rebuild state a batch reads (TDBEM_ERR_STATE).
+void not_batched(const Context& c, const char* what)
+{
+    if (c.batch_refs > 0)
+        throw std::logic_error(std::string(what) + ": the context is in a batch");
+}
+
 template <typename F>
 int guarded(tdbem_context* ctx, F&& f)
 {
@@ -345,11 +352,16 @@ int tdbem_create(int nnd, const double* coords, int nel, const int* ien, int ety
     }
 }
 
-void tdbem_destroy(tdbem_context* ctx)
+int tdbem_destroy(tdbem_context* ctx)
 {
-    if (!ctx) return;
+    if (!ctx) return TDBEM_OK;
+    if (ctx->c.batch_refs > 0) {
+        ctx->c.err = "tdbem_destroy: the context is in a batch";
+        return TDBEM_ERR_STATE;
+    }
     (void)hipStreamSynchronize(ctx->c.stream);
     delete ctx;
+    return TDBEM_OK;
 }
 
 const char* tdbem_last_error(const tdbem_context* ctx)
@@ -394,6 +406,7 @@ int tdbem_assemble_step(tdbem_context* ctx, int step, int lower, int upper, doub
 int tdbem_build_lag_store(tdbem_context* ctx)
 {
     return guarded(ctx, [&](Context& c) {
+        not_batched(c, "tdbem_build_lag_store");
         if (c.kernel != TDBEM_KERNEL_STEP)
             throw std::invalid_argument("the lag store is built for the Step kernel");
         build_lag_store(c);
@@ -405,7 +418,8 @@ int64_t tdbem_lag_store_bytes(tdbem_context* ctx)
     int64_t v = -1;
     int rc = guarded(ctx, [&](Context& c) {
         if (c.store_ready)
-            v = (int64_t)(c.store.vals.bytes() + c.store.idx.bytes());
+            v = (int64_t)(c.store.vals.bytes() + c.store.idx.bytes() + c.tp.vals.bytes() +
+                          c.tp.ent.bytes());
         else
             v = predict_store_bytes(c);
     });
@@ -421,9 +435,34 @@ int tdbem_get_lag(tdbem_context* ctx, int k, double* G, double* H)
         DBuf<double> g, h;
         g.alloc(N * N);
         h.alloc(N * N);
-        lag_dense(c, k, g.p, h.p);
+        if (c.store_precision == 1)
+            tile_rows(c, k, 0, c.nnd - 1, g.p, h.p);
+        else
+            lag_dense(c, k, g.p, h.p);
         TD_HIP(hipMemcpyAsync(G, g.p, N * N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         TD_HIP(hipMemcpyAsync(H, h.p, N * N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        TD_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int tdbem_get_lag_rows(tdbem_context* ctx, int k, int lower, int upper, double* G, double* H)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!c.store_ready) throw std::logic_error("lag store not built");
+        if (k < 0 || k > c.Kc) throw std::invalid_argument("lag index out of range");
+        if (lower < 0 || upper >= c.nnd || upper < lower)
+            throw std::invalid_argument("row range out of bounds");
+        if (!G || !H) throw std::invalid_argument("null output");
+        const size_t n = (size_t)3 * (upper - lower + 1) * 3 * c.nnd;
+        DBuf<double> g(c.stream), h(c.stream);
+        g.alloc(n);
+        h.alloc(n);
+        if (c.store_precision == 1)
+            tile_rows(c, k, lower, upper, g.p, h.p);
+        else
+            lag_rows(c, k, lower, upper, g.p, h.p);
+        TD_HIP(hipMemcpyAsync(G, g.p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        TD_HIP(hipMemcpyAsync(H, h.p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         TD_HIP(hipStreamSynchronize(c.stream));
     });
 }
@@ -433,6 +472,7 @@ int tdbem_march_init(tdbem_context* ctx, double a1, double a2, int n_ubc, const 
                      int nrhs, const double* amp, const double* t_rise)
 {
     return guarded(ctx, [&](Context& c) {
+        not_batched(c, "tdbem_march_init");
         if (c.kernel != TDBEM_KERNEL_STEP)
             throw std::invalid_argument("the march runs the Step kernel (SolverStepLinearT)");
         if (!(a1 >= 0 && a1 <= 1 && a2 >= 0 && a2 < 1 && a1 + a2 <= 1))
@@ -453,6 +493,7 @@ int tdbem_march_init(tdbem_context* ctx, double a1, double a2, int n_ubc, const 
 int tdbem_march_run(tdbem_context* ctx, int nsteps)
 {
     return guarded(ctx, [&](Context& c) {
+        not_batched(c, "tdbem_march_run");
         if (!c.march) throw std::logic_error("march not initialised");
         march_run(c, nsteps);
     });
@@ -461,6 +502,7 @@ int tdbem_march_run(tdbem_context* ctx, int nsteps)
 int tdbem_march_enqueue(tdbem_context* ctx, int nsteps)
 {
     return guarded(ctx, [&](Context& c) {
+        not_batched(c, "tdbem_march_enqueue");
         if (!c.march) throw std::logic_error("march not initialised");
         march_run(c, nsteps, false);
     });
@@ -508,6 +550,7 @@ int tdbem_synchronize(tdbem_context* ctx)
 int tdbem_set_precision(tdbem_context* ctx, int precision)
 {
     return guarded(ctx, [&](Context& c) {
+        not_batched(c, "tdbem_set_precision");
         if (precision != TDBEM_PRECISION_FP64 && precision != TDBEM_PRECISION_MIXED_BF16X3)
             throw std::invalid_argument("unknown precision");
         c.precision = precision;
@@ -604,7 +647,7 @@ int tdbem_get_timings(tdbem_context* ctx, double* out)
         const double v[TDBEM_NUM_TIMINGS] = {
             t.store_ms, t.init_ms, t.march_ms, t.lag_kernel_ms, t.gather_ms, t.conv_ms,
             t.solve_ms, (double)t.lag_launches, (double)t.conv_launches, (double)t.active_points,
-            (double)c.store.nonzero_entries, (double)c.store.n_entries, (double)c.tp.nsub,
+            (double)c.pair_lags, (double)c.store.n_entries, (double)c.tp.nsub,
             t.reduce_ms, (double)c.tp.nblocks, (double)c.tp.nentries};
         std::memcpy(out, v, sizeof(v));
     });
@@ -613,7 +656,7 @@ int tdbem_get_timings(tdbem_context* ctx, double* out)
 
 int tdbem_batch_create(tdbem_context* const* ctxs, int count, tdbem_batch** out)
 {
-    if (!ctxs || count < 1 || !out) return TDBEM_ERR_ARG;
+    if ((!ctxs && count > 0) || count < 0 || !out) return TDBEM_ERR_ARG;
     *out = nullptr;
     return batch_guarded([&] {
         std::vector<Context*> v;
@@ -626,6 +669,20 @@ int tdbem_batch_create(tdbem_context* const* ctxs, int count, tdbem_batch** out)
         *out = b.release();
     });
 }
+
+int tdbem_batch_add(tdbem_batch* b, tdbem_context* ctx)
+{
+    if (!b || !ctx) return TDBEM_ERR_ARG;
+    return batch_guarded([&] { batch_add(*b->b, &ctx->c); });
+}
+
+int tdbem_batch_remove(tdbem_batch* b, tdbem_context* ctx)
+{
+    if (!b || !ctx) return TDBEM_ERR_ARG;
+    return batch_guarded([&] { batch_remove(*b->b, &ctx->c); });
+}
+
+int tdbem_batch_size(const tdbem_batch* b) { return b ? batch_size(*b->b) : TDBEM_ERR_ARG; }
 
 int tdbem_batch_tune(tdbem_batch* b, double* report)
 {
@@ -656,7 +713,10 @@ int tdbem_batch_destroy(tdbem_batch* b)
 
 int tdbem_set_max_step(tdbem_context* ctx, int max_step)
 {
-    return guarded(ctx, [&](Context& c) { set_max_step(c, max_step); });
+    return guarded(ctx, [&](Context& c) {
+        not_batched(c, "tdbem_set_max_step");
+        set_max_step(c, max_step);
+    });
 }
 
 int tdbem_mrt_create(tdbem_context* region1, tdbem_context* region2, int max_step, int n_iface,
@@ -665,6 +725,8 @@ int tdbem_mrt_create(tdbem_context* region1, tdbem_context* region2, int max_ste
     if (!region1 || !region2 || !out) return TDBEM_ERR_ARG;
     *out = nullptr;
     return batch_guarded([&] {
+        not_batched(region1->c, "tdbem_mrt_create");
+        not_batched(region2->c, "tdbem_mrt_create");
         std::unique_ptr<tdbem_mrt> m(new tdbem_mrt());
         m->m = mrt_create(&region1->c, &region2->c, max_step, n_iface, iface1, iface2);
         *out = m.release();

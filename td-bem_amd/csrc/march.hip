@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <memory>
 #include <stdexcept>
 #include <vector>
 
@@ -336,12 +337,20 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
         throw std::invalid_argument("FP64 march: nrhs must be 1, 2, 4 or 8");
     if (c.precision == 1 && !(nrhs == 16 || nrhs == 32 || nrhs == 48 || nrhs == 64))
         throw std::invalid_argument("mixed march: nrhs must be 16, 32, 48 or 64");
+    const int N = 3 * c.nnd;
+    // the boundary data is checked before anything is freed or built: a rejected call leaves
+    // the context as it was
+    for (int q = 0; q < n_fbc; q++)
+        if (fbc_dof[q] < 0 || fbc_dof[q] >= N) throw std::invalid_argument("FBC dof out of range");
+    for (int q = 0; q < n_ubc; q++)
+        if (ubc_dof[q] < 0 || ubc_dof[q] >= N) throw std::invalid_argument("UBC dof out of range");
     if (!c.store_ready) build_lag_store(c);
     const auto t0 = std::chrono::steady_clock::now();
     march_free(c);
-    c.march = new March();
-    March& M = *c.march;
-    const int N = 3 * c.nnd;
+    // the new march state becomes the context's only when it is complete (a failure below
+    // leaves no march, never a partly built one)
+    std::unique_ptr<March> owner(new March());
+    March& M = *owner;
     M.N = N;
     M.nrhs = nrhs;
     M.a1 = a1;
@@ -359,13 +368,11 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
     std::vector<double> uv(N, 0.0), fv(N, 0.0), yF(N, 0.0), yU(N, 0.0);
     for (int q = 0; q < n_fbc; q++) {
         const int d = fbc_dof[q];
-        if (d < 0 || d >= N) throw std::runtime_error("FBC dof out of range");
         isf[d] = 1;
         fv[d] = fbc_val[q];
     }
     for (int q = 0; q < n_ubc; q++) {
         const int d = ubc_dof[q];
-        if (d < 0 || d >= N) throw std::runtime_error("UBC dof out of range");
         isu[d] = 1;
         uv[d] = ubc_val[q];
     }
@@ -473,6 +480,7 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
     TD_HIP(hipMemsetAsync(M.T.p, 0, M.T.bytes(), c.stream));
     TD_HIP(hipStreamSynchronize(c.stream));
     M.s = 0;
+    c.march = owner.release();
     c.tm.init_ms +=
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }

@@ -47,56 +47,105 @@ __device__ __forceinline__ int row_of_chunk(const int64_t* row_chunk, int nnd, i
     return lo;
 }
 
-// FP64 node-pair store -> FP32 sub-tiles, gathered: one wavefront per stored sub-tile
-// (row tile I, column tile J, lag k, sub-tile u = 3 m + kt).  Lane L holds block row 16 m + L % 16
-// and the 8 block columns 32 kt + 8 (L / 16) .. + 7 (16x16x32 A-operand order); each element is
-// looked up through the node pair's lag range and first entry, and the lane writes columns
-// 0..3 of its eight to the sub-tile's first 1 KiB line and 4..7 to the second.  Block row 3 ii + p, column 6 jj + 3 part + q holds G[p][q]
-// (part 0) or -H[p][q] (part 1) of node pair (perm[16 I + ii], perm[16 J + jj]).
+// Tile store from the lag kernel's slots: one wavefront per stored sub-tile (row tile I, column
+// tile J, lag k, sub-tile u = 3 m + kt) of the row tiles whose rows the slots hold (positions
+// P0.. of the bisection order).  Lane L holds block row 16 m + L % 16 and the 8 block columns
+// 32 kt + 8 (L / 16) .. + 7 (16x16x32 A-operand order) and writes columns 0..3 of its eight to the
+// sub-tile's first 1 KiB line and 4..7 to the second.  Block row 3 ii + p, column 6 jj + 3 part + q
+// holds G[p][q] (part 0) or -H[p][q] (part 1) of node pair (i, j) = (perm[16 I + ii],
+// perm[16 J + jj]) at lag k: the sum over the elements around j of their slots (i, e, k) in the
+// node's adjacency order, plus the C-term and the exterior-problem term on the diagonal block at
+// k <= 1 -- the same FP64 sums as the FP64 store's gather (k_store_gather), rounded once to FP32.
 __global__ void __launch_bounds__(256)
-    k_tp_gather(const double* __restrict__ vals, const int2* __restrict__ hull,
-                const int64_t* __restrict__ off, const int* __restrict__ perm, int nnd,
-                const uint2* __restrict__ sub_id, int64_t nsub, f32x4* __restrict__ tp)
+    k_tp_from_slots(const double* __restrict__ scratch, int64_t nslots,
+                    const int64_t* __restrict__ pair_slot, const int2* __restrict__ pair_k,
+                    const int* __restrict__ adj_off, const int* __restrict__ adj,
+                    const double* __restrict__ cterm, double ext, const int* __restrict__ perm,
+                    int P0, int nel, const uint2* __restrict__ sub_id, int64_t s0, int64_t s1,
+                    f32x4* __restrict__ tp)
 {
-    const int64_t sidx = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (sidx >= nsub) return;
+    const int64_t sidx = s0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (sidx >= s1) return;
     const int lane = threadIdx.x & 63;
     const uint2 id = sub_id[sidx];
     const int I = (int)(id.x >> 16), J = (int)(id.x & 0xFFFFu);
     const int k = (int)(id.y >> 4), u = (int)(id.y & 15u);
     const int m = u / 3, kt = u % 3;
     const int r = 16 * m + (lane & 15);
-    const int i = perm[I * kTile + r / 3], p = r % 3;
+    const int pos = I * kTile + r / 3;
+    const int i = perm[pos], p = r % 3;
+    const int64_t li = pos - P0;
     const int sym[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
     f32x4 out[2];
-    // the lane's 8 columns belong to at most 3 column nodes: look each node pair up once
-    int last = -1;
-    int64_t en = -1;   // entry of (i, j, k), -1 when the pair has no lag k
 #pragma unroll
     for (int e = 0; e < 8; e++) {
         const int c = 32 * kt + 8 * (lane >> 4) + e;
         const int jj = c / 6;
         const int part = (c % 6) / 3, q = c % 3;
-        if (jj != last) {
-            last = jj;
-            en = -1;
-            const int j = perm[J * kTile + jj];
-            if (i >= 0 && j >= 0) {
-                const int64_t pr = (int64_t)i * nnd + j;
-                const int2 h = hull[pr];
-                if (h.x <= k && k <= h.y) en = off[pr] + (k - h.x);
-            }
-        }
+        const int j = perm[J * kTile + jj];
         double v = 0.0;
-        if (en >= 0) {
+        if (i >= 0 && j >= 0) {
             const int comp = part == 0 ? sym[p][q] : 6 + 3 * p + q;
-            v = vals[((en / kChunk) * kVals + comp) * kChunk + en % kChunk];
+            for (int a = adj_off[j]; a < adj_off[j + 1]; a++) {
+                const int el = adj[a] >> 3, loc = adj[a] & 7;
+                const int2 pk = pair_k[(int64_t)i * nel + el];
+                if (k < pk.x || k > pk.y) continue;
+                const int64_t sl = pair_slot[li * nel + el] + (k - pk.x);
+                v += scratch[(int64_t)(loc * kVals + comp) * nslots + sl];
+            }
+            if (part == 1 && j == i && k <= 1) v += cterm[9 * i + 3 * p + q] + (p == q ? ext : 0.0);
             if (part) v = -v;
         }
         out[e >> 2][e & 3] = (float)v;
     }
 #pragma unroll
     for (int h = 0; h < 2; h++) tp[(sidx * 2 + h) * 64 + lane] = out[h];
+}
+
+// Rows lower..upper of the dense G and H of lag k from the tile store (the FP32 values): one
+// workgroup per merged-list entry, one wavefront per row tile of the entry's group.
+__global__ void __launch_bounds__(64 * kGroup)
+    k_tile_to_dense(const f32x4* __restrict__ tp, const TPEntry* __restrict__ ent,
+                    const int64_t* __restrict__ grp_off, int nG, const int* __restrict__ perm,
+                    int nT, int k, int lower, int upper, int64_t N, double* __restrict__ G,
+                    double* __restrict__ H)
+{
+    const int64_t e = blockIdx.x;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const TPEntry E = ent[e];
+    if ((int)(E.id & kMaxLag) != k) return;
+    int lo = 0, hi = nG;   // group of entry e: the last g with grp_off[g] <= e
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (grp_off[mid] <= e)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const int I = lo * kGroup + w, J = (int)(E.id >> kKBits);
+    if (I >= nT) return;
+    const unsigned msk = E.msk[w];
+    for (int u = 0; u < 9; u++) {
+        if (!((msk >> u) & 1u)) continue;
+        const int64_t sidx = E.sub[w] + __builtin_popcount(msk & ((1u << u) - 1u));
+        const int m = u / 3, kt = u % 3;
+        const int r = 16 * m + (lane & 15);
+        const int i = perm[I * kTile + r / 3], p = r % 3;
+        if (i < lower || i > upper) continue;
+#pragma unroll
+        for (int e8 = 0; e8 < 8; e8++) {
+            const int c = 32 * kt + 8 * (lane >> 4) + e8;
+            const int jj = c / 6, part = (c % 6) / 3, q = c % 3;
+            const int j = perm[J * kTile + jj];
+            if (j < 0) continue;
+            const float v = tp[(sidx * 2 + (e8 >> 2)) * 64 + lane][e8 & 3];
+            const int64_t o = (int64_t)(3 * (i - lower) + p) * N + 3 * j + q;
+            if (part == 0)
+                G[o] = v;
+            else
+                H[o] = -(double)v;
+        }
+    }
 }
 
 // FP32 copies (see zcopy_elem) of the ring slots of steps s - 2, s - 1, s (the ones the
@@ -186,11 +235,10 @@ void bisect(std::vector<int>& idx, int lo, int hi, const std::vector<double>& X)
 
 }  // namespace
 
-void tp_build(Context& c)
+void tp_plan_structure(Context& c)
 {
-    if (!c.store_ready) build_lag_store(c);
     TPStore& T = c.tp;
-    if (T.ready) return;
+    if (T.planned) return;
     const int nnd = c.nnd;
     std::vector<int> order(nnd);
     std::iota(order.begin(), order.end(), 0);
@@ -346,22 +394,64 @@ void tp_build(Context& c)
     T.d_perm.upload(T.perm.data(), T.perm.size(), c.stream);
     T.d_pos.upload(T.pos.data(), T.pos.size(), c.stream);
     T.ent.upload(ent.data(), ent.size(), c.stream);
-    T.vals.alloc((size_t)std::max<int64_t>(nsub, 1) * kSubFloats);
-    DBuf<uint2> d_sub_id(c.stream);
-    if (nsub > 0) {
-        if (!c.store.d_hull.p || !c.store.d_off.p)
-            throw std::logic_error("tile-pair copy needs the store's node-pair tables");
-        d_sub_id.upload(sub_id.data(), sub_id.size(), c.stream);
-        hipLaunchKernelGGL(k_tp_gather, dim3((unsigned)((nsub + 3) / 4)), dim3(256), 0, c.stream,
-                           c.store.vals.p, c.store.d_hull.p, c.store.d_off.p, T.d_perm.p, nnd,
-                           d_sub_id.p, nsub, reinterpret_cast<f32x4*>(T.vals.p));
-    }
-    TD_HIP(hipGetLastError());
+    T.d_grp_off.upload(goff.data(), goff.size(), c.stream);
+    // stored sub-tiles are ordered by tile pair (I, J): those of row tile I are a range
+    T.h_tile_sub.assign(nT + 1, 0);
+    for (int I = 0; I <= nT; I++) T.h_tile_sub[I] = qsub[(size_t)I * nT];
+    T.h_sub_id = std::move(sub_id);
     TD_HIP(hipStreamSynchronize(c.stream));
-    c.store.d_hull.reset();
-    c.store.d_off.reset();
-    if ((int64_t)nnd * nnd * 24 > kPinnedKeepBytes) c.pin_hull.reserve(0);
+    T.planned = true;
+}
+
+void tp_alloc(Context& c)
+{
+    TPStore& T = c.tp;
+    if (!T.planned) throw std::logic_error("tile store not planned");
+    T.vals.alloc((size_t)std::max<int64_t>(T.nsub, 1) * kSubFloats);
+    T.d_sub_id.upload(T.h_sub_id.data(), T.h_sub_id.size(), c.stream);
+    std::vector<uint2>().swap(T.h_sub_id);
+}
+
+void tp_gather_slots(Context& c, int I0, int I1, const double* scratch, int64_t nslots,
+                     const int64_t* pair_slot, double ext)
+{
+    TPStore& T = c.tp;
+    if (!T.planned || !T.vals.p) throw std::logic_error("tile store not planned");
+    const int64_t s0 = T.h_tile_sub[I0], s1 = T.h_tile_sub[I1];
+    if (s1 <= s0) return;
+    hipLaunchKernelGGL(k_tp_from_slots, dim3((unsigned)((s1 - s0 + 3) / 4)), dim3(256), 0,
+                       c.stream, scratch, nslots, pair_slot, c.pair_k.p, c.adj_off.p, c.adj.p,
+                       c.cterm.p, ext, T.d_perm.p, I0 * kTile, c.nel, T.d_sub_id.p, s0, s1,
+                       reinterpret_cast<f32x4*>(T.vals.p));
+    TD_HIP(hipGetLastError());
+}
+
+void tp_finish(Context& c)
+{
+    TPStore& T = c.tp;
+    TD_HIP(hipStreamSynchronize(c.stream));
+    T.d_sub_id.reset();
     T.ready = true;
+}
+
+void tp_build(Context& c)
+{
+    if (!c.store_ready) build_lag_store(c);
+    if (!c.tp.ready) throw std::logic_error("mixed-precision lag store without its tile store");
+}
+
+void tile_rows(Context& c, int k, int lower, int upper, double* dG, double* dH)
+{
+    const TPStore& T = c.tp;
+    const int64_t N = 3LL * c.nnd;
+    const int64_t n = 3LL * (upper - lower + 1) * N;
+    TD_HIP(hipMemsetAsync(dG, 0, n * sizeof(double), c.stream));
+    TD_HIP(hipMemsetAsync(dH, 0, n * sizeof(double), c.stream));
+    if (!T.ready || T.nentries == 0) return;
+    hipLaunchKernelGGL(k_tile_to_dense, dim3((unsigned)T.nentries), dim3(64 * kGroup), 0,
+                       c.stream, reinterpret_cast<const f32x4*>(T.vals.p), T.ent.p,
+                       T.d_grp_off.p, T.nG, T.d_perm.p, T.nT, k, lower, upper, N, dG, dH);
+    TD_HIP(hipGetLastError());
 }
 
 void tp_prepare_step(Context& c, const double* ring, int N, int nrhs, int s, int R, float* zs,

@@ -155,6 +155,7 @@ void set_max_step(Context& c, int K)
     c.store_ready = false;
     c.ranges_ready = false;
     c.tp.ready = false;
+    c.tp.planned = false;
 }
 
 Mrt* mrt_create(Context* c1, Context* c2, int max_step, int nz, const int* if1, const int* if2)

@@ -72,59 +72,6 @@ __device__ __forceinline__ void accumulate_hl(R t, R dt, R inv_dt, R r, R inv_r2
     S[5] += cI * I6 + cJ * J6;
 }
 
-// FP32 form of hat_integrals for the mixed-precision assembly.  With a = t - r/c (formed in FP64
-// by the caller) and d2 = 2 r / c, (t - tau)^2 / r^2 - 1/c^2 = (a - tau)(a - tau + d2) / r^2, so
-//   I1 = 1/(dt r^2) int_0^b (dt - tau)(a - tau)(a - tau + d2) dtau,
-//   J1 = 1/(dt r^2) int_0^b tau (a - tau)(a - tau + d2) dtau,
-// cubic integrands whose factors are all >= 0 on [0, b] (b <= a): two-point Gauss-Legendre is
-// exact for them and adds only positive terms, so nothing cancels near the wave's arrival (the
-// expanded form above loses the digits of t^2/r^2 - 1/c^2 there).  I3, I5, J3, J5 as above.
-__device__ __forceinline__ void hat_integrals_f32(float a, float dt, float inv_dt, float d2,
-                                                  float inv_r2, float& I1, float& I3, float& I5,
-                                                  float& J1, float& J3, float& J5)
-{
-    float b;
-    if (dt <= a) {
-        b = dt;
-        I5 = 0.f;
-        J5 = 0.f;
-    } else if (0.f <= a) {
-        b = a;
-        I5 = (dt - a) * inv_dt;
-        J5 = a * inv_dt;
-    } else {
-        b = 0.f;
-        I5 = 0.f;
-        J5 = 0.f;
-    }
-    const float h = 0.5f * b, g = 0.57735026918962576f * h;  // 1/sqrt(3)
-    const float tm = h - g, tp = h + g;
-    const float em = (a - tm) * (a - tm + d2), ep = (a - tp) * (a - tp + d2);
-    const float sc = h * inv_r2 * inv_dt;
-    I1 = ((dt - tm) * em + (dt - tp) * ep) * sc;
-    J1 = (tm * em + tp * ep) * sc;
-    I3 = (dt * b - 0.5f * b * b) * inv_dt;
-    J3 = 0.5f * b * b * inv_dt;
-}
-
-// S[0..5] += cI * (I1..I6)(t) + cJ * (J1..J6)(t) in FP32; the retarded times t - r/c in FP64.
-__device__ __forceinline__ void accumulate_hl_f32(double t, double r, float dt, float inv_dt,
-                                                  float inv_r2, double inv_cp, double inv_cs,
-                                                  float cI, float cJ, float* S)
-{
-    float I1, I3, I5, J1, J3, J5, I2, I4, I6, J2, J4, J6;
-    hat_integrals_f32((float)(t - r * inv_cp), dt, inv_dt, (float)(2.0 * r * inv_cp), inv_r2, I1,
-                      I3, I5, J1, J3, J5);
-    hat_integrals_f32((float)(t - r * inv_cs), dt, inv_dt, (float)(2.0 * r * inv_cs), inv_r2, I2,
-                      I4, I6, J2, J4, J6);
-    S[0] += cI * I1 + cJ * J1;
-    S[1] += cI * I2 + cJ * J2;
-    S[2] += cI * I3 + cJ * J3;
-    S[3] += cI * I4 + cJ * J4;
-    S[4] += cI * I5 + cJ * J5;
-    S[5] += cI * I6 + cJ * J6;
-}
-
 // From S to the 15 block values of one point, already transposed into the global block layout
 // (row = collocation component p, column = node component q):
 //   G[p][q] = U[q][p] (symmetric: g00 g01 g02 g11 g12 g22)

@@ -200,10 +200,6 @@ struct LagStore {
     DBuf<int64_t> row_valid;      // [nnd] entries before padding, per row node
     std::vector<int64_t> h_row_chunk, h_row_valid;
     int64_t nonzero_entries = 0;  // before padding
-    // node-pair tables on the device, kept from the build of a mixed-precision store until the
-    // tile-pair copy has been gathered from it: lag range and first entry of each pair
-    DBuf<int2> d_hull;            // [nnd*nnd]
-    DBuf<int64_t> d_off;          // [nnd*nnd]
 };
 
 // Row tiles per group of the mixed-precision convolution: one wavefront each in a workgroup.
@@ -239,7 +235,14 @@ struct TPStore {
     int nchunks = 0;
     DBuf<int4> chunk;                // (group, first entry, end entry, 0)
     DBuf<int> grp_chunk;             // [nG + 1] chunks of each group
-    bool ready = false;
+    DBuf<int64_t> d_grp_off;         // [nG + 1] first merged-list entry of each group
+    // planning (tp_plan_structure) before the sub-tiles are filled: the stored sub-tiles'
+    // (I << 16 | J, k << 4 | u), ordered by tile pair, and the first sub-tile of each row tile
+    std::vector<uint2> h_sub_id;
+    DBuf<uint2> d_sub_id;
+    std::vector<int64_t> h_tile_sub; // [nT + 1]
+    bool planned = false;            // structure planned from the node-pair hulls
+    bool ready = false;              // sub-tiles filled
 };
 
 struct Context {
@@ -273,12 +276,20 @@ struct Context {
 
     LagStore store;
     bool store_ready = false;
+    // in-shell (row node, column node, lag) entries of the full store; the FP64 store holds all
+    // of them (store_lags = Kc), the mixed-precision build only those of lags 0 and 1, the rest
+    // living in the tile store (store_lags = 1)
+    int64_t pair_lags = 0;
+    int store_lags = 0;
     TPStore tp;
     int precision = 0;  // 0: FP64 march, 1: split-BF16 MFMA convolution (mixed)
     int store_precision = 0;  // precision the lag store was assembled with
 
     // march state (march.hip)
     struct March* march = nullptr;
+    // batches holding this context (batch.hip): while nonzero, the calls that would free or
+    // re-initialise the march state or the store the batch reads are refused
+    int batch_refs = 0;
 
     // Optional timing of the hot kernels with HIP events on the context stream.
     int profile = 0;   // 0 off, 1 every timed kernel, 2 the history convolution only
@@ -301,6 +312,8 @@ void assemble_step_dense(Context& c, int step, int lower, int upper, double* dG1
                          double* dH1, double* dH2);
 void build_lag_store(Context& c);
 void lag_dense(Context& c, int k, double* dG, double* dH);   // [N][N] each, device
+// rows of collocation nodes lower..upper of lag k: [3 (upper - lower + 1)][N] each, device
+void lag_rows(Context& c, int k, int lower, int upper, double* dG, double* dH);
 int64_t predict_store_bytes(Context& c);
 
 // march.hip
@@ -368,6 +381,11 @@ void march_set_step(Context& c, int s);
 struct Batch;
 Batch* batch_create(const std::vector<Context*>& ctxs);
 void batch_destroy(Batch* b);
+// membership changes take effect at the next run or tune (a new plan and graph); a member's
+// batch_refs is held until it leaves or the batch is destroyed
+void batch_add(Batch& b, Context* c);
+void batch_remove(Batch& b, Context* c);
+int batch_size(const Batch& b);
 // times the tunables on the batch; report = {rounds, apply waves, conv ms, apply ms}
 void batch_tune(Batch& b, double* report);
 void batch_set_params(Batch& b, int rounds, int apply_waves);
@@ -375,7 +393,19 @@ void batch_set_params(Batch& b, int rounds, int apply_waves);
 int batch_run(Batch& b, int steps_per_graph, int max_replays);
 
 // mixed.hip
+// Tile store of the mixed-precision march.  tp_plan_structure plans it from the node-pair hulls
+// (c.pin_hull) on the host: bisection order, stored sub-tiles, merged lists and their work;
+// tp_alloc allocates the sub-tiles; tp_gather_slots fills those of row tiles [I0, I1) from the
+// lag kernel's slots of their rows (batch rows = bisection positions 16 I0 ..); tp_finish marks
+// it complete.  tp_build checks that a built mixed-precision store has its tiles.
+void tp_plan_structure(Context& c);
+void tp_alloc(Context& c);
+void tp_gather_slots(Context& c, int I0, int I1, const double* scratch, int64_t nslots,
+                     const int64_t* pair_slot, double ext);
+void tp_finish(Context& c);
 void tp_build(Context& c);
+// rows lower..upper of lag k from the tile store (FP32 values), dense [3 (upper-lower+1)][N]
+void tile_rows(Context& c, int k, int lower, int upper, double* dG, double* dH);
 // start of step s of the mixed march (one launch): FP32 copies of ring slots s - 2 .. s in the
 // B-operand order of the mixed convolution, and the load factors fac[r] = f_r(t_load)
 void tp_prepare_step(Context& c, const double* ring, int N, int nrhs, int s, int R, float* zs,

@@ -47,7 +47,6 @@ def lib():
                                    C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int,
                                    C.POINTER(vp)]
         L.tdbem_destroy.argtypes = [vp]
-        L.tdbem_destroy.restype = None
         L.tdbem_last_error.argtypes = [vp]
         L.tdbem_last_error.restype = C.c_char_p
         L.tdbem_max_step.argtypes = [vp]
@@ -57,6 +56,7 @@ def lib():
         L.tdbem_lag_store_bytes.argtypes = [vp]
         L.tdbem_lag_store_bytes.restype = C.c_int64
         L.tdbem_get_lag.argtypes = [vp, C.c_int, dp, dp]
+        L.tdbem_get_lag_rows.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
         L.tdbem_march_init.argtypes = [vp, C.c_double, C.c_double, C.c_int, ip, dp, C.c_int, ip,
                                        dp, C.c_int, dp, dp]
         L.tdbem_march_run.argtypes = [vp, C.c_int]
@@ -72,6 +72,9 @@ def lib():
         L.tdbem_mfma_f64_selftest.argtypes = [dp, dp, dp]
         L.tdbem_dense_solve.argtypes = [C.c_int, dp, C.c_int, dp, C.c_int]
         L.tdbem_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
+        L.tdbem_batch_add.argtypes = [vp, vp]
+        L.tdbem_batch_remove.argtypes = [vp, vp]
+        L.tdbem_batch_size.argtypes = [vp]
         L.tdbem_batch_tune.argtypes = [vp, dp]
         L.tdbem_batch_set_params.argtypes = [vp, C.c_int, C.c_int]
         L.tdbem_batch_run.argtypes = [vp, C.c_int, C.c_int, ip]
@@ -131,8 +134,11 @@ class Context:
             raise TdbemError(rc, lib().tdbem_last_error(self._h).decode())
 
     def close(self):
+        """Frees the context; raises TdbemError (ERR_STATE) while it is a member of a batch."""
         if getattr(self, "_h", None):
-            lib().tdbem_destroy(self._h)
+            rc = lib().tdbem_destroy(self._h)
+            if rc:
+                raise TdbemError(rc, lib().tdbem_last_error(self._h).decode())
             self._h = None
 
     def __del__(self):
@@ -175,6 +181,14 @@ class Context:
         G = np.zeros((self.N, self.N))
         H = np.zeros((self.N, self.N))
         self._check(lib().tdbem_get_lag(self._h, k, _d(G), _d(H)))
+        return G, H
+
+    def get_lag_rows(self, k, lower, upper):
+        """Rows of collocation nodes lower..upper of lag k: G, H [3 (upper - lower + 1)][N]."""
+        G = np.zeros((3 * (upper - lower + 1), self.N))
+        H = np.zeros_like(G)
+        self._check(lib().tdbem_get_lag_rows(self._h, int(k), int(lower), int(upper), _d(G),
+                                             _d(H)))
         return G, H
 
     def march_init(self, a1=0.25, a2=0.25, ubc_dof=(), ubc_val=(), fbc_dof=(), fbc_val=(),
@@ -364,11 +378,13 @@ def prepare_contexts(problems, loads, precision=PRECISION_MIXED, workers=8):
 
 class Batch:
     """Scenarios marched together (tdbem_batch_*): each context after a mixed-precision
-    march_init with the same number of right-hand sides; results stay in the contexts."""
+    march_init with the same number of right-hand sides; results stay in the contexts.  Members
+    may join (add) and leave (remove) between runs; a member cannot be closed or re-initialised
+    until it has left."""
 
-    def __init__(self, contexts):
+    def __init__(self, contexts=()):
         self.contexts = list(contexts)
-        arr = (C.c_void_p * len(self.contexts))(*[c._h for c in self.contexts])
+        arr = (C.c_void_p * max(1, len(self.contexts)))(*[c._h for c in self.contexts])
         h = C.c_void_p()
         rc = lib().tdbem_batch_create(arr, len(self.contexts), C.byref(h))
         if rc:
@@ -378,6 +394,18 @@ class Batch:
     def _check(self, rc):
         if rc:
             raise TdbemError(rc, lib().tdbem_last_error(None).decode())
+
+    def add(self, ctx):
+        """Admits a context (its first step is solved on its own stream if not yet done)."""
+        self._check(lib().tdbem_batch_add(self._h, ctx._h))
+        self.contexts.append(ctx)
+
+    def remove(self, ctx):
+        self._check(lib().tdbem_batch_remove(self._h, ctx._h))
+        self.contexts.remove(ctx)
+
+    def __len__(self):
+        return lib().tdbem_batch_size(self._h)
 
     def tune(self):
         """Times the tunables on this batch and keeps the fastest."""

@@ -23,6 +23,28 @@ def rel_fro_set(got, ref):
     return float(np.sqrt(num / den)) if den > 0 else float(np.sqrt(num))
 
 
+def shell_parity(got, ref):
+    """Parity of lag matrices that the GPU holds only inside each node pair's shell (exact zeros
+    outside) with the oracle's dense ones, which -- like the reference -- evaluate the Step kernel
+    HL(t) - HL(t - dt) at every lag: past a pair's static time (t - r/c_s >= dt at every
+    quadrature point) the two terms are equal and the difference is rounding noise, eps t^2 / r^2
+    at the near-singular points, growing with the number of lags.  Returns (in_shell, outside):
+    the relative Frobenius error over the entries the GPU holds, and the norm of the oracle's
+    entries where the GPU holds zeros, both relative to the oracle's norm over the whole set."""
+    num = den = out = 0.0
+    for g, r in zip(got, ref):
+        zero = g == 0
+        out += float(np.linalg.norm(r[zero])) ** 2
+        num += float(np.linalg.norm((g - r)[~zero])) ** 2
+        den += float(np.linalg.norm(r)) ** 2
+    return float(np.sqrt(num / den)), float(np.sqrt(out / den))
+
+
+def shell_masked(lags, got):
+    """The oracle's lags with the entries where the GPU store holds zeros set to zero."""
+    return np.stack([np.where(g == 0, 0.0, r) for g, r in zip(got, lags)])
+
+
 def plane_1e():
     """geometry/HalfSpacePlane_1E.txt: one unit square element, E=5e7, nu=0.25, rho=1730,
     dt=5e-3, if_ext=1, traction 1 on the z dofs."""

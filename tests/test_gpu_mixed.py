@@ -61,12 +61,14 @@ def test_mixed_march_matches_fp64(gpu, case, nrhs):
 
 
 @pytest.mark.parametrize("case", ["sphere", "bar"])
-def test_fp32_regular_quadrature_store(gpu, case):
-    """Mixed precision assembles regular far pairs with FP32 time integrals and kernels; the lag
-    store stays within 1e-6 of the FP64 store over all lags (relative Frobenius), and switching
-    back to FP64 rebuilds the FP64 store exactly."""
+def test_mixed_tile_store_is_the_fp64_store_rounded(gpu, case):
+    """The mixed-precision build assembles in FP64 and keeps every lag only as FP32 sub-tiles
+    (filled straight from the lag kernel's slots, no FP64 copy of the whole store): each stored
+    value is the FP64 store's value rounded once to FP32, so the lags agree to FP32 rounding
+    (relative Frobenius over all lags <= 1e-7, and not exactly); switching back to FP64
+    rebuilds the FP64 store exactly."""
     p = {"sphere": helpers.small_sphere, "bar": helpers.small_bar}[case](numstep=40)
-    print(f"[gpu case] fp32 quadrature store {case}", flush=True)
+    print(f"[gpu case] mixed tile store {case}", flush=True)
     with api.Context.from_problem(p) as c64, api.Context.from_problem(p) as c32:
         c64.build_lag_store()
         c32.set_precision(api.PRECISION_MIXED)
@@ -75,11 +77,14 @@ def test_fp32_regular_quadrature_store(gpu, case):
         for k in range(c64.max_step + 1):
             G64, H64 = c64.get_lag(k)
             G32, H32 = c32.get_lag(k)
+            # the same FP64 sums, rounded once
+            assert np.array_equal(G32, G64.astype(np.float32).astype(np.float64)), k
+            assert np.array_equal(H32, H64.astype(np.float32).astype(np.float64)), k
             num += np.linalg.norm(G32 - G64) ** 2 + np.linalg.norm(H32 - H64) ** 2
             den += np.linalg.norm(G64) ** 2 + np.linalg.norm(H64) ** 2
         err = np.sqrt(num / den)
-        print(f"fp32 store rel err {err:.3e}")
-        assert 0 < err <= 1e-6
+        print(f"tile store rel err {err:.3e}")
+        assert 0 < err <= 1e-7
         c32.set_precision(api.PRECISION_FP64)
         c32.build_lag_store()
         for k in range(c64.max_step + 1):
