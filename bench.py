@@ -53,9 +53,11 @@ def parse():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenario leg")
-    ap.add_argument("--batch-scenarios", type=int, default=64)
-    ap.add_argument("--serve-requests", type=int, default=16)
-    ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrivals per second")
+    # the first scenarios / requests of the seeded stated-size sequences; the whole 64-scenario
+    # and 200-request runs take some minutes each (--batch-scenarios 64 --serve-requests 200)
+    ap.add_argument("--batch-scenarios", type=int, default=12)
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (3 launches) to scale HBM counters by")
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -122,8 +124,11 @@ def timed_solves(ctx, p, warmup, steps, amp=None, rise=None):
     return time.perf_counter() - t0, ctx.timings()
 
 
+_T0 = time.perf_counter()
+
+
 def log(msg):
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    print(f"[bench {time.perf_counter() - _T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
 def copy_calibration(api):
@@ -138,77 +143,48 @@ def copy_calibration(api):
                           "avg_ms": round(rd, 4), "GBs": round((8 << 30) / rd / 1e6, 1)}}
 
 
-def serve_leg(api, configs, serve, nreq, rate):
-    """Continuous-batching serving of mixed-precision solves (bar and sphere geometries of 72 to
-    864 elements, 100 to 300 steps, 16 or 32 load cases, Poisson arrivals), against the same
-    requests solved one after another on the plain FP64 path with one load case each; then the
-    same requests at 2x, 4x and 8x the arrival rate (rate_sweep)."""
+def _engine_summary(res, summary, probs):
+    ok = [r for r in res if not r.error]
+    out = dict(summary)
+    out.update({
+        "elements_range": [int(min(p.nel for p in probs)), int(max(p.nel for p in probs))],
+        "elements_total": int(sum(p.nel for p in probs)),
+        "steps_range": [int(min(p.numstep for p in probs)), int(max(p.numstep for p in probs))],
+        "predicted_resident_bytes_total": int(sum(r.predicted_resident for r in ok)),
+        "predicted_resident_bytes_max": int(max((r.predicted_resident for r in ok), default=0)),
+    })
+    return out
+
+
+def batch_leg(api, configs, serve, count, nrhs=16):
+    """Uneven half-space cavity scenarios at the stated sizes (free plane 8x8 + spherical cavity of
+    radius U[0.5, 1.5] at depth U[2, 4]; elements U[400, 4000], steps U[100, 800], nu U[0.2,
+    0.35], seed 4), nrhs pressure histories each, all arriving at once: the continuous-batching
+    engine admits them by their predicted device bytes, sets them up 4 at a time while the
+    admitted ones march together in one batch (hipGraph replays of 8 steps of every member), and
+    lets the next ones in as finished ones leave.  Time-steps/s over the whole job, assembly
+    included."""
+    scen = configs.halfspace_scenarios(count=count, seed=4)
+    reqs = serve.poisson_requests(scen, rate=None, nrhs=nrhs, seed=40)
+    res, summary = serve.Engine(nrhs=nrhs, setup_workers=4, steps_per_graph=8,
+                                tune=True).run(reqs)
+    out = _engine_summary(res, summary, scen)
+    out.update({"scenarios": count, "nrhs": nrhs})
+    return out
+
+
+def serve_leg(api, configs, serve, nreq, rate, nrhs=32):
+    """Serving at the stated sizes: Poisson arrivals (rate per second, seed 5) of solves of the
+    bar and sphere geometries, element counts log-uniform in [300, 6000], steps U[100, 1000],
+    32 pressure histories each, through the continuous-batching engine (admission by predicted
+    device bytes against the device's free memory, set-ups beside the batched march)."""
     probs = configs.serving_mix(nreq, seed=5)
-    reqs = serve.poisson_requests(probs, rate=rate, seed=5)
-    res, summary = serve.Server(max_active=8).run(reqs)
-    t0 = time.perf_counter()
-    steps = 0
-    for p in probs:
-        with api.Context.from_problem(p) as ctx:
-            ctx.build_lag_store()
-            ctx.march_init(p.a1, p.a2, p.ubc_dof, p.ubc_val, p.fbc_dof, p.fbc_val)
-            ctx.march_run()
-            ctx.synchronize()
-        steps += p.numstep - 1
-    serial = time.perf_counter() - t0
-    summary.update({"arrival_rate_per_s": rate,
-                    "elements": [int(p.nel) for p in probs],
-                    "plain_fp64_serial": {"time_steps": steps, "seconds": round(serial, 3),
-                                          "time_steps_per_s": round(steps / serial, 2)}})
-    summary["speedup_vs_plain_fp64_serial"] = round(summary["time_steps_per_s"] /
-                                                    (steps / serial), 2)
-    # the same requests at higher arrival rates: throughput up to the device's, latency after
-    sweep = []
-    for r in (2 * rate, 4 * rate, 8 * rate):
-        _, s_r = serve.Server(max_active=8).run(serve.poisson_requests(probs, rate=r, seed=5))
-        sweep.append({"arrival_rate_per_s": r, "time_steps_per_s": s_r["time_steps_per_s"],
-                      "latency_p50_s": s_r["latency_p50_s"], "latency_p99_s": s_r["latency_p99_s"],
-                      "makespan_s": s_r["makespan_s"], "failed": s_r["failed"]})
-    summary["rate_sweep"] = sweep
-    return summary
-
-
-def batch_leg(api, configs, count, nrhs=16):
-    """Uneven half-space cavity scenarios (free plane 8x8 + spherical cavity of radius U[0.5,
-    1.5] at depth U[2, 4], 200 to 800 elements, 100 to 400 steps, nu U[0.2, 0.35], seed 4),
-    each with nrhs pressure histories, marched together by one batch: assembly and step-0
-    solves per scenario (8 scenarios' set-ups at a time on their own streams), autotuning,
-    then hipGraph replays of 16 steps of all scenarios."""
-    scen = configs.halfspace_scenarios(count=count, seed=4, el_range=(200, 800),
-                                       step_range=(100, 400))
-    t0 = time.perf_counter()
-    loads = [configs.rhs_set(nrhs, p.dt, seed=40 + i) for i, p in enumerate(scen)]
-    # set-ups of 8 scenarios at a time (own streams, concurrent on the device)
-    ctxs = api.prepare_contexts(scen, loads, precision=api.PRECISION_MIXED, workers=8)
-    store = sum(c.lag_store_bytes() for c in ctxs)
-    t_setup = time.perf_counter() - t0
-    b = api.Batch(ctxs)
-    t1 = time.perf_counter()
-    tuned = b.tune()
-    t_tune = time.perf_counter() - t1
-    t2 = time.perf_counter()
-    launches = b.run(steps_per_graph=16)
-    t_march = time.perf_counter() - t2
-    total = time.perf_counter() - t0
-    assert all(c.step_index == p.numstep - 1 for c, p in zip(ctxs, scen))
-    steps = sum((p.numstep - 1) * nrhs for p in scen)
-    b.close()
-    for c in ctxs:
-        c.close()
-    return {"scenarios": count, "nrhs": nrhs, "elements_total": int(sum(p.nel for p in scen)),
-            "elements_range": [int(min(p.nel for p in scen)), int(max(p.nel for p in scen))],
-            "steps_range": [int(min(p.numstep for p in scen)), int(max(p.numstep for p in scen))],
-            "fp64_store_bytes": int(store), "time_steps": int(steps),
-            "time_steps_per_s": round(steps / total, 2),
-            "march_time_steps_per_s": round(steps / t_march, 2),
-            "seconds": {"assembly_and_step0": round(t_setup, 3), "tune": round(t_tune, 3),
-                        "march": round(t_march, 3)},
-            "graph_launches": launches, "tuned": tuned}
+    reqs = serve.poisson_requests(probs, rate=rate, nrhs=nrhs, seed=5)
+    res, summary = serve.Engine(nrhs=nrhs, setup_workers=4, steps_per_graph=8,
+                                tune=True).run(reqs)
+    out = _engine_summary(res, summary, probs)
+    out.update({"arrival_rate_per_s": rate, "nrhs": nrhs})
+    return out
 
 
 def roofline(kernel, avg_ms, nbytes, traffic=None):
@@ -351,16 +327,19 @@ def main():
             out["ms_per_step"] = out["plain_fp64"]["ms_per_step"]
             out["dtype"] = "f64"
             out["roofline"] = out["plain_fp64"]["roofline"]
+    from tdbem import serve
     if not args.no_batch:
-        log(f"batched leg: {args.batch_scenarios} half-space scenarios")
-        out["batched"] = batch_leg(api, configs, args.batch_scenarios)
+        log(f"batched leg: {args.batch_scenarios} half-space scenarios at the stated sizes")
+        out["batched"] = batch_leg(api, configs, serve, args.batch_scenarios)
         if "plain_fp64" in out:
             out["batched"]["speedup_vs_plain_fp64"] = round(
                 out["batched"]["time_steps_per_s"] / out["plain_fp64"]["value"], 2)
     if not args.no_serve:
         log(f"serving leg: {args.serve_requests} requests at {args.serve_rate}/s")
-        from tdbem import serve
         out["serving"] = serve_leg(api, configs, serve, args.serve_requests, args.serve_rate)
+        if "plain_fp64" in out:
+            out["serving"]["speedup_vs_plain_fp64"] = round(
+                out["serving"]["time_steps_per_s"] / out["plain_fp64"]["value"], 2)
     if args.copy_calibration:
         log("copy calibration kernel")
         out["copy_calibration"] = copy_calibration(api)

@@ -76,6 +76,16 @@ int tdbem_assemble_step_device(tdbem_context* ctx, int step, int lower, int uppe
 int tdbem_build_lag_store(tdbem_context* ctx);
 /* Bytes of the shell store (predicted when not yet built). */
 int64_t tdbem_lag_store_bytes(tdbem_context* ctx);
+/* Device bytes a solve of this context needs at nrhs right-hand sides in the current precision,
+ * from the store planner's exact counts (node-pair lags, stored sub-tiles, slot batches) and the
+ * march's allocations, without building anything: *resident while it marches (tables, store,
+ * march state), *peak at any point of its set-up (with the build's node-pair tables and slot
+ * buffers, or the dense step-0 matrices).  Admission control for serving (no counterpart in the
+ * reference).  The device's memory pool may keep up to 16 GiB of freed set-up memory besides. */
+int tdbem_predict_bytes(tdbem_context* ctx, int nrhs, int64_t* resident, int64_t* peak);
+/* Free and total device memory after a device synchronisation with the memory pool trimmed
+ * (hipMemGetInfo): what the predictions above are checked against. */
+int tdbem_device_memory(int64_t* free_bytes, int64_t* total_bytes);
 /* Dense N x N (N = 3 nnd) lag k of G and H on the host: fG_DB[k], fH_DB[k]. */
 int tdbem_get_lag(tdbem_context* ctx, int k, double* G, double* H);
 /* The rows of collocation nodes lower..upper of lag k, [3 (upper - lower + 1)][N] each: one

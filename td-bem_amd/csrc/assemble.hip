@@ -1180,9 +1180,65 @@ void build_mixed(Context& c)
 
 }  // namespace
 
-int64_t predict_store_bytes(Context& c)
+namespace {
+
+// Largest slot count and row count of the batches slot_pass forms over `order` (see there).
+template <int ENND>
+void slot_batches(Context& c, const std::vector<int>& order, int align, const int2* hpk,
+                  int64_t& max_slots, int& max_rows)
+{
+    const bool ident = order.empty();
+    const int npos = ident ? c.nnd : (int)order.size();
+    std::vector<int64_t> rs(npos, 0);
+    parallel_for(npos, [&](int64_t lo, int64_t hi) {
+        for (int64_t q = lo; q < hi; q++) {
+            const int i = ident ? (int)q : order[q];
+            if (i < 0) continue;
+            int64_t n = 0;
+            for (int e = 0; e < c.nel; e++) {
+                const int2 k = hpk[(int64_t)i * c.nel + e];
+                if (k.y >= k.x) n += k.y - k.x + 1;
+            }
+            rs[q] = n;
+        }
+    });
+    const int64_t slot_bytes = (int64_t)ENND * kVals * sizeof(double);
+    max_slots = 1;
+    max_rows = 1;
+    for (int b0 = 0; b0 < npos;) {
+        int b1 = b0;
+        int64_t slots = 0;
+        while (b1 < npos) {
+            const int g1 = std::min(npos, b1 + align);
+            int64_t add = 0;
+            for (int q = b1; q < g1; q++) add += rs[q];
+            if (b1 > b0 && (slots + add) * slot_bytes > kScratchBytes) break;
+            slots += add;
+            b1 = g1;
+        }
+        max_slots = std::max(max_slots, slots);
+        max_rows = std::max(max_rows, b1 - b0);
+        b0 = b1;
+    }
+}
+
+// Device bytes of one slot pass's buffers (scratch, slot lists, row map).
+int64_t slot_pass_bytes(const Context& c, int64_t max_slots, int max_rows)
+{
+    return max_slots * ((int64_t)c.ennd * kVals * 8 + 4 + 8) + (int64_t)max_rows * (c.nel * 8 + 4);
+}
+
+}  // namespace
+
+void predict_store(Context& c, int64_t& store, int64_t& transient)
 {
     compute_pair_ranges(c);
+    const int64_t nn = (int64_t)c.nnd * c.nnd, np = (int64_t)c.nnd * c.nel;
+    const int64_t rows_bytes = (int64_t)(c.nnd + 1) * 8 + (int64_t)c.nnd * 8;
+    const int64_t entry_bytes = kChunk * (int64_t)(kVals * sizeof(double) + sizeof(uint32_t));
+    const int2* hpk = host_pair_ranges(c);
+    int64_t max_slots = 0;
+    int max_rows = 0;
     if (c.precision == 1) {
         if (!c.tp.planned) {
             ShellPlan Pf;
@@ -1190,11 +1246,41 @@ int64_t predict_store_bytes(Context& c)
             c.pair_lags = Pf.nonzero;
             tp_plan_structure(c);
         }
-        return c.tp.nsub * (int64_t)kSubFloats * 4 + c.tp.nentries * (int64_t)sizeof(TPEntry);
+        const TPStore& T = c.tp;
+        // the FP64 store of lags 0 and 1
+        DBuf<int2> pk01(c.stream);
+        pk01.alloc(np);
+        hipLaunchKernelGGL(k_clip_pairs, dim3(grid_for(np)), dim3(kBlock), 0, c.stream,
+                           c.pair_k.p, np, 1, pk01.p);
+        TD_HIP(hipGetLastError());
+        ShellPlan P01;
+        plan_shells(c, P01, pk01.p, false);
+        const int64_t mini = P01.row_chunk[c.nnd] * entry_bytes + rows_bytes;
+        store = T.nsub * (int64_t)kSubFloats * 4 + T.nentries * (int64_t)sizeof(TPEntry) +
+                (int64_t)T.nT * kTile * 4 + (int64_t)c.nnd * 4 + (int64_t)(T.nG + 1) * 8 + mini;
+        if (c.ennd == 4)
+            slot_batches<4>(c, T.perm, kTile, hpk, max_slots, max_rows);
+        else
+            slot_batches<8>(c, T.perm, kTile, hpk, max_slots, max_rows);
+        // the tile pass (sub-tile list and slot buffers) outweighs the lags-0..1 pass
+        transient = std::max(nn * 16 + np * 8, T.nsub * 8 + slot_pass_bytes(c, max_slots, max_rows));
+    } else {
+        ShellPlan P;
+        plan_shells(c, P, c.pair_k.p, false);
+        store = P.row_chunk[c.nnd] * entry_bytes + rows_bytes;
+        if (c.ennd == 4)
+            slot_batches<4>(c, std::vector<int>(), 1, hpk, max_slots, max_rows);
+        else
+            slot_batches<8>(c, std::vector<int>(), 1, hpk, max_slots, max_rows);
+        transient = std::max(nn * 16, slot_pass_bytes(c, max_slots, max_rows));
     }
-    ShellPlan P;
-    plan_shells(c, P, c.pair_k.p, false);
-    return P.row_chunk[c.nnd] * kChunk * (int64_t)(kVals * sizeof(double) + sizeof(uint32_t));
+}
+
+int64_t predict_store_bytes(Context& c)
+{
+    int64_t store = 0, transient = 0;
+    predict_store(c, store, transient);
+    return store;
 }
 
 void build_lag_store(Context& c)

@@ -184,10 +184,6 @@ void batch_plan(Batch& B)
     B.drop_graph();
     if (B.S == 0) return;
     const int NT = B.nrhs / 16;
-    const int target = B.rounds * tp_resident_groups(B.nrhs, B.device);
-    std::vector<double> work(B.S);
-    double total = 0;
-    for (int sc = 0; sc < B.S; sc++) total += work[sc] = tp_work(*B.ctx[sc]);
     std::vector<int4> ch;
     std::vector<int> gch;
     std::vector<int> oz(B.S + 1, 0), orr(B.S + 1, 0), orhs(B.S + 1, 0), oa(B.S + 1, 0);
@@ -195,8 +191,8 @@ void batch_plan(Batch& B)
         Context& c = *B.ctx[sc];
         BatchScen& d = B.h_desc[sc];
         d.gch0 = (int)gch.size();
-        const int share = total > 0 ? (int)std::lround(target * work[sc] / total) : 1;
-        tp_chunks(c, share, sc, ch, gch);
+        // each scenario is cut as it would be alone (its results do not depend on the mix)
+        tp_chunks(c, tp_chunk_target(c, B.rounds), sc, ch, gch);
         d.nsplit = apply_splits(d.N, B.apply_waves);
         const size_t need = (size_t)d.nsplit * d.nrhs * d.Np;
         auto& pb = B.parts[&c];

@@ -467,6 +467,54 @@ int tdbem_get_lag_rows(tdbem_context* ctx, int k, int lower, int upper, double* 
     });
 }
 
+int tdbem_predict_bytes(tdbem_context* ctx, int nrhs, int64_t* resident, int64_t* peak)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!resident || !peak) throw std::invalid_argument("null output");
+        if (c.kernel != TDBEM_KERNEL_STEP)
+            throw std::invalid_argument("the lag store is built for the Step kernel");
+        if (c.precision == TDBEM_PRECISION_FP64 && nrhs != 1 && nrhs != 2 && nrhs != 4 &&
+            nrhs != 8)
+            throw std::invalid_argument("FP64 march: nrhs must be 1, 2, 4 or 8");
+        if (c.precision == TDBEM_PRECISION_MIXED_BF16X3 && nrhs != 16 && nrhs != 32 &&
+            nrhs != 48 && nrhs != 64)
+            throw std::invalid_argument("mixed march: nrhs must be 16, 32, 48 or 64");
+        const int64_t nq2 = (int64_t)c.nq * c.nq;
+        // what the context holds from its creation and the assembly tables of the first build
+        const int64_t base = (int64_t)c.nnd * 3 * 8 + (int64_t)c.nel * c.ennd * 4 + 2LL * c.nq * 8 +
+                             c.nel * nq2 * (7 + c.ennd) * 8 + (int64_t)(c.nnd + 1) * 4 +
+                             (int64_t)c.nel * c.ennd * 4 + (int64_t)c.nnd * 9 * 8 +
+                             (int64_t)c.nnd * c.nel * 8;
+        int64_t store = 0, transient = 0;
+        predict_store(c, store, transient);
+        const int64_t state = march_state_bytes(c, nrhs);
+        *resident = base + store + state;
+        *peak = std::max(base + store + transient,
+                         base + store + state + march_init_transient_bytes(c));
+    });
+}
+
+int tdbem_device_memory(int64_t* free_bytes, int64_t* total_bytes)
+{
+    if (!free_bytes || !total_bytes) return TDBEM_ERR_ARG;
+    try {
+        int dev = 0;
+        TD_HIP(hipGetDevice(&dev));
+        TD_HIP(hipDeviceSynchronize());
+        hipMemPool_t pool = nullptr;
+        TD_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
+        TD_HIP(hipMemPoolTrimTo(pool, 0));
+        size_t f = 0, t = 0;
+        TD_HIP(hipMemGetInfo(&f, &t));
+        *free_bytes = (int64_t)f;
+        *total_bytes = (int64_t)t;
+        return TDBEM_OK;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        return TDBEM_ERR_HIP;
+    }
+}
+
 int tdbem_march_init(tdbem_context* ctx, double a1, double a2, int n_ubc, const int* ubc_dof,
                      const double* ubc_val, int n_fbc, const int* fbc_dof, const double* fbc_val,
                      int nrhs, const double* amp, const double* t_rise)

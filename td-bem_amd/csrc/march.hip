@@ -487,6 +487,42 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
 
 int march_step_count(const Context& c) { return c.march ? c.march->s : 0; }
 
+int64_t march_state_bytes(Context& c, int nrhs)
+{
+    // the allocations of march_init that outlive it, term by term
+    const int64_t N = 3LL * c.nnd, R = c.Kc + 2, Np = frag_rows((int)N);
+    const int64_t nsplit = apply_splits((int)N);
+    int64_t b = 0;
+    b += frag_elems((int)N, (int)N) * 8;                 // Af
+    b += frag_elems((int)N, nrhs) * 8;                   // Xf
+    b += nsplit * nrhs * Np * 8;                         // parts
+    b += 4 * N * 8;                                      // gF, gU, wF, wU
+    b += 2 * N * 4 + 2 * N * 8;                          // is_ubc, is_fbc, ubc_val, fbc_val
+    b += R * nrhs * 2 * N * 8;                           // ring
+    b += 3LL * nrhs * 8;                                 // fac, amp, rise
+    b += 2LL * std::max(c.numstep, 1) * nrhs * N * 8;    // U, T
+    if (c.precision == 1) {
+        const TPStore& T = c.tp;
+        b += R * tp_zslot_elems(c, nrhs) * 4;            // zs
+        std::vector<int4> ch;
+        std::vector<int> gch;
+        tp_chunks(c, tp_chunk_target(c, 4), 0, ch, gch);
+        b += (int64_t)ch.size() * kGroup * 3 * (nrhs / 16) * 256 * 8;   // partial
+        b += (int64_t)ch.size() * 16 + (int64_t)(T.nG + 1) * 4;         // chunk, grp_chunk
+    } else {
+        b += (int64_t)nrhs * N * 8;                      // hist
+    }
+    return b;
+}
+
+int64_t march_init_transient_bytes(const Context& c)
+{
+    // the seven dense N x N matrices of the step-0 set-up, the pivots and the first solve's
+    // refinement buffers
+    const int64_t N = 3LL * c.nnd;
+    return 7 * N * N * 8 + N * 4 + 2 * frag_elems((int)N, 2) * 8 + 16 * N * 8;
+}
+
 void march_describe(Context& c, BatchScen& d)
 {
     if (!c.march || c.march->precision != 1)

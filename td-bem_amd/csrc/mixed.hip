@@ -407,9 +407,12 @@ void tp_alloc(Context& c)
 {
     TPStore& T = c.tp;
     if (!T.planned) throw std::logic_error("tile store not planned");
+    if ((int64_t)T.h_sub_id.size() != T.nsub)
+        throw std::logic_error("tile store plan without its sub-tile list");
     T.vals.alloc((size_t)std::max<int64_t>(T.nsub, 1) * kSubFloats);
     T.d_sub_id.upload(T.h_sub_id.data(), T.h_sub_id.size(), c.stream);
-    std::vector<uint2>().swap(T.h_sub_id);
+    // a large list is not kept on the host: the next build plans again
+    if (T.nsub * (int64_t)sizeof(uint2) > kPinnedKeepBytes) std::vector<uint2>().swap(T.h_sub_id);
 }
 
 void tp_gather_slots(Context& c, int I0, int I1, const double* scratch, int64_t nslots,
@@ -417,6 +420,11 @@ void tp_gather_slots(Context& c, int I0, int I1, const double* scratch, int64_t 
 {
     TPStore& T = c.tp;
     if (!T.planned || !T.vals.p) throw std::logic_error("tile store not planned");
+    // the kernel reads the sub-tile list and writes the sub-tiles: both must be the plan's size
+    if (!T.d_sub_id.p || (int64_t)T.d_sub_id.n != T.nsub ||
+        (int64_t)T.vals.n != std::max<int64_t>(T.nsub, 1) * kSubFloats)
+        throw std::logic_error("tile store buffers do not match its plan");
+    if (I0 < 0 || I1 > T.nT || I0 > I1) throw std::logic_error("row tiles out of range");
     const int64_t s0 = T.h_tile_sub[I0], s1 = T.h_tile_sub[I1];
     if (s1 <= s0) return;
     hipLaunchKernelGGL(k_tp_from_slots, dim3((unsigned)((s1 - s0 + 3) / 4)), dim3(256), 0,
@@ -431,6 +439,8 @@ void tp_finish(Context& c)
     TPStore& T = c.tp;
     TD_HIP(hipStreamSynchronize(c.stream));
     T.d_sub_id.reset();
+    // the plan stays valid for the next build only while its sub-tile list is kept
+    if ((int64_t)T.h_sub_id.size() != T.nsub) T.planned = false;
     T.ready = true;
 }
 
@@ -520,15 +530,25 @@ void tp_chunks(const Context& c, int target, int sc, std::vector<int4>& ch, std:
     gch[gbase + T.nG] = (int)ch.size();
 }
 
+// Chunks of a context's merged lists: one per kChunkWork / rounds units of entry work (about
+// 2 000 for the 1536-quad sphere at rounds = 4, four rounds of the workgroups an MI355X holds at
+// once), at least one per group.  A function of the context alone -- not of the device or of the
+// other scenarios of a batch -- so a scenario's partial sums, and its results bit for bit, are
+// the same marched alone and in any batch at the same rounds.
+constexpr double kChunkWork = 2400.0;
+
+int tp_chunk_target(const Context& c, int rounds)
+{
+    return std::max(c.tp.nG, (int)std::lround(rounds * tp_work(c) / kChunkWork));
+}
+
 void tp_plan(Context& c, int nrhs, int rounds)
 {
     TPStore& T = c.tp;
     if (T.plan_nrhs == nrhs && T.plan_rounds == rounds) return;
-    // `rounds` rounds of resident workgroups, shared among the groups by their work
-    const int target = rounds * tp_resident_groups(nrhs, c.device);
     std::vector<int4> ch;
     std::vector<int> gch;
-    tp_chunks(c, target, 0, ch, gch);
+    tp_chunks(c, tp_chunk_target(c, rounds), 0, ch, gch);
     T.nchunks = (int)ch.size();
     T.chunk.upload(ch.data(), ch.size(), c.stream);
     T.grp_chunk.upload(gch.data(), gch.size(), c.stream);

@@ -315,6 +315,9 @@ void lag_dense(Context& c, int k, double* dG, double* dH);   // [N][N] each, dev
 // rows of collocation nodes lower..upper of lag k: [3 (upper - lower + 1)][N] each, device
 void lag_rows(Context& c, int k, int lower, int upper, double* dG, double* dH);
 int64_t predict_store_bytes(Context& c);
+// device bytes of the store the next build_lag_store makes (store) and the most its build holds
+// besides (transient: node-pair tables, slot buffers, sub-tile list), planned without building
+void predict_store(Context& c, int64_t& store, int64_t& transient);
 
 // march.hip
 void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
@@ -325,6 +328,10 @@ int march_step_count(const Context& c);
 void march_history(Context& c, int rhs, double* U, double* T);
 void march_state(Context& c, int rhs, int s, double* u, double* t);
 void march_free(Context& c);
+// device bytes of march_init's state for nrhs right-hand sides (held until the march is freed)
+// and of its temporaries (the dense step-0 matrices); the store must be planned (predict_store)
+int64_t march_state_bytes(Context& c, int nrhs);
+int64_t march_init_transient_bytes(const Context& c);
 // k_conv: hist[r][3i+p] = sum over the FP64 lag store of G z^t - H z^u, z from ring slot
 // (s - k) mod R of the node-major layout [R][2][3 nnd][nrhs]; nrhs in {1, 2, 4, 8}
 void conv_fp64(Context& c, const double* ring, int R, int s, int nrhs, double* hist,
@@ -413,6 +420,8 @@ void tp_prepare_step(Context& c, const double* ring, int N, int nrhs, int s, int
 int64_t tp_zslot_elems(const Context& c, int nrhs);
 // plans the chunks of k_conv_mixed for nrhs right-hand sides (after tp_build)
 void tp_plan(Context& c, int nrhs, int rounds = 4);
+// chunk count of context c's convolution at `rounds` (a function of its work alone)
+int tp_chunk_target(const Context& c, int rounds);
 // appends `target` work-balanced chunks of context c's merged lists (chunk.w = sc) and the
 // absolute first chunk of each of its groups (nG + 1 entries) to gch
 void tp_chunks(const Context& c, int target, int sc, std::vector<int4>& chunk,

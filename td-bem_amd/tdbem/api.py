@@ -57,6 +57,9 @@ def lib():
         L.tdbem_lag_store_bytes.restype = C.c_int64
         L.tdbem_get_lag.argtypes = [vp, C.c_int, dp, dp]
         L.tdbem_get_lag_rows.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
+        L.tdbem_predict_bytes.argtypes = [vp, C.c_int, C.POINTER(C.c_int64),
+                                          C.POINTER(C.c_int64)]
+        L.tdbem_device_memory.argtypes = [C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.tdbem_march_init.argtypes = [vp, C.c_double, C.c_double, C.c_int, ip, dp, C.c_int, ip,
                                        dp, C.c_int, dp, dp]
         L.tdbem_march_run.argtypes = [vp, C.c_int]
@@ -176,6 +179,13 @@ class Context:
         if v < 0:
             self._check(int(v))
         return int(v)
+
+    def predict_bytes(self, nrhs):
+        """(resident, peak) device bytes of a solve at nrhs right-hand sides in the current
+        precision, from the store planner's exact counts (tdbem_predict_bytes)."""
+        r, p = C.c_int64(0), C.c_int64(0)
+        self._check(lib().tdbem_predict_bytes(self._h, int(nrhs), C.byref(r), C.byref(p)))
+        return r.value, p.value
 
     def get_lag(self, k):
         G = np.zeros((self.N, self.N))
@@ -308,6 +318,15 @@ def stream_read(nbytes, launches=3):
     if rc:
         raise TdbemError(rc, lib().tdbem_last_error(None).decode())
     return ms.value
+
+
+def device_memory():
+    """(free, total) device bytes after a device synchronisation with the memory pool trimmed."""
+    f, t = C.c_int64(0), C.c_int64(0)
+    rc = lib().tdbem_device_memory(C.byref(f), C.byref(t))
+    if rc:
+        raise TdbemError(rc, lib().tdbem_last_error(None).decode())
+    return f.value, t.value
 
 
 def mfma_f64_selftest(A, B):

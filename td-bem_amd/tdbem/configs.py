@@ -150,16 +150,21 @@ def halfspace_scenarios(count=64, seed=4, el_range=(400, 4000), step_range=(100,
     return out
 
 
-def serving_mix(count=16, seed=5, step_range=(100, 300)):
-    """Solve requests for the serving loop, drawn from the bar and sphere geometries at several
-    sizes (72 to 864 elements), step counts U[step_range], 16 or 32 load cases each."""
+def serving_mix(count=16, seed=5, el_range=(300, 6000), step_range=(100, 1000)):
+    """Solve requests for the serving loop from the bar and sphere geometries: element counts
+    log-uniform in el_range (many small solves, few large ones), each taken to the nearest mesh of
+    its geometry inside the range (bar: 18 n^2 quads, n = 5..18 for the default range; sphere:
+    6 n^2, n = 8..31), step counts U[step_range]."""
     rng = np.random.default_rng(seed)
-    shapes = [("bar", 0.5), ("bar", 0.25), ("sphere", 8), ("sphere", 10), ("sphere", 12)]
+    lo, hi = el_range
     out = []
     for _ in range(count):
-        kind, size = shapes[int(rng.integers(len(shapes)))]
+        kind = "bar" if rng.uniform() < 0.5 else "sphere"
+        target = float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
         steps = int(rng.integers(step_range[0], step_range[1] + 1))
-        p = bar(h=size, numstep=steps) if kind == "bar" else sphere(n=size, numstep=steps)
-        p.meta["nrhs"] = int(rng.choice([16, 32]))
+        per = 18 if kind == "bar" else 6
+        sizes = [n for n in range(2, 64) if lo <= per * n * n <= hi]
+        n = min(sizes, key=lambda m: abs(per * m * m - target))
+        p = bar(h=1.0 / n, numstep=steps) if kind == "bar" else sphere(n=n, numstep=steps)
         out.append(p)
     return out

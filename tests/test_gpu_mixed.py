@@ -91,3 +91,26 @@ def test_mixed_tile_store_is_the_fp64_store_rounded(gpu, case):
             G64, H64 = c64.get_lag(k)
             G, H = c32.get_lag(k)
             assert np.array_equal(G, G64) and np.array_equal(H, H64)
+
+
+def test_mixed_store_rebuilt_on_one_context(gpu):
+    """Repeated solves on one context (as the benchmark times them): the second and third builds
+    of the mixed-precision store reuse or redo the tile plan and give the same histories bit for
+    bit (regression: a rebuild once launched the tile gather without its sub-tile list)."""
+    p = helpers.small_sphere(numstep=20)
+    rng = np.random.default_rng(4)
+    amp = rng.uniform(0.5, 2.0, 16)
+    rise = np.zeros(16)
+    print("[gpu case] mixed store rebuilt three times", flush=True)
+    with api.Context.from_problem(p) as ctx:
+        ctx.set_precision(api.PRECISION_MIXED)
+        runs = []
+        for _ in range(3):
+            ctx.build_lag_store()
+            ctx.march_init(p.a1, p.a2, p.ubc_dof, p.ubc_val, p.fbc_dof, p.fbc_val, amp=amp,
+                           t_rise=rise)
+            ctx.march_run()
+            runs.append([ctx.history(r) for r in (0, 15)])
+    for run in runs[1:]:
+        for (u, t), (u0, t0) in zip(run, runs[0]):
+            assert np.array_equal(u, u0) and np.array_equal(t, t0)
