@@ -69,6 +69,17 @@ int tdbem_assemble_step(tdbem_context* ctx, int step, int lower, int upper, doub
                         double* G2, double* H1, double* H2);
 int tdbem_assemble_step_device(tdbem_context* ctx, int step, int lower, int upper, double* G1,
                                double* G2, double* H1, double* H2);
+/* Allocation-free variant of tdbem_assemble_step_device (no counterpart in the reference): for
+ * callers that cannot allocate device memory -- code under hipGraph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  Same arguments and results; it calls no allocating
+ * routine and does not wait: the kernels are queued on the context's stream (tdbem_synchronize
+ * waits for them).  The element slots of each row batch live in a 64 MiB device array of the
+ * library, one per GPU, held until the process ends: a mesh whose one collocation row needs more
+ * (more than 69 905 4-node or 34 952 8-node elements) is refused with TDBEM_ERR_ARG.  All calls
+ * on one GPU share that array: calls that overlap on one GPU (contexts on different streams)
+ * race, and it keeps the last call's contents. */
+int tdbem_assemble_step_device_noalloc(tdbem_context* ctx, int step, int lower, int upper,
+                                       double* G1, double* G2, double* H1, double* H2);
 
 /* The lag matrices that SolverStepLinearT::UpdateGHLinear accumulates over steps
  * 1..min(numstep, MaxStep) (src/SolverStepLinearT.cpp:58-97), built on the device in one pass and

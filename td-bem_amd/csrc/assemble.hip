@@ -778,8 +778,9 @@ constexpr int64_t kScratchBytes = 1LL << 30;
 }  // namespace
 
 void assemble_step_dense(Context& c, int step, int lower, int upper, double* dG1, double* dG2,
-                         double* dH1, double* dH2)
+                         double* dH1, double* dH2, double* work, int64_t work_doubles)
 {
+    if (work && !c.cterm_ready) throw std::logic_error("C-term not computed");
     compute_cterm(c);
     // C-term (Kelvin * dt/2 summed over the elements) into H1 and H2: Step kernel at step 1
     // only, Heaviside every step; B-spline and ramp every step with the phi_1 / phi_2 weights of
@@ -816,11 +817,18 @@ void assemble_step_dense(Context& c, int step, int lower, int upper, double* dG1
         c2s = g2 / (0.5 * c.dt);
     }
     const double ext = (c.if_ext && step == 1) ? 0.5 * c.dt : 0.0;
-    const int64_t per_node = (int64_t)c.nel * 2 * c.ennd * kVals * sizeof(double);
-    const int rows_per_batch = (int)std::max<int64_t>(1, kScratchBytes / per_node);
-    DBuf<double> scratch(c.stream);
-    scratch.alloc((size_t)std::min(rows_per_batch, upper - lower + 1) * c.nel * 2 * c.ennd *
-                  kVals);
+    // per-pair slots of one row batch: in a buffer of this call, or in the caller's workspace
+    // (work, work_doubles: nothing is allocated and the call does not wait for its kernels)
+    const int64_t per_node = (int64_t)c.nel * 2 * c.ennd * kVals;
+    if (work && per_node > work_doubles)
+        throw std::invalid_argument("one collocation row's element slots exceed the workspace");
+    const int rows_per_batch =
+        (int)std::max<int64_t>(1, (work ? work_doubles * (int64_t)sizeof(double) : kScratchBytes) /
+                                      (per_node * (int64_t)sizeof(double)));
+    DBuf<double> own(c.stream);
+    if (!work)
+        own.alloc((size_t)std::min(rows_per_batch, upper - lower + 1) * per_node);
+    double* const slots = work ? work : own.p;
     const int64_t ncol = 3LL * c.nnd;
     for (int b0 = lower; b0 <= upper; b0 += rows_per_batch) {
         const int nb = std::min(rows_per_batch, upper - b0 + 1);
@@ -829,24 +837,24 @@ void assemble_step_dense(Context& c, int step, int lower, int upper, double* dG1
         if (c.ennd == 4) {
             hipLaunchKernelGGL(k_pair_step<4>, dim3(grid_for(nslots)), dim3(kBlock), 0, c.stream,
                                c.coords.p, c.ien.p, c.geo.p, c.gauss.p, c.nq, c.nel, c.dt, c.mat,
-                               step, c.kernel, b0, nb, scratch.p);
+                               step, c.kernel, b0, nb, slots);
             hipLaunchKernelGGL(k_dense_gather<4>, dim3(grid_for((int64_t)nb * c.nnd)),
-                               dim3(kBlock), 0, c.stream, scratch.p, c.adj_off.p, c.adj.p,
+                               dim3(kBlock), 0, c.stream, slots, c.adj_off.p, c.adj.p,
                                c.cterm.p, c.nnd, c.nel, b0, nb, c1s, c2s, ext, dG1 + roff,
                                dG2 + roff, dH1 + roff, dH2 + roff);
         } else {
             hipLaunchKernelGGL(k_pair_step<8>, dim3(grid_for(nslots)), dim3(kBlock), 0, c.stream,
                                c.coords.p, c.ien.p, c.geo.p, c.gauss.p, c.nq, c.nel, c.dt, c.mat,
-                               step, c.kernel, b0, nb, scratch.p);
+                               step, c.kernel, b0, nb, slots);
             hipLaunchKernelGGL(k_dense_gather<8>, dim3(grid_for((int64_t)nb * c.nnd)),
-                               dim3(kBlock), 0, c.stream, scratch.p, c.adj_off.p, c.adj.p,
+                               dim3(kBlock), 0, c.stream, slots, c.adj_off.p, c.adj.p,
                                c.cterm.p, c.nnd, c.nel, b0, nb, c1s, c2s, ext, dG1 + roff,
                                dG2 + roff, dH1 + roff, dH2 + roff);
         }
         TD_HIP(hipGetLastError());
     }
-    // the scratch buffer must outlive the kernels that use it
-    TD_HIP(hipStreamSynchronize(c.stream));
+    // a buffer of this call must outlive the kernels that use it
+    if (!work) TD_HIP(hipStreamSynchronize(c.stream));
 }
 
 namespace {

@@ -338,6 +338,10 @@ int tdbem_create(int nnd, const double* coords, int nel, const int* ien, int ety
         std::memcpy(g.data() + nq, gw, nq * sizeof(double));
         c.gauss.upload(g.data(), g.size(), c.stream);
         prepare_geometry(c);
+        // the C-term now, so no later assembly call needs to allocate it; and the device's
+        // workspace address for the allocation-free entry point
+        compute_cterm(c);
+        noalloc_prepare(c.device);
         TD_HIP(hipStreamSynchronize(c.stream));
         *out = ctx;
         return TDBEM_OK;
@@ -380,6 +384,18 @@ int tdbem_assemble_step_device(tdbem_context* ctx, int step, int lower, int uppe
             throw std::invalid_argument("row range out of bounds");
         if (!G1 || !G2 || !H1 || !H2) throw std::invalid_argument("null output");
         assemble_step_dense(c, step, lower, upper, G1, G2, H1, H2);
+    });
+}
+
+int tdbem_assemble_step_device_noalloc(tdbem_context* ctx, int step, int lower, int upper,
+                                       double* G1, double* G2, double* H1, double* H2)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (step < 1) throw std::invalid_argument("step must be >= 1");
+        if (lower < 0 || upper >= c.nnd || upper < lower)
+            throw std::invalid_argument("row range out of bounds");
+        if (!G1 || !G2 || !H1 || !H2) throw std::invalid_argument("null output");
+        assemble_step_dense_noalloc(c, step, lower, upper, G1, G2, H1, H2);
     });
 }
 

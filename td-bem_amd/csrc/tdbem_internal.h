@@ -308,8 +308,19 @@ struct Context {
 void prepare_geometry(Context& c);
 void compute_cterm(Context& c);
 void compute_pair_ranges(Context& c);
+// Dense rows of one step (G_H_Driver).  work == nullptr: the element slots live in a buffer of
+// the call, which waits for its kernels; else in the caller's device workspace of work_doubles
+// (no allocation, no wait; the C-term must already be computed).
 void assemble_step_dense(Context& c, int step, int lower, int upper, double* dG1, double* dG2,
-                         double* dH1, double* dH2);
+                         double* dH1, double* dH2, double* work = nullptr,
+                         int64_t work_doubles = 0);
+// The allocation-free variant: the workspace is the library's per-GPU static array (noalloc.hip)
+void assemble_step_dense_noalloc(Context& c, int step, int lower, int upper, double* dG1,
+                                 double* dG2, double* dH1, double* dH2);
+// bytes of that array; noalloc_prepare looks its address up on a device (tdbem_create does, once
+// per device, so the entry point itself only reads the kept address)
+int64_t noalloc_scratch_bytes();
+void noalloc_prepare(int device);
 void build_lag_store(Context& c);
 void lag_dense(Context& c, int k, double* dG, double* dH);   // [N][N] each, device
 // rows of collocation nodes lower..upper of lag k: [3 (upper - lower + 1)][N] each, device
