@@ -85,6 +85,21 @@ int tdbem_assemble_step_device_noalloc(tdbem_context* ctx, int step, int lower, 
  * 1..min(numstep, MaxStep) (src/SolverStepLinearT.cpp:58-97), built on the device in one pass and
  * kept only inside each node pair's wavefront shell. */
 int tdbem_build_lag_store(tdbem_context* ctx);
+/* SolverStepLinearT::UpdateGHLinear(G_0, G_1, H_0, H_1) at fCurrStep = step
+ * (src/SolverStepLinearT.cpp:58-97) with caller-supplied blocks: the rows of collocation nodes
+ * lower..upper ([3 (upper - lower + 1)][3 nnd] each, host) of the step's phi_1 part (G0, H0:
+ * main.cpp's G_0, H_0 = fG_1, fH_1) and phi_2 part (G1, H1).  As the reference does, the phi_2
+ * blocks are inserted into lag 0 at step 1 and added to lag step - 1 after, the phi_1 blocks
+ * inserted into lag step; steps past the kept lags (MaxStep, or numstep) are ignored.  The first
+ * call replaces the store the device assembles itself (and drops a march on it) by a dense lag
+ * database on the device (the reference's (MaxStep + 1) x 2 N x N footprint); the march builds
+ * its shell store from it before each step that needs newer lags, so steps and updates may
+ * alternate as in main.cpp:89-128.  March step s needs the blocks of step s + 1 (lag s complete),
+ * march_init those of steps 1 and 2.  FP64 precision only; G's 3 x 3 node blocks must be
+ * symmetric (to 1e-12): the store keeps their upper triangle.  tdbem_build_lag_store returns to
+ * the device's own assembly. */
+int tdbem_update_gh_linear(tdbem_context* ctx, int step, int lower, int upper, const double* G0,
+                           const double* G1, const double* H0, const double* H1);
 /* Bytes of the shell store (predicted when not yet built). */
 int64_t tdbem_lag_store_bytes(tdbem_context* ctx);
 /* Device bytes a solve of this context needs at nrhs right-hand sides in the current precision,

@@ -878,6 +878,8 @@ struct ShellPlan {
     int64_t nonzero = 0;
 };
 
+void plan_from_hull(Context& c, ShellPlan& P, bool offsets);
+
 // Node-pair hulls (lag range of each (i, j), from the pair ranges dpk of the elements around j)
 // into the pinned c.pin_hull, entries per row node and their total; with offsets = true also the
 // store layout (row chunks, first entry of every pair in c.pin_off).
@@ -894,6 +896,14 @@ void plan_shells(Context& c, ShellPlan& P, const int2* dpk, bool offsets = true)
     TD_HIP(hipMemcpyAsync(P.hull, hull.p, nn * sizeof(int2), hipMemcpyDeviceToHost,
                           c.stream));
     TD_HIP(hipStreamSynchronize(c.stream));
+    plan_from_hull(c, P, offsets);
+}
+
+// Entries per row node, row chunks and (offsets = true) each pair's first entry, from the
+// node-pair hulls in P.hull.
+void plan_from_hull(Context& c, ShellPlan& P, bool offsets)
+{
+    const int64_t nn = (int64_t)c.nnd * c.nnd;
     P.row_chunk.assign(c.nnd + 1, 0);
     P.row_valid.assign(c.nnd, 0);
     P.nonzero = 0;
@@ -1105,9 +1115,9 @@ void fill_store(Context& c, const ShellPlan& P, const int2* hpk, const int2* dpk
                     });
 }
 
-// Allocates the FP64 store of layout P (pair hulls in P.hull, offsets in P.off) and fills it.
-template <int ENND>
-void build_fp64_store(Context& c, const ShellPlan& P, const int2* hpk, const int2* dpk)
+// Allocates the FP64 store of layout P (pair hulls in P.hull, offsets in P.off) and its entry
+// index; the values are left to the caller.
+void alloc_store(Context& c, const ShellPlan& P)
 {
     LagStore& S = c.store;
     S.n_chunks = P.row_chunk[c.nnd];
@@ -1130,6 +1140,13 @@ void build_fp64_store(Context& c, const ShellPlan& P, const int2* hpk, const int
                            d_off.p, c.nnd, S.idx.p);
         TD_HIP(hipGetLastError());
     }
+}
+
+// Allocates the FP64 store of layout P and fills it from the lag kernel's slots.
+template <int ENND>
+void build_fp64_store(Context& c, const ShellPlan& P, const int2* hpk, const int2* dpk)
+{
+    alloc_store(c, P);
     fill_store<ENND>(c, P, hpk, dpk);
 }
 
@@ -1187,6 +1204,19 @@ void build_mixed(Context& c)
 }
 
 }  // namespace
+
+void alloc_store_for_hull(Context& c, const int2* hull)
+{
+    const int64_t nn = (int64_t)c.nnd * c.nnd;
+    c.pin_hull.reserve(nn);
+    std::copy(hull, hull + nn, c.pin_hull.p);
+    ShellPlan P;
+    P.hull = c.pin_hull.p;
+    plan_from_hull(c, P, true);
+    c.pair_lags = P.nonzero;
+    alloc_store(c, P);
+    TD_HIP(hipStreamSynchronize(c.stream));
+}
 
 namespace {
 

@@ -425,6 +425,7 @@ int tdbem_build_lag_store(tdbem_context* ctx)
         not_batched(c, "tdbem_build_lag_store");
         if (c.kernel != TDBEM_KERNEL_STEP)
             throw std::invalid_argument("the lag store is built for the Step kernel");
+        caller_drop(c);   // the device assembles the lags itself
         build_lag_store(c);
     });
 }
@@ -445,6 +446,7 @@ int64_t tdbem_lag_store_bytes(tdbem_context* ctx)
 int tdbem_get_lag(tdbem_context* ctx, int k, double* G, double* H)
 {
     return guarded(ctx, [&](Context& c) {
+        if (c.caller_last > 0 && (c.caller_stale || !c.store_ready)) caller_store(c);
         if (!c.store_ready) throw std::logic_error("lag store not built");
         if (k < 0 || k > c.Kc) throw std::invalid_argument("lag index out of range");
         const size_t N = 3 * (size_t)c.nnd;
@@ -464,6 +466,7 @@ int tdbem_get_lag(tdbem_context* ctx, int k, double* G, double* H)
 int tdbem_get_lag_rows(tdbem_context* ctx, int k, int lower, int upper, double* G, double* H)
 {
     return guarded(ctx, [&](Context& c) {
+        if (c.caller_last > 0 && (c.caller_stale || !c.store_ready)) caller_store(c);
         if (!c.store_ready) throw std::logic_error("lag store not built");
         if (k < 0 || k > c.Kc) throw std::invalid_argument("lag index out of range");
         if (lower < 0 || upper >= c.nnd || upper < lower)
@@ -480,6 +483,15 @@ int tdbem_get_lag_rows(tdbem_context* ctx, int k, int lower, int upper, double* 
         TD_HIP(hipMemcpyAsync(G, g.p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         TD_HIP(hipMemcpyAsync(H, h.p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         TD_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int tdbem_update_gh_linear(tdbem_context* ctx, int step, int lower, int upper, const double* G0,
+                           const double* G1, const double* H0, const double* H1)
+{
+    return guarded(ctx, [&](Context& c) {
+        not_batched(c, "tdbem_update_gh_linear");
+        caller_update(c, step, lower, upper, G0, G1, H0, H1);
     });
 }
 
@@ -615,6 +627,7 @@ int tdbem_set_precision(tdbem_context* ctx, int precision)
 {
     return guarded(ctx, [&](Context& c) {
         not_batched(c, "tdbem_set_precision");
+        if (precision != c.precision) caller_drop(c);
         if (precision != TDBEM_PRECISION_FP64 && precision != TDBEM_PRECISION_MIXED_BF16X3)
             throw std::invalid_argument("unknown precision");
         c.precision = precision;
@@ -779,6 +792,7 @@ int tdbem_set_max_step(tdbem_context* ctx, int max_step)
 {
     return guarded(ctx, [&](Context& c) {
         not_batched(c, "tdbem_set_max_step");
+        caller_drop(c);
         set_max_step(c, max_step);
     });
 }

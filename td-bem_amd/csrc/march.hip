@@ -344,7 +344,15 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
         if (fbc_dof[q] < 0 || fbc_dof[q] >= N) throw std::invalid_argument("FBC dof out of range");
     for (int q = 0; q < n_ubc; q++)
         if (ubc_dof[q] < 0 || ubc_dof[q] >= N) throw std::invalid_argument("UBC dof out of range");
-    if (!c.store_ready) build_lag_store(c);
+    if (c.caller_last > 0) {
+        // caller-supplied lags: the step-0 matrices need M_0 and M_1 complete, i.e. the blocks
+        // of steps 1 and 2 (of step 1 alone when the store keeps one lag)
+        if (c.caller_last < std::min(2, c.Kc))
+            throw std::logic_error("UpdateGHLinear of steps 1 and 2 must come before the march");
+        if (c.caller_stale || !c.store_ready) caller_store(c);
+    } else if (!c.store_ready) {
+        build_lag_store(c);
+    }
     const auto t0 = std::chrono::steady_clock::now();
     march_free(c);
     // the new march state becomes the context's only when it is complete (a failure below
@@ -578,6 +586,18 @@ void march_run(Context& c, int nsteps, bool wait)
     reduce_timer.reserve(steps);
     for (int it = 0; it < nsteps; it++) {
         if (M.s + 1 >= c.numstep) break;  // histories hold x_0 .. x_{numstep-1}
+        if (c.caller_last > 0) {
+            // caller-supplied lags: step s sums lags up to min(s, Kc); lag k is complete once
+            // the blocks of step k + 1 are in (or k = Kc, which has no phi_2 part)
+            const int complete = c.caller_last >= c.Kc ? c.Kc : c.caller_last - 1;
+            if (std::min(M.s + 1, c.Kc) > complete)
+                throw std::logic_error("UpdateGHLinear of step " + std::to_string(M.s + 2) +
+                                       " is needed before this step");
+            if (c.caller_stale) {
+                TD_HIP(hipStreamSynchronize(c.stream));
+                caller_store(c);
+            }
+        }
         const int s = ++M.s;
         const bool mixed = M.precision == 1 && s >= 2;
         if (!mixed)

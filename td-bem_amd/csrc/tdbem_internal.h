@@ -281,6 +281,12 @@ struct Context {
     // living in the tile store (store_lags = 1)
     int64_t pair_lags = 0;
     int store_lags = 0;
+    // lag matrices supplied by the caller step by step (UpdateGHLinear): a dense database of
+    // Kc + 1 lags of G and H on the device, the last step supplied, and whether the store was
+    // built from it before the latest supply
+    DBuf<double> caller_db;   // [Kc + 1][2][N][N]
+    int caller_last = 0;      // 0: not in this mode
+    bool caller_stale = false;
     TPStore tp;
     int precision = 0;  // 0: FP64 march, 1: split-BF16 MFMA convolution (mixed)
     int store_precision = 0;  // precision the lag store was assembled with
@@ -322,6 +328,18 @@ void assemble_step_dense_noalloc(Context& c, int step, int lower, int upper, dou
 int64_t noalloc_scratch_bytes();
 void noalloc_prepare(int device);
 void build_lag_store(Context& c);
+// Allocates the FP64 store whose node pair (i, j) holds lags hull[i * nnd + j] (.x..y; empty
+// when y < x) and fills its entry index; the values are left to the caller (caller.hip).
+void alloc_store_for_hull(Context& c, const int2* hull);
+
+// caller.hip: the lag matrices of caller-supplied per-step blocks (UpdateGHLinear)
+// caller_update: the blocks of one step (rows of collocation nodes lower..upper, host) into the
+// dense lag database on the device, as UpdateGHLinear accumulates them; caller_store: the FP64
+// store from that database (lags with any nonzero 3 x 3 block); caller_drop: leaves the mode.
+void caller_update(Context& c, int step, int lower, int upper, const double* G0,
+                   const double* G1, const double* H0, const double* H1);
+void caller_store(Context& c);
+void caller_drop(Context& c);
 void lag_dense(Context& c, int k, double* dG, double* dH);   // [N][N] each, device
 // rows of collocation nodes lower..upper of lag k: [3 (upper - lower + 1)][N] each, device
 void lag_rows(Context& c, int k, int lower, int upper, double* dG, double* dH);

@@ -406,16 +406,24 @@ void SolverStepLinearT::GetDofLowHigh(int* low, int* high) const
     *high = 3 * (fModel->GetNND() - 1) + 2;
 }
 
-void SolverStepLinearT::UpdateGHLinear(const double*, const double*, const double*, const double*)
+void SolverStepLinearT::UpdateGHLinear(const double* G_0, const double* G_1, const double* H_0,
+                                       const double* H_1)
 {
-    // The lag matrices are assembled on the GPU in one pass by the first Solve; the host copies
-    // that the reference threads through here are not needed.
+    // With the step's blocks (the rows of this solver's nodes): the reference's accumulation
+    // into the lag matrices, on the device (tdbem_update_gh_linear).  Without them (null): the
+    // lag matrices are assembled on the GPU in one pass by the first Solve.
+    if (!G_0 || !G_1 || !H_0 || !H_1) return;
+    int low, high;
+    GetNodeLowHigh(&low, &high);
+    check(fCtx, tdbem_update_gh_linear(fCtx, fCurrStep, low, high, G_0, G_1, H_0, H_1));
 }
 
 void SolverStepLinearT::Solve(double a1, double a2)
 {
     if (!(a1 >= 0 && a1 <= 1 && a2 >= 0 && a2 < 1 && a1 + a2 <= 1)) fail("Alpha is out of range");
-    if (!fInit) {
+    // step 1 solves nothing (src/SolverStepLinearT.cpp:349-354): the march starts at step 2, when
+    // lags 0 and 1 of supplied blocks are complete
+    if (!fInit && fCurrStep >= 2) {
         int nu, nf;
         const int *ud, *fd;
         const double *uv, *fv;
@@ -423,8 +431,8 @@ void SolverStepLinearT::Solve(double a1, double a2)
         check(fCtx, tdbem_march_init(fCtx, a1, a2, nu, ud, uv, nf, fd, fv, 1, nullptr, nullptr));
         fInit = true;
     }
-    // step 1 solves nothing (src/SolverStepLinearT.cpp:349-354); step n solves x_{n-1}
-    while (tdbem_march_step_index(fCtx) < fCurrStep - 1)
+    // step n solves x_{n-1}
+    while (fInit && tdbem_march_step_index(fCtx) < fCurrStep - 1)
         check(fCtx, tdbem_march_run(fCtx, 1));
 }
 

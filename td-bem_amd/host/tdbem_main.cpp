@@ -1,6 +1,6 @@
 // tdbem_main.cpp — command-line driver with the step loop of src/main.cpp:22-149 on the GPU.
 //
-//   tdbem_main input.txt [--write-gh] [--write-gh-binary] [--history out.bin]
+//   tdbem_main input.txt [--write-gh] [--write-gh-binary] [--history out.bin] [--caller-lags]
 //   tdbem_main region1.txt --mrt region2.txt [--averaging] [--max-step K] [--no-vtk]
 //
 // Reads the deck and alpha.txt (a1 a2) from the working directory like the reference, marches
@@ -14,7 +14,10 @@
 // UpdateGHLinear / SetLoad / Solve / RecordResult: Solve_Direct as SolverMRT::Solve does, or
 // Solve_Averaging with --averaging; fMaxStep 10 unless --max-step; displacement.txt gets the
 // reference's recorded region-2 dof and BEM_1st_<n>.vtk / BEM_2nd_<n>.vtk the region fields
-// (not with --no-vtk).  Unlike the reference, errors exit non-zero.
+// (not with --no-vtk).  --caller-lags runs main.cpp's own data flow: G_H_Driver assembles each
+// step's blocks (ElastoDynamicsT) and UpdateGHLinear hands them to the solver, which accumulates
+// the lag matrices from them (tdbem_update_gh_linear) instead of assembling its store itself.
+// Unlike the reference, errors exit non-zero.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -32,7 +35,7 @@ int main(int argc, char** argv)
         std::cerr << "Input file not specified, try ./TD_BEM input.txt" << std::endl;
         return 2;
     }
-    bool write_gh = false, write_bin = false, averaging = false, vtk = true;
+    bool write_gh = false, write_bin = false, averaging = false, vtk = true, caller_lags = false;
     const char* history = nullptr;
     const char* mrt = nullptr;
     int max_step = 10;
@@ -47,6 +50,8 @@ int main(int argc, char** argv)
             mrt = argv[++a];
         else if (!std::strcmp(argv[a], "--averaging"))
             averaging = true;
+        else if (!std::strcmp(argv[a], "--caller-lags"))
+            caller_lags = true;
         else if (!std::strcmp(argv[a], "--no-vtk"))
             vtk = false;
         else if (!std::strcmp(argv[a], "--max-step") && a + 1 < argc)
@@ -100,12 +105,13 @@ int main(int argc, char** argv)
                   << "alpha_1=" << a1 << "  alpha_2=" << a2 << "\n"
                   << "***********************************" << std::endl;
 
-        ElastoDynamicsT* ED = (write_gh || write_bin) ? new ElastoDynamicsT(&model) : nullptr;
+        ElastoDynamicsT* ED =
+            (write_gh || write_bin || caller_lags) ? new ElastoDynamicsT(&model) : nullptr;
         const int max_step = solver.GetMaxStep();
         for (int curr_step = 1; curr_step <= num_step; curr_step++) {
+            double *G_0 = nullptr, *G_1 = nullptr, *H_0 = nullptr, *H_1 = nullptr;
             if (ED && curr_step <= max_step) {
                 ED->G_H_Driver(curr_step * dt, dt);
-                double *G_0, *G_1, *H_0, *H_1;
                 ED->GetGHLinear(&G_0, &G_1, &H_0, &H_1);
                 const int rows = dof_high - dof_low + 1, cols = 3 * model.GetNND();
                 if (write_gh) {
@@ -122,7 +128,10 @@ int main(int argc, char** argv)
                 }
             }
             solver.SetCurrStep(curr_step);
-            solver.UpdateGHLinear(nullptr, nullptr, nullptr, nullptr);
+            if (caller_lags)
+                solver.UpdateGHLinear(G_0, G_1, H_0, H_1);   // main.cpp:121
+            else
+                solver.UpdateGHLinear(nullptr, nullptr, nullptr, nullptr);
             solver.SetLoad();
             solver.Solve(a1, a2);
             solver.RecordResult();

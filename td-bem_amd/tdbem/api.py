@@ -57,6 +57,7 @@ def lib():
         L.tdbem_lag_store_bytes.restype = C.c_int64
         L.tdbem_get_lag.argtypes = [vp, C.c_int, dp, dp]
         L.tdbem_get_lag_rows.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
+        L.tdbem_update_gh_linear.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp, dp, dp]
         L.tdbem_predict_bytes.argtypes = [vp, C.c_int, C.POINTER(C.c_int64),
                                           C.POINTER(C.c_int64)]
         L.tdbem_device_memory.argtypes = [C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
@@ -179,6 +180,17 @@ class Context:
         if v < 0:
             self._check(int(v))
         return int(v)
+
+    def update_gh_linear(self, step, G0, G1, H0, H1, lower=0, upper=None):
+        """UpdateGHLinear at fCurrStep = step with caller-supplied row blocks: G0/H0 the phi_1
+        part, G1/H1 the phi_2 part of the step, rows of collocation nodes lower..upper."""
+        upper = self.nnd - 1 if upper is None else upper
+        blocks = [np.ascontiguousarray(b, dtype=np.float64) for b in (G0, G1, H0, H1)]
+        for b in blocks:
+            if b.size != 3 * (upper - lower + 1) * self.N:
+                raise ValueError("block size does not match the row range")
+        self._check(lib().tdbem_update_gh_linear(self._h, int(step), int(lower), int(upper),
+                                                 *[_d(b) for b in blocks]))
 
     def predict_bytes(self, nrhs):
         """(resident, peak) device bytes of a solve at nrhs right-hand sides in the current
