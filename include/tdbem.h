@@ -266,6 +266,45 @@ int tdbem_mrt_get_state(tdbem_mrt* m, int s, double* x, double* u, double* t);
 int tdbem_mrt_get_timings(const tdbem_mrt* m, double* init_ms, double* run_ms);
 int tdbem_mrt_destroy(tdbem_mrt* m);
 
+/* Element matrices of (collocation point, element) pairs of no mesh: the TDElementT interface
+ * (src/TDElementT.h:16-115) as a reusable evaluator.  tdbem_element_create is the constructor
+ * TDElementT(type) of the element class the kernel names (TDBEM_KERNEL_STEP:
+ * TDElementStepLinearT, TDBEM_KERNEL_HEAVISIDE: TDElementHeavisideLinearT, TDBEM_KERNEL_BSPLINE
+ * and TDBEM_KERNEL_RAMP: TDElementRampHeaviLinearT) with SetNQ(nq) and
+ * SetMaterial(E, nu, density) (TDElementT.h:19-30); it holds a stream and buffers on the current
+ * device that later calls reuse.  Errors: TDBEM_ERR_ARG for an element type, rule, kernel or
+ * material the reference refuses. */
+typedef struct tdbem_element tdbem_element;
+int tdbem_element_create(int etype, int kernel, int nq, double E, double nu, double density,
+                         tdbem_element** out);
+int tdbem_element_destroy(tdbem_element* el);
+/* Message of the element's last failed call; with el NULL, of the last failed create. */
+const char* tdbem_element_last_error(const tdbem_element* el);
+
+/* SetCoords(coords[p]), SetCollocation(collocation[p]), SetTime(t, t_1), then FormGH_Regular()
+ * for singular[p] == -1 or FormGH_Singular(singular[p]), then GetGHLinear (TDElementT.h:22-52;
+ * the call sequence of src/ElastoDynamicsT.cpp:139-192), for npairs pairs at once.  Host arrays:
+ * coords [npairs][ennd][3], collocation [npairs][3], singular [npairs]; out GE_1, GE_2, HE_1, HE_2
+ * [npairs][3][3 ennd] and C_1, C_2 [npairs][3][3], row-major as the element's own arrays (_1: the
+ * phi_1 hat, _2: the phi_2 hat).  The Step element's C block is nonzero only at t == t_1 (its
+ * first step); the Heaviside and pulse elements fill C for regular pairs and move it into the
+ * singular node's HE columns for singular ones, as the reference's elements do.  As in the
+ * reference, the singular node is taken from the caller and is not checked against the geometry.
+ * TDBEM_ERR_ARG: a singular id outside -1 .. ennd-1 (the reference throws), t_1 <= 0, null
+ * arrays.  Waits for its copies. */
+int tdbem_element_form(tdbem_element* el, int64_t npairs, const double* coords,
+                       const double* collocation, const int* singular, double t, double t_1,
+                       double* GE_1, double* GE_2, double* HE_1, double* HE_2, double* C_1,
+                       double* C_2);
+/* The same on device arrays, queued on the element's stream without waiting (no counterpart in
+ * the reference: for callers whose pairs are already in device memory).  out = GE_1 | GE_2 |
+ * HE_1 | HE_2 | C_1 | C_2 back to back (npairs * (36 ennd + 18) doubles).  A pair whose
+ * singular id is out of range gets NaN.  tdbem_element_synchronize waits for the queued work. */
+int tdbem_element_form_device(tdbem_element* el, int64_t npairs, const double* coords,
+                              const double* collocation, const int* singular, double t,
+                              double t_1, double* out);
+int tdbem_element_synchronize(tdbem_element* el);
+
 /* Instrumentation (no counterpart in the reference, which has no timers): level
  * TDBEM_PROFILE_ALL (any nonzero level other than TDBEM_PROFILE_CONV) brackets the lag kernel,
  * the store gather, the history convolution, the reduction and the step solve with HIP events on

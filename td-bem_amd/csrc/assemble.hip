@@ -307,13 +307,40 @@ __global__ void k_fill_idx(const int2* __restrict__ hull, const int64_t* __restr
 // (pair, part) gives phi_1 (part 0) or phi_2 (part 1) of step n; for the Heaviside kernel
 // (src/GreenFunctionT.cpp:126-129) there is no (t - dt) level.
 // ---------------------------------------------------------------------------------------------
-// FP64 integration of a pair over the points of for_each_point (regular rule for sing < 0,
-// Duffy triangles / sub-elements for a singular node), accumulated into out[ENND][15].
-template <int ENND>
-__device__ __forceinline__ int integrate_fp64(const double* X, int sing, const double* coll,
-                                              const double* geo_e, const double* gauss, int nq,
-                                              const TimeCombo& tc, double dt, const Material& m,
-                                              double* out, int64_t ostride)
+// The rules of for_each_point with the regular points' shape functions evaluated in place of
+// the mesh's geometry table (the element entry integrates pairs of no mesh).
+template <int ENND, typename F>
+__device__ __forceinline__ void for_each_point_direct(const double* X, int sing,
+                                                      const double* gauss, int nq, F&& f)
+{
+    if (sing >= 0) {
+        for_each_point<ENND>(X, sing, nullptr, gauss, nq, f);
+        return;
+    }
+    for (int q1 = 0; q1 < nq; q1++)
+        for (int q2 = 0; q2 < nq; q2++) {
+            double N[ENND], x[3], n[3], J;
+            shape_eval<ENND>(X, gauss[q1], gauss[q2], N, x, n, J);
+            f(x, n, gauss[nq + q1] * gauss[nq + q2] * J, N);
+        }
+}
+
+// Per-point addition after a point's kernel blocks are accumulated: none.
+struct NoPointExtra {
+    template <typename Acc>
+    __device__ __forceinline__ void operator()(const double*, const double*, double, Acc&) const
+    {
+    }
+};
+
+// FP64 integration of a pair over the points that points(f) visits (for_each_point: regular
+// rule for sing < 0, Duffy triangles / sub-elements for a singular node), accumulated into
+// out[ENND][15].  extra(x, n, wJ, acc) runs at every point after its blocks are added.
+template <int ENND, typename Points, typename Extra = NoPointExtra>
+__device__ __forceinline__ int integrate_points(Points&& points, const double* coll,
+                                                const TimeCombo& tc, double dt,
+                                                const Material& m, double* out, int64_t ostride,
+                                                Extra&& extra = Extra{})
 {
     int active = 0;
     const double inv_dt = 1.0 / dt;
@@ -322,26 +349,27 @@ __device__ __forceinline__ int integrate_fp64(const double* X, int sing, const d
     for (int a = 0; a < ENND; a++)
 #pragma unroll
         for (int v = 0; v < kVals; v++) acc[a][v] = 0.0;
-    for_each_point<ENND>(X, sing, geo_e, gauss, nq,
-                         [&](const double* x, const double* n, double wJ, const double* N) {
-                             double d[3] = {x[0] - coll[0], x[1] - coll[1], x[2] - coll[2]};
-                             const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-                             double S[6];
-                             if (!eval_s(tc, r, dt, inv_dt, m, S)) return;
-                             active++;
-                             const double ir = 1.0 / r;
-                             d[0] *= ir;
-                             d[1] *= ir;
-                             d[2] *= ir;
-                             double blk[kVals];
-                             blocks_from_s<double>(S, d, n, r, ir, ir * ir, m, blk);
+    points([&](const double* x, const double* n, double wJ, const double* N) {
+        double d[3] = {x[0] - coll[0], x[1] - coll[1], x[2] - coll[2]};
+        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        double S[6];
+        if (eval_s(tc, r, dt, inv_dt, m, S)) {
+            active++;
+            const double ir = 1.0 / r;
+            d[0] *= ir;
+            d[1] *= ir;
+            d[2] *= ir;
+            double blk[kVals];
+            blocks_from_s<double>(S, d, n, r, ir, ir * ir, m, blk);
 #pragma unroll
-                             for (int a = 0; a < ENND; a++) {
-                                 const double wN = wJ * N[a];
+            for (int a = 0; a < ENND; a++) {
+                const double wN = wJ * N[a];
 #pragma unroll
-                                 for (int v = 0; v < kVals; v++) acc[a][v] += wN * blk[v];
-                             }
-                         });
+                for (int v = 0; v < kVals; v++) acc[a][v] += wN * blk[v];
+            }
+        }
+        extra(x, n, wJ, acc);
+    });
 #pragma unroll
     for (int a = 0; a < ENND; a++)
 #pragma unroll
@@ -363,7 +391,9 @@ __device__ __forceinline__ int integrate_pair(const double* coords, const int* i
     load_element<ENND>(coords, ien, e, X);
     const int sing = REGULAR ? -1 : singular_node<ENND>(X, coll);
     const double* geo_e = geo + (size_t)e * nq * nq * (7 + ENND);
-    return integrate_fp64<ENND>(X, sing, coll, geo_e, gauss, nq, tc, dt, m, out, ostride);
+    return integrate_points<ENND>(
+        [&](auto&& f) { for_each_point<ENND>(X, sing, geo_e, gauss, nq, f); }, coll, tc, dt, m,
+        out, ostride);
 }
 
 // The reference's 20-point time rule (src/GreenFunctionT.cpp:1042-1164, 9 digits), for the
@@ -418,6 +448,57 @@ __constant__ double kTime40w[40] = {
 // rule), and kernel 6, the ramped Heaviside (Int_Stokes_RampedHeaviside_Linear, :589-622,
 // 40-point time rule).  phi_{part+1} blocks of pair (i, e) at time t at every spatial point; the
 // Kelvin parts go to the C-term (assemble_step_dense).
+template <int ENND, typename Points, typename Extra = NoPointExtra>
+__device__ __forceinline__ void pulse_points(Points&& points, const double* coll, double t,
+                                             double dt, const Material& m, int part, int kernel,
+                                             double* out, Extra&& extra = Extra{})
+{
+    double acc[ENND][kVals];
+#pragma unroll
+    for (int a = 0; a < ENND; a++)
+#pragma unroll
+        for (int v = 0; v < kVals; v++) acc[a][v] = 0.0;
+    const bool ramp = kernel == 6;
+    const double bw = 0.7 * dt, slope = 2 / dt;
+    const int ntq = ramp ? 40 : 20;
+    points([&](const double* x, const double* n, double wJ, const double* N) {
+        double d[3] = {x[0] - coll[0], x[1] - coll[1], x[2] - coll[2]};
+        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const double ir = 1.0 / r;
+        d[0] *= ir;
+        d[1] *= ir;
+        d[2] *= ir;
+        double blk[kVals];
+#pragma unroll
+        for (int v = 0; v < kVals; v++) blk[v] = 0.0;
+        for (int q = 0; q < ntq; q++) {
+            const double ab = ramp ? kTime40x[q] : kTime20x[q];
+            const double w = ramp ? kTime40w[q] : kTime20w[q];
+            const double phi = part == 0 ? 0.5 * (1 - ab) : 0.5 * (1 + ab);
+            const double tt = 0.5 * (1 + ab) * dt;
+            double b[kVals];
+            if (ramp)
+                ramp_blocks(t - tt, slope, r, d, n, m, b);
+            else
+                bspline_blocks(t - tt, bw, r, d, n, m, b);
+            const double wt = w * (dt * 0.5) * phi;
+#pragma unroll
+            for (int v = 0; v < kVals; v++) blk[v] += wt * b[v];
+        }
+#pragma unroll
+        for (int a = 0; a < ENND; a++) {
+            const double wN = wJ * N[a];
+#pragma unroll
+            for (int v = 0; v < kVals; v++) acc[a][v] += wN * blk[v];
+        }
+        extra(x, n, wJ, acc);
+    });
+#pragma unroll
+    for (int a = 0; a < ENND; a++)
+#pragma unroll
+        for (int v = 0; v < kVals; v++) out[a * kVals + v] = acc[a][v];
+}
+
 template <int ENND>
 __device__ __forceinline__ void integrate_pair_pulse(const double* coords, const int* ien,
                                                      const double* geo, const double* gauss,
@@ -429,50 +510,9 @@ __device__ __forceinline__ void integrate_pair_pulse(const double* coords, const
     double X[3 * ENND];
     load_element<ENND>(coords, ien, e, X);
     const int sing = singular_node<ENND>(X, coll);
-    double acc[ENND][kVals];
-#pragma unroll
-    for (int a = 0; a < ENND; a++)
-#pragma unroll
-        for (int v = 0; v < kVals; v++) acc[a][v] = 0.0;
-    const bool ramp = kernel == 6;
-    const double bw = 0.7 * dt, slope = 2 / dt;
-    const int ntq = ramp ? 40 : 20;
-    for_each_point<ENND>(X, sing, geo + (size_t)e * nq * nq * (7 + ENND), gauss, nq,
-                         [&](const double* x, const double* n, double wJ, const double* N) {
-                             double d[3] = {x[0] - coll[0], x[1] - coll[1], x[2] - coll[2]};
-                             const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-                             const double ir = 1.0 / r;
-                             d[0] *= ir;
-                             d[1] *= ir;
-                             d[2] *= ir;
-                             double blk[kVals];
-#pragma unroll
-                             for (int v = 0; v < kVals; v++) blk[v] = 0.0;
-                             for (int q = 0; q < ntq; q++) {
-                                 const double ab = ramp ? kTime40x[q] : kTime20x[q];
-                                 const double w = ramp ? kTime40w[q] : kTime20w[q];
-                                 const double phi = part == 0 ? 0.5 * (1 - ab) : 0.5 * (1 + ab);
-                                 const double tt = 0.5 * (1 + ab) * dt;
-                                 double b[kVals];
-                                 if (ramp)
-                                     ramp_blocks(t - tt, slope, r, d, n, m, b);
-                                 else
-                                     bspline_blocks(t - tt, bw, r, d, n, m, b);
-                                 const double wt = w * (dt * 0.5) * phi;
-#pragma unroll
-                                 for (int v = 0; v < kVals; v++) blk[v] += wt * b[v];
-                             }
-#pragma unroll
-                             for (int a = 0; a < ENND; a++) {
-                                 const double wN = wJ * N[a];
-#pragma unroll
-                                 for (int v = 0; v < kVals; v++) acc[a][v] += wN * blk[v];
-                             }
-                         });
-#pragma unroll
-    for (int a = 0; a < ENND; a++)
-#pragma unroll
-        for (int v = 0; v < kVals; v++) out[a * kVals + v] = acc[a][v];
+    const double* geo_e = geo + (size_t)e * nq * nq * (7 + ENND);
+    pulse_points<ENND>([&](auto&& f) { for_each_point<ENND>(X, sing, geo_e, gauss, nq, f); },
+                       coll, t, dt, m, part, kernel, out);
 }
 
 // Lag slots of one row batch, value v of element node a of slot s at scratch[(a * 15 + v) *
@@ -556,6 +596,119 @@ __global__ void __launch_bounds__(kBlock)
     tc.static_ok = !heaviside;
     integrate_pair<ENND>(coords, ien, geo, gauss, nq, i, e, tc, dt, m,
                          scratch + s * (ENND * kVals));
+}
+
+// Element matrices of TDElementT::FormGH_Regular / FormGH_Singular (src/TDElementT.h:37-48;
+// src/TDElementStepLinearT.cpp:28-187, src/TDElementHeavisideLinearT.cpp,
+// src/TDElementRampHeaviLinearT.cpp) for npairs (collocation point, element) pairs of no mesh,
+// thread per (pair, part): the phi_{part+1} matrices GE and HE [3][3 ENND] and the Kelvin block
+// C [3][3] (weight cw1 / cw2), by the point rules and kernels of the assembly.  X [npairs][ENND][3],
+// coll [npairs][3], sing [npairs] (-1: regular rule, a: Duffy rules about node a).  out holds six
+// arrays: GE_1, GE_2, HE_1, HE_2 of npairs * 9 ENND doubles, then C_1, C_2 of npairs * 9.
+// A pair whose sing is out of range gets NaN.
+template <int ENND>
+__global__ void __launch_bounds__(kBlock)
+    k_element(const double* __restrict__ X_all, const double* __restrict__ coll_all,
+              const int* __restrict__ sing_all, int64_t npairs, const double* __restrict__ gauss,
+              int nq, int kernel, double t, double dt, Material m, double cw1, double cw2,
+              double* __restrict__ out)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= npairs * 2) return;
+    const int part = (int)(s & 1);
+    const int64_t p = s >> 1;
+    constexpr int W = 9 * ENND;
+    double* GE = out + (part * npairs + p) * W;
+    double* HE = out + ((2 + part) * npairs + p) * W;
+    double* C = out + 4 * npairs * W + (part * npairs + p) * 9;
+    const int sing = sing_all[p];
+    if (sing < -1 || sing >= ENND) {
+        const double nan = __builtin_nan("");
+        for (int v = 0; v < W; v++) GE[v] = HE[v] = nan;
+        for (int v = 0; v < 9; v++) C[v] = nan;
+        return;
+    }
+    double X[3 * ENND], coll[3];
+#pragma unroll
+    for (int v = 0; v < 3 * ENND; v++) X[v] = X_all[p * 3 * ENND + v];
+#pragma unroll
+    for (int d = 0; d < 3; d++) coll[d] = coll_all[3 * p + d];
+    auto points = [&](auto&& f) { for_each_point_direct<ENND>(X, sing, gauss, nq, f); };
+    const double cw = part == 0 ? cw1 : cw2;
+    // Kelvin part at one point: KT = Kelvin traction * dt/2, transposed as the C block takes it
+    auto kelvin_at = [&](const double* x, const double* n, double* KT) {
+        double d[3] = {x[0] - coll[0], x[1] - coll[1], x[2] - coll[2]};
+        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const double ir = 1.0 / r;
+        d[0] *= ir;
+        d[1] *= ir;
+        d[2] *= ir;
+        kelvin_block(d, n, r, dt, m, KT);
+    };
+    // Singular elements of the Heaviside and pulse kernels subtract their Kelvin part from the
+    // singular node's H columns at every point, after the point's Stokes part, instead of adding
+    // it to C (src/TDElementHeavisideLinearT.cpp:489-497, src/TDElementRampHeaviLinearT.cpp:
+    // 499-506): near the node the two nearly cancel, and the per-point difference keeps the
+    // digits a separate sum would lose.
+    const bool into_h = sing >= 0 && kernel != 4;
+    auto extra = [&](const double* x, const double* n, double wJ, auto& acc) {
+        if (!into_h || cw == 0.0) return;
+        double KT[9];
+        kelvin_at(x, n, KT);
+        const double f = cw * wJ;
+#pragma unroll
+        for (int a = 0; a < ENND; a++)
+            if (a == sing)
+#pragma unroll
+                for (int v = 0; v < 9; v++) acc[a][6 + v] -= f * KT[v];
+    };
+    double acc[ENND * kVals];
+    if (kernel == 5 || kernel == 6) {
+        pulse_points<ENND>(points, coll, t, dt, m, part, kernel, acc, extra);
+    } else {
+        // Stokes_Step_Linear (HL(t) - HL(t - dt)) or the Heaviside kernel's HL(t), as k_pair_step
+        const bool heaviside = kernel == 2;
+        TimeCombo tc;
+        tc.t[0] = t - dt;
+        tc.t[1] = t;
+        tc.t[2] = 0.0;
+        const double prev = heaviside ? 0.0 : -1.0;
+        tc.cI[0] = part == 0 ? prev : 0.0;
+        tc.cJ[0] = part == 1 ? prev : 0.0;
+        tc.cI[1] = part == 0 ? 1.0 : 0.0;
+        tc.cJ[1] = part == 1 ? 1.0 : 0.0;
+        tc.cI[2] = 0.0;
+        tc.cJ[2] = 0.0;
+        tc.static_ok = !heaviside;
+        integrate_points<ENND>(points, coll, tc, dt, m, acc, 1, extra);
+    }
+    // C = cw * (-sum wJ K^T) for the others
+    double c[9];
+#pragma unroll
+    for (int v = 0; v < 9; v++) c[v] = 0.0;
+    if (cw != 0.0 && !into_h) {
+        points([&](const double* x, const double* n, double wJ, const double*) {
+            double KT[9];
+            kelvin_at(x, n, KT);
+#pragma unroll
+            for (int v = 0; v < 9; v++) c[v] -= wJ * KT[v];
+        });
+#pragma unroll
+        for (int v = 0; v < 9; v++) c[v] *= cw;
+    }
+#pragma unroll
+    for (int a = 0; a < ENND; a++)
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const int lo = r < q ? r : q, hi = r < q ? q : r;
+                const int o = lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
+                GE[r * 3 * ENND + 3 * a + q] = acc[a * kVals + o];
+                HE[r * 3 * ENND + 3 * a + q] = acc[a * kVals + 6 + 3 * r + q];
+            }
+#pragma unroll
+    for (int v = 0; v < 9; v++) C[v] = c[v];
 }
 
 // Dense row blocks of one step: thread per (row node il, column node j).
@@ -777,45 +930,53 @@ namespace {
 constexpr int64_t kScratchBytes = 1LL << 30;
 }  // namespace
 
-void assemble_step_dense(Context& c, int step, int lower, int upper, double* dG1, double* dG2,
-                         double* dH1, double* dH2, double* work, int64_t work_doubles)
+// Weights of the Kelvin block (the C-term) in the phi_1 and phi_2 matrices at time t: the Step
+// kernel at its first step only (first: fT == fT_1, src/TDElementStepLinearT.cpp:103), Heaviside
+// every step; B-spline and ramp every step with the phi_1 / phi_2 weights of their load pulse,
+// Int_CubicB_Linear (src/GreenFunctionT.cpp:733-759) and Int_RampedHeaviside_Linear (:705-731),
+// over dt/2.
+void kelvin_weights(int kernel, double t, double dt, bool first, double& c1s, double& c2s)
 {
-    if (work && !c.cterm_ready) throw std::logic_error("C-term not computed");
-    compute_cterm(c);
-    // C-term (Kelvin * dt/2 summed over the elements) into H1 and H2: Step kernel at step 1
-    // only, Heaviside every step; B-spline and ramp every step with the phi_1 / phi_2 weights of
-    // their load pulse, Int_CubicB_Linear (src/GreenFunctionT.cpp:733-759) and
-    // Int_RampedHeaviside_Linear (:705-731), over dt/2
-    double c1s = 0.0, c2s = 0.0;
-    if (c.kernel == 2 || (c.kernel == 4 && step == 1)) {
+    c1s = c2s = 0.0;
+    if (kernel == 2 || (kernel == 4 && first)) {
         c1s = c2s = 1.0;
-    } else if (c.kernel == 5) {
-        const double t = step * c.dt, bw = 0.7 * c.dt, J = c.dt * 0.5;
+    } else if (kernel == 5) {
+        const double bw = 0.7 * dt, J = dt * 0.5;
         double g1 = 0.0, g2 = 0.0;
         for (int q = 0; q < 20; q++) {
             const double ab = kTime20xHost[q], w = kTime20wHost[q];
             const double phi1 = 0.5 * (1 - ab), phi2 = 0.5 * (1 + ab);
             double g, dg;
-            cubic_b1(t - phi2 * c.dt, bw, g, dg);
+            cubic_b1(t - phi2 * dt, bw, g, dg);
             g1 += phi1 * g * w * J;
             g2 += phi2 * g * w * J;
         }
-        c1s = g1 / (0.5 * c.dt);
-        c2s = g2 / (0.5 * c.dt);
-    } else if (c.kernel == 6) {
-        const double t = step * c.dt, k = 2 / c.dt, J = c.dt * 0.5;
+        c1s = g1 / (0.5 * dt);
+        c2s = g2 / (0.5 * dt);
+    } else if (kernel == 6) {
+        const double k = 2 / dt, J = dt * 0.5;
         const double *gx, *gw;
         gauss_table(40, &gx, &gw);
         double g1 = 0.0, g2 = 0.0;
         for (int q = 0; q < 40; q++) {
             const double phi1 = 0.5 * (1 - gx[q]), phi2 = 0.5 * (1 + gx[q]);
-            const double g = ramp_load(t - phi2 * c.dt, k);
+            const double g = ramp_load(t - phi2 * dt, k);
             g1 += phi1 * g * gw[q] * J;
             g2 += phi2 * g * gw[q] * J;
         }
-        c1s = g1 / (0.5 * c.dt);
-        c2s = g2 / (0.5 * c.dt);
+        c1s = g1 / (0.5 * dt);
+        c2s = g2 / (0.5 * dt);
     }
+}
+
+void assemble_step_dense(Context& c, int step, int lower, int upper, double* dG1, double* dG2,
+                         double* dH1, double* dH2, double* work, int64_t work_doubles)
+{
+    if (work && !c.cterm_ready) throw std::logic_error("C-term not computed");
+    compute_cterm(c);
+    // C-term (Kelvin * dt/2 summed over the elements) into H1 and H2
+    double c1s, c2s;
+    kelvin_weights(c.kernel, step * c.dt, c.dt, step == 1, c1s, c2s);
     const double ext = (c.if_ext && step == 1) ? 0.5 * c.dt : 0.0;
     // per-pair slots of one row batch: in a buffer of this call, or in the caller's workspace
     // (work, work_doubles: nothing is allocated and the call does not wait for its kernels)
@@ -855,6 +1016,24 @@ void assemble_step_dense(Context& c, int step, int lower, int upper, double* dG1
     }
     // a buffer of this call must outlive the kernels that use it
     if (!work) TD_HIP(hipStreamSynchronize(c.stream));
+}
+
+void element_form(Element& el, int64_t npairs, const double* X, const double* coll,
+                  const int* sing, double t, double t_1, double* out)
+{
+    if (npairs <= 0) return;
+    // the Step kernel's Kelvin block at its first step only: fT == fT_1, compared exactly as the
+    // reference compares them
+    double cw1, cw2;
+    kelvin_weights(el.kernel, t, t_1, t == t_1, cw1, cw2);
+    const unsigned grid = grid_for(2 * npairs);
+    if (el.ennd == 4)
+        hipLaunchKernelGGL(k_element<4>, dim3(grid), dim3(kBlock), 0, el.stream, X, coll, sing,
+                           npairs, el.gauss.p, el.nq, el.kernel, t, t_1, el.mat, cw1, cw2, out);
+    else
+        hipLaunchKernelGGL(k_element<8>, dim3(grid), dim3(kBlock), 0, el.stream, X, coll, sing,
+                           npairs, el.gauss.p, el.nq, el.kernel, t, t_1, el.mat, cw1, cw2, out);
+    TD_HIP(hipGetLastError());
 }
 
 namespace {

@@ -20,6 +20,10 @@ struct tdbem_batch {
     ~tdbem_batch() { tdbem::batch_destroy(b); }
 };
 
+struct tdbem_element {
+    tdbem::Element e;
+};
+
 struct tdbem_mrt {
     tdbem::Mrt* m = nullptr;
     ~tdbem_mrt() { tdbem::mrt_destroy(m); }
@@ -31,6 +35,28 @@ void check_hip(hipError_t e, const char* what)
 {
     if (e != hipSuccess)
         throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e) + " in " + what);
+}
+
+Material make_material(double E, double nu, double rho)
+{
+    Material m;
+    m.E = E;
+    m.nu = nu;
+    m.rho = rho;
+    m.mu = E / (2 * (1 + nu));
+    const double lam = E * nu / ((1 + nu) * (1 - 2 * nu));
+    m.cp = std::sqrt((lam + 2 * m.mu) / rho);
+    m.cs = std::sqrt(m.mu / rho);
+    return m;
+}
+
+Element::~Element()
+{
+    gauss.reset();
+    in.reset();
+    out.reset();
+    sing.reset();
+    if (stream) (void)hipStreamDestroy(stream);
 }
 
 Context::~Context()
@@ -278,6 +304,15 @@ int batch_guarded(F&& f)
         return TDBEM_ERR_HIP;
     }
 }
+
+template <typename F>
+int element_guarded(tdbem_element* el, F&& f)
+{
+    if (!el) return TDBEM_ERR_ARG;
+    const int rc = batch_guarded([&] { f(el->e); });
+    el->e.err = rc == TDBEM_OK ? std::string() : g_create_error;
+    return rc;
+}
 }  // namespace
 
 extern "C" {
@@ -320,13 +355,7 @@ int tdbem_create(int nnd, const double* coords, int nel, const int* ien, int ety
         c.if_ext = if_ext ? 1 : 0;
         c.numstep = numstep;
         c.dt = dt;
-        c.mat.E = E;
-        c.mat.nu = nu;
-        c.mat.rho = rho;
-        c.mat.mu = E / (2 * (1 + nu));
-        const double lam = E * nu / ((1 + nu) * (1 - 2 * nu));
-        c.mat.cp = std::sqrt((lam + 2 * c.mat.mu) / rho);
-        c.mat.cs = std::sqrt(c.mat.mu / rho);
+        c.mat = make_material(E, nu, rho);
         c.h_coords.assign(coords, coords + 3 * (size_t)nnd);
         c.h_ien.assign(ien, ien + (size_t)nel * ennd);
         c.K = max_step(c.h_coords, nnd, E, nu, rho, dt);
@@ -852,6 +881,118 @@ int tdbem_mrt_destroy(tdbem_mrt* m)
 {
     delete m;
     return TDBEM_OK;
+}
+
+int tdbem_element_create(int etype, int kernel, int nq, double E, double nu, double density,
+                         tdbem_element** out)
+{
+    if (!out) return TDBEM_ERR_ARG;
+    *out = nullptr;
+    tdbem_element* el = nullptr;
+    const int rc = batch_guarded([&] {
+        const double *gx, *gw;
+        if (etype != 1 && etype != 2)
+            throw std::invalid_argument(
+                "TDElementT::TDElementT, unsupported element type. There is no sub element for "
+                "this element");
+        if (!gauss_table(nq, &gx, &gw))
+            throw std::invalid_argument(
+                "ShapeFunctionT::SetNQ, Unsupported number of Gaussian points");
+        if (kernel != TDBEM_KERNEL_STEP && kernel != TDBEM_KERNEL_HEAVISIDE &&
+            kernel != TDBEM_KERNEL_BSPLINE && kernel != TDBEM_KERNEL_RAMP)
+            throw std::invalid_argument("unsupported temporal kernel");
+        if (!(E > 0) || !(density > 0) || !(nu > -1 && nu < 0.5))
+            throw std::invalid_argument("bad material");
+        el = new tdbem_element();
+        Element& e = el->e;
+        TD_HIP(hipGetDevice(&e.device));
+        TD_HIP(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
+        keep_pool_memory(e.device);
+        e.etype = etype;
+        e.ennd = etype == 1 ? 4 : 8;
+        e.nq = nq;
+        e.kernel = kernel;
+        e.mat = make_material(E, nu, density);
+        std::vector<double> g(2 * nq);
+        std::memcpy(g.data(), gx, nq * sizeof(double));
+        std::memcpy(g.data() + nq, gw, nq * sizeof(double));
+        e.gauss.upload(g.data(), g.size(), e.stream);
+        TD_HIP(hipStreamSynchronize(e.stream));
+    });
+    if (rc != TDBEM_OK) {
+        delete el;
+        return rc;
+    }
+    *out = el;
+    return TDBEM_OK;
+}
+
+int tdbem_element_destroy(tdbem_element* el)
+{
+    delete el;
+    return TDBEM_OK;
+}
+
+const char* tdbem_element_last_error(const tdbem_element* el)
+{
+    return el ? el->e.err.c_str() : g_create_error.c_str();
+}
+
+int tdbem_element_form(tdbem_element* el, int64_t npairs, const double* coords,
+                       const double* collocation, const int* singular, double t, double t_1,
+                       double* GE_1, double* GE_2, double* HE_1, double* HE_2, double* C_1,
+                       double* C_2)
+{
+    return element_guarded(el, [&](Element& e) {
+        if (npairs < 0) throw std::invalid_argument("npairs < 0");
+        if (npairs == 0) return;
+        if (!coords || !collocation || !singular || !GE_1 || !GE_2 || !HE_1 || !HE_2 || !C_1 ||
+            !C_2)
+            throw std::invalid_argument("null array");
+        if (!(t_1 > 0) || !std::isfinite(t)) throw std::invalid_argument("bad time");
+        for (int64_t p = 0; p < npairs; p++)
+            if (singular[p] < -1 || singular[p] >= e.ennd)
+                throw std::invalid_argument(
+                    "TDElementT::FormGH_Singular, collocation id is out of range");
+        const int64_t nx = npairs * 3 * e.ennd, w = npairs * 9 * e.ennd;
+        e.in.reserve((size_t)(nx + 3 * npairs));
+        e.sing.reserve((size_t)npairs);
+        e.out.reserve((size_t)(4 * w + 18 * npairs));
+        TD_HIP(hipMemcpyAsync(e.in.p, coords, nx * sizeof(double), hipMemcpyHostToDevice,
+                              e.stream));
+        TD_HIP(hipMemcpyAsync(e.in.p + nx, collocation, 3 * npairs * sizeof(double),
+                              hipMemcpyHostToDevice, e.stream));
+        TD_HIP(hipMemcpyAsync(e.sing.p, singular, npairs * sizeof(int), hipMemcpyHostToDevice,
+                              e.stream));
+        element_form(e, npairs, e.in.p, e.in.p + nx, e.sing.p, t, t_1, e.out.p);
+        double* dst[6] = {GE_1, GE_2, HE_1, HE_2, C_1, C_2};
+        for (int b = 0; b < 6; b++) {
+            const int64_t off = b < 4 ? b * w : 4 * w + (b - 4) * 9 * npairs;
+            const int64_t len = b < 4 ? w : 9 * npairs;
+            TD_HIP(hipMemcpyAsync(dst[b], e.out.p + off, len * sizeof(double),
+                                  hipMemcpyDeviceToHost, e.stream));
+        }
+        TD_HIP(hipStreamSynchronize(e.stream));
+    });
+}
+
+int tdbem_element_form_device(tdbem_element* el, int64_t npairs, const double* coords,
+                              const double* collocation, const int* singular, double t,
+                              double t_1, double* out)
+{
+    return element_guarded(el, [&](Element& e) {
+        if (npairs < 0) throw std::invalid_argument("npairs < 0");
+        if (npairs == 0) return;
+        if (!coords || !collocation || !singular || !out)
+            throw std::invalid_argument("null array");
+        if (!(t_1 > 0) || !std::isfinite(t)) throw std::invalid_argument("bad time");
+        element_form(e, npairs, coords, collocation, singular, t, t_1, out);
+    });
+}
+
+int tdbem_element_synchronize(tdbem_element* el)
+{
+    return element_guarded(el, [&](Element& e) { TD_HIP(hipStreamSynchronize(e.stream)); });
 }
 
 }  // extern "C"

@@ -310,7 +310,29 @@ struct Context {
     ~Context();
 };
 
+// Material constants with the wave speeds (capi.hip).
+Material make_material(double E, double nu, double rho);
+
+// A reusable element evaluator: TDElementT (src/TDElementT.h:16-115) for (collocation point,
+// element) pairs of no mesh, on one device and stream; its buffers grow with the largest call.
+struct Element {
+    int device = 0, etype = 1, ennd = 4, nq = 6, kernel = 4;
+    Material mat{};
+    hipStream_t stream = nullptr;
+    DBuf<double> gauss;             // [nq] abscissae then [nq] weights
+    DBuf<double> in, out;           // staging of host-pointer calls
+    DBuf<int> sing;
+    std::string err;
+    ~Element();
+};
+
 // assemble.hip
+// Element matrices of npairs pairs (device pointers, stream-ordered): X [npairs][ennd][3], coll
+// [npairs][3], sing [npairs]; out: GE_1, GE_2, HE_1, HE_2 (npairs * 9 ennd each), C_1, C_2
+// (npairs * 9 each), the arrays of TDElementT::GetGHLinear after SetTime(t, t_1) and
+// FormGH_Regular (sing -1) or FormGH_Singular(sing).
+void element_form(Element& el, int64_t npairs, const double* X, const double* coll,
+                  const int* sing, double t, double t_1, double* out);
 void prepare_geometry(Context& c);
 void compute_cterm(Context& c);
 void compute_pair_ranges(Context& c);

@@ -236,69 +236,34 @@ TDElementT::TDElementT(int type, int kernel) : fType(type), fKernel(kernel)
     fC_2 = fC_1;
 }
 
-TDElementT::~TDElementT() = default;
+TDElementT::~TDElementT()
+{
+    if (fEval) tdbem_element_destroy(fEval);
+}
 
 void TDElementT::Form(int singular)
 {
-    // one element plus the collocation point as an extra node: the element blocks are the
-    // columns of the element nodes, the C-term lands on the extra node's diagonal block
-    const int ennd = fType == 1 ? 4 : 8;
     if (!fCoords || !fCollocation) fail("TDElementT: coordinates or collocation point not set");
-    std::vector<double> X(3 * (ennd + 1));
-    std::memcpy(X.data(), fCoords, sizeof(double) * 3 * ennd);
-    std::memcpy(X.data() + 3 * ennd, fCollocation, sizeof(double) * 3);
-    std::vector<int> ien(ennd);
-    for (int a = 0; a < ennd; a++) ien[a] = a;
-    int sing = -1;
-    for (int a = 0; a < ennd; a++) {
-        double r2 = 0;
-        for (int d = 0; d < 3; d++) r2 += std::pow(fCoords[3 * a + d] - fCollocation[d], 2);
-        if (r2 < 1e-9) sing = a;
-    }
-    if (sing != singular) fail("TDElementT: the singular node does not match the geometry");
-    const int step = (int)std::llround(fT / fT_1);
-    if (step < 1) fail("TDElementT: time must be a positive multiple of the step");
-    tdbem_context* ctx = nullptr;
-    if (tdbem_create(ennd + 1, X.data(), 1, ien.data(), fType, fNQ, fE, fNu, fDensity, 0, fT_1,
-                     step, fKernel, &ctx) != TDBEM_OK) {
-        static thread_local std::string msg;
-        msg = tdbem_last_error(nullptr);
-        throw msg.c_str();
-    }
-    const int ncol = 3 * (ennd + 1);
-    std::vector<double> g1(3 * ncol), g2(3 * ncol), h1(3 * ncol), h2(3 * ncol);
-    const int rc = tdbem_assemble_step(ctx, step, ennd, ennd, g1.data(), g2.data(), h1.data(),
-                                       h2.data());
-    if (rc != TDBEM_OK) {
-        static thread_local std::string msg;
-        msg = tdbem_last_error(ctx);
-        tdbem_destroy(ctx);
-        throw msg.c_str();
-    }
-    tdbem_destroy(ctx);
-    for (int i = 0; i < 3; i++) {
-        for (int c = 0; c < 3 * ennd; c++) {
-            fGE_1[i * 3 * ennd + c] = g1[i * ncol + c];
-            fGE_2[i * 3 * ennd + c] = g2[i * ncol + c];
-            fHE_1[i * 3 * ennd + c] = h1[i * ncol + c];
-            fHE_2[i * 3 * ennd + c] = h2[i * ncol + c];
+    if (!fEval || fEvalNQ != fNQ || fEvalMat[0] != fE || fEvalMat[1] != fNu ||
+        fEvalMat[2] != fDensity) {
+        if (fEval) tdbem_element_destroy(fEval);
+        fEval = nullptr;
+        if (tdbem_element_create(fType, fKernel, fNQ, fE, fNu, fDensity, &fEval) != TDBEM_OK) {
+            static thread_local std::string msg;
+            msg = tdbem_element_last_error(nullptr);
+            throw msg.c_str();
         }
-        for (int j = 0; j < 3; j++) {
-            fC_1[3 * i + j] = h1[i * ncol + 3 * ennd + j];
-            fC_2[3 * i + j] = h2[i * ncol + 3 * ennd + j];
-        }
+        fEvalNQ = fNQ;
+        fEvalMat[0] = fE;
+        fEvalMat[1] = fNu;
+        fEvalMat[2] = fDensity;
     }
-    // Heaviside and B-spline singular elements subtract their Kelvin part from the singular
-    // node's H columns instead of C (src/TDElementHeavisideLinearT.cpp:489-497,
-    // src/TDElementRampHeaviLinearT.cpp:499-506); the Step kernel keeps it in C
-    if (singular >= 0 && fKernel != TDBEM_KERNEL_STEP) {
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) {
-                fHE_1[i * 3 * ennd + 3 * singular + j] += fC_1[3 * i + j];
-                fHE_2[i * 3 * ennd + 3 * singular + j] += fC_2[3 * i + j];
-                fC_1[3 * i + j] = 0.0;
-                fC_2[3 * i + j] = 0.0;
-            }
+    if (tdbem_element_form(fEval, 1, fCoords, fCollocation, &singular, fT, fT_1, fGE_1.data(),
+                           fGE_2.data(), fHE_1.data(), fHE_2.data(), fC_1.data(),
+                           fC_2.data()) != TDBEM_OK) {
+        static thread_local std::string msg;
+        msg = tdbem_element_last_error(fEval);
+        throw msg.c_str();
     }
 }
 

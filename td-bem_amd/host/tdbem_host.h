@@ -90,14 +90,15 @@ private:
     std::vector<double> fUBCv_local, fFBCv_local;
 };
 
-// One (collocation point, element) pair, src/TDElementT.h:16-115.  Evaluated on the GPU through
-// a one-element problem whose extra node is the collocation point (or the element node it
-// coincides with), so singular detection, the Duffy rules and the C-term are those of the
-// assembly path.
+// One (collocation point, element) pair, src/TDElementT.h:16-115.  Evaluated on the GPU by a
+// reusable element evaluator (tdbem_element_form), made at the first Form and made again when
+// the rule or the material changes.
 class TDElementT {
 public:
     explicit TDElementT(int type, int kernel);
     virtual ~TDElementT();
+    TDElementT(const TDElementT&) = delete;
+    TDElementT& operator=(const TDElementT&) = delete;
     void SetCollocation(double* collocation) { fCollocation = collocation; }
     void SetCoords(double* coords) { fCoords = coords; }
     virtual void SetTime(double t, double t_1)
@@ -119,6 +120,9 @@ public:
 
 protected:
     void Form(int singular);
+    tdbem_element* fEval = nullptr;
+    int fEvalNQ = 0;
+    double fEvalMat[3] = {0, 0, 0};
     int fType, fKernel, fNQ = 6;
     double fE = 1, fNu = 0, fDensity = 1, fT = 1, fT_1 = 1;
     double* fCollocation = nullptr;

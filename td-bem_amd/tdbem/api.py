@@ -97,6 +97,16 @@ def lib():
         L.tdbem_mrt_get_state.argtypes = [vp, C.c_int, dp, dp, dp]
         L.tdbem_mrt_get_timings.argtypes = [vp, dp, dp]
         L.tdbem_mrt_destroy.argtypes = [vp]
+        L.tdbem_element_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                           C.c_double, C.POINTER(vp)]
+        L.tdbem_element_destroy.argtypes = [vp]
+        L.tdbem_element_last_error.argtypes = [vp]
+        L.tdbem_element_last_error.restype = C.c_char_p
+        L.tdbem_element_form.argtypes = [vp, C.c_int64, dp, dp, ip, C.c_double, C.c_double,
+                                         dp, dp, dp, dp, dp, dp]
+        L.tdbem_element_form_device.argtypes = [vp, C.c_int64, vp, vp, vp, C.c_double,
+                                                C.c_double, vp]
+        L.tdbem_element_synchronize.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -457,6 +467,74 @@ class Batch:
     def close(self):
         if getattr(self, "_h", None):
             lib().tdbem_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Element:
+    """TDElementT (src/TDElementT.h:16-115) as a reusable evaluator of (collocation point,
+    element) pairs: kernel picks the element class (KERNEL_STEP: TDElementStepLinearT,
+    KERNEL_HEAVISIDE: TDElementHeavisideLinearT, KERNEL_BSPLINE / KERNEL_RAMP:
+    TDElementRampHeaviLinearT)."""
+
+    def __init__(self, etype=1, kernel=KERNEL_STEP, nq=6, *, E, nu, rho):
+        h = C.c_void_p()
+        rc = lib().tdbem_element_create(int(etype), int(kernel), int(nq), float(E), float(nu),
+                                        float(rho), C.byref(h))
+        if rc:
+            raise TdbemError(rc, lib().tdbem_element_last_error(None).decode())
+        self._h = h
+        self.ennd = 4 if etype == 1 else 8
+
+    def _check(self, rc):
+        if rc:
+            raise TdbemError(rc, lib().tdbem_element_last_error(self._h).decode())
+
+    def form(self, coords, collocation, singular, t, t_1):
+        """SetCoords / SetCollocation / SetTime(t, t_1) / FormGH_Regular (singular -1) or
+        FormGH_Singular(singular) / GetGHLinear for each pair.  coords [n][ennd][3] (or one
+        [ennd][3]), collocation [n][3], singular [n].  Returns GE_1, GE_2, HE_1, HE_2
+        [n][3][3 ennd] and C_1, C_2 [n][3][3] (without the leading n for a single pair)."""
+        X = np.ascontiguousarray(coords, dtype=np.float64)
+        single = X.ndim == 2
+        X = X.reshape(-1, self.ennd, 3)
+        n = X.shape[0]
+        col = np.ascontiguousarray(collocation, dtype=np.float64).reshape(n, 3)
+        sg = np.ascontiguousarray(np.broadcast_to(np.asarray(singular, dtype=np.int32), (n,)))
+        out = [np.empty((n, 3, 3 * self.ennd)) for _ in range(4)] + \
+              [np.empty((n, 3, 3)) for _ in range(2)]
+        self._check(lib().tdbem_element_form(self._h, n, _d(X), _d(col), _i(sg), float(t),
+                                             float(t_1), *[_d(o) for o in out]))
+        return [o[0] for o in out] if single else out
+
+    def form_device(self, coords, collocation, singular, t, t_1, out):
+        """tdbem_element_form_device on torch tensors already on the device (float64 coords
+        [n][ennd][3] and collocation [n][3], int32 singular [n], float64 out of
+        n * (36 ennd + 18)); queued on the element's stream, see synchronize()."""
+        n = collocation.shape[0]
+        assert coords.numel() == n * 3 * self.ennd and out.numel() == n * (36 * self.ennd + 18)
+        assert singular.numel() == n and singular.dtype.itemsize == 4
+        self._check(lib().tdbem_element_form_device(
+            self._h, n, C.c_void_p(coords.data_ptr()), C.c_void_p(collocation.data_ptr()),
+            C.c_void_p(singular.data_ptr()), float(t), float(t_1), C.c_void_p(out.data_ptr())))
+
+    def synchronize(self):
+        self._check(lib().tdbem_element_synchronize(self._h))
+
+    def close(self):
+        if self._h:
+            lib().tdbem_element_destroy(self._h)
             self._h = None
 
     def __del__(self):

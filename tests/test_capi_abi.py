@@ -33,6 +33,14 @@ def test_create_without_gpu_fails_loudly():
                     rho=1, dt=0.1, numstep=3)
 
 
+def test_element_create_without_gpu_fails_loudly():
+    from conftest import gpu_available
+    if gpu_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(api.TdbemError):
+        api.Element(1, api.KERNEL_STEP, 6, E=1.0, nu=0.25, rho=1.0)
+
+
 def test_product_does_not_reference_the_oracle():
     pkg = ROOT / "td-bem_amd"
     for f in list(pkg.rglob("*.py")) + list(pkg.rglob("*.hip")) + list(pkg.rglob("*.h")) + \
