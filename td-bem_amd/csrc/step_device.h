@@ -182,6 +182,15 @@ __device__ __forceinline__ void conv_chunk(const f32x4* __restrict__ tp,
             zload(ent[e + 1].id, knext);
         }
         const bf16x8* Z = zl[cur] + lane;
+        // The leading product and the corrections accumulate apart in FP32, over the entry's
+        // three k-tiles (96 columns) where the registers allow it (kChain; the 48-rhs variant
+        // would lose its second wave per SIMD), else over one sub-tile, and then move to FP64.
+        constexpr bool kChain = NT != 3;
+        f32x4 dm[3][NT], dc[3][NT];
+#pragma unroll
+        for (int m = 0; m < 3; m++)
+#pragma unroll
+            for (int n = 0; n < NT; n++) dm[m][n] = dc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kt = 0; kt < 3; kt++) {
             const unsigned rows = (msk >> kt) & 0x49u;   // bits kt, 3 + kt, 6 + kt
@@ -198,27 +207,40 @@ __device__ __forceinline__ void conv_chunk(const f32x4* __restrict__ tp,
                     bf16x8 a1, a2, a3;
                     split_f32x8(a[u][0], a[u][1], a1, a2, a3);
                     // the leading product a1 z1 and the seven corrections (smallest first) sum
-                    // in separate FP32 registers; both move to FP64 after each sub-tile.
-                    // Dropped: a3 z3, below 2^-28 of |a| |z|.
+                    // in separate FP32 registers.  Dropped: a3 z3, below 2^-28 of |a| |z|.
 #pragma unroll
                     for (int n = 0; n < NT; n++) {
                         const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
-                        const f32x4 dm =
-                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, z[0][n], zero, 0, 0, 0);
-                        f32x4 dc =
-                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, z[1][n], zero, 0, 0, 0);
-                        dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, z[2][n], dc, 0, 0, 0);
-                        dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, z[2][n], dc, 0, 0, 0);
-                        dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, z[0][n], dc, 0, 0, 0);
-                        dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, z[1][n], dc, 0, 0, 0);
-                        dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, z[0][n], dc, 0, 0, 0);
-                        dc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, z[1][n], dc, 0, 0, 0);
+                        f32x4 d = kChain ? dm[m][n] : zero, c = kChain ? dc[m][n] : zero;
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, z[0][n], d, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, z[1][n], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, z[2][n], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, z[2][n], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, z[0][n], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, z[1][n], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, z[0][n], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, z[1][n], c, 0, 0, 0);
+                        if constexpr (kChain) {
+                            dm[m][n] = d;
+                            dc[m][n] = c;
+                        } else {
 #pragma unroll
-                        for (int v = 0; v < 4; v++) acc[m][n][v] += (double)dm[v] + (double)dc[v];
+                            for (int v = 0; v < 4; v++)
+                                acc[m][n][v] += (double)d[v] + (double)c[v];
+                        }
                     }
                 }
             }
             if (more) aload(mnext, fnext, kt);
+        }
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            if (!kChain || !((msk >> (3 * m)) & 7u)) continue;
+#pragma unroll
+            for (int n = 0; n < NT; n++)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    acc[m][n][v] += (double)dm[m][n][v] + (double)dc[m][n][v];
         }
         if (more) zstore(cur ^ 1, knext);
         __syncthreads();
