@@ -64,7 +64,9 @@ __global__ void __launch_bounds__(256)
                     int P0, int nel, const uint2* __restrict__ sub_id, int64_t s0, int64_t s1,
                     f32x4* __restrict__ tp)
 {
-    const int64_t sidx = s0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    // consecutive sub-tiles (the lags of one tile pair) on one XCD, whose L2 then serves the
+    // neighbouring slots of the next lags
+    const int64_t sidx = s0 + (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (sidx >= s1) return;
     const int lane = threadIdx.x & 63;
     const uint2 id = sub_id[sidx];
