@@ -128,9 +128,9 @@ struct TimeCombo {
 };
 
 __device__ __forceinline__ bool eval_s(const TimeCombo& tc, double r, double dt, double inv_dt,
-                                       const Material& m, double* S)
+                                       const Material& m, double* S, double& ir)
 {
-    const double inv_cp = 1.0 / m.cp, inv_cs = 1.0 / m.cs;
+    const double inv_cp = m.inv_cp, inv_cs = m.inv_cs;
     const double ta = r * inv_cp;         // P-wave arrival
     const double ts = r * inv_cs + dt;    // static from here on
     bool any = false, all_static = true;
@@ -141,14 +141,15 @@ __device__ __forceinline__ bool eval_s(const TimeCombo& tc, double r, double dt,
         if (tc.t[l] < ts) all_static = false;
     }
     if (!any || (tc.static_ok && all_static)) return false;
-    const double inv_r2 = 1.0 / (r * r);
+    ir = 1.0 / r;
+    const double inv_r2 = ir * ir;
 #pragma unroll
     for (int l = 0; l < 6; l++) S[l] = 0.0;
 #pragma unroll
     for (int l = 0; l < 3; l++) {
         if ((tc.cI[l] == 0.0 && tc.cJ[l] == 0.0) || !(tc.t[l] > ta)) continue;
-        accumulate_hl<double>(tc.t[l], dt, inv_dt, r, inv_r2, inv_cp, inv_cs, inv_cp * inv_cp,
-                              inv_cs * inv_cs, tc.cI[l], tc.cJ[l], S);
+        accumulate_hl<double>(tc.t[l], dt, inv_dt, r, inv_r2, inv_cp, inv_cs, m.inv_cp2,
+                              m.inv_cs2, tc.cI[l], tc.cJ[l], S);
     }
     return true;
 }
@@ -352,10 +353,9 @@ __device__ __forceinline__ int integrate_points(Points&& points, const double* c
     points([&](const double* x, const double* n, double wJ, const double* N) {
         double d[3] = {x[0] - coll[0], x[1] - coll[1], x[2] - coll[2]};
         const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        double S[6];
-        if (eval_s(tc, r, dt, inv_dt, m, S)) {
+        double S[6], ir;
+        if (eval_s(tc, r, dt, inv_dt, m, S, ir)) {
             active++;
-            const double ir = 1.0 / r;
             d[0] *= ir;
             d[1] *= ir;
             d[2] *= ir;

@@ -47,6 +47,13 @@ Material make_material(double E, double nu, double rho)
     const double lam = E * nu / ((1 + nu) * (1 - 2 * nu));
     m.cp = std::sqrt((lam + 2 * m.mu) / rho);
     m.cs = std::sqrt(m.mu / rho);
+    m.inv_cp = 1.0 / m.cp;
+    m.inv_cs = 1.0 / m.cs;
+    m.inv_cp2 = 1.0 / (m.cp * m.cp);
+    m.inv_cs2 = 1.0 / (m.cs * m.cs);
+    m.k2 = (m.cs / m.cp) * (m.cs / m.cp);
+    m.k3 = m.k2 * (m.cs / m.cp);
+    m.inv_4pi_rho = 1.0 / (4 * kPi * rho);
     return m;
 }
 

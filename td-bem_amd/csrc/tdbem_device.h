@@ -18,6 +18,10 @@ constexpr double kPi = 3.14159265358979323846;
 
 struct Material {
     double E, nu, rho, mu, cp, cs;
+    // derived once on the host (make_material), so that the kernels multiply instead of divide
+    double inv_cp, inv_cs, inv_cp2, inv_cs2;   // 1/c_p, 1/c_s, 1/c_p^2, 1/c_s^2
+    double k2, k3;                             // (c_s/c_p)^2, (c_s/c_p)^3
+    double inv_4pi_rho;                        // 1 / (4 pi rho)
 };
 
 // Six closed-form integrals of one wave against the phi_1 ("I") and phi_2 ("J") hats:
@@ -81,25 +85,25 @@ template <typename R>
 __device__ __forceinline__ void blocks_from_s(const R* S, const R* rh, const R* n, R r,
                                               R inv_r, R inv_r2, const Material& m, R* out)
 {
-    const R cs = R(m.cs), cp = R(m.cp);
-    const R inv_cs2 = R(1) / (cs * cs);
-    const R k2 = (cs / cp) * (cs / cp);
-    const R k3 = k2 * (cs / cp);
+    const R cs = R(m.cs);
+    const R inv_cs2 = R(m.inv_cs2);
+    const R k2 = R(m.k2);
+    const R k3 = R(m.k3);
     const R A1 = R(0.5) * (S[0] - S[1]);
-    const R A2 = S[2] / (cp * cp) - S[3] * inv_cs2;
+    const R A2 = S[2] * R(m.inv_cp2) - S[3] * inv_cs2;
     const R A3 = S[3];
     const R B2 = S[3] - k2 * S[2];
     const R B3 = S[5] - k3 * S[4];
-    const R B4 = S[2] + r / cp * S[4];
-    const R B5 = S[3] + r / cs * S[5];
+    const R B4 = S[2] + r * R(m.inv_cp) * S[4];
+    const R B5 = S[3] + r * R(m.inv_cs) * S[5];
     const R rdn = rh[0] * n[0] + rh[1] * n[1] + rh[2] * n[2];
-    const R uscale = R(1) / (R(4 * kPi) * R(m.rho) * r);
+    const R uscale = R(m.inv_4pi_rho) * inv_r;
     const R ua = (R(3) * A1 + A2) * uscale;
     const R ub = (-A1 + A3 * inv_cs2) * uscale;
     const R w = (R(6) * cs * cs * A1 - R(2) * B2) * inv_r2;
     const R t4 = R(1) / R(4 * kPi);
     const R al = (rdn * (-R(30) * cs * cs * A1 + R(12) * B2) * inv_r2 +
-                  R(2) * rdn * B3 * inv_r / cs) * t4;
+                  R(2) * rdn * B3 * inv_r * R(m.inv_cs)) * t4;
     const R be = (w - (R(1) - R(2) * k2) * B4 * inv_r2) * t4;
     const R ga = (w - B5 * inv_r2) * t4;
     const R de = (rdn * w - rdn * B5 * inv_r2) * t4;
