@@ -11,18 +11,33 @@ One bench "step" is one complete solve: lag-store assembly of all 66 lags, the s
 factorisations, and the 599 marching solves (src/main.cpp:89-128).  The mesh is resident on the
 GPU before the timed region.  value = time-steps solved per second (summed over right-hand sides)
 of the mixed form; plain_fp64 gives the same for the plain path (one load, as the reference runs)
-and fp64_8rhs for the FP64 path at 8 right-hand sides, its best rate.  The mixed error is max over
-8 right-hand sides of ||u_mixed - u_fp64||_F / ||u_fp64||_F (and the same for t).
+and fp64_8rhs for the FP64 path at 8 right-hand sides, its best rate.  The mixed error is the max
+over all 32 right-hand sides of ||u_mixed - u_fp64||_F / ||u_fp64||_F (and the same for t)
+against FP64 solves of the same loads; above 1e-6 the line is marked invalid and the exit code
+is 3.
 
 roofline: the mixed path's dominant kernel k_conv_mixed, HBM-bound: algorithmic bytes per launch =
-stored 16 x 32 FP32 sub-tiles x 2048 B + merged-list entries x 32 B + the FP32 history copy
-read once, over its mean duration from HIP events on the context stream;
-plain_fp64.roofline gives k_conv the same way (stored entries x 124 B).
-Printed as one JSON line; --gpus N > 1 exits (one GPU per box in this build).
+the in-shell node-pair lags x 18 FP32 values (G's 9 and H's 9 of each 3 x 3 node block) + the
+merged-list entries x 32 B + the FP32 history copy read once, over its mean duration from HIP
+events on the context stream; stored_bytes_per_launch counts whole stored 16 x 32 sub-tiles
+instead.  traffic: HBM bytes per launch measured in this run by two rocprofv3 --pmc passes
+(FETCH_SIZE, WRITE_SIZE) over a short run of the same workload (--traffic-probe), each scaled
+by what the counter reads for a 1 GiB k_stream_copy; null when rocprofv3 is absent (--no-pmc
+skips it).  plain_fp64.roofline gives k_conv the same way (stored entries x 124 B).  assembly:
+the lag kernel k_pair_lags against the FP64 vector peak, ASM_FLOPS_PER_POINT flops per active
+(quadrature point, lag) evaluation plus ASM_FLOPS_PER_LEVEL per time level evaluated there.
+batched and serving: the continuous-batching engine over half-space scenarios and Poisson
+request streams at their stated sizes.  Printed as one JSON line; --gpus N > 1 exits (one GPU
+per box in this build).
 """
 import argparse
 import json
+import os
+import shutil
+import signal
+import subprocess
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -35,10 +50,16 @@ MI355X_HBM_GBS = 8000.0
 TP_SUBTILE_BYTES = 16 * 32 * 4           # FP32 16x32 sub-tile
 TP_ENTRY_BYTES = 32                      # merged-list step: 4 first sub-tiles, 4 masks, J/k, k-tile mask
 FP64_ENTRY_BYTES = 15 * 8 + 4            # 15 values + index per (i, j, k)
-# HBM bytes per launch (read + written) from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE)
-# on this workload, scaled by the counters of a 1 GiB k_stream_copy (factors 2.0 and 1.0);
-# profiles/k_conv_traffic_pmc_sphere1536.md holds the runs.
-MEASURED_TRAFFIC = {"k_conv_mixed": 6.928e9 + 99.3e6, "k_conv": 7.444e9 + 0.1e6}
+MI355X_FP64_TFLOPS = 78.6
+# FP64 flops of the lag kernel as written (integrate_points, eval_s, hat_integrals,
+# blocks_from_s; +, -, x, / and sqrt 1 each, FMA 2), 4-node elements: per active (point, lag)
+# evaluation the point's distance, reciprocals and direction (17), the 15 block values from the
+# six time integrals (150) and their accumulation into 4 nodes x 15 values (124); per time level
+# evaluated there the two waves' hat integrals (2 x 47) and their six-term accumulation (24).
+ASM_FLOPS_PER_POINT = 291
+ASM_FLOPS_PER_LEVEL = 118
+# rocprofv3 kernel names of the convolution kernels and the calibration copy
+PMC_KERNELS = {"k_conv_mixed": "k_conv_mixed<", "k_conv": "k_conv<1>", "copy": "k_stream_copy"}
 
 
 def parse():
@@ -61,6 +82,10 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (3 launches) to scale HBM counters by")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-pmc", action="store_true",
+                    help="skip the two rocprofv3 --pmc passes that measure roofline.traffic")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="(internal) the short run the --pmc passes profile")
     return ap.parse_args()
 
 
@@ -187,6 +212,74 @@ def serve_leg(api, configs, serve, nreq, rate, nrhs=32):
     return out
 
 
+def traffic_probe(nrhs):
+    """What the counter passes profile: the 1 GiB calibration copy, then one mixed solve at nrhs
+    right-hand sides and one plain FP64 solve of the bench workload, 80 steps (every launch past
+    step MaxStep streams the whole store, as at 600 steps)."""
+    from tdbem import api, configs
+    api.stream_copy(1 << 30, 2)
+    p = configs.sphere(n=16, numstep=80)
+    amp, rise = configs.rhs_set(nrhs, p.dt, seed=3)
+    with api.Context.from_problem(p) as ctx:
+        ctx.set_precision(api.PRECISION_MIXED)
+        timed_solves(ctx, p, 0, 1, amp, rise)
+    with api.Context.from_problem(p) as ctx:
+        timed_solves(ctx, p, 0, 1)
+
+
+def measure_traffic(nrhs):
+    """HBM bytes per launch of k_conv_mixed and k_conv: two rocprofv3 --pmc passes (FETCH_SIZE,
+    then WRITE_SIZE, KiB per dispatch) over traffic_probe, each counter scaled by its reading for
+    the 1 GiB k_stream_copy (1 GiB read, 1 GiB written).  The mean over the last 10 launches of
+    each kernel.  Returns ({kernel: bytes}, note)."""
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return {}, "rocprofv3 not found: traffic not measured"
+    kib = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            out_dir = os.path.join(d, counter)
+            cmd = [exe, "--pmc", counter, "--kernel-include-regex", "k_conv|k_stream_copy",
+                   "-f", "csv", "-d", out_dir, "-o", "pmc", "--", sys.executable, "-u",
+                   str(ROOT / "bench.py"), "--traffic-probe", "--nrhs", str(nrhs)]
+            env = dict(os.environ, TMPDIR="/tmp")
+            proc = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL,
+                                    stderr=subprocess.DEVNULL, start_new_session=True)
+            try:
+                rc = proc.wait(timeout=240)
+            except subprocess.TimeoutExpired:
+                os.killpg(proc.pid, signal.SIGKILL)
+                proc.wait()
+                return {}, f"rocprofv3 --pmc {counter} pass timed out"
+            csvs = list(Path(out_dir).rglob("*counter_collection.csv"))
+            if not csvs:
+                return {}, f"rocprofv3 --pmc {counter} pass wrote no counters (exit {rc})"
+            import csv
+            vals = {k: [] for k in PMC_KERNELS}
+            with open(csvs[0]) as f:
+                for row in csv.DictReader(f):
+                    if row["Counter_Name"] != counter:
+                        continue
+                    for k, pat in PMC_KERNELS.items():
+                        if pat in row["Kernel_Name"]:
+                            vals[k].append(float(row["Counter_Value"]))
+            if not vals["copy"]:
+                return {}, f"rocprofv3 --pmc {counter}: no calibration dispatch"
+            kib[counter] = {k: (sum(v[-10:]) / len(v[-10:]) if v else None)
+                            for k, v in vals.items()}
+    f_read = (1 << 30) / (kib["FETCH_SIZE"]["copy"] * 1024)
+    f_write = (1 << 30) / (kib["WRITE_SIZE"]["copy"] * 1024)
+    res = {}
+    for k in ("k_conv_mixed", "k_conv"):
+        rd, wr = kib["FETCH_SIZE"][k], kib["WRITE_SIZE"][k]
+        if rd is not None and wr is not None:
+            res[k] = {"bytes": rd * 1024 * f_read + wr * 1024 * f_write,
+                      "read": rd * 1024 * f_read, "written": wr * 1024 * f_write}
+    note = (f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes over --traffic-probe in this run; "
+            f"scale factors {f_read:.4f} / {f_write:.4f} from a 1 GiB k_stream_copy")
+    return res, note
+
+
 def roofline(kernel, avg_ms, nbytes, traffic=None):
     ach = nbytes / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": MI355X_HBM_GBS, "unit": "GB/s",
@@ -196,6 +289,9 @@ def roofline(kernel, avg_ms, nbytes, traffic=None):
 
 def main():
     args = parse()
+    if args.traffic_probe:
+        traffic_probe(args.nrhs)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
@@ -253,8 +349,19 @@ def main():
                       "solve_avg_ms": round(tm_all["solve_ms"] / max(tm_all["conv_launches"], 1), 4),
                       "lag_kernel_ms": round(tm_all["lag_kernel_ms"], 3)},
         })
+        # the lag kernel of the same store build (one solve, every kernel timed)
+        asm_flops = (tm_all["active_points"] * ASM_FLOPS_PER_POINT +
+                     tm_all["active_levels"] * ASM_FLOPS_PER_LEVEL)
+        asm_tf = asm_flops / (tm_all["lag_kernel_ms"] * 1e-3) / 1e12
+        out["assembly"] = {"bound": "fp64-valu", "kernel": "k_pair_lags (lag-store assembly)",
+                           "achieved": round(asm_tf, 2), "peak": MI355X_FP64_TFLOPS,
+                           "unit": "TFLOP/s", "frac": round(asm_tf / MI355X_FP64_TFLOPS, 4),
+                           "kernel_ms_per_store": round(tm_all["lag_kernel_ms"], 3),
+                           "active_points": int(tm_all["active_points"]),
+                           "active_levels": int(tm_all["active_levels"]),
+                           "flops_per_store": int(asm_flops)}
         out["roofline"] = roofline("k_conv_mixed (split-BF16 MFMA history convolution)", conv_ms,
-                                   inshell_bytes, MEASURED_TRAFFIC["k_conv_mixed"])
+                                   inshell_bytes)
         out["roofline"]["stored_bytes_per_launch"] = int(stored_bytes)
         out["roofline"]["frac_of_stored_bytes"] = round(
             stored_bytes / (conv_ms * 1e-3) / 1e9 / MI355X_HBM_GBS, 4)
@@ -312,8 +419,7 @@ def main():
             "active_points_per_solve": int(tm64_all["active_points"]),
             "store": {"entries": int(tm64["store_nonzero"]), "bytes": store_bytes,
                       "dense_bytes": 2 * (K + 1) * p.ndof ** 2 * 8},
-            "roofline": roofline("k_conv (plain FP64 history convolution)", conv64, bytes64,
-                                 MEASURED_TRAFFIC["k_conv"]),
+            "roofline": roofline("k_conv (plain FP64 history convolution)", conv64, bytes64),
             # Lame static radial displacement p a / (4 mu) = 0.625 at late time
             "u_r_mean_final": float(np.mean(np.einsum("ij,ij->i", U[-1].reshape(-1, 3),
                                                       p.coords))),
@@ -343,6 +449,19 @@ def main():
     if args.copy_calibration:
         log("copy calibration kernel")
         out["copy_calibration"] = copy_calibration(api)
+    if not args.no_pmc and ("roofline" in out):
+        log("HBM traffic: two rocprofv3 --pmc passes over a short run")
+        traffic, note = measure_traffic(R)
+        for key, k in (("roofline", "k_conv_mixed"), ("plain_fp64", "k_conv")):
+            rl = out[key] if key == "roofline" else out.get(key, {}).get("roofline")
+            if rl is None or k not in rl.get("kernel", ""):
+                continue
+            rl["traffic"] = round(traffic[k]["bytes"]) if k in traffic else None
+            rl["traffic_source"] = note
+            if k in traffic:
+                rl["traffic_read"] = round(traffic[k]["read"])
+                rl["traffic_written"] = round(traffic[k]["written"])
+                rl["traffic_over_algorithmic"] = round(traffic[k]["bytes"] / rl["bytes_per_launch"], 3)
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle sample)")
         out["cpu_baseline"] = cpu_baseline(p, args.cpu_threads)

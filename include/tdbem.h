@@ -316,9 +316,10 @@ int tdbem_element_synchronize(tdbem_element* el);
  * store_nonzero (stored (i, j, k) blocks before padding), store_padded, tile_subtiles (16 x 32
  * split-BF16 sub-tiles of the mixed-precision store, 0 until built), reduce_ms (the mixed path's
  * reduction of partial sums into the step solve's right-hand side, summed event times), tile_blocks (48 x 96 blocks with a stored
- * sub-tile), tile_entries (steps of the merged per-group lists).  conv_ms covers k_conv or
- * k_conv_mixed alone. */
-#define TDBEM_NUM_TIMINGS 16
+ * sub-tile), tile_entries (steps of the merged per-group lists), active_levels (time levels
+ * evaluated at those points, one to three each).  conv_ms covers k_conv or k_conv_mixed
+ * alone. */
+#define TDBEM_NUM_TIMINGS 17
 #define TDBEM_PROFILE_ALL 1
 #define TDBEM_PROFILE_CONV 2
 int tdbem_set_profiling(tdbem_context* ctx, int level);

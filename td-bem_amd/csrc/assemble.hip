@@ -117,7 +117,7 @@ __device__ __forceinline__ void for_each_point(const double* X, int sing, const 
 }
 
 // Time-integral combination S for one point: sum over three time levels of cI*I(t) + cJ*J(t).
-// Returns false when the combination is zero: every level with a nonzero coefficient is before
+// Returns the number of levels evaluated (and ir = 1/r), 0 when the combination is zero: every level with a nonzero coefficient is before
 // the P-wave arrival (t <= r/c_p), or (when the coefficients sum to zero, as for the Step
 // kernel) every such level is in the static regime t - r/c_s >= dt, where HL(t) no longer
 // depends on t (src/GreenFunctionT.cpp:251-314 with b = dt for both waves).
@@ -127,8 +127,8 @@ struct TimeCombo {
     bool static_ok;
 };
 
-__device__ __forceinline__ bool eval_s(const TimeCombo& tc, double r, double dt, double inv_dt,
-                                       const Material& m, double* S, double& ir)
+__device__ __forceinline__ int eval_s(const TimeCombo& tc, double r, double dt, double inv_dt,
+                                      const Material& m, double* S, double& ir)
 {
     const double inv_cp = m.inv_cp, inv_cs = m.inv_cs;
     const double ta = r * inv_cp;         // P-wave arrival
@@ -140,18 +140,20 @@ __device__ __forceinline__ bool eval_s(const TimeCombo& tc, double r, double dt,
         if (tc.t[l] > ta) any = true;
         if (tc.t[l] < ts) all_static = false;
     }
-    if (!any || (tc.static_ok && all_static)) return false;
+    if (!any || (tc.static_ok && all_static)) return 0;
     ir = 1.0 / r;
     const double inv_r2 = ir * ir;
 #pragma unroll
     for (int l = 0; l < 6; l++) S[l] = 0.0;
+    int levels = 0;
 #pragma unroll
     for (int l = 0; l < 3; l++) {
         if ((tc.cI[l] == 0.0 && tc.cJ[l] == 0.0) || !(tc.t[l] > ta)) continue;
         accumulate_hl<double>(tc.t[l], dt, inv_dt, r, inv_r2, inv_cp, inv_cs, m.inv_cp2,
                               m.inv_cs2, tc.cI[l], tc.cJ[l], S);
+        levels++;
     }
-    return true;
+    return levels;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -336,14 +338,15 @@ struct NoPointExtra {
 
 // FP64 integration of a pair over the points that points(f) visits (for_each_point: regular
 // rule for sing < 0, Duffy triangles / sub-elements for a singular node), accumulated into
-// out[ENND][15].  extra(x, n, wJ, acc) runs at every point after its blocks are added.
+// out[ENND][15].  extra(x, n, wJ, acc) runs at every point after its blocks are added.  Returns
+// (points with a nonzero time combination, time levels evaluated at them).
 template <int ENND, typename Points, typename Extra = NoPointExtra>
-__device__ __forceinline__ int integrate_points(Points&& points, const double* coll,
-                                                const TimeCombo& tc, double dt,
-                                                const Material& m, double* out, int64_t ostride,
-                                                Extra&& extra = Extra{})
+__device__ __forceinline__ int2 integrate_points(Points&& points, const double* coll,
+                                                 const TimeCombo& tc, double dt,
+                                                 const Material& m, double* out, int64_t ostride,
+                                                 Extra&& extra = Extra{})
 {
-    int active = 0;
+    int active = 0, levels = 0;
     const double inv_dt = 1.0 / dt;
     double acc[ENND][kVals];
 #pragma unroll
@@ -354,8 +357,10 @@ __device__ __forceinline__ int integrate_points(Points&& points, const double* c
         double d[3] = {x[0] - coll[0], x[1] - coll[1], x[2] - coll[2]};
         const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
         double S[6], ir;
-        if (eval_s(tc, r, dt, inv_dt, m, S, ir)) {
+        const int nl = eval_s(tc, r, dt, inv_dt, m, S, ir);
+        if (nl) {
             active++;
+            levels += nl;
             d[0] *= ir;
             d[1] *= ir;
             d[2] *= ir;
@@ -374,14 +379,14 @@ __device__ __forceinline__ int integrate_points(Points&& points, const double* c
     for (int a = 0; a < ENND; a++)
 #pragma unroll
         for (int v = 0; v < kVals; v++) out[(a * kVals + v) * ostride] = acc[a][v];
-    return active;
+    return make_int2(active, levels);
 }
 
 // Value v of node a goes to out[(a * 15 + v) * ostride].  REGULAR: the caller guarantees that
 // no node of e lies at the collocation point, so the singular rules are not compiled in (the
 // regular path then fits two waves per SIMD).
 template <int ENND, bool REGULAR = false>
-__device__ __forceinline__ int integrate_pair(const double* coords, const int* ien,
+__device__ __forceinline__ int2 integrate_pair(const double* coords, const int* ien,
                                               const double* geo, const double* gauss, int nq,
                                               int i, int e, const TimeCombo& tc, double dt,
                                               const Material& m, double* out, int64_t ostride = 1)
@@ -559,9 +564,12 @@ __global__ void __launch_bounds__(kBlock, (ENND == 4 && !SINGULAR) ? 2 : 1)
     tc.cI[2] = 0.0;
     tc.cJ[2] = hasJ;
     tc.static_ok = true;
-    const int active = integrate_pair<ENND, !SINGULAR>(coords, ien, geo, gauss, nq, i, e, tc, dt,
-                                                       m, scratch + s, nslots);
-    if (counter) atomicAdd(counter, (unsigned long long)active);
+    const int2 n = integrate_pair<ENND, !SINGULAR>(coords, ien, geo, gauss, nq, i, e, tc, dt, m,
+                                                   scratch + s, nslots);
+    if (counter) {
+        atomicAdd(counter, (unsigned long long)n.x);
+        atomicAdd(counter + 1, (unsigned long long)n.y);
+    }
 }
 
 template <int ENND>
@@ -1136,8 +1144,8 @@ void slot_pass(Context& c, const std::vector<int>& order, int align, const int2*
     const bool prof = c.profile == 1;
     KernelTimer lag_timer(c.stream, prof), gather_timer(c.stream, prof);
     if (prof && !c.counters.p) {
-        c.counters.alloc(1);
-        TD_HIP(hipMemsetAsync(c.counters.p, 0, sizeof(unsigned long long), c.stream));
+        c.counters.alloc(2);
+        TD_HIP(hipMemsetAsync(c.counters.p, 0, 2 * sizeof(unsigned long long), c.stream));
     }
     // slots (lags of in-shell element pairs) per position, then batches of whole `align` groups
     // of positions whose slots fit the scratch budget
@@ -1508,7 +1516,7 @@ void build_lag_store(Context& c)
     c.tp.ready = false;
     c.store_precision = c.precision;
     if (c.profile == 1 && c.counters.p)
-        TD_HIP(hipMemsetAsync(c.counters.p, 0, sizeof(unsigned long long), c.stream));
+        TD_HIP(hipMemsetAsync(c.counters.p, 0, 2 * sizeof(unsigned long long), c.stream));
     compute_cterm(c);
     compute_pair_ranges(c);
     if (c.precision == 1) {
@@ -1534,9 +1542,10 @@ void build_lag_store(Context& c)
     }
     TD_HIP(hipStreamSynchronize(c.stream));
     if (c.profile == 1 && c.counters.p) {
-        unsigned long long n = 0;
-        TD_HIP(hipMemcpy(&n, c.counters.p, sizeof(n), hipMemcpyDeviceToHost));
-        c.tm.active_points += (int64_t)n;
+        unsigned long long n[2] = {0, 0};
+        TD_HIP(hipMemcpy(n, c.counters.p, sizeof(n), hipMemcpyDeviceToHost));
+        c.tm.active_points += (int64_t)n[0];
+        c.tm.active_levels += (int64_t)n[1];
     }
     // the pinned planning copies stay for the next build unless they are large
     if ((int64_t)c.nnd * c.nnd * 24 > kPinnedKeepBytes) {

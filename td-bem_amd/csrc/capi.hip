@@ -761,7 +761,7 @@ int tdbem_get_timings(tdbem_context* ctx, double* out)
             t.store_ms, t.init_ms, t.march_ms, t.lag_kernel_ms, t.gather_ms, t.conv_ms,
             t.solve_ms, (double)t.lag_launches, (double)t.conv_launches, (double)t.active_points,
             (double)c.pair_lags, (double)c.store.n_entries, (double)c.tp.nsub,
-            t.reduce_ms, (double)c.tp.nblocks, (double)c.tp.nentries};
+            t.reduce_ms, (double)c.tp.nblocks, (double)c.tp.nentries, (double)t.active_levels};
         std::memcpy(out, v, sizeof(v));
     });
 }

@@ -304,6 +304,7 @@ struct Context {
         double lag_kernel_ms = 0, gather_ms = 0, conv_ms = 0, solve_ms = 0, reduce_ms = 0;
         int64_t lag_launches = 0, conv_launches = 0;
         int64_t active_points = 0;  // (quadrature point, lag) evaluations of the lag kernel
+        int64_t active_levels = 0;  // time levels evaluated at them (1 to 3 each)
     } tm;
     DBuf<unsigned long long> counters;  // device counters of the lag kernel
 
