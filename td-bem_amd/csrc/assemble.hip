@@ -293,11 +293,13 @@ __global__ void k_node_hull(const int2* __restrict__ pair_k, const int* __restri
     hull[tid] = hi < lo ? make_int2(1, 0) : make_int2(lo, hi);
 }
 
+// Entry index (j << kKBits | k) of every in-shell (row, j, k) from the pair hulls and first
+// entries of nrows rows ([nrows][nnd] each).
 __global__ void k_fill_idx(const int2* __restrict__ hull, const int64_t* __restrict__ off,
-                           int nnd, uint32_t* __restrict__ idx)
+                           int nnd, int64_t nrows, uint32_t* __restrict__ idx)
 {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= (int64_t)nnd * nnd) return;
+    if (tid >= nrows * nnd) return;
     const int j = (int)(tid % nnd);
     const int2 h = hull[tid];
     const int64_t o = off[tid];
@@ -801,24 +803,28 @@ __global__ void __launch_bounds__(kChunk)
                    const int2* __restrict__ pair_k, const int* __restrict__ adj_off,
                    const int* __restrict__ adj, const double* __restrict__ cterm,
                    const int64_t* __restrict__ row_chunk, const int64_t* __restrict__ row_valid,
-                   const uint32_t* __restrict__ idx, int nnd, int nel, int i0, double ext,
-                   int64_t chunk0, int64_t nslots, double* __restrict__ vals)
+                   const uint32_t* __restrict__ idx, int nrows, int nel, int i0, double ext,
+                   int64_t chunk0, int64_t nslots, double* __restrict__ vals,
+                   const int* __restrict__ rows)
 {
+    // row r of the store layout: node r (slots from node i0 on), or with `rows` node rows[r]
+    // (a batch of the lag kernel's slot pass in its own row order, slots from position 0)
     const int64_t c = chunk0 + blockIdx.x;
-    const int i = row_of_chunk(row_chunk, nnd, c);
+    const int r = row_of_chunk(row_chunk, nrows, c);
+    const int i = rows ? rows[r] : r;
     const int lane = threadIdx.x;
-    const int64_t in_row = (c - row_chunk[i]) * kChunk + lane;
+    const int64_t in_row = (c - row_chunk[r]) * kChunk + lane;
     double v[kVals];
 #pragma unroll
     for (int t = 0; t < kVals; t++) v[t] = 0.0;
-    if (in_row < row_valid[i]) {
+    if (in_row < row_valid[r]) {
         const uint32_t id = idx[c * kChunk + lane];
         const int j = (int)(id >> kKBits), k = (int)(id & kMaxLag);
         for (int a = adj_off[j]; a < adj_off[j + 1]; a++) {
             const int e = adj[a] >> 3, loc = adj[a] & 7;
             const int2 pk = pair_k[(int64_t)i * nel + e];
             if (k < pk.x || k > pk.y) continue;
-            const int64_t p = (int64_t)(i - i0) * nel + e;
+            const int64_t p = (int64_t)(rows ? r : i - i0) * nel + e;
             const double* s = scratch + (pair_slot[p] + (k - pk.x)) + loc * kVals * nslots;
 #pragma unroll
             for (int t = 0; t < kVals; t++) v[t] += s[t * nslots];
@@ -1297,7 +1303,7 @@ void fill_store(Context& c, const ShellPlan& P, const int2* hpk, const int2* dpk
                                            dim3(kChunk), 0, c.stream, scratch, pair_slot, dpk,
                                            c.adj_off.p, c.adj.p, c.cterm.p, S.row_chunk.p,
                                            S.row_valid.p, S.idx.p, c.nnd, c.nel, i0, ext, ch0,
-                                           nslots, S.vals.p);
+                                           nslots, S.vals.p, (const int*)nullptr);
                         TD_HIP(hipGetLastError());
                     });
 }
@@ -1324,7 +1330,7 @@ void alloc_store(Context& c, const ShellPlan& P)
         d_off.upload(P.off, nn, c.stream);
         d_hull.upload(P.hull, nn, c.stream);
         hipLaunchKernelGGL(k_fill_idx, dim3(grid_for(nn)), dim3(kBlock), 0, c.stream, d_hull.p,
-                           d_off.p, c.nnd, S.idx.p);
+                           d_off.p, c.nnd, (int64_t)c.nnd, S.idx.p);
         TD_HIP(hipGetLastError());
     }
 }
@@ -1351,18 +1357,111 @@ const int2* host_pair_ranges(Context& c)
 // Mixed-precision store: the tile store of every lag, filled straight from the lag kernel's
 // slots in row tiles of the bisection order (no FP64 copy of the whole store is ever held),
 // and an FP64 store of lags 0 and 1 only, from which march_init forms the step-0 matrices.
+// Rows of one batch of the tile pass in the FP64 store's layout: the entries of row li (node
+// rows[li], -1 = none) from chunk rc[li] on, rv[li] of them, each pair's lag range and first
+// entry in hull / off [li][nnd].  Two host sets, reused after the event of their upload, as the
+// slot pass's lists.
+struct SegmentRows {
+    std::vector<int64_t> rc, rv, off;
+    std::vector<int2> hull;
+    std::vector<int> rows;
+};
+
 template <int ENND>
 void build_mixed(Context& c)
 {
     const int64_t np = (int64_t)c.nnd * c.nel;
-    if (!c.tp.planned) {
-        ShellPlan Pf;
-        plan_shells(c, Pf, c.pair_k.p, false);
-        c.pair_lags = Pf.nonzero;
-        tp_plan_structure(c);
-    }
+    const int nnd = c.nnd;
+    // node-pair hulls of every lag (host: the plan and each batch's segment layout)
+    ShellPlan Pf;
+    plan_shells(c, Pf, c.pair_k.p, false);
+    c.pair_lags = Pf.nonzero;
+    if (!c.tp.planned) tp_plan_structure(c);
     const int2* hpk = host_pair_ranges(c);
-    // lags 0 and 1 in FP64
+    // every lag into the tile store, row tiles in bisection order: per batch of the lag kernel,
+    // its rows' FP64 store entries (k_store_gather into a segment), then the sub-tiles of its row
+    // tiles from the segment
+    tp_alloc(c);
+    const double ext = c.if_ext ? 0.5 * c.dt : 0.0;
+    {
+        DBuf<double> seg(c.stream);
+        DBuf<uint32_t> seg_idx(c.stream);
+        DBuf<int2> seg_hull(c.stream);
+        DBuf<int64_t> seg_off(c.stream), seg_rc(c.stream), seg_rv(c.stream);
+        DBuf<int> seg_rows(c.stream);
+        SegmentRows h[2];
+        hipEvent_t uploaded[2];
+        TD_HIP(hipEventCreateWithFlags(&uploaded[0], hipEventDisableTiming));
+        TD_HIP(hipEventCreateWithFlags(&uploaded[1], hipEventDisableTiming));
+        int nbatch = 0;
+        try {
+            slot_pass<ENND>(
+                c, c.tp.perm, kTile, hpk, c.pair_k.p,
+                [&](int b0, int b1, const double* scratch, int64_t nslots,
+                    const int64_t* pair_slot) {
+                    const int set = nbatch & 1;
+                    if (nbatch >= 2) TD_HIP(hipEventSynchronize(uploaded[set]));
+                    nbatch++;
+                    SegmentRows& H = h[set];
+                    const int nb = b1 - b0;
+                    H.rc.assign(nb + 1, 0);
+                    H.rv.assign(nb, 0);
+                    H.rows.resize(nb);
+                    for (int li = 0; li < nb; li++) {
+                        const int i = c.tp.perm[b0 + li];
+                        H.rows[li] = i;
+                        H.rv[li] = i >= 0 ? Pf.row_valid[i] : 0;
+                        H.rc[li + 1] = H.rc[li] + (H.rv[li] + kChunk - 1) / kChunk;
+                    }
+                    H.off.resize((size_t)nb * nnd);
+                    H.hull.resize((size_t)nb * nnd);
+                    parallel_for(nb, [&](int64_t lo, int64_t hi) {
+                        for (int64_t li = lo; li < hi; li++) {
+                            const int i = H.rows[li];
+                            int64_t o = H.rc[li] * kChunk;
+                            for (int j = 0; j < nnd; j++) {
+                                const int2 hj = i >= 0 ? Pf.hull[(int64_t)i * nnd + j]
+                                                       : make_int2(1, 0);
+                                H.hull[li * nnd + j] = hj;
+                                H.off[li * nnd + j] = o;
+                                if (hj.y >= hj.x) o += hj.y - hj.x + 1;
+                            }
+                        }
+                    });
+                    const int64_t chunks = H.rc[nb];
+                    seg_rc.upload(H.rc.data(), H.rc.size(), c.stream);
+                    seg_rv.upload(H.rv.data(), H.rv.size(), c.stream);
+                    seg_rows.upload(H.rows.data(), H.rows.size(), c.stream);
+                    seg_hull.upload(H.hull.data(), H.hull.size(), c.stream);
+                    seg_off.upload(H.off.data(), H.off.size(), c.stream);
+                    TD_HIP(hipEventRecord(uploaded[set], c.stream));
+                    if (chunks == 0) return;
+                    seg.reserve((size_t)chunks * kChunk * kVals);
+                    seg_idx.reserve((size_t)chunks * kChunk);
+                    const int64_t ncell = (int64_t)nb * nnd;
+                    hipLaunchKernelGGL(k_fill_idx, dim3(grid_for(ncell)), dim3(kBlock), 0,
+                                       c.stream, seg_hull.p, seg_off.p, nnd, (int64_t)nb,
+                                       seg_idx.p);
+                    hipLaunchKernelGGL(k_store_gather<ENND>, dim3((unsigned)chunks),
+                                       dim3(kChunk), 0, c.stream, scratch, pair_slot,
+                                       c.pair_k.p, c.adj_off.p, c.adj.p, c.cterm.p, seg_rc.p,
+                                       seg_rv.p, seg_idx.p, nb, c.nel, 0, ext, (int64_t)0,
+                                       nslots, seg.p, (const int*)seg_rows.p);
+                    TD_HIP(hipGetLastError());
+                    tp_gather_segment(c, b0 / kTile, b1 / kTile, seg.p, seg_hull.p, seg_off.p);
+                });
+        } catch (...) {
+            (void)hipStreamSynchronize(c.stream);
+            (void)hipEventDestroy(uploaded[0]);
+            (void)hipEventDestroy(uploaded[1]);
+            throw;
+        }
+        TD_HIP(hipStreamSynchronize(c.stream));
+        TD_HIP(hipEventDestroy(uploaded[0]));
+        TD_HIP(hipEventDestroy(uploaded[1]));
+    }
+    tp_finish(c);
+    // lags 0 and 1 in FP64 (their planning replaces the pinned node-pair hulls)
     {
         DBuf<int2> pk01(c.stream);
         pk01.alloc(np);
@@ -1378,16 +1477,6 @@ void build_mixed(Context& c)
         plan_shells(c, P01, pk01.p);
         build_fp64_store<ENND>(c, P01, hpk01.data(), pk01.p);
     }
-    // every lag into the tile store, row tiles in bisection order
-    tp_alloc(c);
-    const double ext = c.if_ext ? 0.5 * c.dt : 0.0;
-    slot_pass<ENND>(c, c.tp.perm, kTile, hpk, c.pair_k.p,
-                    [&](int b0, int b1, const double* scratch, int64_t nslots,
-                        const int64_t* pair_slot) {
-                        tp_gather_slots(c, b0 / kTile, b1 / kTile, scratch, nslots, pair_slot,
-                                        ext);
-                    });
-    tp_finish(c);
 }
 
 }  // namespace
@@ -1410,7 +1499,8 @@ namespace {
 // Largest slot count and row count of the batches slot_pass forms over `order` (see there).
 template <int ENND>
 void slot_batches(Context& c, const std::vector<int>& order, int align, const int2* hpk,
-                  int64_t& max_slots, int& max_rows)
+                  int64_t& max_slots, int& max_rows, const std::vector<int64_t>* row_entries = nullptr,
+                  int64_t* max_seg_chunks = nullptr)
 {
     const bool ident = order.empty();
     const int npos = ident ? c.nnd : (int)order.size();
@@ -1443,6 +1533,15 @@ void slot_batches(Context& c, const std::vector<int>& order, int align, const in
         }
         max_slots = std::max(max_slots, slots);
         max_rows = std::max(max_rows, b1 - b0);
+        if (row_entries && max_seg_chunks) {
+            // the batch's rows in the FP64 store's layout (the tile pass's segment)
+            int64_t ch = 0;
+            for (int q = b0; q < b1; q++) {
+                const int i = ident ? q : order[q];
+                if (i >= 0) ch += ((*row_entries)[i] + kChunk - 1) / kChunk;
+            }
+            *max_seg_chunks = std::max(*max_seg_chunks, ch);
+        }
         b0 = b1;
     }
 }
@@ -1465,12 +1564,10 @@ void predict_store(Context& c, int64_t& store, int64_t& transient)
     int64_t max_slots = 0;
     int max_rows = 0;
     if (c.precision == 1) {
-        if (!c.tp.planned) {
-            ShellPlan Pf;
-            plan_shells(c, Pf, c.pair_k.p, false);
-            c.pair_lags = Pf.nonzero;
-            tp_plan_structure(c);
-        }
+        ShellPlan Pf;
+        plan_shells(c, Pf, c.pair_k.p, false);
+        c.pair_lags = Pf.nonzero;
+        if (!c.tp.planned) tp_plan_structure(c);
         const TPStore& T = c.tp;
         // the FP64 store of lags 0 and 1
         DBuf<int2> pk01(c.stream);
@@ -1483,12 +1580,17 @@ void predict_store(Context& c, int64_t& store, int64_t& transient)
         const int64_t mini = P01.row_chunk[c.nnd] * entry_bytes + rows_bytes;
         store = T.nsub * (int64_t)kSubFloats * 4 + T.nentries * (int64_t)sizeof(TPEntry) +
                 (int64_t)T.nT * kTile * 4 + (int64_t)c.nnd * 4 + (int64_t)(T.nG + 1) * 8 + mini;
+        int64_t seg_chunks = 0;
         if (c.ennd == 4)
-            slot_batches<4>(c, T.perm, kTile, hpk, max_slots, max_rows);
+            slot_batches<4>(c, T.perm, kTile, hpk, max_slots, max_rows, &Pf.row_valid, &seg_chunks);
         else
-            slot_batches<8>(c, T.perm, kTile, hpk, max_slots, max_rows);
-        // the tile pass (sub-tile list and slot buffers) outweighs the lags-0..1 pass
-        transient = std::max(nn * 16 + np * 8, T.nsub * 8 + slot_pass_bytes(c, max_slots, max_rows));
+            slot_batches<8>(c, T.perm, kTile, hpk, max_slots, max_rows, &Pf.row_valid, &seg_chunks);
+        // the tile pass (sub-tile list, slot buffers and one batch's FP64 segment: values, entry
+        // index, row tables and the pairs' hulls and first entries) or the lags-0..1 pass
+        const int64_t segment = seg_chunks * kChunk * (kVals * 8 + 4) +
+                                (int64_t)max_rows * c.nnd * 16 + (int64_t)max_rows * 28 + 8;
+        transient = std::max(nn * 16 + np * 8,
+                             T.nsub * 8 + slot_pass_bytes(c, max_slots, max_rows) + segment);
     } else {
         ShellPlan P;
         plan_shells(c, P, c.pair_k.p, false);

@@ -47,25 +47,25 @@ __device__ __forceinline__ int row_of_chunk(const int64_t* row_chunk, int nnd, i
     return lo;
 }
 
-// Tile store from the lag kernel's slots: one wavefront per stored sub-tile (row tile I, column
-// tile J, lag k, sub-tile u = 3 m + kt) of the row tiles whose rows the slots hold (positions
-// P0.. of the bisection order).  Lane L holds block row 16 m + L % 16 and the 8 block columns
-// 32 kt + 8 (L / 16) .. + 7 (16x16x32 A-operand order) and writes columns 0..3 of its eight to the
-// sub-tile's first 1 KiB line and 4..7 to the second.  Block row 3 ii + p, column 6 jj + 3 part + q
-// holds G[p][q] (part 0) or -H[p][q] (part 1) of node pair (i, j) = (perm[16 I + ii],
-// perm[16 J + jj]) at lag k: the sum over the elements around j of their slots (i, e, k) in the
-// node's adjacency order, plus the C-term and the exterior-problem term on the diagonal block at
-// k <= 1 -- the same FP64 sums as the FP64 store's gather (k_store_gather), rounded once to FP32.
+// Tile store from a segment of the FP64 node-pair store: the rows of the positions P0 .. of the
+// bisection order (one batch of the lag kernel's slot pass), in the store's layout ([chunk][15]
+// [64] entries, each row's (j, k) entries in order), with each pair's lag range hull[li][j] and
+// first entry off[li][j] (li = position - P0).  One wavefront per stored sub-tile (row tile I,
+// column tile J, lag k, sub-tile u = 3 m + kt) of the row tiles the segment holds.  Lane L holds
+// block row 16 m + L % 16 and the 8 block columns 32 kt + 8 (L / 16) .. + 7 (16x16x32 A-operand
+// order); it looks each of its (at most two) node pairs up once, reads the values -- the
+// segment's entries of consecutive lags are consecutive, so the sub-tiles of one tile pair's
+// lags, which neighbouring workgroups on one XCD take, share their lines -- and writes columns
+// 0..3 of its eight to the sub-tile's first 1 KiB line and 4..7 to the second.  Block row
+// 3 ii + p, column 6 jj + 3 part + q holds G[p][q] (part 0) or -H[p][q] (part 1) of node pair
+// (perm[16 I + ii], perm[16 J + jj]): the FP64 store's value rounded once to FP32.
 __global__ void __launch_bounds__(256)
-    k_tp_from_slots(const double* __restrict__ scratch, int64_t nslots,
-                    const int64_t* __restrict__ pair_slot, const int2* __restrict__ pair_k,
-                    const int* __restrict__ adj_off, const int* __restrict__ adj,
-                    const double* __restrict__ cterm, double ext, const int* __restrict__ perm,
-                    int P0, int nel, const uint2* __restrict__ sub_id, int64_t s0, int64_t s1,
-                    f32x4* __restrict__ tp)
+    k_tp_from_segment(const double* __restrict__ seg, const int2* __restrict__ hull,
+                      const int64_t* __restrict__ off, const int* __restrict__ perm, int P0,
+                      int nnd, const uint2* __restrict__ sub_id, int64_t s0, int64_t s1,
+                      f32x4* __restrict__ tp)
 {
-    // consecutive sub-tiles (the lags of one tile pair) on one XCD, whose L2 then serves the
-    // neighbouring slots of the next lags
+    // consecutive sub-tiles (the lags of one tile pair) on one XCD
     const int64_t sidx = s0 + (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (sidx >= s1) return;
     const int lane = threadIdx.x & 63;
@@ -79,23 +79,26 @@ __global__ void __launch_bounds__(256)
     const int64_t li = pos - P0;
     const int sym[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
     f32x4 out[2];
+    int last = -1;
+    int64_t en = -1;   // entry of (i, j, k), -1 when the pair has no lag k
 #pragma unroll
     for (int e = 0; e < 8; e++) {
         const int c = 32 * kt + 8 * (lane >> 4) + e;
         const int jj = c / 6;
         const int part = (c % 6) / 3, q = c % 3;
-        const int j = perm[J * kTile + jj];
-        double v = 0.0;
-        if (i >= 0 && j >= 0) {
-            const int comp = part == 0 ? sym[p][q] : 6 + 3 * p + q;
-            for (int a = adj_off[j]; a < adj_off[j + 1]; a++) {
-                const int el = adj[a] >> 3, loc = adj[a] & 7;
-                const int2 pk = pair_k[(int64_t)i * nel + el];
-                if (k < pk.x || k > pk.y) continue;
-                const int64_t sl = pair_slot[li * nel + el] + (k - pk.x);
-                v += scratch[(int64_t)(loc * kVals + comp) * nslots + sl];
+        if (jj != last) {
+            last = jj;
+            en = -1;
+            const int j = perm[J * kTile + jj];
+            if (i >= 0 && j >= 0) {
+                const int2 h = hull[li * nnd + j];
+                if (h.x <= k && k <= h.y) en = off[li * nnd + j] + (k - h.x);
             }
-            if (part == 1 && j == i && k <= 1) v += cterm[9 * i + 3 * p + q] + (p == q ? ext : 0.0);
+        }
+        double v = 0.0;
+        if (en >= 0) {
+            const int comp = part == 0 ? sym[p][q] : 6 + 3 * p + q;
+            v = seg[((en / kChunk) * kVals + comp) * kChunk + en % kChunk];
             if (part) v = -v;
         }
         out[e >> 2][e & 3] = (float)v;
@@ -449,8 +452,8 @@ void tp_alloc(Context& c)
     if (T.nsub * (int64_t)sizeof(uint2) > kPinnedKeepBytes) std::vector<uint2>().swap(T.h_sub_id);
 }
 
-void tp_gather_slots(Context& c, int I0, int I1, const double* scratch, int64_t nslots,
-                     const int64_t* pair_slot, double ext)
+void tp_gather_segment(Context& c, int I0, int I1, const double* seg, const int2* hull,
+                       const int64_t* off)
 {
     TPStore& T = c.tp;
     if (!T.planned || !T.vals.p) throw std::logic_error("tile store not planned");
@@ -461,10 +464,9 @@ void tp_gather_slots(Context& c, int I0, int I1, const double* scratch, int64_t 
     if (I0 < 0 || I1 > T.nT || I0 > I1) throw std::logic_error("row tiles out of range");
     const int64_t s0 = T.h_tile_sub[I0], s1 = T.h_tile_sub[I1];
     if (s1 <= s0) return;
-    hipLaunchKernelGGL(k_tp_from_slots, dim3((unsigned)((s1 - s0 + 3) / 4)), dim3(256), 0,
-                       c.stream, scratch, nslots, pair_slot, c.pair_k.p, c.adj_off.p, c.adj.p,
-                       c.cterm.p, ext, T.d_perm.p, I0 * kTile, c.nel, T.d_sub_id.p, s0, s1,
-                       reinterpret_cast<f32x4*>(T.vals.p));
+    hipLaunchKernelGGL(k_tp_from_segment, dim3((unsigned)((s1 - s0 + 3) / 4)), dim3(256), 0,
+                       c.stream, seg, hull, off, T.d_perm.p, I0 * kTile, c.nnd, T.d_sub_id.p, s0,
+                       s1, reinterpret_cast<f32x4*>(T.vals.p));
     TD_HIP(hipGetLastError());
 }
 

@@ -454,13 +454,14 @@ int batch_run(Batch& b, int steps_per_graph, int max_replays);
 // mixed.hip
 // Tile store of the mixed-precision march.  tp_plan_structure plans it from the node-pair hulls
 // (c.pin_hull) on the host: bisection order, stored sub-tiles, merged lists and their work;
-// tp_alloc allocates the sub-tiles; tp_gather_slots fills those of row tiles [I0, I1) from the
-// lag kernel's slots of their rows (batch rows = bisection positions 16 I0 ..); tp_finish marks
-// it complete.  tp_build checks that a built mixed-precision store has its tiles.
+// tp_alloc allocates the sub-tiles; tp_gather_segment fills those of row tiles [I0, I1) from a
+// segment of the FP64 store holding their rows (bisection positions 16 I0 ..: values seg, pair
+// hulls and first entries [row][nnd]); tp_finish marks it complete.  tp_build checks that a
+// built mixed-precision store has its tiles.
 void tp_plan_structure(Context& c);
 void tp_alloc(Context& c);
-void tp_gather_slots(Context& c, int I0, int I1, const double* scratch, int64_t nslots,
-                     const int64_t* pair_slot, double ext);
+void tp_gather_segment(Context& c, int I0, int I1, const double* seg, const int2* hull,
+                       const int64_t* off);
 void tp_finish(Context& c);
 void tp_build(Context& c);
 // rows lower..upper of lag k from the tile store (FP32 values), dense [3 (upper-lower+1)][N]
