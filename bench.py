@@ -192,7 +192,7 @@ def batch_leg(api, configs, serve, count, nrhs=16):
     scen = configs.halfspace_scenarios(count=count, seed=4)
     reqs = serve.poisson_requests(scen, rate=None, nrhs=nrhs, seed=40)
     res, summary = serve.Engine(nrhs=nrhs, setup_workers=4, steps_per_graph=8,
-                                tune=True).run(reqs)
+                                tune=True, progress=log).run(reqs)
     out = _engine_summary(res, summary, scen)
     out.update({"scenarios": count, "nrhs": nrhs})
     return out
@@ -206,7 +206,7 @@ def serve_leg(api, configs, serve, nreq, rate, nrhs=32):
     probs = configs.serving_mix(nreq, seed=5)
     reqs = serve.poisson_requests(probs, rate=rate, nrhs=nrhs, seed=5)
     res, summary = serve.Engine(nrhs=nrhs, setup_workers=4, steps_per_graph=8,
-                                tune=True).run(reqs)
+                                tune=True, progress=log).run(reqs)
     out = _engine_summary(res, summary, probs)
     out.update({"arrival_rate_per_s": rate, "nrhs": nrhs})
     return out

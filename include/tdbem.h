@@ -147,12 +147,12 @@ int tdbem_get_state(tdbem_context* ctx, int rhs, int s, double* u, double* t);
 /* Arithmetic of the lag store and the history convolution for the next tdbem_build_lag_store /
  * tdbem_march_init (no counterpart in the reference, which is FP64 throughout):
  *   TDBEM_PRECISION_FP64 (default): the plain path, nrhs in {1, 2, 4, 8};
- *   TDBEM_PRECISION_MIXED_BF16X3: regular element pairs farther than two element diameters are
- *   integrated with FP32 time integrals and kernels (FP64 distances, FP64 sums); the lags are
- *   stored as FP32 16 x 32 sub-tiles (only those a node pair reaches), each value split exactly
- *   into three BF16 terms in registers before its MFMA products, which are accumulated in FP32
- *   per sub-tile and in FP64 across sub-tiles and lags, nrhs in {16, 32, 48, 64}.  Singular and
- *   near-singular pairs, the C-term, the step-0 LU/inverse and the step solve stay FP64.
+ *   TDBEM_PRECISION_MIXED_BF16X3: the lags are assembled in FP64 as on the plain path and
+ *   rounded once into FP32 16 x 32 sub-tiles (only those a node pair reaches), each value split
+ *   exactly into three BF16 terms in registers before its MFMA products, which are accumulated
+ *   in FP32 over one (column tile, lag) step and in FP64 across steps, nrhs in {16, 32, 48, 64}.
+ *   The FP64 store of the lags is not kept (only lags 0 and 1, for the step-0 matrices); the
+ *   step-0 LU/inverse and the step solve stay FP64.
  * Changing the precision discards a lag store built with the other one.  tdbem_assemble_step
  * is always FP64. */
 enum { TDBEM_PRECISION_FP64 = 0, TDBEM_PRECISION_MIXED_BF16X3 = 1 };

@@ -91,10 +91,11 @@ class Engine:
     setup_workers: set-ups in flight at once; steps_per_graph: steps of all members per graph
     replay (the granularity at which members join and leave); tune: time the batch's tunables
     once, when it first holds two members (else the defaults, which the single-context path
-    uses too); keep_histories: read every member's u and t histories when it finishes."""
+    uses too); keep_histories: read every member's u and t histories when it finishes;
+    progress: called with a line of text as each request finishes or fails."""
 
     def __init__(self, nrhs=32, hbm_budget=None, setup_workers=4, steps_per_graph=8,
-                 tune=False, keep_histories=False, poll=0.002):
+                 tune=False, keep_histories=False, poll=0.002, progress=None):
         self.nrhs = int(nrhs)
         self.hbm_budget = hbm_budget
         self.setup_workers = max(1, int(setup_workers))
@@ -102,6 +103,7 @@ class Engine:
         self.tune = tune
         self.keep = keep_histories
         self.poll = poll
+        self.progress = progress
 
     def run(self, requests):
         """Serves the requests against their arrival times; returns the results in request
@@ -130,10 +132,19 @@ class Engine:
         t0 = time.perf_counter()
         clock = lambda: time.perf_counter() - t0  # noqa: E731
 
+        done = [0]
+
+        def report(req, what):
+            done[0] += 1
+            if self.progress:
+                self.progress(f"request {req.rid} {what} ({done[0]}/{len(requests)} done, "
+                              f"{len(members)} in the batch)")
+
         def fail(req, ctx, msg):
             res = results[req.rid]
             res.error = msg
             res.finished = clock()
+            report(req, f"failed: {msg}")
             if ctx is not None:
                 try:
                     ctx.close()
@@ -203,6 +214,8 @@ class Engine:
                             res.histories = [ctx.history(r) for r in range(self.nrhs)]
                         ctx.close()
                         used -= peak
+                        report(req, f"finished, {req.problem.nel} elements x "
+                                    f"{req.problem.numstep} steps")
                 elif setups:
                     wait(list(setups), timeout=0.05, return_when=FIRST_COMPLETED)
                 elif waiting:
