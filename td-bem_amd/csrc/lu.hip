@@ -12,10 +12,16 @@
 // restores the matrix and runs blocked right-looking LU with partial pivoting: a recursive panel
 // factorisation with one-workgroup register leaves, row interchanges applied to the other
 // columns, a triangular solve for the U12 block row and the same GEMM.  Solves use the same
-// pieces column-blocked over the right-hand sides.
+// pieces column-blocked over the right-hand sides.  The large trailing-matrix and block-solve
+// updates are plain GEMMs and go to rocBLAS (dgemm, atomics not allowed, so results are the same
+// run to run); the small ones inside the panels stay on the LDS-tiled kernel below.
 #include <algorithm>
+#include <map>
 #include <stdexcept>
+#include <string>
 #include <vector>
+
+#include <rocblas/rocblas.h>
 
 #include "step_device.h"
 
@@ -386,10 +392,44 @@ __global__ void __launch_bounds__(256)
         }
 }
 
+// One rocBLAS handle per host thread and device, created on first use (rocBLAS handles are
+// not shared between threads) and kept until the process ends; each call sets the stream.
+rocblas_handle blas_handle(hipStream_t s)
+{
+    thread_local std::map<int, rocblas_handle> handles;
+    int dev = 0;
+    TD_HIP(hipGetDevice(&dev));
+    rocblas_handle& h = handles[dev];
+    if (!h) {
+        if (rocblas_create_handle(&h) != rocblas_status_success) {
+            h = nullptr;
+            throw std::runtime_error("HIP error: rocblas_create_handle failed");
+        }
+        rocblas_set_atomics_mode(h, rocblas_atomics_not_allowed);
+    }
+    if (rocblas_set_stream(h, s) != rocblas_status_success)
+        throw std::runtime_error("HIP error: rocblas_set_stream failed");
+    return h;
+}
+
+// Updates at least this large (m n k) go to rocBLAS.
+constexpr double kBlasMinWork = 64.0 * 1024 * 1024;
+
 void gemm_sub(int m, int n, int k, const double* A, int lda, const double* B, int ldb, double* C,
               int ldc, hipStream_t s)
 {
     if (m <= 0 || n <= 0 || k <= 0) return;
+    if ((double)m * n * k >= kBlasMinWork) {
+        // row-major C -= A B is column-major C^T -= B^T A^T: the same buffers, operands swapped
+        const double alpha = -1.0, beta = 1.0;
+        const rocblas_status st =
+            rocblas_dgemm(blas_handle(s), rocblas_operation_none, rocblas_operation_none, n, m, k,
+                          &alpha, B, ldb, A, lda, &beta, C, ldc);
+        if (st != rocblas_status_success)
+            throw std::runtime_error("HIP error: rocblas_dgemm failed (status " +
+                                     std::to_string((int)st) + ")");
+        return;
+    }
     dim3 grid((n + TN - 1) / TN, (m + TM - 1) / TM);
     hipLaunchKernelGGL(k_gemm_sub, grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, C, ldc);
     TD_HIP(hipGetLastError());
