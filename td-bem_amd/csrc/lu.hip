@@ -17,6 +17,7 @@
 // run to run); the small ones inside the panels stay on the LDS-tiled kernel below.
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -392,13 +393,20 @@ __global__ void __launch_bounds__(256)
         }
 }
 
-// One rocBLAS handle per host thread and device, created on first use (rocBLAS handles are
-// not shared between threads) and kept until the process ends; each call sets the stream.
-rocblas_handle blas_handle(hipStream_t s)
+// Updates at least this large (m n k) go to rocBLAS.
+constexpr double kBlasMinWork = 64.0 * 1024 * 1024;
+
+// C -= A B on rocBLAS.  One handle per device for the whole process, created on first use and
+// kept until the process ends (it holds the library's kernels and workspace on the device, about
+// 0.2 GB); threads share it under a lock held while the stream is set and the product enqueued.
+void blas_gemm_sub(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                   double* C, int ldc, hipStream_t s)
 {
-    thread_local std::map<int, rocblas_handle> handles;
+    static std::mutex mu;
+    static std::map<int, rocblas_handle> handles;
     int dev = 0;
     TD_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
     rocblas_handle& h = handles[dev];
     if (!h) {
         if (rocblas_create_handle(&h) != rocblas_status_success) {
@@ -409,25 +417,21 @@ rocblas_handle blas_handle(hipStream_t s)
     }
     if (rocblas_set_stream(h, s) != rocblas_status_success)
         throw std::runtime_error("HIP error: rocblas_set_stream failed");
-    return h;
+    // row-major C -= A B is column-major C^T -= B^T A^T: the same buffers, operands swapped
+    const double alpha = -1.0, beta = 1.0;
+    const rocblas_status st = rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n,
+                                            m, k, &alpha, B, ldb, A, lda, &beta, C, ldc);
+    if (st != rocblas_status_success)
+        throw std::runtime_error("HIP error: rocblas_dgemm failed (status " +
+                                 std::to_string((int)st) + ")");
 }
-
-// Updates at least this large (m n k) go to rocBLAS.
-constexpr double kBlasMinWork = 64.0 * 1024 * 1024;
 
 void gemm_sub(int m, int n, int k, const double* A, int lda, const double* B, int ldb, double* C,
               int ldc, hipStream_t s)
 {
     if (m <= 0 || n <= 0 || k <= 0) return;
     if ((double)m * n * k >= kBlasMinWork) {
-        // row-major C -= A B is column-major C^T -= B^T A^T: the same buffers, operands swapped
-        const double alpha = -1.0, beta = 1.0;
-        const rocblas_status st =
-            rocblas_dgemm(blas_handle(s), rocblas_operation_none, rocblas_operation_none, n, m, k,
-                          &alpha, B, ldb, A, lda, &beta, C, ldc);
-        if (st != rocblas_status_success)
-            throw std::runtime_error("HIP error: rocblas_dgemm failed (status " +
-                                     std::to_string((int)st) + ")");
+        blas_gemm_sub(m, n, k, A, lda, B, ldb, C, ldc, s);
         return;
     }
     dim3 grid((n + TN - 1) / TN, (m + TM - 1) / TM);
