@@ -59,7 +59,7 @@ MI355X_FP64_TFLOPS = 78.6
 ASM_FLOPS_PER_POINT = 291
 ASM_FLOPS_PER_LEVEL = 118
 # rocprofv3 kernel names of the convolution kernels and the calibration copy
-PMC_KERNELS = {"k_conv_mixed": "k_conv_mixed<", "k_conv": "k_conv<1>", "copy": "k_stream_copy"}
+PMC_KERNELS = {"k_conv_mixed": "k_conv_mixed(", "k_conv": "k_conv<1>", "copy": "k_stream_copy"}
 
 
 def parse():
@@ -323,15 +323,29 @@ def main():
             _, tm_all = timed_solves(ctx, p, 0, 1, amp, rise)
         nconv = max(tm["conv_launches"], 1)
         conv_ms = tm["conv_ms"] / nconv
-        # per launch, the algorithmic bytes: the in-shell node-pair lags (18 FP32 values each,
-        # G's 9 and H's 9, as the 48 x 96 blocks hold them), the merged lists, and the FP32
-        # history copy read once ((K + 1) slots x 96 columns per tile x R x 4 B)
+        # per launch (time-blocked at R <= 48: the near lags k < 3 for step s and the far lags of
+        # parity s mod 2 for steps s and s + 1, so a launch streams the near lags and half the
+        # far ones; the mean of the two parities), the algorithmic bytes: the in-shell node-pair
+        # lags it reads (18 FP32 values each, G's 9 and H's 9, as the 48 x 96 blocks hold them),
+        # its merged lists, and the FP32 history copy read once ((K + 1) slots x 96 columns per
+        # tile x R x 4 B)
+        blocked = R <= 48
         nT = -(-p.nnd // 16)
         hist_bytes = (K + 1) * nT * 96 * R * 4
-        list_bytes = tm["tile_entries"] * TP_ENTRY_BYTES
-        inshell_bytes = tm["store_nonzero"] * 18 * 4 + list_bytes + hist_bytes
+        if blocked:
+            lags_launch = tm["pair_lags_near"] + 0.5 * (tm["pair_lags_even"] + tm["pair_lags_odd"])
+            subs_launch = tm["subtiles_near"] + 0.5 * (tm["subtiles_even"] + tm["subtiles_odd"])
+            ents_launch = 0.5 * (tm["entries_parity0"] + tm["entries_parity1"])
+        else:
+            lags_launch, subs_launch = tm["store_nonzero"], tm["tile_subtiles"]
+            ents_launch = tm["tile_entries"]
+        list_bytes = ents_launch * TP_ENTRY_BYTES
+        inshell_bytes = lags_launch * 18 * 4 + list_bytes + hist_bytes
         # what the kernel streams: whole stored 16 x 32 sub-tiles, zeros inside them included
-        stored_bytes = tm["tile_subtiles"] * TP_SUBTILE_BYTES + list_bytes + hist_bytes
+        stored_bytes = subs_launch * TP_SUBTILE_BYTES + list_bytes + hist_bytes
+        # one step's convolution as an unblocked kernel reads it: every in-shell node-pair lag
+        step_bytes = (tm["store_nonzero"] * 18 * 4 + tm["tile_entries"] * TP_ENTRY_BYTES +
+                      hist_bytes)
         out.update({
             "value": round(args.steps * (p.numstep - 1) * R / wall, 2),
             "ms_per_step": round(wall / args.steps * 1e3, 3),
@@ -365,6 +379,13 @@ def main():
         out["roofline"]["stored_bytes_per_launch"] = int(stored_bytes)
         out["roofline"]["frac_of_stored_bytes"] = round(
             stored_bytes / (conv_ms * 1e-3) / 1e9 / MI355X_HBM_GBS, 4)
+        out["roofline"]["time_blocked"] = blocked
+        # one step's in-shell bytes (the whole store, as an unblocked kernel streams it every
+        # step) over the time per step: the rate an unblocked kernel would need to keep pace
+        out["roofline"]["step_bytes"] = int(step_bytes)
+        out["roofline"]["step_equivalent_GBs"] = round(step_bytes / (conv_ms * 1e-3) / 1e9, 1)
+        out["roofline"]["step_equivalent_frac"] = round(
+            step_bytes / (conv_ms * 1e-3) / 1e9 / MI355X_HBM_GBS, 4)
 
         # FP64 at 8 right-hand sides: the FP64 path's best rate (timed), then the error reference
         # for every one of the 32 right-hand sides (four untimed 8-RHS solves in all)

@@ -317,9 +317,12 @@ int tdbem_element_synchronize(tdbem_element* el);
  * split-BF16 sub-tiles of the mixed-precision store, 0 until built), reduce_ms (the mixed path's
  * reduction of partial sums into the step solve's right-hand side, summed event times), tile_blocks (48 x 96 blocks with a stored
  * sub-tile), tile_entries (steps of the merged per-group lists), active_levels (time levels
- * evaluated at those points, one to three each).  conv_ms covers k_conv or k_conv_mixed
- * alone. */
-#define TDBEM_NUM_TIMINGS 17
+ * evaluated at those points, one to three each), then, of the mixed-precision store, the
+ * in-shell node-pair lags of the near lags (k < 3), of the far lags of even and of odd k, the
+ * stored sub-tiles of the same three classes, and the entries of the merged lists of lag parity
+ * 0 and 1 (the time-blocked convolution streams the near lags and the far lags of one parity at
+ * each step).  conv_ms covers k_conv or k_conv_mixed alone. */
+#define TDBEM_NUM_TIMINGS 25
 #define TDBEM_PROFILE_ALL 1
 #define TDBEM_PROFILE_CONV 2
 int tdbem_set_profiling(tdbem_context* ctx, int level);

@@ -77,26 +77,37 @@ __global__ void k_b_zcopy(const BatchScen* __restrict__ D, const int* __restrict
     zcopy_elem(d.ring, d.N, d.nrhs, d.nT, d.perm, slot, d.nrhs / 16, d.zs, t - m * per);
 }
 
-// One workgroup per chunk (chunk.w = scenario), one wavefront per row tile of its group.
-template <int NT>
-__global__ void __launch_bounds__(64 * kGroup, NT <= 2 ? 2 : 1)
-    k_b_conv(const BatchScen* __restrict__ D, const int4* __restrict__ chunk,
-             const int* __restrict__ sdev, const int* __restrict__ act, double* __restrict__ partial)
+// One workgroup per (chunk slot, right-hand-side tile), one wavefront per row tile of the
+// chunk's group (conv_chunk).  A member with n0 chunks of lag parity 0 and n1 of parity 1 has
+// max(n0, n1) slots (slot_off: first slot of each member); at step s slot j of the member runs
+// its chunk j of parity s mod 2, if there is one, so the grid holds no workgroup of the parity
+// that does not run.
+__global__ void __launch_bounds__(64 * kGroup, 2)
+    k_b_conv(const BatchScen* __restrict__ D, const int* __restrict__ slot_off, int S,
+             const int4* __restrict__ chunk, int NT, const int* __restrict__ sdev,
+             const int* __restrict__ act, double* __restrict__ partial)
 {
-    const int c = xcd_chunk(blockIdx.x, gridDim.x);
+    const int L = xcd_chunk(blockIdx.x, gridDim.x);
+    const int slot = L / NT, nt0 = L % NT;
+    const int sc = scen_of(slot_off, S, slot);
+    if (!act[sc]) return;
+    const BatchScen& d = D[sc];
+    const int s = sdev[sc], p = s & 1;
+    const int j = slot - slot_off[sc];
+    if (j >= (p ? d.nch1 : d.nch0)) return;
+    const int c = d.ch0 + (p ? d.nch0 : 0) + j;
     const int4 cw = chunk[c];
-    if (!act[cw.w]) return;
-    const BatchScen& d = D[cw.w];
     const int w = threadIdx.x >> 6;
-    conv_chunk<NT>(reinterpret_cast<const f32x4*>(d.tp), d.ent,
-                   reinterpret_cast<const f32x4*>(d.zs), d.nT, d.R, sdev[cw.w], cw.y, cw.z, w,
-                   cw.x * kGroup + w < d.nT,
-                   partial + (((int64_t)c * kGroup + w) * 3 * NT) * 256);
+    const int g = cw.x & ((1 << kChunkParityShift) - 1);
+    double* out = partial + ((int64_t)c * 2 * kGroup + w) * 3 * NT * 256;
+    conv_chunk(reinterpret_cast<const f32x4*>(d.tp), d.ent, reinterpret_cast<const f32x4*>(d.zs),
+               d.nT, NT, nt0, d.R, s, cw.y, cw.z, w, g * kGroup + w < d.nT, out,
+               out + (int64_t)kGroup * 3 * NT * 256);
 }
 
 // the reduction of the partial sums, written as the step solve's right-hand side
 __global__ void k_b_reduce_rhs(const BatchScen* __restrict__ D, const int* __restrict__ off,
-                               int S, const int* __restrict__ act,
+                               int S, const int* __restrict__ sdev, const int* __restrict__ act,
                                const double* __restrict__ partial, const int* __restrict__ gch)
 {
     const int sc = scen_of(off, S, blockIdx.x);
@@ -104,8 +115,8 @@ __global__ void k_b_reduce_rhs(const BatchScen* __restrict__ D, const int* __res
     const BatchScen& d = D[sc];
     const int64_t t = (int64_t)(blockIdx.x - off[sc]) * kThreads + threadIdx.x;
     if (t >= (int64_t)d.nT * kRows * d.nrhs) return;
-    reduce_rhs_elem(partial, gch + d.gch0, d.nrhs / 16, d.perm, d.N, d.nrhs, d.gF, d.gU, d.fac,
-                    d.Xf, t);
+    reduce_rhs_elem(partial, gch + d.gch0, d.nG, sdev[sc] & 1, d.nrhs / 16, d.perm, d.N, d.nrhs,
+                    d.gF, d.gU, d.fac, d.carry, d.Xf, t);
 }
 
 template <int NT>
@@ -149,6 +160,8 @@ struct Batch {
     DBuf<int> off_z, off_r, off_rhs, off_a;
     int nb_z = 0, nb_r = 0, nb_rhs = 0, nb_a = 0;
     DBuf<int4> chunk;
+    DBuf<int> slot_off;          // [S + 1] first convolution slot of each member
+    int nslots = 0;
     DBuf<int> gch;
     DBuf<double> partial;
     int nchunks = 0;
@@ -186,13 +199,20 @@ void batch_plan(Batch& B)
     const int NT = B.nrhs / 16;
     std::vector<int4> ch;
     std::vector<int> gch;
-    std::vector<int> oz(B.S + 1, 0), orr(B.S + 1, 0), orhs(B.S + 1, 0), oa(B.S + 1, 0);
+    std::vector<int> oz(B.S + 1, 0), orr(B.S + 1, 0), orhs(B.S + 1, 0), oa(B.S + 1, 0),
+        os(B.S + 1, 0);
     for (int sc = 0; sc < B.S; sc++) {
         Context& c = *B.ctx[sc];
         BatchScen& d = B.h_desc[sc];
         d.gch0 = (int)gch.size();
-        // each scenario is cut as it would be alone (its results do not depend on the mix)
+        // each scenario is cut as it would be alone (its results do not depend on the mix);
+        // its chunks of parity 0 come first, then those of parity 1
+        d.ch0 = (int)ch.size();
         tp_chunks(c, tp_chunk_target(c, B.rounds), sc, ch, gch);
+        d.nch0 = 0;
+        for (size_t q = d.ch0; q < ch.size(); q++) d.nch0 += ((ch[q].x >> kChunkParityShift) & 1) == 0;
+        d.nch1 = (int)ch.size() - d.ch0 - d.nch0;
+        os[sc + 1] = os[sc] + std::max(d.nch0, d.nch1);
         d.nsplit = apply_splits(d.N, B.apply_waves);
         const size_t need = (size_t)d.nsplit * d.nrhs * d.Np;
         auto& pb = B.parts[&c];
@@ -206,8 +226,10 @@ void batch_plan(Batch& B)
     }
     B.nchunks = (int)ch.size();
     B.chunk.upload(ch.data(), ch.size(), B.stream);
+    B.slot_off.upload(os.data(), os.size(), B.stream);
+    B.nslots = os[B.S];
     B.gch.upload(gch.data(), gch.size(), B.stream);
-    B.partial.alloc((size_t)B.nchunks * kGroup * 3 * NT * 256);
+    B.partial.alloc((size_t)B.nchunks * 2 * kGroup * 3 * NT * 256);
     B.off_z.upload(oz.data(), oz.size(), B.stream);
     B.off_r.upload(orr.data(), orr.size(), B.stream);
     B.off_rhs.upload(orhs.data(), orhs.size(), B.stream);
@@ -222,14 +244,13 @@ void batch_plan(Batch& B)
 
 void launch_conv(Batch& B)
 {
-    const dim3 g((unsigned)B.nchunks), b(64 * kGroup);
-    switch (B.nrhs / 16) {
-    case 1: hipLaunchKernelGGL(k_b_conv<1>, g, b, 0, B.stream, B.desc.p, B.chunk.p, B.sdev.p, B.act.p, B.partial.p); break;
-    case 2: hipLaunchKernelGGL(k_b_conv<2>, g, b, 0, B.stream, B.desc.p, B.chunk.p, B.sdev.p, B.act.p, B.partial.p); break;
-    case 3: hipLaunchKernelGGL(k_b_conv<3>, g, b, 0, B.stream, B.desc.p, B.chunk.p, B.sdev.p, B.act.p, B.partial.p); break;
-    case 4: hipLaunchKernelGGL(k_b_conv<4>, g, b, 0, B.stream, B.desc.p, B.chunk.p, B.sdev.p, B.act.p, B.partial.p); break;
-    default: throw std::invalid_argument("batch: nrhs must be 16, 32, 48 or 64");
-    }
+    const int NT = B.nrhs / 16;
+    if (NT < 1 || NT > 4 || B.nrhs % 16)
+        throw std::invalid_argument("batch: nrhs must be 16, 32, 48 or 64");
+    if (B.nslots == 0) return;
+    hipLaunchKernelGGL(k_b_conv, dim3((unsigned)(B.nslots * NT)), dim3(64 * kGroup), 0,
+                       B.stream, B.desc.p, B.slot_off.p, B.S, B.chunk.p, NT, B.sdev.p, B.act.p,
+                       B.partial.p);
     TD_HIP(hipGetLastError());
 }
 
@@ -254,7 +275,7 @@ void launch_step(Batch& B)
                        B.off_z.p, B.S, B.sdev.p, B.act.p);
     launch_conv(B);
     hipLaunchKernelGGL(k_b_reduce_rhs, dim3(B.nb_r), dim3(kThreads), 0, B.stream, B.desc.p,
-                       B.off_r.p, B.S, B.act.p, B.partial.p, B.gch.p);
+                       B.off_r.p, B.S, B.sdev.p, B.act.p, B.partial.p, B.gch.p);
     launch_apply(B);
     hipLaunchKernelGGL(k_b_finalize, dim3(B.nb_rhs), dim3(kThreads), 0, B.stream, B.desc.p,
                        B.off_rhs.p, B.S, B.sdev.p, B.act.p);
@@ -311,6 +332,7 @@ void batch_refresh(Batch& B)
         d.ent = c.tp.ent.p;
         d.perm = c.tp.d_perm.p;
         d.nT = c.tp.nT;
+        d.nG = c.tp.nG;
     }
     for (auto it = B.parts.begin(); it != B.parts.end();) {
         if (std::find(B.ctx.begin(), B.ctx.end(), it->first) == B.ctx.end())

@@ -761,7 +761,11 @@ int tdbem_get_timings(tdbem_context* ctx, double* out)
             t.store_ms, t.init_ms, t.march_ms, t.lag_kernel_ms, t.gather_ms, t.conv_ms,
             t.solve_ms, (double)t.lag_launches, (double)t.conv_launches, (double)t.active_points,
             (double)c.pair_lags, (double)c.store.n_entries, (double)c.tp.nsub,
-            t.reduce_ms, (double)c.tp.nblocks, (double)c.tp.nentries, (double)t.active_levels};
+            t.reduce_ms, (double)c.tp.nblocks, (double)c.tp.nentries, (double)t.active_levels,
+            (double)c.tp.pair_lags_class[0], (double)c.tp.pair_lags_class[1],
+            (double)c.tp.pair_lags_class[2], (double)c.tp.sub_class[0],
+            (double)c.tp.sub_class[1], (double)c.tp.sub_class[2], (double)c.tp.ent_parity[0],
+            (double)c.tp.ent_parity[1]};
         std::memcpy(out, v, sizeof(v));
     });
 }

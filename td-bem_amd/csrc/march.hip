@@ -44,6 +44,7 @@ struct March {
     int precision = 0;             // 1: split-BF16 MFMA convolution
     DBuf<float> zs;                // FP32 mirror of the ring in MFMA operand order (mixed)
     DBuf<double> partial;          // per-wave partial sums of the mixed convolution
+    DBuf<double> carry;            // time-blocked: the next step's history part, summed
     int64_t zslot = 0;
 };
 
@@ -469,6 +470,8 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
         M.zs.alloc((size_t)M.R * M.zslot);
         TD_HIP(hipMemsetAsync(M.zs.p, 0, M.zs.bytes(), c.stream));
         M.partial.alloc((size_t)tp_partial_elems(c, nrhs));
+        M.carry.alloc((size_t)tp_carry_elems(c, nrhs));
+        TD_HIP(hipMemsetAsync(M.carry.p, 0, M.carry.bytes(), c.stream));
     }
     M.ring.alloc((size_t)M.R * nrhs * 2 * N);
     TD_HIP(hipMemsetAsync(M.ring.p, 0, M.ring.bytes(), c.stream));
@@ -510,13 +513,14 @@ int64_t march_state_bytes(Context& c, int nrhs)
     b += 3LL * nrhs * 8;                                 // fac, amp, rise
     b += 2LL * std::max(c.numstep, 1) * nrhs * N * 8;    // U, T
     if (c.precision == 1) {
-        const TPStore& T = c.tp;
         b += R * tp_zslot_elems(c, nrhs) * 4;            // zs
         std::vector<int4> ch;
         std::vector<int> gch;
         tp_chunks(c, tp_chunk_target(c, 4), 0, ch, gch);
-        b += (int64_t)ch.size() * kGroup * 3 * (nrhs / 16) * 256 * 8;   // partial
-        b += (int64_t)ch.size() * 16 + (int64_t)(T.nG + 1) * 4;         // chunk, grp_chunk
+        const int NT = nrhs / 16;
+        b += (int64_t)ch.size() * 2 * kGroup * 3 * NT * 256 * 8;   // partial
+        b += (int64_t)ch.size() * 16 + (int64_t)gch.size() * 4;   // chunk, grp_chunk
+        b += tp_carry_elems(c, nrhs) * 8;                         // carry
     } else {
         b += (int64_t)nrhs * N * 8;                      // hist
     }
@@ -562,6 +566,7 @@ void march_describe(Context& c, BatchScen& d)
     d.a2 = M.a2;
     d.a3 = M.a3;
     d.zs = M.zs.p;
+    d.carry = M.carry.p;
 }
 
 void march_set_step(Context& c, int s)
@@ -616,7 +621,8 @@ void march_run(Context& c, int nsteps, bool wait)
                 tp_conv(c, M.zs.p, M.nrhs, M.R, s, M.partial.p);
                 conv_timer.end();
                 reduce_timer.begin();
-                tp_reduce_rhs(c, M.partial.p, M.nrhs, N, M.gF.p, M.gU.p, M.fac.p, M.Xf.p);
+                tp_reduce_rhs(c, M.partial.p, M.nrhs, N, s, M.gF.p, M.gU.p, M.fac.p,
+                              M.carry.p, M.Xf.p);
                 reduce_timer.end();
             } else {
                 conv_timer.begin();

@@ -23,6 +23,7 @@
 // so rounding does not grow with the number of blocks and lags summed.  Storing FP32 instead of
 // the three bf16 terms moves 2 KiB instead of 3 KiB per sub-tile through HBM, the bound.
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <numeric>
 #include <stdexcept>
@@ -108,10 +109,11 @@ __global__ void __launch_bounds__(256)
 }
 
 // Rows lower..upper of the dense G and H of lag k from the tile store (the FP32 values): one
-// workgroup per merged-list entry, one wavefront per row tile of the entry's group.
+// workgroup per merged-list entry, one wavefront per row tile of the entry's group (a near lag
+// sits in both lists of its group and is written twice, with the same values).
 __global__ void __launch_bounds__(64 * kGroup)
     k_tile_to_dense(const f32x4* __restrict__ tp, const TPEntry* __restrict__ ent,
-                    const int64_t* __restrict__ grp_off, int nG, const int* __restrict__ perm,
+                    const int64_t* __restrict__ grp_off, int nL, const int* __restrict__ perm,
                     int nT, int k, int lower, int upper, int64_t N, double* __restrict__ G,
                     double* __restrict__ H)
 {
@@ -119,7 +121,7 @@ __global__ void __launch_bounds__(64 * kGroup)
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const TPEntry E = ent[e];
     if ((int)(E.id & kMaxLag) != k) return;
-    int lo = 0, hi = nG;   // group of entry e: the last g with grp_off[g] <= e
+    int lo = 0, hi = nL;   // list of entry e: the last L with grp_off[L] <= e; group L / 2
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
         if (grp_off[mid] <= e)
@@ -127,7 +129,7 @@ __global__ void __launch_bounds__(64 * kGroup)
         else
             hi = mid;
     }
-    const int I = lo * kGroup + w, J = (int)(E.id >> kKBits);
+    const int I = (lo >> 1) * kGroup + w, J = (int)(E.id >> kKBits);
     if (I >= nT) return;
     const unsigned msk = E.msk[w];
     for (int u = 0; u < 9; u++) {
@@ -172,31 +174,37 @@ __global__ void k_zcopy3(const double* __restrict__ ring, int N, int nrhs, int n
     zcopy_elem(ring, N, nrhs, nT, perm, slot, NT, zs, t - m * per);
 }
 
-// One workgroup per chunk of a group's merged list, one wavefront per row tile (conv_chunk).
-template <int NT>
-__global__ void __launch_bounds__(64 * kGroup, NT <= 2 ? 2 : 1)
+// One workgroup per (chunk, right-hand-side tile): chunks base .. of the plan, the NT
+// workgroups of a chunk next to each other on one XCD, one wavefront per row tile of the chunk's
+// group (conv_chunk).  A launch runs the chunks of the lag parity of step s.
+__global__ void __launch_bounds__(64 * kGroup, 2)
     k_conv_mixed(const f32x4* __restrict__ tp, const TPEntry* __restrict__ ent,
-                 const int4* __restrict__ chunk, const f32x4* __restrict__ zs, int nT, int Rring,
-                 int s, double* __restrict__ partial)
+                 const int4* __restrict__ chunk, int base, const f32x4* __restrict__ zs, int nT,
+                 int NT, int Rring, int s, double* __restrict__ partial)
 {
-    const int c = xcd_chunk(blockIdx.x, gridDim.x);
+    const int L = xcd_chunk(blockIdx.x, gridDim.x);
+    const int c = base + L / NT, nt0 = L % NT;
     const int4 cw = chunk[c];
     const int w = threadIdx.x >> 6;
-    // partial sums of chunk c: [c][wave][3][NT][64][4]
-    conv_chunk<NT>(tp, ent, zs, nT, Rring, s, cw.y, cw.z, w, cw.x * kGroup + w < nT,
-                   partial + (((int64_t)c * kGroup + w) * 3 * NT) * 256);
+    const int g = cw.x & ((1 << kChunkParityShift) - 1);
+    // partial sums of chunk c: [c][target][row tile][3][NT][64][4]
+    double* out = partial + ((int64_t)c * 2 * kGroup + w) * 3 * NT * 256;
+    conv_chunk(tp, ent, zs, nT, NT, nt0, Rring, s, cw.y, cw.z, w, g * kGroup + w < nT, out,
+               out + (int64_t)kGroup * 3 * NT * 256);
 }
 
-// Step solve right-hand side from the partial sums of the history (see reduce_rhs_elem).
+// Step solve right-hand side from the partial sums of the history at step s (see
+// reduce_rhs_elem).
 __global__ void k_tp_reduce_rhs(const double* __restrict__ partial, int nT,
-                                const int* __restrict__ grp_chunk, int NT,
+                                const int* __restrict__ grp_chunk, int nG, int s, int NT,
                                 const int* __restrict__ perm, int N, int nrhs,
                                 const double* __restrict__ gF, const double* __restrict__ gU,
-                                const double* __restrict__ fac, double* __restrict__ Xf)
+                                const double* __restrict__ fac, double* __restrict__ carry,
+                                double* __restrict__ Xf)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nT * kRows * nrhs) return;
-    reduce_rhs_elem(partial, grp_chunk, NT, perm, N, nrhs, gF, gU, fac, Xf, t);
+    reduce_rhs_elem(partial, grp_chunk, nG, s & 1, NT, perm, N, nrhs, gF, gU, fac, carry, Xf, t);
 }
 
 // One 16x16x32 bf16 product with the lane layouts assumed above, for the layout self-test.
@@ -372,11 +380,12 @@ void tp_plan_structure(Context& c)
     for (int64_t x : qnz) nz += x;
     T.nblocks = nz;
     T.nsub = nsub;
-    // merged (J, k) lists of the groups of kGroup row tiles, built per group in parallel
-    std::vector<std::vector<TPEntry>> gent(T.nG);
+    // merged (J, k) lists of the groups of kGroup row tiles, built per group in parallel: list
+    // 2 g + p holds the near lags (k < kNearLags) and the far lags of parity p (the time-blocked
+    // convolution runs the lists of one parity per step)
+    std::vector<std::vector<TPEntry>> gent(2 * (size_t)T.nG);
     parallel_for(T.nG, [&](int64_t glo, int64_t ghi) {
         for (int64_t g = glo; g < ghi; g++) {
-            auto& list = gent[g];
             for (int J = 0; J < nT; J++) {
                 int klo = 1 << 30, khi = -1;
                 for (int w = 0; w < kGroup; w++) {
@@ -405,27 +414,74 @@ void tp_plan_structure(Context& c)
                         for (int m = 0; m < 3; m++) e.kmask |= (msk[b] >> (3 * m)) & 7u;
                         any = true;
                     }
-                    if (any) list.push_back(e);
+                    if (!any) continue;
+                    if (k < kNearLags) {
+                        gent[2 * g].push_back(e);
+                        gent[2 * g + 1].push_back(e);
+                    } else {
+                        gent[2 * g + (k & 1)].push_back(e);
+                    }
                 }
             }
         }
     });
     std::vector<TPEntry> ent;
-    std::vector<int64_t> goff(T.nG + 1, 0);
-    for (int g = 0; g < T.nG; g++) {
-        goff[g] = (int64_t)ent.size();
-        ent.insert(ent.end(), gent[g].begin(), gent[g].end());
+    std::vector<int64_t> goff(2 * (size_t)T.nG + 1, 0);
+    for (size_t L = 0; L < gent.size(); L++) {
+        goff[L] = (int64_t)ent.size();
+        ent.insert(ent.end(), gent[L].begin(), gent[L].end());
     }
-    goff[T.nG] = (int64_t)ent.size();
+    goff[gent.size()] = (int64_t)ent.size();
     T.nentries = (int64_t)ent.size();
     T.h_grp_off = goff;
-    // cost of an entry: its stored sub-tiles, the staged history k-tiles, a fixed part
+    // cost of an entry: its stored sub-tiles, the staged history k-tiles (a far lag's for two
+    // targets), a fixed part
     T.h_work.resize(ent.size());
     for (size_t e = 0; e < ent.size(); e++) {
         int nsubs = 0;
         for (int w = 0; w < kGroup; w++) nsubs = std::max(nsubs, __builtin_popcount(ent[e].msk[w]));
+        const int tg = (int)(ent[e].id & kMaxLag) < kNearLags ? 1 : 2;
         // the slowest wavefront of the step, the history staging, the step's latency
-        T.h_work[e] = (float)(nsubs + 0.5 * __builtin_popcount(ent[e].kmask) + 4.0);
+        T.h_work[e] = (float)(nsubs + 0.5 * tg * __builtin_popcount(ent[e].kmask) + 4.0);
+    }
+    // per lag class (near, far even, far odd): in-shell node-pair lags and stored sub-tiles
+    // (what a launch of either parity streams), entries per parity list
+    {
+        const HostBuf<int2>& Hh = c.pin_hull;
+        std::vector<std::array<int64_t, 3>> pl(nT);
+        parallel_for(nT, [&](int64_t Ilo, int64_t Ihi) {
+            for (int64_t I = Ilo; I < Ihi; I++) {
+                std::array<int64_t, 3> cnt{0, 0, 0};
+                for (int ii = 0; ii < kTile; ii++) {
+                    const int i = T.perm[I * kTile + ii];
+                    if (i < 0) continue;
+                    for (int j = 0; j < nnd; j++) {
+                        const int2 h = Hh[(size_t)i * nnd + j];
+                        if (h.y < h.x) continue;
+                        cnt[0] += std::max(0, std::min(h.y, kNearLags - 1) - h.x + 1);
+                        const int lo = std::max(h.x, kNearLags);
+                        if (h.y < lo) continue;
+                        // lags lo..h.y: the even ones, then the odd ones
+                        const int ev = h.y / 2 - (lo + 1) / 2 + 1;
+                        cnt[1] += ev;
+                        cnt[2] += (h.y - lo + 1) - ev;
+                    }
+                }
+                pl[I] = cnt;
+            }
+        });
+        for (int q = 0; q < 3; q++) T.pair_lags_class[q] = T.sub_class[q] = 0;
+        for (int I = 0; I < nT; I++)
+            for (int q = 0; q < 3; q++) T.pair_lags_class[q] += pl[I][q];
+        for (const uint2& id : sub_id) {
+            const int k = (int)(id.y >> 4);
+            T.sub_class[k < kNearLags ? 0 : 1 + (k & 1)]++;
+        }
+        for (int p = 0; p < 2; p++) {
+            T.ent_parity[p] = 0;
+            for (int g = 0; g < T.nG; g++)
+                T.ent_parity[p] += goff[2 * g + p + 1] - goff[2 * g + p];
+        }
     }
     T.plan_nrhs = 0;
     T.d_perm.upload(T.perm.data(), T.perm.size(), c.stream);
@@ -496,7 +552,7 @@ void tile_rows(Context& c, int k, int lower, int upper, double* dG, double* dH)
     if (!T.ready || T.nentries == 0) return;
     hipLaunchKernelGGL(k_tile_to_dense, dim3((unsigned)T.nentries), dim3(64 * kGroup), 0,
                        c.stream, reinterpret_cast<const f32x4*>(T.vals.p), T.ent.p,
-                       T.d_grp_off.p, T.nG, T.d_perm.p, T.nT, k, lower, upper, N, dG, dH);
+                       T.d_grp_off.p, 2 * T.nG, T.d_perm.p, T.nT, k, lower, upper, N, dG, dH);
     TD_HIP(hipGetLastError());
 }
 
@@ -514,16 +570,11 @@ int64_t tp_zslot_elems(const Context& c, int nrhs) { return (int64_t)c.tp.nT * 3
 
 int tp_resident_groups(int nrhs, int device)
 {
+    if (nrhs < 16 || nrhs > 64 || nrhs % 16)
+        throw std::invalid_argument("mixed march needs nrhs in {16, 32, 48, 64}");
     int per_cu = 1, cus = 1;
-    const void* fn = nullptr;
-    switch (nrhs / 16) {
-    case 1: fn = reinterpret_cast<const void*>(&k_conv_mixed<1>); break;
-    case 2: fn = reinterpret_cast<const void*>(&k_conv_mixed<2>); break;
-    case 3: fn = reinterpret_cast<const void*>(&k_conv_mixed<3>); break;
-    case 4: fn = reinterpret_cast<const void*>(&k_conv_mixed<4>); break;
-    default: throw std::invalid_argument("mixed march needs nrhs in {16, 32, 48, 64}");
-    }
-    TD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * kGroup, 0));
+    TD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(&k_conv_mixed), 64 * kGroup, 0));
     TD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     return std::max(1, per_cu) * std::max(1, cus);
 }
@@ -538,32 +589,36 @@ double tp_work(const Context& c)
 void tp_chunks(const Context& c, int target, int sc, std::vector<int4>& ch, std::vector<int>& gch)
 {
     const TPStore& T = c.tp;
-    target = std::max(target, T.nG);
+    target = std::max(target, 2 * T.nG);
     const double total = tp_work(c);
     const size_t gbase = gch.size();
-    gch.resize(gbase + T.nG + 1, 0);
-    for (int g = 0; g < T.nG; g++) {
-        const int first = (int)ch.size();
-        gch[gbase + g] = first;
-        const int64_t e0 = T.h_grp_off[g], e1 = T.h_grp_off[g + 1];
-        double wg = 0;
-        for (int64_t e = e0; e < e1; e++) wg += T.h_work[e];
-        const int n = std::max(1, (int)std::lround(total > 0 ? target * wg / total : 1.0));
-        const double step = wg / n;
-        double acc = 0;
-        int64_t start = e0;
-        for (int64_t e = e0; e < e1; e++) {
-            acc += T.h_work[e];
-            // close the chunk once it holds its share (the last one takes the rest)
-            const int made = (int)ch.size() - first;
-            if (acc >= step * (made + 1) && made < n - 1) {
-                ch.push_back(make_int4(g, (int)start, (int)(e + 1), sc));
-                start = e + 1;
+    gch.resize(gbase + 2 * ((size_t)T.nG + 1), 0);
+    for (int p = 0; p < 2; p++) {
+        int* gc = gch.data() + gbase + (size_t)p * (T.nG + 1);
+        for (int g = 0; g < T.nG; g++) {
+            const int first = (int)ch.size();
+            gc[g] = first;
+            const int64_t e0 = T.h_grp_off[2 * g + p], e1 = T.h_grp_off[2 * g + p + 1];
+            double wg = 0;
+            for (int64_t e = e0; e < e1; e++) wg += T.h_work[e];
+            const int n = std::max(1, (int)std::lround(total > 0 ? target * wg / total : 1.0));
+            const double step = wg / n;
+            const int gx = g | (p << kChunkParityShift);
+            double acc = 0;
+            int64_t start = e0;
+            for (int64_t e = e0; e < e1; e++) {
+                acc += T.h_work[e];
+                // close the chunk once it holds its share (the last one takes the rest)
+                const int made = (int)ch.size() - first;
+                if (acc >= step * (made + 1) && made < n - 1) {
+                    ch.push_back(make_int4(gx, (int)start, (int)(e + 1), sc));
+                    start = e + 1;
+                }
             }
+            ch.push_back(make_int4(gx, (int)start, (int)e1, sc));
         }
-        ch.push_back(make_int4(g, (int)start, (int)e1, sc));
+        gc[T.nG] = (int)ch.size();
     }
-    gch[gbase + T.nG] = (int)ch.size();
 }
 
 // Chunks of a context's merged lists: one per kChunkWork / rounds units of entry work (about
@@ -586,6 +641,9 @@ void tp_plan(Context& c, int nrhs, int rounds)
     std::vector<int> gch;
     tp_chunks(c, tp_chunk_target(c, rounds), 0, ch, gch);
     T.nchunks = (int)ch.size();
+    T.nch[1] = 0;
+    for (const int4& q : ch) T.nch[1] += (q.x >> kChunkParityShift) & 1;
+    T.nch[0] = T.nchunks - T.nch[1];
     T.chunk.upload(ch.data(), ch.size(), c.stream);
     T.grp_chunk.upload(gch.data(), gch.size(), c.stream);
     TD_HIP(hipStreamSynchronize(c.stream));
@@ -597,36 +655,37 @@ void tp_conv(Context& c, const float* zs, int nrhs, int Rring, int s, double* pa
 {
     const TPStore& T = c.tp;
     if (T.plan_nrhs != nrhs) throw std::logic_error("mixed convolution not planned for nrhs");
+    if (nrhs < 16 || nrhs > 64 || nrhs % 16)
+        throw std::runtime_error("mixed march needs nrhs in {16, 32, 48, 64}");
     const int NT = nrhs / 16;
-    const dim3 grid((unsigned)T.nchunks), block(64 * kGroup);
-    const f32x4* tpv = reinterpret_cast<const f32x4*>(T.vals.p);
-    const f32x4* z = reinterpret_cast<const f32x4*>(zs);
-    switch (NT) {
-    case 1: hipLaunchKernelGGL(k_conv_mixed<1>, grid, block, 0, c.stream, tpv, T.ent.p, T.chunk.p, z, T.nT, Rring, s, partial); break;
-    case 2: hipLaunchKernelGGL(k_conv_mixed<2>, grid, block, 0, c.stream, tpv, T.ent.p, T.chunk.p, z, T.nT, Rring, s, partial); break;
-    case 3: hipLaunchKernelGGL(k_conv_mixed<3>, grid, block, 0, c.stream, tpv, T.ent.p, T.chunk.p, z, T.nT, Rring, s, partial); break;
-    case 4: hipLaunchKernelGGL(k_conv_mixed<4>, grid, block, 0, c.stream, tpv, T.ent.p, T.chunk.p, z, T.nT, Rring, s, partial); break;
-    default: throw std::runtime_error("mixed march needs nrhs in {16, 32, 48, 64}");
-    }
+    // the chunks of lag parity s mod 2 (the plan holds parity 0's first), NT workgroups each
+    const int base = (s & 1) ? T.nch[0] : 0;
+    const int n = T.nch[s & 1];
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_conv_mixed, dim3((unsigned)(n * NT)), dim3(64 * kGroup), 0, c.stream,
+                       reinterpret_cast<const f32x4*>(T.vals.p), T.ent.p, T.chunk.p, base,
+                       reinterpret_cast<const f32x4*>(zs), T.nT, NT, Rring, s, partial);
     TD_HIP(hipGetLastError());
 }
 
-void tp_reduce_rhs(Context& c, const double* partial, int nrhs, int N, const double* gF,
-                   const double* gU, const double* fac, double* Xf)
+void tp_reduce_rhs(Context& c, const double* partial, int nrhs, int N, int s, const double* gF,
+                   const double* gU, const double* fac, double* carry, double* Xf)
 {
     const TPStore& T = c.tp;
     const int NT = nrhs / 16;
     const int64_t n = (int64_t)T.nT * kRows * nrhs;
     hipLaunchKernelGGL(k_tp_reduce_rhs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                       c.stream, partial, T.nT, T.grp_chunk.p, NT, T.d_perm.p, N, nrhs, gF, gU,
-                       fac, Xf);
+                       c.stream, partial, T.nT, T.grp_chunk.p, T.nG, s, NT, T.d_perm.p, N, nrhs,
+                       gF, gU, fac, carry, Xf);
     TD_HIP(hipGetLastError());
 }
 
 int64_t tp_partial_elems(const Context& c, int nrhs)
 {
-    return (int64_t)c.tp.nchunks * kGroup * 3 * (nrhs / 16) * 256;
+    return (int64_t)c.tp.nchunks * 2 * kGroup * 3 * (nrhs / 16) * 256;
 }
+
+int64_t tp_carry_elems(const Context& c, int nrhs) { return (int64_t)c.tp.nT * kRows * nrhs; }
 
 void mfma_selftest(const float* hA, const float* hB, float* hD)
 {

@@ -82,73 +82,101 @@ __device__ __forceinline__ void zcopy_elem(const double* __restrict__ ring, int 
         (float)v;
 }
 
-// One wavefront's share of a convolution chunk: the steps [e0, e1) of its group's merged list,
-// for row tile `w` of the group, at step s; writes its 48 x (16 NT) FP64 partial sums to `out`
-// ([3][NT][64][4]) unless the row tile does not exist.  All wavefronts of the workgroup
-// call it together (it has a barrier per step): the history k-tiles of a step are loaded from
-// HBM once per workgroup into one of two LDS buffers, one step ahead of their use, and shared by
-// the group's row tiles; each wavefront streams its own stored FP32 sub-tiles and splits each
-// into three bf16 operands in registers (split_f32x8) just before its products.
-template <int NT>
+// Lags below kNearLags are the near lags of the two-step (time-blocked) convolution: their
+// history at step s + 1 is not known at step s (z_0, z_1 and y_{s-1} need x_s), so they are
+// convolved at every step for that step alone; a lag k >= kNearLags is convolved every other
+// step, at the steps s with s = k (mod 2), for both s (history slot s - k) and s + 1 (slot
+// s + 1 - k, complete at step s since s + 1 - k <= s - 2).  Each stored sub-tile is thus read
+// once per two steps.
+constexpr int kNearLags = 3;
+
+// One workgroup's share of a convolution chunk: the steps [e0, e1) of one of its group's merged
+// lists (the near lags and the far lags of one parity), at step s, for the 16 load cases of
+// right-hand-side tile nt0 of NT.  Wavefront w takes row tile w of the group and both target
+// steps, s and s + 1 (s alone for a near lag), and writes its 48 x 16 FP64 partial sums of
+// target t to the columns of tile nt0 of out[t] ([3][NT][64][4]) unless the row tile does not
+// exist.  The NT workgroups of a chunk run next to each other on one XCD and read the same
+// stored sub-tiles at about the same time, so those come from HBM once and from L2 after.
+// All wavefronts of the workgroup call it together (a barrier per step): the history k-tiles of
+// a step (for each target) are loaded once per workgroup into one of two LDS buffers, one step
+// ahead of their use, and shared by the group's row tiles; each wavefront streams its own stored
+// FP32 sub-tiles and splits each into three bf16 operands in registers (split_f32x8) just before
+// its products, which serve both targets.
 __device__ __forceinline__ void conv_chunk(const f32x4* __restrict__ tp,
                                            const TPEntry* __restrict__ ent,
-                                           const f32x4* __restrict__ zs, int nT, int Rring, int s,
-                                           int64_t e0, int64_t e1, int w, bool write,
-                                           double* __restrict__ out)
+                                           const f32x4* __restrict__ zs, int nT, int NT, int nt0,
+                                           int Rring, int s, int64_t e0, int64_t e1, int w,
+                                           bool write, double* __restrict__ out0,
+                                           double* __restrict__ out1)
 {
     // one (slot, column tile) history block: in HBM, [k-tile][rhs tile] units of 64 lanes x 8
-    // FP32; in LDS, split exactly into [term][k-tile][rhs tile] lines of 64 x 16 B (bf16x8)
-    constexpr int kLines = 9 * NT;
-    constexpr int kUnits = 3 * NT;
-    constexpr int kPer = (kUnits + kGroup - 1) / kGroup;   // units per wavefront
-    __shared__ bf16x8 zl[2][kLines * 64];
+    // FP32; in LDS (the workgroup's rhs tile), split exactly into [term][k-tile] lines of
+    // 64 x 16 B (bf16x8), one block per target
+    constexpr int kUnits = 3;                              // per target: k-tiles
+    constexpr int kPer = (2 * kUnits + kGroup - 1) / kGroup;  // units per wavefront
+    __shared__ bf16x8 zl[2][2][9 * 64];
+    // the store, the lists and the history are global memory: loads through global (not flat)
+    // instructions even where the pointers come from a descriptor table, so they count against
+    // the vector-memory counter only and an LDS wait does not wait for them
+    typedef const __attribute__((address_space(1))) f32x4 gf32x4;
+    typedef const __attribute__((address_space(1))) TPEntry gentry;
+    gf32x4* tpg = (gf32x4*)tp;
+    gf32x4* zsg = (gf32x4*)zs;
+    gentry* entg = (gentry*)ent;
     const int lane = threadIdx.x & 63;
-    double acc[3][NT][4];
-#pragma unroll
-    for (int m = 0; m < 3; m++)
-#pragma unroll
-        for (int n = 0; n < NT; n++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) acc[m][n][v] = 0.0;
-    // wavefront w stages units w, w + 4, ... of a block; unit U = k-tile * NT + rhs tile
-    f32x4 zn[kPer][2];
-    auto zload = [&](uint32_t id, unsigned kmask) {
-        const int J = (int)(id >> kKBits), k = (int)(id & kMaxLag);
-        int slot = (s - k) % Rring;
-        if (slot < 0) slot += Rring;
-        const f32x4* Zg = zs + ((int64_t)slot * nT + J) * (kUnits * 128) + lane;
-#pragma unroll
-        for (int q = 0; q < kPer; q++) {
-            const int U = q * kGroup + w;
-            if (U < kUnits && ((kmask >> (U / NT)) & 1u)) {
-                zn[q][0] = Zg[(U * 2) * 64];
-                zn[q][1] = Zg[(U * 2 + 1) * 64];
-            }
-        }
-    };
-    auto zstore = [&](int buf, unsigned kmask) {
-#pragma unroll
-        for (int q = 0; q < kPer; q++) {
-            const int U = q * kGroup + w;
-            if (U < kUnits && ((kmask >> (U / NT)) & 1u)) {
-                bf16x8 t0, t1, t2;
-                split_f32x8(zn[q][0], zn[q][1], t0, t1, t2);
-                const int kt = U / NT, nt = U % NT;
-                zl[buf][((0 * 3 + kt) * NT + nt) * 64 + lane] = t0;
-                zl[buf][((1 * 3 + kt) * NT + nt) * 64 + lane] = t1;
-                zl[buf][((2 * 3 + kt) * NT + nt) * 64 + lane] = t2;
-            }
-        }
-    };
     // w is the same in every lane of a wavefront: the entry fields are scalar loads
     w = __builtin_amdgcn_readfirstlane(w);
-    // this wavefront's stored sub-tiles of an entry (read once: non-temporal, so they do not
-    // evict history blocks) go to a[u] by sub-tile position u; the next entry's sub-tiles of
-    // k-tile kt are requested as soon as the products of k-tile kt have consumed a[kt], a[3+kt],
-    // a[6+kt], so their latency overlaps the rest of the entry and the barrier
+    double acc[2][3][4];
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int m = 0; m < 3; m++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) acc[t][m][v] = 0.0;
+    // wavefront w stages units w, w + 4 of an entry's blocks; unit U = target * 3 + k-tile; the
+    // target-(s + 1) block of a near lag is not staged
+    f32x4 zn[kPer][2];
+    const int64_t zblock = 3LL * NT * 128;   // f32x4 per (slot, column tile)
+    auto zload = [&](uint32_t id, unsigned kmask) {
+        const int J = (int)(id >> kKBits), k = (int)(id & kMaxLag);
+#pragma unroll
+        for (int q = 0; q < kPer; q++) {
+            const int U = q * kGroup + w;
+            const int t = U / kUnits, kt = U % kUnits;
+            if (U < 2 * kUnits && ((kmask >> kt) & 1u) && !(t == 1 && k < kNearLags)) {
+                int slot = (s + t - k) % Rring;
+                if (slot < 0) slot += Rring;
+                gf32x4* Zg = zsg + ((int64_t)slot * nT + J) * zblock +
+                             (int64_t)(kt * NT + nt0) * 128 + lane;
+                zn[q][0] = Zg[0];
+                zn[q][1] = Zg[64];
+            }
+        }
+    };
+    auto zstore = [&](int buf, uint32_t id, unsigned kmask) {
+        const int k = (int)(id & kMaxLag);
+#pragma unroll
+        for (int q = 0; q < kPer; q++) {
+            const int U = q * kGroup + w;
+            const int t = U / kUnits, kt = U % kUnits;
+            if (U < 2 * kUnits && ((kmask >> kt) & 1u) && !(t == 1 && k < kNearLags)) {
+                bf16x8 t0, t1, t2;
+                split_f32x8(zn[q][0], zn[q][1], t0, t1, t2);
+                zl[buf][t][(0 * 3 + kt) * 64 + lane] = t0;
+                zl[buf][t][(1 * 3 + kt) * 64 + lane] = t1;
+                zl[buf][t][(2 * 3 + kt) * 64 + lane] = t2;
+            }
+        }
+    };
+    // this wavefront's stored sub-tiles of an entry go to a[u] by sub-tile position u; the next
+    // entry's sub-tiles of k-tile kt are requested as soon as the products of k-tile kt have
+    // consumed a[kt], a[3+kt], a[6+kt], so their latency overlaps the rest of the entry and the
+    // barrier.  With one rhs tile (NT = 1) nothing else reads them: non-temporal loads, so they
+    // do not evict history blocks; otherwise the chunk's other workgroups read them from L2.
     f32x4 a[9][2];
+    const bool shared = NT > 1;
     auto aload = [&](unsigned mk, uint32_t first, int kt) {
-        const f32x4* A = tp + (int64_t)first * (kSubFloats / 4) + lane;
+        gf32x4* A = tpg + (int64_t)first * (kSubFloats / 4) + lane;
 #pragma unroll
         for (int m = 0; m < 3; m++) {
             const int u = 3 * m + kt;
@@ -156,50 +184,53 @@ __device__ __forceinline__ void conv_chunk(const f32x4* __restrict__ tp,
                 const int q = __builtin_popcount(mk & ((1u << u) - 1u));
 #pragma unroll
                 for (int h = 0; h < 2; h++)
-                    a[u][h] = __builtin_nontemporal_load(&A[(q * 2 + h) * 64]);
+                    a[u][h] = shared ? A[(q * 2 + h) * 64]
+                                     : __builtin_nontemporal_load(&A[(q * 2 + h) * 64]);
             }
         }
     };
     if (e0 < e1) {
-        zload(ent[e0].id, ent[e0].kmask);
-        zstore(0, ent[e0].kmask);
-        const unsigned m0 = ent[e0].msk[w];
-        const uint32_t f0 = ent[e0].sub[w];
+        const gentry& E0 = entg[e0];
+        zload(E0.id, E0.kmask);
+        zstore(0, E0.id, E0.kmask);
+        const unsigned m0 = E0.msk[w];
+        const uint32_t f0 = E0.sub[w];
 #pragma unroll
         for (int kt = 0; kt < 3; kt++) aload(m0, f0, kt);
     }
     __syncthreads();
     for (int64_t e = e0; e < e1; e++) {
         const int cur = (int)((e - e0) & 1);
-        const unsigned msk = ent[e].msk[w];
+        const gentry& E = entg[e];
+        const unsigned msk = E.msk[w];
+        const bool far = (int)(E.id & kMaxLag) >= kNearLags;
         const bool more = e + 1 < e1;
         unsigned knext = 0, mnext = 0;
-        uint32_t fnext = 0;
+        uint32_t fnext = 0, idnext = 0;
         if (more) {
-            knext = ent[e + 1].kmask;
-            mnext = ent[e + 1].msk[w];
-            fnext = ent[e + 1].sub[w];
-            zload(ent[e + 1].id, knext);
+            const gentry& En = entg[e + 1];
+            idnext = En.id;
+            knext = En.kmask;
+            mnext = En.msk[w];
+            fnext = En.sub[w];
+            zload(idnext, knext);
         }
-        const bf16x8* Z = zl[cur] + lane;
         // The leading product and the corrections accumulate apart in FP32, over the entry's
-        // three k-tiles (96 columns) where the registers allow it (kChain; the 48-rhs variant
-        // would lose its second wave per SIMD), else over one sub-tile, and then move to FP64.
-        constexpr bool kChain = NT != 3;
-        f32x4 dm[3][NT], dc[3][NT];
+        // three k-tiles (96 columns), and then move to FP64.
+        f32x4 dm[2][3], dc[2][3];
 #pragma unroll
-        for (int m = 0; m < 3; m++)
+        for (int t = 0; t < 2; t++)
 #pragma unroll
-            for (int n = 0; n < NT; n++) dm[m][n] = dc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int m = 0; m < 3; m++) dm[t][m] = dc[t][m] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kt = 0; kt < 3; kt++) {
             const unsigned rows = (msk >> kt) & 0x49u;   // bits kt, 3 + kt, 6 + kt
             if (rows) {
-                bf16x8 z[3][NT];
+                bf16x8 z[2][3];
 #pragma unroll
-                for (int t = 0; t < 3; t++)
+                for (int t = 0; t < 2; t++)
 #pragma unroll
-                    for (int n = 0; n < NT; n++) z[t][n] = Z[((t * 3 + kt) * NT + n) * 64];
+                    for (int q = 0; q < 3; q++) z[t][q] = zl[cur][t][(q * 3 + kt) * 64 + lane];
 #pragma unroll
                 for (int m = 0; m < 3; m++) {
                     const int u = 3 * m + kt;
@@ -209,60 +240,61 @@ __device__ __forceinline__ void conv_chunk(const f32x4* __restrict__ tp,
                     // the leading product a1 z1 and the seven corrections (smallest first) sum
                     // in separate FP32 registers.  Dropped: a3 z3, below 2^-28 of |a| |z|.
 #pragma unroll
-                    for (int n = 0; n < NT; n++) {
-                        const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
-                        f32x4 d = kChain ? dm[m][n] : zero, c = kChain ? dc[m][n] : zero;
-                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, z[0][n], d, 0, 0, 0);
-                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, z[1][n], c, 0, 0, 0);
-                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, z[2][n], c, 0, 0, 0);
-                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, z[2][n], c, 0, 0, 0);
-                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, z[0][n], c, 0, 0, 0);
-                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, z[1][n], c, 0, 0, 0);
-                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, z[0][n], c, 0, 0, 0);
-                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, z[1][n], c, 0, 0, 0);
-                        if constexpr (kChain) {
-                            dm[m][n] = d;
-                            dc[m][n] = c;
-                        } else {
-#pragma unroll
-                            for (int v = 0; v < 4; v++)
-                                acc[m][n][v] += (double)d[v] + (double)c[v];
-                        }
+                    for (int t = 0; t < 2; t++) {
+                        if (t == 1 && !far) continue;
+                        f32x4 d = dm[t][m], c = dc[t][m];
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, z[t][0], d, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, z[t][1], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, z[t][2], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, z[t][2], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, z[t][0], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, z[t][1], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, z[t][0], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, z[t][1], c, 0, 0, 0);
+                        dm[t][m] = d;
+                        dc[t][m] = c;
                     }
                 }
             }
             if (more) aload(mnext, fnext, kt);
         }
 #pragma unroll
-        for (int m = 0; m < 3; m++) {
-            if (!kChain || !((msk >> (3 * m)) & 7u)) continue;
+        for (int t = 0; t < 2; t++) {
+            if (t == 1 && !far) continue;
 #pragma unroll
-            for (int n = 0; n < NT; n++)
+            for (int m = 0; m < 3; m++) {
+                if (!((msk >> (3 * m)) & 7u)) continue;
 #pragma unroll
                 for (int v = 0; v < 4; v++)
-                    acc[m][n][v] += (double)dm[m][n][v] + (double)dc[m][n][v];
+                    acc[t][m][v] += (double)dm[t][m][v] + (double)dc[t][m][v];
+            }
         }
-        if (more) zstore(cur ^ 1, knext);
+        if (more) zstore(cur ^ 1, idnext, knext);
         __syncthreads();
     }
     if (!write) return;
 #pragma unroll
     for (int m = 0; m < 3; m++)
 #pragma unroll
-        for (int n = 0; n < NT; n++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) out[((m * NT + n) * 64 + lane) * 4 + v] = acc[m][n][v];
+        for (int v = 0; v < 4; v++) {
+            const int64_t o = ((int64_t)(m * NT + nt0) * 64 + lane) * 4 + v;
+            out0[o] = acc[0][m][v];
+            out1[o] = acc[1][m][v];
+        }
 }
 
-// Element t (< nT * 48 * nrhs) of hist[r][dof] = the fixed-order sum over the chunks of the row
-// tile's group; partial of chunk c, wave w: [c][w][3][NT][64][4], 16x16 output fragment
-// lane = col + 16 * (row / 4), element row % 4.  t runs through a row tile's partials in that
-// order, so the reads of consecutive threads are contiguous.  grp_chunk: first chunk of each
-// group.  Returns false for the padding rows of the last tile (no dof).
+// Element t (< nT * 48 * nrhs) of the history sums of step s (sum0) and of the part of step
+// s + 1 convolved at step s (sum1): fixed-order sums over the chunks of the row tile's group that
+// ran at step s (those of the lists of lag parity p = s mod 2).  Partial of chunk c, target tg,
+// row tile w of its group: [c][tg][w][3][NT][64][4], 16x16 output fragment lane = col + 16 *
+// (row / 4), element row % 4; t runs through a row tile's partials in that order, so the reads
+// of consecutive threads are contiguous.  grp_chunk: [2][nG + 1], the first chunk of each group
+// in the lists of parity 0 and 1.  Returns false for the padding rows of the last tile (no dof).
 __device__ __forceinline__ bool reduce_sum(const double* __restrict__ partial,
-                                           const int* __restrict__ grp_chunk, int NT,
-                                           const int* __restrict__ perm, int nrhs, int64_t t,
-                                           int& r, int& dof, double& sum)
+                                           const int* __restrict__ grp_chunk, int nG, int p,
+                                           int NT, const int* __restrict__ perm, int nrhs,
+                                           int64_t t, int& r, int& dof, double& sum0,
+                                           double& sum1)
 {
     const int per = kRows * nrhs;   // = 3 NT 256
     const int I = (int)(t / per), o = (int)(t % per);
@@ -274,22 +306,15 @@ __device__ __forceinline__ bool reduce_sum(const double* __restrict__ partial,
     if (node < 0) return false;
     dof = 3 * node + row % 3;
     const int g = I / kGroup, w = I % kGroup;
-    double acc = 0.0;
-    for (int c = grp_chunk[g]; c < grp_chunk[g + 1]; c++)
-        acc += partial[((int64_t)c * kGroup + w) * per + o];
-    sum = acc;
+    double a0 = 0.0, a1 = 0.0;
+    const int* gc = grp_chunk + p * (nG + 1);
+    for (int c = gc[g]; c < gc[g + 1]; c++) {
+        a0 += partial[(((int64_t)c * 2) * kGroup + w) * per + o];
+        a1 += partial[(((int64_t)c * 2 + 1) * kGroup + w) * per + o];
+    }
+    sum0 = a0;
+    sum1 = a1;
     return true;
-}
-
-__device__ __forceinline__ void reduce_elem(const double* __restrict__ partial,
-                                            const int* __restrict__ grp_chunk, int NT,
-                                            const int* __restrict__ perm, int N, int nrhs,
-                                            double* __restrict__ hist, int64_t t)
-{
-    int r, d;
-    double sum;
-    if (reduce_sum(partial, grp_chunk, NT, perm, nrhs, t, r, d, sum))
-        hist[(int64_t)r * N + d] = sum;
 }
 
 // Element t (< N * nrhs): r = f_r gF + gU + hist, written in the fragment order of the step solve.
@@ -308,21 +333,30 @@ __device__ __forceinline__ void rhs_elem(const double* __restrict__ gF,
     Xf[frag_index(N, r, d)] = rhs_value(fac[r], gF[d], gU[d], hist[t]);
 }
 
-// reduce_elem and rhs_elem in one pass: the summed history goes straight into the step solve's
-// right-hand side (every dof of every rhs is one element t of the reduction).
+// reduce_sum and rhs_elem in one pass: the summed history goes straight into the step solve's
+// right-hand side (every dof of every rhs is one element t of the reduction).  carry[t] holds
+// the part of step s's history convolved at step s - 1 and receives the part of step s + 1's
+// convolved now (the same thread reads, then writes, element t).
 __device__ __forceinline__ void reduce_rhs_elem(const double* __restrict__ partial,
-                                                const int* __restrict__ grp_chunk, int NT,
+                                                const int* __restrict__ grp_chunk, int nG, int p,
+                                                int NT,
                                                 const int* __restrict__ perm, int N, int nrhs,
                                                 const double* __restrict__ gF,
                                                 const double* __restrict__ gU,
                                                 const double* __restrict__ fac,
+                                                double* __restrict__ carry,
                                                 double* __restrict__ Xf, int64_t t)
 {
     int r, d;
-    double sum;
-    if (reduce_sum(partial, grp_chunk, NT, perm, nrhs, t, r, d, sum))
-        Xf[frag_index(N, r, d)] = rhs_value(fac[r], gF[d], gU[d], sum);
+    double sum0, sum1;
+    if (!reduce_sum(partial, grp_chunk, nG, p, NT, perm, nrhs, t, r, d, sum0, sum1)) return;
+    sum0 += carry[t];
+    carry[t] = sum1;
+    Xf[frag_index(N, r, d)] = rhs_value(fac[r], gF[d], gU[d], sum0);
 }
+
+// Chunk of a convolution plan: (group | lag parity << 24, first entry, end entry, scenario).
+constexpr int kChunkParityShift = 24;
 
 // Wavefront g (< frag_rows(n) / 16 * nsplit) of parts[p][r][i] = sum over the columns j of
 // split p of A[i][j] X[r][j], A and X in fragment order: per 32-column fragment it loads 4 KiB of

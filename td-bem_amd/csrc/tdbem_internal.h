@@ -226,16 +226,21 @@ struct TPStore {
     int64_t nsub = 0;                // stored sub-tiles
     int64_t nentries = 0;            // merged-list entries over all groups
     DBuf<float> vals;                // [nsub][2][64][4] FP32, 16x16x32 A-operand order
-    DBuf<TPEntry> ent;               // merged lists, group after group
-    std::vector<int64_t> h_grp_off;  // [nG + 1]
+    // merged lists, two per group: list 2 g + p holds the near lags (k < kNearLags) and the far
+    // lags of parity p of group g, lists in order
+    DBuf<TPEntry> ent;
+    std::vector<int64_t> h_grp_off;  // [2 nG + 1] first entry of each list
     std::vector<float> h_work;       // per entry: relative cost (bytes moved)
+    // per lag class (near, far even, far odd): in-shell node-pair lags, stored sub-tiles; entries
+    // of the parity-0 and parity-1 lists
+    int64_t pair_lags_class[3] = {0, 0, 0}, sub_class[3] = {0, 0, 0}, ent_parity[2] = {0, 0};
     // launch plan of k_conv_mixed for one right-hand-side count: one workgroup per chunk, chunks
-    // of equal work inside each group, as many chunks as workgroups resident at once
+    // of equal work inside each list, the parity-0 lists' chunks first
     int plan_nrhs = 0, plan_rounds = 0;
-    int nchunks = 0;
-    DBuf<int4> chunk;                // (group, first entry, end entry, 0)
-    DBuf<int> grp_chunk;             // [nG + 1] chunks of each group
-    DBuf<int64_t> d_grp_off;         // [nG + 1] first merged-list entry of each group
+    int nchunks = 0, nch[2] = {0, 0};
+    DBuf<int4> chunk;                // (group | parity << 24, first entry, end entry, 0)
+    DBuf<int> grp_chunk;             // [2][nG + 1] first chunk of each group per parity
+    DBuf<int64_t> d_grp_off;         // [2 nG + 1] first entry of each list
     // planning (tp_plan_structure) before the sub-tiles are filled: the stored sub-tiles'
     // (I << 16 | J, k << 4 | u), ordered by tile pair, and the first sub-tile of each row tile
     std::vector<uint2> h_sub_id;
@@ -429,7 +434,11 @@ struct BatchScen {
     const TPEntry* ent;
     float* zs;
     const int* perm;
-    int nT, gch0;           // gch0: this scenario's first entry in the batch's group-chunk table
+    double* carry;          // time-blocked convolution: the next step's part of the history sum
+    int nT, nG, gch0;       // gch0: this scenario's first entry in the batch's group-chunk table
+    // its chunks in the batch's chunk table: the first, and how many of lag parity 0 (those
+    // first) and 1
+    int ch0, nch0, nch1;
 };
 // fills the march part of d (requires a mixed-precision march_init); march_set_step stores the
 // step a batch reached
@@ -483,12 +492,16 @@ void tp_chunks(const Context& c, int target, int sc, std::vector<int4>& chunk,
 int tp_resident_groups(int nrhs, int device);
 double tp_work(const Context& c);
 int64_t tp_partial_elems(const Context& c, int nrhs);
-// k_conv_mixed: per-wave FP64 partial sums of the history at step s
+// doubles of the time-blocked convolution's carry (the part of the next step's history sum
+// convolved at this step), zero before the first step
+int64_t tp_carry_elems(const Context& c, int nrhs);
+// k_conv_mixed: per-wave FP64 partial sums of the history at step s (time-blocked: the far lags
+// of parity s mod 2 for steps s and s + 1, the near lags for step s)
 void tp_conv(Context& c, const float* zs, int nrhs, int Rring, int s, double* partial);
-// fixed-order sum of the partials, written as the step solve's right-hand side
-// Xf = f_r gF + gU + (history sum) in fragment order
-void tp_reduce_rhs(Context& c, const double* partial, int nrhs, int N, const double* gF,
-                   const double* gU, const double* fac, double* Xf);
+// fixed-order sum of the partials of step s, written as the step solve's right-hand side
+// Xf = f_r gF + gU + (history sum + carry) in fragment order; the carry takes the next step's
+void tp_reduce_rhs(Context& c, const double* partial, int nrhs, int N, int s, const double* gF,
+                   const double* gU, const double* fac, double* carry, double* Xf);
 void mfma_selftest(const float* A, const float* B, float* D);
 double stream_copy(size_t bytes, int launches);
 double stream_read(size_t bytes, int launches);

@@ -280,7 +280,9 @@ class Context:
     TIMING_KEYS = ("store_ms", "init_ms", "march_ms", "lag_kernel_ms", "gather_ms", "conv_ms",
                    "solve_ms", "lag_launches", "conv_launches", "active_points", "store_nonzero",
                    "store_padded", "tile_subtiles", "reduce_ms", "tile_blocks", "tile_entries",
-                   "active_levels")
+                   "active_levels", "pair_lags_near", "pair_lags_even", "pair_lags_odd",
+                   "subtiles_near", "subtiles_even", "subtiles_odd", "entries_parity0",
+                   "entries_parity1")
 
     def set_profiling(self, on=True, conv_only=False):
         """Event timing of the kernels (tdbem_set_profiling); conv_only times the history
