@@ -82,6 +82,9 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (3 launches) to scale HBM counters by")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--fp64-serial-every", type=int, default=2,
+                    help="plain FP64 serial baseline of the batched leg on every n-th scenario "
+                         "(the serving leg on every 2n-th request); 0 skips it")
     ap.add_argument("--no-pmc", action="store_true",
                     help="skip the two rocprofv3 --pmc passes that measure roofline.traffic")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -177,14 +180,17 @@ def _engine_summary(res, summary, probs):
         "steps_range": [int(min(p.numstep for p in probs)), int(max(p.numstep for p in probs))],
         "predicted_resident_bytes_total": int(sum(r.predicted_resident for r in ok)),
         "predicted_resident_bytes_max": int(max((r.predicted_resident for r in ok), default=0)),
+        "setup_s_total": round(sum(r.setup_s for r in ok), 2),
+        "pair_lags_total": int(sum(r.pair_lags for r in ok)),
     })
     return out
 
 
-def batch_leg(api, configs, serve, count, nrhs=16):
+def batch_leg(api, configs, serve, count, nrhs=32):
     """Uneven half-space cavity scenarios at the stated sizes (free plane 8x8 + spherical cavity of
     radius U[0.5, 1.5] at depth U[2, 4]; elements U[400, 4000], steps U[100, 800], nu U[0.2,
-    0.35], seed 4), nrhs pressure histories each, all arriving at once: the continuous-batching
+    0.35], seed 4), nrhs pressure histories each (32, as the sphere workload), all arriving at
+    once: the continuous-batching
     engine admits them by their predicted device bytes, sets them up 4 at a time while the
     admitted ones march together in one batch (hipGraph replays of 8 steps of every member), and
     lets the next ones in as finished ones leave.  Time-steps/s over the whole job, assembly
@@ -210,6 +216,40 @@ def serve_leg(api, configs, serve, nreq, rate, nrhs=32):
     out = _engine_summary(res, summary, probs)
     out.update({"arrival_rate_per_s": rate, "nrhs": nrhs})
     return out
+
+
+def fp64_serial(api, probs, every, steps=8):
+    """The plain FP64 path (one load per solve, as the reference runs) on the leg's own requests,
+    one after another on an idle device: requests 0, every, 2 every, ... each set up (FP64 lag
+    store, step-0 factorisations) and marched for 2 + `steps` steps, the per-step time of the
+    last `steps` (every step past the first streams the whole store, so it is constant) carried to
+    the request's step count.  time_steps_per_s = sum of (numstep - 1) / sum of the estimated
+    solve times."""
+    rows = []
+    for i in range(0, len(probs), every):
+        p = probs[i]
+        with api.Context.from_problem(p) as ctx:
+            t0 = time.perf_counter()
+            ctx.build_lag_store()
+            ctx.march_init(p.a1, p.a2, p.ubc_dof, p.ubc_val, p.fbc_dof, p.fbc_val)
+            ctx.synchronize()
+            t1 = time.perf_counter()
+            ctx.march_run(2)
+            ctx.synchronize()
+            t2 = time.perf_counter()
+            n = min(steps, p.numstep - 3)
+            ctx.march_run(n)
+            ctx.synchronize()
+            t3 = time.perf_counter()
+        step = (t3 - t2) / max(n, 1)
+        rows.append({"request": i, "elements": int(p.nel), "steps": int(p.numstep),
+                     "setup_s": round(t1 - t0, 3), "step_ms": round(step * 1e3, 3),
+                     "solve_s": round((t2 - t0) + (p.numstep - 3) * step, 3)})
+    total = sum(r["solve_s"] for r in rows)
+    tsteps = sum(r["steps"] - 1 for r in rows)
+    return {"requests_measured": len(rows), "every": every, "steps_timed": steps,
+            "time_steps_per_s": round(tsteps / total, 2) if total > 0 else 0.0,
+            "serial_solve_s": round(total, 2), "per_request": rows}
 
 
 def traffic_probe(nrhs):
@@ -458,15 +498,23 @@ def main():
     if not args.no_batch:
         log(f"batched leg: {args.batch_scenarios} half-space scenarios at the stated sizes")
         out["batched"] = batch_leg(api, configs, serve, args.batch_scenarios)
-        if "plain_fp64" in out:
-            out["batched"]["speedup_vs_plain_fp64"] = round(
-                out["batched"]["time_steps_per_s"] / out["plain_fp64"]["value"], 2)
+        if args.fp64_serial_every > 0:
+            log("plain FP64 serial solves of the batched leg's scenarios")
+            scen = configs.halfspace_scenarios(count=args.batch_scenarios, seed=4)
+            ser = fp64_serial(api, scen, args.fp64_serial_every)
+            out["batched"]["plain_fp64_serial"] = ser
+            out["batched"]["speedup_vs_plain_fp64_serial"] = round(
+                out["batched"]["time_steps_per_s"] / ser["time_steps_per_s"], 2)
     if not args.no_serve:
         log(f"serving leg: {args.serve_requests} requests at {args.serve_rate}/s")
         out["serving"] = serve_leg(api, configs, serve, args.serve_requests, args.serve_rate)
-        if "plain_fp64" in out:
-            out["serving"]["speedup_vs_plain_fp64"] = round(
-                out["serving"]["time_steps_per_s"] / out["plain_fp64"]["value"], 2)
+        if args.fp64_serial_every > 0:
+            log("plain FP64 serial solves of the serving leg's requests")
+            ser = fp64_serial(api, configs.serving_mix(args.serve_requests, seed=5),
+                              2 * args.fp64_serial_every)
+            out["serving"]["plain_fp64_serial"] = ser
+            out["serving"]["speedup_vs_plain_fp64_serial"] = round(
+                out["serving"]["time_steps_per_s"] / ser["time_steps_per_s"], 2)
     if args.copy_calibration:
         log("copy calibration kernel")
         out["copy_calibration"] = copy_calibration(api)

@@ -54,6 +54,8 @@ class Result:
     predicted_resident: int = 0
     nel: int = 0
     numstep: int = 0
+    pair_lags: int = 0         # in-shell (row node, column node, lag) entries of its store
+    setup_s: float = 0.0       # set-up wall time (lag and tile store, step-0 solve)
     error: str = ""
     histories: list = field(default_factory=list)
 
@@ -76,11 +78,13 @@ def _predict(req, nrhs):
 
 
 def _setup(req, ctx):
+    """Set-up of an admitted request on its context; returns its wall time."""
     p = req.problem
+    t0 = time.perf_counter()
     ctx.build_lag_store()
     ctx.march_init(p.a1, p.a2, p.ubc_dof, p.ubc_val, p.fbc_dof, p.fbc_val, amp=req.amp,
                    t_rise=req.rise)
-    return ctx
+    return time.perf_counter() - t0
 
 
 class Engine:
@@ -184,7 +188,8 @@ class Engine:
                 for f in [f for f in setups if f.done()]:
                     req, ctx, peak = setups.pop(f)
                     try:
-                        f.result()
+                        results[req.rid].setup_s = f.result()
+                        results[req.rid].pair_lags = int(ctx.timings()["store_nonzero"])
                         batch.add(ctx)
                     except Exception as e:  # noqa: BLE001
                         used -= peak

@@ -137,3 +137,33 @@ def test_concurrent_set_up_matches_sequential(gpu):
     finally:
         for c in ctxs:
             c.close()
+
+
+def test_batch_members_joining_at_either_step_parity(gpu):
+    """The mixed convolution is time-blocked: at step s it convolves the far lags of parity
+    s mod 2 for steps s and s + 1 and carries the step-(s + 1) part to the next step.  Members
+    that marched alone for an even or an odd number of steps before joining (so the batch holds
+    both parities at once), and that leave and march alone again, reproduce their march alone,
+    bit for bit at the single-context tunables (rounds 4, 4096 solve wavefronts)."""
+    probs = _scenarios()[:4]
+    ref = _alone(probs)
+    ctxs = [_start(p, 100 + i) for i, p in enumerate(probs)]
+    print("[gpu case] batch: members joining at steps of both parities", flush=True)
+    for i, ctx in enumerate(ctxs):
+        ctx.march_run(3 + i)            # steps 1 .. 3 + i alone: parities 1, 0, 1, 0
+    with api.Batch(ctxs[:2]) as b:
+        b.set_params(4, 4096)
+        b.run(steps_per_graph=5, max_replays=1)
+        b.add(ctxs[2])
+        b.add(ctxs[3])
+        b.run(steps_per_graph=3, max_replays=2)
+        b.remove(ctxs[0])
+        b.run(steps_per_graph=4, max_replays=1)
+    for ctx, rf, p in zip(ctxs, ref, probs):
+        ctx.march_run()                 # the rest alone
+        assert ctx.step_index == p.numstep - 1
+        for r in range(NRHS):
+            U, T = ctx.history(r)
+            np.testing.assert_array_equal(U, rf[r][0])
+            np.testing.assert_array_equal(T, rf[r][1])
+        ctx.close()
