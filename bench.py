@@ -18,7 +18,7 @@ is 3.
 
 roofline: the mixed path's dominant kernel k_conv_mixed, HBM-bound: algorithmic bytes per launch =
 the in-shell node-pair lags x 18 FP32 values (G's 9 and H's 9 of each 3 x 3 node block) + the
-merged-list entries x 32 B + the FP32 history copy read once, over its mean duration from HIP
+merged-list entries x 32 B + the history copy (split bf16, 6 B a value) read once, over its mean duration from HIP
 events on the context stream; stored_bytes_per_launch counts whole stored 16 x 32 sub-tiles
 instead.  traffic: HBM bytes per launch measured in this run by two rocprofv3 --pmc passes
 (FETCH_SIZE, WRITE_SIZE) over a short run of the same workload (--traffic-probe), each scaled
@@ -363,15 +363,15 @@ def main():
             _, tm_all = timed_solves(ctx, p, 0, 1, amp, rise)
         nconv = max(tm["conv_launches"], 1)
         conv_ms = tm["conv_ms"] / nconv
-        # per launch (time-blocked at R <= 48: the near lags k < 3 for step s and the far lags of
+        # per launch (time-blocked: the near lags k < 3 for step s and the far lags of
         # parity s mod 2 for steps s and s + 1, so a launch streams the near lags and half the
         # far ones; the mean of the two parities), the algorithmic bytes: the in-shell node-pair
         # lags it reads (18 FP32 values each, G's 9 and H's 9, as the 48 x 96 blocks hold them),
-        # its merged lists, and the FP32 history copy read once ((K + 1) slots x 96 columns per
-        # tile x R x 4 B)
-        blocked = R <= 48
+        # its merged lists, and the history copy read once ((K + 1) slots x 96 columns per
+        # tile x R x 6 B, three bf16 terms a value)
+        blocked = True
         nT = -(-p.nnd // 16)
-        hist_bytes = (K + 1) * nT * 96 * R * 4
+        hist_bytes = (K + 1) * nT * 96 * R * 6
         if blocked:
             lags_launch = tm["pair_lags_near"] + 0.5 * (tm["pair_lags_even"] + tm["pair_lags_odd"])
             subs_launch = tm["subtiles_near"] + 0.5 * (tm["subtiles_even"] + tm["subtiles_odd"])

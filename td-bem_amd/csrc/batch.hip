@@ -74,7 +74,8 @@ __global__ void k_b_zcopy(const BatchScen* __restrict__ D, const int* __restrict
     const int m = (int)(t / per);
     int slot = (sdev[sc] - 2 + m) % d.R;
     if (slot < 0) slot += d.R;
-    zcopy_elem(d.ring, d.N, d.nrhs, d.nT, d.perm, slot, d.nrhs / 16, d.zs, t - m * per);
+    zcopy_elem(d.ring, d.N, d.nrhs, d.nT, d.perm, slot, d.nrhs / 16,
+               reinterpret_cast<unsigned short*>(d.zs), t - m * per);
 }
 
 // One workgroup per (chunk slot, right-hand-side tile), one wavefront per row tile of the
@@ -100,7 +101,7 @@ __global__ void __launch_bounds__(64 * kGroup, 2)
     const int w = threadIdx.x >> 6;
     const int g = cw.x & ((1 << kChunkParityShift) - 1);
     double* out = partial + ((int64_t)c * 2 * kGroup + w) * 3 * NT * 256;
-    conv_chunk(reinterpret_cast<const f32x4*>(d.tp), d.ent, reinterpret_cast<const f32x4*>(d.zs),
+    conv_chunk(reinterpret_cast<const f32x4*>(d.tp), d.ent, reinterpret_cast<const bf16x8*>(d.zs),
                d.nT, NT, nt0, d.R, s, cw.y, cw.z, w, g * kGroup + w < d.nT, out,
                out + (int64_t)kGroup * 3 * NT * 256);
 }

@@ -42,7 +42,7 @@ struct March {
     DBuf<double> fac;       // [nrhs] load factor of the current step
     DBuf<double> d_amp, d_rise;
     int precision = 0;             // 1: split-BF16 MFMA convolution
-    DBuf<float> zs;                // FP32 mirror of the ring in MFMA operand order (mixed)
+    DBuf<float> zs;                // the ring as split bf16 terms in MFMA operand order (mixed)
     DBuf<double> partial;          // per-wave partial sums of the mixed convolution
     DBuf<double> carry;            // time-blocked: the next step's history part, summed
     int64_t zslot = 0;
@@ -613,7 +613,7 @@ void march_run(Context& c, int nsteps, bool wait)
                                M.fac.p, N, M.Np, M.nrhs, M.parts.p);
         } else {
             if (mixed) {
-                // per step: FP32 history copy + load factors, convolution, reduction into the
+                // per step: split-bf16 history copy + load factors, convolution, reduction into the
                 // right-hand side, step solve, finalize
                 tp_prepare_step(c, M.ring.p, N, M.nrhs, s, M.R, M.zs.p, M.d_amp.p, M.d_rise.p,
                                 s * c.dt, M.fac.p);

@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(64 * kGroup)
 // previous step's finalize changed), and the load factors f_r(t) of the step in block 0.
 __global__ void k_zcopy3(const double* __restrict__ ring, int N, int nrhs, int nT,
                          const int* __restrict__ perm, int s, int R, int NT,
-                         float* __restrict__ zs, const double* __restrict__ amp,
+                         unsigned short* __restrict__ zs, const double* __restrict__ amp,
                          const double* __restrict__ rise, double t_load,
                          double* __restrict__ fac)
 {
@@ -179,7 +179,7 @@ __global__ void k_zcopy3(const double* __restrict__ ring, int N, int nrhs, int n
 // group (conv_chunk).  A launch runs the chunks of the lag parity of step s.
 __global__ void __launch_bounds__(64 * kGroup, 2)
     k_conv_mixed(const f32x4* __restrict__ tp, const TPEntry* __restrict__ ent,
-                 const int4* __restrict__ chunk, int base, const f32x4* __restrict__ zs, int nT,
+                 const int4* __restrict__ chunk, int base, const bf16x8* __restrict__ zs, int nT,
                  int NT, int Rring, int s, double* __restrict__ partial)
 {
     const int L = xcd_chunk(blockIdx.x, gridDim.x);
@@ -562,11 +562,14 @@ void tp_prepare_step(Context& c, const double* ring, int N, int nrhs, int s, int
     const TPStore& T = c.tp;
     const int64_t n = 3LL * T.nT * kCols * nrhs;
     hipLaunchKernelGGL(k_zcopy3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, ring,
-                       N, nrhs, T.nT, T.d_perm.p, s, R, nrhs / 16, zs, amp, rise, t_load, fac);
+                       N, nrhs, T.nT, T.d_perm.p, s, R, nrhs / 16,
+                       reinterpret_cast<unsigned short*>(zs), amp, rise, t_load, fac);
     TD_HIP(hipGetLastError());
 }
 
-int64_t tp_zslot_elems(const Context& c, int nrhs) { return (int64_t)c.tp.nT * 3 * (nrhs / 16) * 512; }
+// FP32-sized words per ring slot of the history copy: per (column tile, k-tile, rhs tile) three
+// bf16 terms of 64 lanes x 8
+int64_t tp_zslot_elems(const Context& c, int nrhs) { return (int64_t)c.tp.nT * 3 * (nrhs / 16) * 768; }
 
 int tp_resident_groups(int nrhs, int device)
 {
@@ -664,7 +667,7 @@ void tp_conv(Context& c, const float* zs, int nrhs, int Rring, int s, double* pa
     if (n == 0) return;
     hipLaunchKernelGGL(k_conv_mixed, dim3((unsigned)(n * NT)), dim3(64 * kGroup), 0, c.stream,
                        reinterpret_cast<const f32x4*>(T.vals.p), T.ent.p, T.chunk.p, base,
-                       reinterpret_cast<const f32x4*>(zs), T.nT, NT, Rring, s, partial);
+                       reinterpret_cast<const bf16x8*>(zs), T.nT, NT, Rring, s, partial);
     TD_HIP(hipGetLastError());
 }
 

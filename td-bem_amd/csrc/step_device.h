@@ -60,13 +60,14 @@ __device__ __forceinline__ void split_f32x8(const f32x4& a0, const f32x4& a1, bf
     }
 }
 
-// Element t (< nT * 96 * nrhs) of the FP32 copy of ring slot `slot`, in B-operand order
-// (lane = rhs + 16 * (k / 8), element k % 8), laid out [slot][J][k-tile][rhs tile][2][64 lanes][4]
-// (elements 0-3 of all lanes, then 4-7).  The ring holds [slot][part][N][rhs] with part 0 = u,
-// part 1 = t; block column part 0 multiplies z^t.
+// Element t (< nT * 96 * nrhs) of the copy of ring slot `slot`, rounded to FP32 and split
+// exactly into three bf16 terms as split_f32x8 splits a value (once per step here, not in every
+// workgroup that stages it), in B-operand order (lane = rhs + 16 * (k / 8), element k % 8),
+// laid out [slot][J][k-tile][rhs tile][term][64 lanes][8].  The ring holds [slot][part][N][rhs]
+// with part 0 = u, part 1 = t; block column part 0 multiplies z^t.
 __device__ __forceinline__ void zcopy_elem(const double* __restrict__ ring, int N, int nrhs,
                                            int nT, const int* __restrict__ perm, int slot,
-                                           int NT, float* __restrict__ zs, int64_t t)
+                                           int NT, unsigned short* __restrict__ zs, int64_t t)
 {
     const int r = (int)(t % nrhs);
     const int c = (int)((t / nrhs) % kCols);
@@ -78,8 +79,15 @@ __device__ __forceinline__ void zcopy_elem(const double* __restrict__ ring, int 
         node < 0 ? 0.0 : ring[(((int64_t)slot * 2 + rpart) * N + 3 * node + q) * nrhs + r];
     const int kt = c >> 5, kk = c & 31, nt = r >> 4, n = r & 15;
     const int lane = n + 16 * (kk >> 3), e = kk & 7;
-    zs[((((int64_t)slot * nT + J) * 3 + kt) * NT + nt) * 512 + (e >> 2) * 256 + lane * 4 + (e & 3)] =
-        (float)v;
+    const float x = (float)v;
+    const unsigned xb = __float_as_uint(x);
+    const float r1 = x - __uint_as_float(xb & 0xFFFF0000u);
+    const unsigned rb = __float_as_uint(r1);
+    const float r2 = r1 - __uint_as_float(rb & 0xFFFF0000u);
+    const int64_t base = ((((int64_t)slot * nT + J) * 3 + kt) * NT + nt) * 3;
+    zs[((base + 0) * 64 + lane) * 8 + e] = (unsigned short)(xb >> 16);
+    zs[((base + 1) * 64 + lane) * 8 + e] = (unsigned short)(rb >> 16);
+    zs[((base + 2) * 64 + lane) * 8 + e] = (unsigned short)(__float_as_uint(r2) >> 16);
 }
 
 // Lags below kNearLags are the near lags of the two-step (time-blocked) convolution: their
@@ -104,14 +112,14 @@ constexpr int kNearLags = 3;
 // its products, which serve both targets.
 __device__ __forceinline__ void conv_chunk(const f32x4* __restrict__ tp,
                                            const TPEntry* __restrict__ ent,
-                                           const f32x4* __restrict__ zs, int nT, int NT, int nt0,
+                                           const bf16x8* __restrict__ zs, int nT, int NT, int nt0,
                                            int Rring, int s, int64_t e0, int64_t e1, int w,
                                            bool write, double* __restrict__ out0,
                                            double* __restrict__ out1)
 {
-    // one (slot, column tile) history block: in HBM, [k-tile][rhs tile] units of 64 lanes x 8
-    // FP32; in LDS (the workgroup's rhs tile), split exactly into [term][k-tile] lines of
-    // 64 x 16 B (bf16x8), one block per target
+    // one (slot, column tile) history block: in HBM, [k-tile][rhs tile][term] lines of 64 lanes
+    // x 8 bf16 (zcopy_elem); in LDS (the workgroup's rhs tile), [term][k-tile] lines, one block
+    // per target
     constexpr int kUnits = 3;                              // per target: k-tiles
     constexpr int kPer = (2 * kUnits + kGroup - 1) / kGroup;  // units per wavefront
     __shared__ bf16x8 zl[2][2][9 * 64];
@@ -119,9 +127,10 @@ __device__ __forceinline__ void conv_chunk(const f32x4* __restrict__ tp,
     // instructions even where the pointers come from a descriptor table, so they count against
     // the vector-memory counter only and an LDS wait does not wait for them
     typedef const __attribute__((address_space(1))) f32x4 gf32x4;
+    typedef const __attribute__((address_space(1))) bf16x8 gbf16x8;
     typedef const __attribute__((address_space(1))) TPEntry gentry;
     gf32x4* tpg = (gf32x4*)tp;
-    gf32x4* zsg = (gf32x4*)zs;
+    gbf16x8* zsg = (gbf16x8*)zs;
     gentry* entg = (gentry*)ent;
     const int lane = threadIdx.x & 63;
     // w is the same in every lane of a wavefront: the entry fields are scalar loads
@@ -135,8 +144,8 @@ __device__ __forceinline__ void conv_chunk(const f32x4* __restrict__ tp,
             for (int v = 0; v < 4; v++) acc[t][m][v] = 0.0;
     // wavefront w stages units w, w + 4 of an entry's blocks; unit U = target * 3 + k-tile; the
     // target-(s + 1) block of a near lag is not staged
-    f32x4 zn[kPer][2];
-    const int64_t zblock = 3LL * NT * 128;   // f32x4 per (slot, column tile)
+    bf16x8 zn[kPer][3];
+    const int64_t zblock = 3LL * NT * 3 * 64;   // bf16x8 per (slot, column tile)
     auto zload = [&](uint32_t id, unsigned kmask) {
         const int J = (int)(id >> kKBits), k = (int)(id & kMaxLag);
 #pragma unroll
@@ -146,10 +155,11 @@ __device__ __forceinline__ void conv_chunk(const f32x4* __restrict__ tp,
             if (U < 2 * kUnits && ((kmask >> kt) & 1u) && !(t == 1 && k < kNearLags)) {
                 int slot = (s + t - k) % Rring;
                 if (slot < 0) slot += Rring;
-                gf32x4* Zg = zsg + ((int64_t)slot * nT + J) * zblock +
-                             (int64_t)(kt * NT + nt0) * 128 + lane;
+                gbf16x8* Zg = zsg + ((int64_t)slot * nT + J) * zblock +
+                              (int64_t)(kt * NT + nt0) * 3 * 64 + lane;
                 zn[q][0] = Zg[0];
                 zn[q][1] = Zg[64];
+                zn[q][2] = Zg[128];
             }
         }
     };
@@ -160,11 +170,9 @@ __device__ __forceinline__ void conv_chunk(const f32x4* __restrict__ tp,
             const int U = q * kGroup + w;
             const int t = U / kUnits, kt = U % kUnits;
             if (U < 2 * kUnits && ((kmask >> kt) & 1u) && !(t == 1 && k < kNearLags)) {
-                bf16x8 t0, t1, t2;
-                split_f32x8(zn[q][0], zn[q][1], t0, t1, t2);
-                zl[buf][t][(0 * 3 + kt) * 64 + lane] = t0;
-                zl[buf][t][(1 * 3 + kt) * 64 + lane] = t1;
-                zl[buf][t][(2 * 3 + kt) * 64 + lane] = t2;
+#pragma unroll
+                for (int term = 0; term < 3; term++)
+                    zl[buf][t][(term * 3 + kt) * 64 + lane] = zn[q][term];
             }
         }
     };

@@ -432,7 +432,7 @@ struct BatchScen {
     // mixed-precision store (mixed.hip)
     const void* tp;
     const TPEntry* ent;
-    float* zs;
+    float* zs;              // the ring as split bf16 terms (zcopy_elem)
     const int* perm;
     double* carry;          // time-blocked convolution: the next step's part of the history sum
     int nT, nG, gch0;       // gch0: this scenario's first entry in the batch's group-chunk table
