@@ -156,6 +156,137 @@ __device__ __forceinline__ int eval_s(const TimeCombo& tc, double r, double dt, 
     return levels;
 }
 
+// The lag combination M_k = phi_1(k) + phi_2(k + 1) of the lag store (src/SolverStepLinearT.cpp:
+// 65-97, each phi the Step kernel HL(t) - HL(t - dt)): per point
+//     S = I(t_k) - I(t_{k-1}) + cJ (J(t_{k+1}) - J(t_k)),   t_l = l dt,
+// cJ = 0 at the last lag k = Kc.  It is the TimeCombo (-1, 0 | 1, -cJ | 0, cJ) with the I
+// coefficients left out: a level contributes only after the P-wave arrival t > r/c_p > 0, and at
+// k = 0 neither t_{k-1} = -dt nor t_k = 0 passes that test.
+struct LagCombo {
+    double t0, t1, t2;   // (k - 1) dt, k dt, (k + 1) dt
+    double cJ;           // 1, or 0 at k = Kc
+};
+
+// r = sqrt(rr) and ir = 1/r from the hardware reciprocal square root (v_rsq_f64, about 23 good
+// bits) by two Goldschmidt iterations on (g, h) -> (sqrt(rr), 1/(2 sqrt(rr))) and one Newton
+// correction of the root (Markstein, "IA-64 and Elementary Functions", 2000): r to within an
+// ulp of the rounded root, ir to a few ulps, in 12 instructions instead of a rounded square root
+// and a divide.  rr is a squared distance between distinct points (normal, positive).
+__device__ __forceinline__ void sqrt_and_recip(double rr, double& r, double& ir)
+{
+    const double y = __builtin_amdgcn_rsq(rr);
+    double g = rr * y, h = 0.5 * y;
+    double e = fma(-h, g, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    e = fma(-h, g, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    r = fma(fma(-g, g, rr), h, g);
+    ir = h + h;
+}
+
+// One wave's hat integrals at one time level in units of dt (hat_integrals x dt; the caller
+// scales the level sum by 1/dt once): the I hat (phi_1) when WANT_I, the J hat (phi_2) when
+// WANT_J.  t_arr = r/c of the wave, as the caller compared it; TREF_POS: the caller knows
+// t > t_arr, so I5 needs no test for t - r/c < 0.
+template <bool WANT_I, bool WANT_J, bool TREF_POS>
+__device__ __forceinline__ void lag_hat(double t, double dt, double t_arr, double inv_c2,
+                                        double ir2_12, double& I1, double& I3, double& I5,
+                                        double& J1, double& J3, double& J5)
+{
+    const double t_ref = t - t_arr;
+    const double b = fmin(fmax(t_ref, 0.0), dt);
+    const double b2 = b * b;
+    const double hb2 = 0.5 * b2;
+    const double t2 = t * t;
+    if constexpr (WANT_I) {
+        const double pI = b * (12.0 * dt * t2 + b * (-(6.0 * t2 + 12.0 * t * dt) +
+                                                     b * ((8.0 * t + 4.0 * dt) - 3.0 * b)));
+        const double q = dt * b - hb2;
+        I1 = pI * ir2_12 - q * inv_c2;
+        I3 = q;
+        I5 = (TREF_POS || t_ref >= 0.0) ? dt - b : 0.0;
+    }
+    if constexpr (WANT_J) {
+        const double pJ = b2 * (-6.0 * t2 + b * (8.0 * t - 3.0 * b));
+        J1 = -pJ * ir2_12 - hb2 * inv_c2;
+        J3 = hb2;
+        J5 = t_ref < dt ? b : 0.0;
+    }
+}
+
+// S of the lag combination at one point (as eval_s, whose zero tests it restates for this
+// combination): levels t_{k+1} (J only), t_k (I and J), t_{k-1} (I only), each formed for the P
+// and the S wave.  Returns the number of levels evaluated.
+__device__ __forceinline__ int eval_s_lag(const LagCombo& lc, double r, double ir, double dt,
+                                          double inv_dt, const Material& m, double* S)
+{
+    const double ta = r * m.inv_cp;        // P-wave arrival
+    const double tsa = r * m.inv_cs;       // S-wave arrival
+    const bool any = lc.cJ != 0.0 ? lc.t2 > ta : lc.t1 > ta;
+    if (!any || lc.t0 >= tsa + dt) return 0;
+    const double ir2_12 = ir * ir * (1.0 / 12.0);
+    const double icp2 = m.inv_cp2, ics2 = m.inv_cs2;
+    double x1, x3, x5, y1, y3, y5;   // P wave
+    double u1, u3, u5, v1, v3, v5;   // S wave
+#pragma unroll
+    for (int l = 0; l < 6; l++) S[l] = 0.0;
+    int levels = 0;
+    if (lc.cJ != 0.0 && lc.t2 > ta) {
+        lag_hat<false, true, true>(lc.t2, dt, ta, icp2, ir2_12, x1, x3, x5, y1, y3, y5);
+        lag_hat<false, true, false>(lc.t2, dt, tsa, ics2, ir2_12, u1, u3, u5, v1, v3, v5);
+        S[0] = y1;
+        S[1] = v1;
+        S[2] = y3;
+        S[3] = v3;
+        S[4] = y5;
+        S[5] = v5;
+        levels++;
+    }
+    if (lc.t1 > ta) {
+        const double cJ = lc.cJ;
+        lag_hat<true, true, true>(lc.t1, dt, ta, icp2, ir2_12, x1, x3, x5, y1, y3, y5);
+        lag_hat<true, true, false>(lc.t1, dt, tsa, ics2, ir2_12, u1, u3, u5, v1, v3, v5);
+        S[0] += x1 - cJ * y1;
+        S[1] += u1 - cJ * v1;
+        S[2] += x3 - cJ * y3;
+        S[3] += u3 - cJ * v3;
+        S[4] += x5 - cJ * y5;
+        S[5] += u5 - cJ * v5;
+        levels++;
+    }
+    if (lc.t0 > ta) {
+        lag_hat<true, false, true>(lc.t0, dt, ta, icp2, ir2_12, x1, x3, x5, y1, y3, y5);
+        lag_hat<true, false, false>(lc.t0, dt, tsa, ics2, ir2_12, u1, u3, u5, v1, v3, v5);
+        S[0] -= x1;
+        S[1] -= u1;
+        S[2] -= x3;
+        S[3] -= u3;
+        S[4] -= x5;
+        S[5] -= u5;
+        levels++;
+    }
+#pragma unroll
+    for (int l = 0; l < 6; l++) S[l] *= inv_dt;
+    return levels;
+}
+
+// Distance, reciprocal and S of one point for either combination.
+__device__ __forceinline__ int point_s(const TimeCombo& tc, double rr, double dt, double inv_dt,
+                                       const Material& m, double* S, double& r, double& ir)
+{
+    r = sqrt(rr);
+    return eval_s(tc, r, dt, inv_dt, m, S, ir);
+}
+
+__device__ __forceinline__ int point_s(const LagCombo& lc, double rr, double dt, double inv_dt,
+                                       const Material& m, double* S, double& r, double& ir)
+{
+    sqrt_and_recip(rr, r, ir);
+    return eval_s_lag(lc, r, ir, dt, inv_dt, m, S);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Geometry table: shape functions, positions, unit normals and w*J at the regular Gauss points.
 // ---------------------------------------------------------------------------------------------
@@ -342,9 +473,9 @@ struct NoPointExtra {
 // rule for sing < 0, Duffy triangles / sub-elements for a singular node), accumulated into
 // out[ENND][15].  extra(x, n, wJ, acc) runs at every point after its blocks are added.  Returns
 // (points with a nonzero time combination, time levels evaluated at them).
-template <int ENND, typename Points, typename Extra = NoPointExtra>
+template <int ENND, typename Points, typename Combo, typename Extra = NoPointExtra>
 __device__ __forceinline__ int2 integrate_points(Points&& points, const double* coll,
-                                                 const TimeCombo& tc, double dt,
+                                                 const Combo& tc, double dt,
                                                  const Material& m, double* out, int64_t ostride,
                                                  Extra&& extra = Extra{})
 {
@@ -357,9 +488,9 @@ __device__ __forceinline__ int2 integrate_points(Points&& points, const double* 
         for (int v = 0; v < kVals; v++) acc[a][v] = 0.0;
     points([&](const double* x, const double* n, double wJ, const double* N) {
         double d[3] = {x[0] - coll[0], x[1] - coll[1], x[2] - coll[2]};
-        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        double S[6], ir;
-        const int nl = eval_s(tc, r, dt, inv_dt, m, S, ir);
+        double S[6], r, ir;
+        const int nl = point_s(tc, d[0] * d[0] + d[1] * d[1] + d[2] * d[2], dt, inv_dt, m, S,
+                               r, ir);
         if (nl) {
             active++;
             levels += nl;
@@ -387,10 +518,10 @@ __device__ __forceinline__ int2 integrate_points(Points&& points, const double* 
 // Value v of node a goes to out[(a * 15 + v) * ostride].  REGULAR: the caller guarantees that
 // no node of e lies at the collocation point, so the singular rules are not compiled in (the
 // regular path then fits two waves per SIMD).
-template <int ENND, bool REGULAR = false>
+template <int ENND, bool REGULAR = false, typename Combo = TimeCombo>
 __device__ __forceinline__ int2 integrate_pair(const double* coords, const int* ien,
                                               const double* geo, const double* gauss, int nq,
-                                              int i, int e, const TimeCombo& tc, double dt,
+                                              int i, int e, const Combo& tc, double dt,
                                               const Material& m, double* out, int64_t ostride = 1)
 {
     const double coll[3] = {coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]};
@@ -554,19 +685,12 @@ __global__ void __launch_bounds__(kBlock, (ENND == 4 && !SINGULAR) ? 2 : 1)
     // rows: the batch's row nodes when they are not the contiguous range i0..
     const int i = rows ? rows[p / nel] : i0 + p / nel, e = p % nel;
     const int k = pair_k[(int64_t)i * nel + e].x + (int)(s - pair_slot[p]);
-    TimeCombo tc;
-    tc.t[0] = (k - 1) * dt;
-    tc.t[1] = k * dt;
-    tc.t[2] = (k + 1) * dt;
-    const double hasI = k >= 1 ? 1.0 : 0.0, hasJ = (k + 1 <= Kc) ? 1.0 : 0.0;
-    tc.cI[0] = -hasI;
-    tc.cJ[0] = 0.0;
-    tc.cI[1] = hasI;
-    tc.cJ[1] = -hasJ;
-    tc.cI[2] = 0.0;
-    tc.cJ[2] = hasJ;
-    tc.static_ok = true;
-    const int2 n = integrate_pair<ENND, !SINGULAR>(coords, ien, geo, gauss, nq, i, e, tc, dt, m,
+    LagCombo lc;
+    lc.t0 = (k - 1) * dt;
+    lc.t1 = k * dt;
+    lc.t2 = (k + 1) * dt;
+    lc.cJ = (k + 1 <= Kc) ? 1.0 : 0.0;
+    const int2 n = integrate_pair<ENND, !SINGULAR>(coords, ien, geo, gauss, nq, i, e, lc, dt, m,
                                                    scratch + s, nslots);
     if (counter) {
         atomicAdd(counter, (unsigned long long)n.x);
