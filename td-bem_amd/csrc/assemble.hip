@@ -437,6 +437,50 @@ __global__ void k_fill_idx(const int2* __restrict__ hull, const int64_t* __restr
     for (int k = h.x; k <= h.y; k++) idx[o + (k - h.x)] = ((uint32_t)j << kKBits) | (uint32_t)k;
 }
 
+// Slot list of one row batch of the lag kernel: slot pair_slot[p] + j, j below the in-shell lag
+// count of pair p = (row r, element e), belongs to p.  rows: the batch's row nodes (-1: a
+// padding row, no slots) when they are not i0, i0 + 1, ...  One thread per pair; each wavefront
+// then writes its 64 pairs' runs one pair at a time with all lanes, so the stores are
+// contiguous.  The blocks have kBlock threads, whole wavefronts.
+__global__ void __launch_bounds__(kBlock)
+    k_slot_fill(const int64_t* __restrict__ pair_slot, const int2* __restrict__ pair_k,
+                const int* __restrict__ rows, int i0, int nel, int64_t npairs,
+                int* __restrict__ slot_pair)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = (int)(threadIdx.x & 63);
+    int cnt = 0;
+    int64_t off = 0;
+    if (p < npairs) {
+        const int i = rows ? rows[p / nel] : i0 + (int)(p / nel);
+        if (i >= 0) {
+            const int2 k = pair_k[(int64_t)i * nel + (int)(p % nel)];
+            cnt = k.y >= k.x ? k.y - k.x + 1 : 0;
+            off = pair_slot[p];
+        }
+    }
+    const int64_t first = p - lane;   // the wavefront's first pair
+    for (int l = 0; l < 64; l++) {
+        const int c = __builtin_amdgcn_readlane(cnt, l);
+        if (c == 0) continue;
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)off, l);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(off >> 32), l);
+        const int64_t o = (int64_t)(((uint64_t)hi << 32) | lo);
+        for (int j = lane; j < c; j += 64) slot_pair[o + j] = (int)(first + l);
+    }
+}
+
+// Slots of the pairs with a node at the collocation point: complemented, so that the regular
+// lag kernel skips them and the singular one takes them.
+__global__ void k_slot_mark(const int64_t* __restrict__ sing_slots, int64_t nsing,
+                            int* __restrict__ slot_pair)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nsing) return;
+    const int64_t s = sing_slots[t];
+    slot_pair[s] = ~slot_pair[s];
+}
+
 // ---------------------------------------------------------------------------------------------
 // Element-pair integration into scratch slots.  Mode LAG: slot = (pair, k) gives
 // M_k = [k>=1](I(t_k) - I(t_{k-1})) + [k+1<=Kc](J(t_{k+1}) - J(t_k)).  Mode STEP: slot =
@@ -1319,11 +1363,13 @@ void slot_pass(Context& c, const std::vector<int>& order, int align, const int2*
     slot_pair.reserve((size_t)max_slots);
     pair_slot.reserve((size_t)max_rows * c.nel);
     if (!ident) d_rows.reserve((size_t)max_rows);
-    // The host builds batch b + 1's slot lists while the device works on batch b: two host list
-    // sets, each reused only after the event that follows its upload; the device buffers are
-    // rewritten in stream order after the previous batch's kernels.
-    std::vector<int64_t> h_pair_slot[2], h_sing[2];
-    std::vector<int> h_slot_pair[2], h_rows[2];
+    // The host builds batch b + 1's pair offsets while the device works on batch b: two pinned
+    // host sets, each reused only after the event that follows its upload; the device buffers are
+    // rewritten in stream order after the previous batch's kernels.  The slot list itself is
+    // expanded on the device (k_slot_fill) from the pairs' first slots.
+    HostBuf<int64_t> h_pair_slot[2];
+    std::vector<int64_t> h_sing[2];
+    std::vector<int> h_rows[2];
     DBuf<int64_t> sing_slots(c.stream);
     hipEvent_t uploaded[2];
     TD_HIP(hipEventCreateWithFlags(&uploaded[0], hipEventDisableTiming));
@@ -1335,26 +1381,27 @@ void slot_pass(Context& c, const std::vector<int>& order, int align, const int2*
         const int nb = b1 - b0;
         if (b >= 2) TD_HIP(hipEventSynchronize(uploaded[set]));
         auto& hp = h_pair_slot[set];
-        auto& hs = h_slot_pair[set];
         auto& hr = h_rows[set];
-        hp.assign((size_t)nb * c.nel, 0);
-        hs.resize((size_t)slots);
+        hp.reserve((size_t)max_rows * c.nel);
         hr.resize(nb);
         std::vector<int64_t> row_start(nb + 1, 0);
         for (int r = 0; r < nb; r++) {
             row_start[r + 1] = row_start[r] + rs[b0 + r];
             hr[r] = node_at(b0 + r);
         }
-        auto& hsing = h_sing[set];
+        // slots of the pairs with a node at the collocation point, per row
+        std::vector<std::vector<int64_t>> near_slots(nb);
         parallel_for(nb, [&](int64_t lo, int64_t hi) {
             for (int64_t r = lo; r < hi; r++) {
                 const int i = hr[r];
-                if (i < 0) continue;
                 int64_t sl = row_start[r];
+                if (i < 0) {
+                    for (int e = 0; e < c.nel; e++) hp[r * c.nel + e] = sl;
+                    continue;
+                }
                 const double* xc = &c.h_coords[3 * (int64_t)i];
                 for (int e = 0; e < c.nel; e++) {
-                    const int64_t p = r * c.nel + e;
-                    hp[p] = sl;
+                    hp[r * c.nel + e] = sl;
                     const int2 k = hpk[(int64_t)i * c.nel + e];
                     if (k.y < k.x) continue;
                     // the device's singular test is a squared distance below 1e-9; a pair goes
@@ -1365,18 +1412,33 @@ void slot_pass(Context& c, const std::vector<int>& order, int align, const int2*
                         const double dx = xa[0] - xc[0], dy = xa[1] - xc[1], dz = xa[2] - xc[2];
                         near = near || dx * dx + dy * dy + dz * dz < 1e-6;
                     }
-                    for (int t = k.x; t <= k.y; t++) hs[sl++] = near ? ~(int)p : (int)p;
+                    const int64_t n = k.y - k.x + 1;
+                    if (near)
+                        for (int64_t t = 0; t < n; t++) near_slots[r].push_back(sl + t);
+                    sl += n;
                 }
             }
         });
+        auto& hsing = h_sing[set];
         hsing.clear();
-        for (int64_t sl = 0; sl < slots; sl++)
-            if (hs[sl] < 0) hsing.push_back(sl);
-        slot_pair.upload(hs.data(), hs.size(), c.stream);
-        pair_slot.upload(hp.data(), hp.size(), c.stream);
+        for (const auto& v : near_slots) hsing.insert(hsing.end(), v.begin(), v.end());
+        const int64_t npairs = (int64_t)nb * c.nel;
+        pair_slot.upload(hp.p, (size_t)npairs, c.stream);
         if (!ident) d_rows.upload(hr.data(), hr.size(), c.stream);
         if (!hsing.empty()) sing_slots.upload(hsing.data(), hsing.size(), c.stream);
         TD_HIP(hipEventRecord(uploaded[set], c.stream));
+        if (slots > 0) {
+            hipLaunchKernelGGL(k_slot_fill, dim3(grid_for(npairs)), dim3(kBlock), 0, c.stream,
+                               pair_slot.p, dpk, ident ? nullptr : d_rows.p, b0, c.nel, npairs,
+                               slot_pair.p);
+            TD_HIP(hipGetLastError());
+            if (!hsing.empty()) {
+                hipLaunchKernelGGL(k_slot_mark, dim3(grid_for((int64_t)hsing.size())),
+                                   dim3(kBlock), 0, c.stream, sing_slots.p,
+                                   (int64_t)hsing.size(), slot_pair.p);
+                TD_HIP(hipGetLastError());
+            }
+        }
         const int* rows = ident ? nullptr : d_rows.p;
         if (slots > 0) {
             lag_timer.begin();
