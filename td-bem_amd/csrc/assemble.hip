@@ -1363,11 +1363,12 @@ void slot_pass(Context& c, const std::vector<int>& order, int align, const int2*
     slot_pair.reserve((size_t)max_slots);
     pair_slot.reserve((size_t)max_rows * c.nel);
     if (!ident) d_rows.reserve((size_t)max_rows);
-    // The host builds batch b + 1's pair offsets while the device works on batch b: two pinned
-    // host sets, each reused only after the event that follows its upload; the device buffers are
+    // The host builds batch b + 1's pair offsets while the device works on batch b: two host
+    // sets, each reused only after the event that follows its upload; the device buffers are
     // rewritten in stream order after the previous batch's kernels.  The slot list itself is
-    // expanded on the device (k_slot_fill) from the pairs' first slots.
-    HostBuf<int64_t> h_pair_slot[2];
+    // expanded on the device (k_slot_fill) from the pairs' first slots.  (Pageable host sets: a
+    // pinned allocation would be freed with hipHostFree, which waits for the whole device.)
+    std::vector<int64_t> h_pair_slot[2];
     std::vector<int64_t> h_sing[2];
     std::vector<int> h_rows[2];
     DBuf<int64_t> sing_slots(c.stream);
@@ -1382,7 +1383,7 @@ void slot_pass(Context& c, const std::vector<int>& order, int align, const int2*
         if (b >= 2) TD_HIP(hipEventSynchronize(uploaded[set]));
         auto& hp = h_pair_slot[set];
         auto& hr = h_rows[set];
-        hp.reserve((size_t)max_rows * c.nel);
+        hp.resize((size_t)nb * c.nel);
         hr.resize(nb);
         std::vector<int64_t> row_start(nb + 1, 0);
         for (int r = 0; r < nb; r++) {
@@ -1423,7 +1424,7 @@ void slot_pass(Context& c, const std::vector<int>& order, int align, const int2*
         hsing.clear();
         for (const auto& v : near_slots) hsing.insert(hsing.end(), v.begin(), v.end());
         const int64_t npairs = (int64_t)nb * c.nel;
-        pair_slot.upload(hp.p, (size_t)npairs, c.stream);
+        pair_slot.upload(hp.data(), (size_t)npairs, c.stream);
         if (!ident) d_rows.upload(hr.data(), hr.size(), c.stream);
         if (!hsing.empty()) sing_slots.upload(hsing.data(), hsing.size(), c.stream);
         TD_HIP(hipEventRecord(uploaded[set], c.stream));

@@ -79,30 +79,37 @@ __global__ void __launch_bounds__(256)
     const int i = perm[pos], p = r % 3;
     const int64_t li = pos - P0;
     const int sym[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-    f32x4 out[2];
-    int last = -1;
-    int64_t en = -1;   // entry of (i, j, k), -1 when the pair has no lag k
+    // the lane's columns c0 .. c0 + 7 (c0 a multiple of 8) cover node columns j0 = c0 / 6 and
+    // j0 + 1; both pairs are looked up at once and every value load is issued before any is used
+    // (the segment holds at least one chunk whenever this kernel runs, so a missing entry reads
+    // entry 0 and is replaced by zero)
+    const int c0 = 32 * kt + 8 * (lane >> 4);
+    const int j0 = c0 / 6;
+    int64_t en[2];
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+        const int j = perm[J * kTile + j0 + b];
+        const int64_t cell = li * nnd + (j >= 0 ? j : 0);
+        const int2 h = hull[cell];
+        const int64_t o = off[cell];
+        en[b] = (i >= 0 && j >= 0 && h.x <= k && k <= h.y) ? o + (k - h.x) : -1;
+    }
+    double v[8];
 #pragma unroll
     for (int e = 0; e < 8; e++) {
-        const int c = 32 * kt + 8 * (lane >> 4) + e;
-        const int jj = c / 6;
+        const int c = c0 + e;
+        const int b = c / 6 - j0;
         const int part = (c % 6) / 3, q = c % 3;
-        if (jj != last) {
-            last = jj;
-            en = -1;
-            const int j = perm[J * kTile + jj];
-            if (i >= 0 && j >= 0) {
-                const int2 h = hull[li * nnd + j];
-                if (h.x <= k && k <= h.y) en = off[li * nnd + j] + (k - h.x);
-            }
-        }
-        double v = 0.0;
-        if (en >= 0) {
-            const int comp = part == 0 ? sym[p][q] : 6 + 3 * p + q;
-            v = seg[((en / kChunk) * kVals + comp) * kChunk + en % kChunk];
-            if (part) v = -v;
-        }
-        out[e >> 2][e & 3] = (float)v;
+        const int comp = part == 0 ? sym[p][q] : 6 + 3 * p + q;
+        const int64_t x = en[b] >= 0 ? en[b] : 0;
+        v[e] = seg[((x / kChunk) * kVals + comp) * kChunk + x % kChunk];
+    }
+    f32x4 out[2];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int c = c0 + e;
+        const int b = c / 6 - j0;
+        out[e >> 2][e & 3] = en[b] >= 0 ? (float)((c % 6) >= 3 ? -v[e] : v[e]) : 0.0f;
     }
 #pragma unroll
     for (int h = 0; h < 2; h++) tp[(sidx * 2 + h) * 64 + lane] = out[h];
