@@ -27,6 +27,10 @@ namespace tdbem {
 namespace {
 
 constexpr int kBlock = 256;
+// The lag kernel's workgroups are single wavefronts of at most 256 registers (two per SIMD for
+// 4-node elements): beside a running march they start wherever one SIMD has room, instead of
+// waiting for a whole CU.
+constexpr int kLagBlock = 64;
 
 inline unsigned grid_for(int64_t n, int block = kBlock)
 {
@@ -704,7 +708,7 @@ __device__ __forceinline__ void integrate_pair_pulse(const double* coords, const
 // over those slots (list sing_slots, nsing entries) with the full integrator, the regular launch
 // over all the others with the regular-only one.
 template <int ENND, bool SINGULAR>
-__global__ void __launch_bounds__(kBlock, (ENND == 4 && !SINGULAR) ? 2 : 1)
+__global__ void __launch_bounds__(kLagBlock, ENND == 4 ? 2 : 1)
     k_pair_lags(const double* __restrict__ coords, const int* __restrict__ ien,
                 const double* __restrict__ geo, const double* __restrict__ gauss, int nq, int nel,
                 double dt, Material m, int Kc, int i0, int64_t nslots,
@@ -1443,7 +1447,8 @@ void slot_pass(Context& c, const std::vector<int>& order, int align, const int2*
         const int* rows = ident ? nullptr : d_rows.p;
         if (slots > 0) {
             lag_timer.begin();
-            hipLaunchKernelGGL((k_pair_lags<ENND, false>), dim3(grid_for(slots)), dim3(kBlock), 0,
+            hipLaunchKernelGGL((k_pair_lags<ENND, false>), dim3(grid_for(slots, kLagBlock)),
+                               dim3(kLagBlock), 0,
                                c.stream, c.coords.p, c.ien.p, c.geo.p, c.gauss.p, c.nq, c.nel,
                                c.dt, c.mat, c.Kc, b0, slots, slot_pair.p, pair_slot.p, dpk,
                                sing_slots.p, (int64_t)0, scratch.p,
@@ -1451,8 +1456,8 @@ void slot_pass(Context& c, const std::vector<int>& order, int align, const int2*
             TD_HIP(hipGetLastError());
             const int64_t nsing = (int64_t)hsing.size();
             if (nsing > 0) {
-                hipLaunchKernelGGL((k_pair_lags<ENND, true>), dim3(grid_for(nsing)),
-                                   dim3(kBlock), 0, c.stream, c.coords.p, c.ien.p, c.geo.p,
+                hipLaunchKernelGGL((k_pair_lags<ENND, true>), dim3(grid_for(nsing, kLagBlock)),
+                                   dim3(kLagBlock), 0, c.stream, c.coords.p, c.ien.p, c.geo.p,
                                    c.gauss.p, c.nq, c.nel, c.dt, c.mat, c.Kc, b0, slots,
                                    slot_pair.p, pair_slot.p, dpk, sing_slots.p, nsing,
                                    scratch.p, prof ? c.counters.p : nullptr, rows);
