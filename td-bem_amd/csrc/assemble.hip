@@ -1661,10 +1661,13 @@ void build_mixed(Context& c)
                            c.pair_k.p, np, 1, pk01.p);
         TD_HIP(hipGetLastError());
         std::vector<int2> hpk01(np);
-        for (int64_t t = 0; t < np; t++) {
-            const int2 v = hpk[t];
-            hpk01[t] = (v.y < v.x || v.x > 1) ? make_int2(1, 0) : make_int2(v.x, std::min(v.y, 1));
-        }
+        parallel_for(np, [&](int64_t lo, int64_t hi) {
+            for (int64_t t = lo; t < hi; t++) {
+                const int2 v = hpk[t];
+                hpk01[t] = (v.y < v.x || v.x > 1) ? make_int2(1, 0)
+                                                  : make_int2(v.x, std::min(v.y, 1));
+            }
+        }, 1 << 16);
         ShellPlan P01;
         plan_shells(c, P01, pk01.p);
         build_fp64_store<ENND>(c, P01, hpk01.data(), pk01.p);
