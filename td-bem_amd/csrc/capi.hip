@@ -68,7 +68,9 @@ Element::~Element()
 
 Context::~Context()
 {
+    // the stream-ordered buffers go before the stream they are ordered on
     march_free(*this);
+    tp.d_sub_id.reset();
     if (stream) (void)hipStreamDestroy(stream);
 }
 

@@ -25,6 +25,15 @@
 namespace tdbem {
 
 struct March {
+    // every buffer stream-ordered on the context's stream: a re-initialisation or the context's
+    // end frees them without a device-wide synchronisation (which would wait for the kernels of
+    // every other context's set-up and march)
+    explicit March(hipStream_t st)
+        : Af(st), Xf(st), parts(st), gF(st), gU(st), wF(st), wU(st), ring(st), hist(st), U(st),
+          T(st), is_ubc(st), is_fbc(st), ubc_val(st), fbc_val(st), fac(st), d_amp(st),
+          d_rise(st), zs(st), partial(st), carry(st)
+    {
+    }
     int N = 0, nrhs = 1, R = 0, s = 0;
     double a1 = 0, a2 = 0, a3 = 0;
     std::vector<double> amp, t_rise;
@@ -358,7 +367,7 @@ void march_init(Context& c, double a1, double a2, int nrhs, const double* amp,
     march_free(c);
     // the new march state becomes the context's only when it is complete (a failure below
     // leaves no march, never a partly built one)
-    std::unique_ptr<March> owner(new March());
+    std::unique_ptr<March> owner(new March(c.stream));
     March& M = *owner;
     M.N = N;
     M.nrhs = nrhs;

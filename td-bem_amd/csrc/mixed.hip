@@ -510,6 +510,9 @@ void tp_alloc(Context& c)
     if ((int64_t)T.h_sub_id.size() != T.nsub)
         throw std::logic_error("tile store plan without its sub-tile list");
     T.vals.alloc((size_t)std::max<int64_t>(T.nsub, 1) * kSubFloats);
+    // the sub-tile list lives for one build only: stream-ordered, so that its free at the end
+    // of the build does not wait for the whole device
+    T.d_sub_id.order_on(c.stream);
     T.d_sub_id.upload(T.h_sub_id.data(), T.h_sub_id.size(), c.stream);
     // a large list is not kept on the host: the next build plans again
     if (T.nsub * (int64_t)sizeof(uint2) > kPinnedKeepBytes) std::vector<uint2>().swap(T.h_sub_id);

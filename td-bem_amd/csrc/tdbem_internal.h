@@ -35,6 +35,13 @@ struct DBuf {
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
     ~DBuf() { reset(); }
+    // stream-ordered on s from the next allocation on (what the buffer holds is freed first)
+    void order_on(hipStream_t s)
+    {
+        reset();
+        ord = s;
+        ordered = true;
+    }
     void reset()
     {
         if (p) {
