@@ -284,6 +284,8 @@ bool refine_first(const double* A2, int N, const double* Af, double c0, const do
     };
     const double bn[2] = {norm(hb, 0), norm(hb, 1)};
     double prev = 1e300;
+    // the residual is read back (and the host waits) after every kCheck iterations only
+    constexpr int kCheck = 4;
     for (int it = 0; it < 200; it++) {
         hipLaunchKernelGGL(k_frag_pack, dim3(blocks(n2)), dim3(256), 0, s, r.p, N, 2, Xf.p);
         apply_frag(Af, N, Xf.p, 2, parts.p, s);
@@ -293,6 +295,7 @@ bool refine_first(const double* A2, int N, const double* Af, double c0, const do
         gemv(A2, N, x + N, ax.p + N, 0.0, s);
         hipLaunchKernelGGL(k_sub, dim3(blocks(n2)), dim3(256), 0, s, b, ax.p, n2, r.p);
         TD_HIP(hipGetLastError());
+        if (it % kCheck != kCheck - 1) continue;
         TD_HIP(hipMemcpyAsync(hr.data(), r.p, n2 * sizeof(double), hipMemcpyDeviceToHost, s));
         TD_HIP(hipMemcpyAsync(hax.data(), ax.p, n2 * sizeof(double), hipMemcpyDeviceToHost, s));
         TD_HIP(hipStreamSynchronize(s));
@@ -304,7 +307,8 @@ bool refine_first(const double* A2, int N, const double* Af, double c0, const do
             if (scale > 0) rel = std::max(rel, rn / scale);
         }
         if (rel <= 1e-15) return true;
-        if (it >= 4 && rel > 0.7 * prev) return rel <= 1e-14;
+        // stalled: kCheck more iterations gained less than a factor 0.7
+        if (it >= 2 * kCheck && rel > 0.7 * prev) return rel <= 1e-14;
         prev = rel;
     }
     return false;
