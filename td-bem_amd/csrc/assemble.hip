@@ -1313,8 +1313,12 @@ void plan_from_hull(Context& c, ShellPlan& P, bool offsets)
 // the gather from its slots; the slot buffers are reused by the next batch in stream order.
 template <int ENND, typename Consume>
 void slot_pass(Context& c, const std::vector<int>& order, int align, const int2* hpk,
-               const int2* dpk, Consume&& consume)
+               const int2* dpk, int kmax, Consume&& consume)
 {
+    // the host ranges clipped to lags <= kmax as k_clip_pairs clips the device ones
+    auto range = [kmax](int2 k) {
+        return (k.y < k.x || k.x > kmax) ? make_int2(1, 0) : make_int2(k.x, std::min(k.y, kmax));
+    };
     const bool ident = order.empty();
     const int npos = ident ? c.nnd : (int)order.size();
     auto node_at = [&](int pos) { return ident ? pos : order[pos]; };
@@ -1334,7 +1338,7 @@ void slot_pass(Context& c, const std::vector<int>& order, int align, const int2*
             if (i < 0) continue;
             int64_t n = 0;
             for (int e = 0; e < c.nel; e++) {
-                const int2 k = hpk[(int64_t)i * c.nel + e];
+                const int2 k = range(hpk[(int64_t)i * c.nel + e]);
                 if (k.y >= k.x) n += k.y - k.x + 1;
             }
             rs[q] = n;
@@ -1407,7 +1411,7 @@ void slot_pass(Context& c, const std::vector<int>& order, int align, const int2*
                 const double* xc = &c.h_coords[3 * (int64_t)i];
                 for (int e = 0; e < c.nel; e++) {
                     hp[r * c.nel + e] = sl;
-                    const int2 k = hpk[(int64_t)i * c.nel + e];
+                    const int2 k = range(hpk[(int64_t)i * c.nel + e]);
                     if (k.y < k.x) continue;
                     // the device's singular test is a squared distance below 1e-9; a pair goes
                     // to the singular launch well before that (squared distance below 1e-6)
@@ -1482,11 +1486,11 @@ void slot_pass(Context& c, const std::vector<int>& order, int align, const int2*
 // The FP64 store of the rows' in-shell lags (pair ranges hpk / dpk, layout P): slots of
 // contiguous row ranges, gathered per (i, j, k) entry by k_store_gather.
 template <int ENND>
-void fill_store(Context& c, const ShellPlan& P, const int2* hpk, const int2* dpk)
+void fill_store(Context& c, const ShellPlan& P, const int2* hpk, const int2* dpk, int kmax)
 {
     LagStore& S = c.store;
     const double ext = c.if_ext ? 0.5 * c.dt : 0.0;
-    slot_pass<ENND>(c, std::vector<int>(), 1, hpk, dpk,
+    slot_pass<ENND>(c, std::vector<int>(), 1, hpk, dpk, kmax,
                     [&](int i0, int i1, const double* scratch, int64_t nslots,
                         const int64_t* pair_slot) {
                         const int64_t ch0 = P.row_chunk[i0], ch1 = P.row_chunk[i1];
@@ -1529,10 +1533,11 @@ void alloc_store(Context& c, const ShellPlan& P)
 
 // Allocates the FP64 store of layout P and fills it from the lag kernel's slots.
 template <int ENND>
-void build_fp64_store(Context& c, const ShellPlan& P, const int2* hpk, const int2* dpk)
+void build_fp64_store(Context& c, const ShellPlan& P, const int2* hpk, const int2* dpk,
+                      int kmax = kMaxLag)
 {
     alloc_store(c, P);
-    fill_store<ENND>(c, P, hpk, dpk);
+    fill_store<ENND>(c, P, hpk, dpk, kmax);
 }
 
 // The pair ranges of c.pair_k on the host (pinned, kept between builds unless large).
@@ -1588,7 +1593,7 @@ void build_mixed(Context& c)
         int nbatch = 0;
         try {
             slot_pass<ENND>(
-                c, c.tp.perm, kTile, hpk, c.pair_k.p,
+                c, c.tp.perm, kTile, hpk, c.pair_k.p, kMaxLag,
                 [&](int b0, int b1, const double* scratch, int64_t nslots,
                     const int64_t* pair_slot) {
                     const int set = nbatch & 1;
@@ -1660,17 +1665,10 @@ void build_mixed(Context& c)
         hipLaunchKernelGGL(k_clip_pairs, dim3(grid_for(np)), dim3(kBlock), 0, c.stream,
                            c.pair_k.p, np, 1, pk01.p);
         TD_HIP(hipGetLastError());
-        std::vector<int2> hpk01(np);
-        parallel_for(np, [&](int64_t lo, int64_t hi) {
-            for (int64_t t = lo; t < hi; t++) {
-                const int2 v = hpk[t];
-                hpk01[t] = (v.y < v.x || v.x > 1) ? make_int2(1, 0)
-                                                  : make_int2(v.x, std::min(v.y, 1));
-            }
-        }, 1 << 16);
         ShellPlan P01;
         plan_shells(c, P01, pk01.p);
-        build_fp64_store<ENND>(c, P01, hpk01.data(), pk01.p);
+        // the slot pass clips the host ranges to lags 0..1 itself, as pk01 on the device
+        build_fp64_store<ENND>(c, P01, hpk, pk01.p, 1);
     }
 }
 
