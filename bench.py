@@ -25,10 +25,12 @@ instead.  traffic: HBM bytes per launch measured in this run by two rocprofv3 --
 by what the counter reads for a 1 GiB k_stream_copy; null when rocprofv3 is absent (--no-pmc
 skips it).  plain_fp64.roofline gives k_conv the same way (stored entries x 124 B).  assembly:
 the lag kernel k_pair_lags against the FP64 vector peak, ASM_FLOPS_PER_POINT flops per active
-(quadrature point, lag) evaluation plus ASM_FLOPS_PER_LEVEL per time level evaluated there.
+(quadrature point, lag) evaluation plus ASM_FLOPS_PER_LEVEL per time level evaluated there (the
+operation counts of the formulas as the oracle states them; the kernel executes fewer: per-level
+constants are hoisted and each level forms only the hats the lag combination needs).
 batched and serving: the continuous-batching engine over half-space scenarios and Poisson
-request streams at their stated sizes.  Printed as one JSON line; --gpus N > 1 exits (one GPU
-per box in this build).
+request streams at their stated sizes.  Printed as one JSON line; --gpus N > 1 exits (the path
+runs on one GPU).
 """
 import argparse
 import json
@@ -333,7 +335,7 @@ def main():
         traffic_probe(args.nrhs)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "the path runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
     from tdbem import api, configs
 
